@@ -1,0 +1,217 @@
+/*
+ * mom5_gpu.h -- C-ABI of the MI355X implementation of MOM5's ocean hot path.
+ *
+ * Each entry point replaces one reference routine of mom-ocean/MOM5 and keeps
+ * its argument list, with the derived-type fields (Grd%, Thickness%, Adv_vel%,
+ * T_prog(n)%, Ext_mode%) flattened to plain pointers.  Paths in the comments
+ * are relative to src/mom5/ of the reference.
+ *
+ * Conventions
+ *  - Every array pointer is a DEVICE pointer, except the pointer tables
+ *    (`const double *const *`), which are HOST arrays of device pointers.
+ *  - Arrays are the Fortran memory images MOM passes around: column-major,
+ *    i fastest, halos included.  A 2-D field is (isd:ied,jsd:jed); a 3-D
+ *    field (isd:ied,jsd:jed,1:nk); wrho_bt and dzwt are (isd:ied,jsd:jed,0:nk).
+ *    A time level is passed as the address of field(isd,jsd,1,tau).
+ *  - Only a one-cell halo is supported (MOM's default ocean domain):
+ *    isd = isc-1, ied = iec+1, jsd = jsc-1, jed = jec+1.
+ *  - One context owns one domain (a whole global grid or one regional
+ *    domain) and one HIP stream.  Calls are asynchronous on that stream;
+ *    mom_gpu_sync() waits.  A context is not thread-safe; separate contexts
+ *    may be driven from separate threads.
+ *  - Every call returns MOM_GPU_OK (0) or a negative error code and leaves a
+ *    message in mom_gpu_last_error(ctx).  No exception crosses the ABI.
+ */
+#ifndef MOM5_GPU_H
+#define MOM5_GPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MOM_GPU_OK               0
+#define MOM_GPU_ERR_ARG         -1   /* bad argument or unsupported option   */
+#define MOM_GPU_ERR_HIP         -2   /* HIP runtime failure                  */
+#define MOM_GPU_ERR_NOMEM       -3   /* device allocation failed             */
+#define MOM_GPU_ERR_GRID        -4   /* inconsistent grid (e.g. tmask/kmt)   */
+
+/* Advection scheme ids, ocean_core/ocean_parameters.F90:148-163. */
+#define MOM_ADVECT_QUICKER       5
+#define MOM_ADVECT_MDFL_SWEBY    9
+#define MOM_ADVECT_DST_LINEAR   10
+
+typedef struct mom_gpu_ctx mom_gpu_ctx_t;
+
+/* Domain bounds (Fortran, 1-based) -- ocean_core/ocean_types.F90:922-935
+ * (ocean_domain_type) with the cyclic flags of ocean_domains.F90:123-230. */
+typedef struct {
+  int isd, ied, jsd, jed;     /* data domain  */
+  int isc, iec, jsc, jec;     /* compute domain */
+  int nk;
+  int cyclic_x, cyclic_y;
+} mom_gpu_domain_t;
+
+/* Grid fields (ocean_core/ocean_types.F90:747-920, ocean_grid_type) on the
+ * data domain.  tmask/umask are (isd:ied,jsd:jed,nk); dzt is (1:nk).
+ * mom_gpu_create() copies what it needs; the caller may free these after. */
+typedef struct {
+  const double *dxt, *dyt, *dat, *datr;
+  const double *dxte, *dyte, *dxtn, *dytn;
+  const double *dxu, *dyu, *dxur, *dyur;
+  const double *f;
+  const int    *kmt, *kmu;
+  const double *tmask, *umask;
+  const double *dzt;
+} mom_gpu_grid_t;
+
+/* Barotropic options (ocean_barotropic.F90:600-630, ocean_barotropic_nml). */
+typedef struct {
+  int    nts;                 /* subcycles per baroclinic step, :1574     */
+  double dtbt;                /* barotropic time step (s)                 */
+  double pred_corr_gamma;     /* :614, default 0.2                        */
+  double rho0;                /* ocean_parameters.F90:194, 1035           */
+  int    stepping_a;          /* 1: barotropic_time_stepping_A, 0: B      */
+  int    initsum_with_bar;    /* :609, default true                       */
+} mom_gpu_bt_params_t;
+
+/* ---- lifecycle -------------------------------------------------------- */
+
+/* Replaces the grid/mask set-up of ocean_tracer_advect_init (quicker_init
+ * :1442-1586, mdfl_init :1644-1691, ocean_tracer_advect.F90:507) and of
+ * ocean_barotropic_init (ocean_barotropic.F90:946, :1255-1288).  Verifies
+ * tmask == (k <= kmt) and umask == (k <= kmu) on the data domain
+ * (ocean_grids.F90:1374-1386). */
+int mom_gpu_create(const mom_gpu_domain_t *dom, const mom_gpu_grid_t *grid,
+                   mom_gpu_ctx_t **ctx);
+int mom_gpu_destroy(mom_gpu_ctx_t *ctx);
+int mom_gpu_sync(mom_gpu_ctx_t *ctx);
+/* Message of the last failing call on ctx (ctx == NULL: last failure of a
+ * call that had no context, e.g. mom_gpu_create). */
+const char *mom_gpu_last_error(const mom_gpu_ctx_t *ctx);
+
+/* Device memory helpers for hosts without a GPU allocator (the Fortran
+ * host keeps the returned addresses as type(c_ptr)). */
+int mom_gpu_malloc(void **ptr, size_t bytes);
+int mom_gpu_free(void *ptr);
+int mom_gpu_memcpy_htod(void *dst, const void *src, size_t bytes);
+int mom_gpu_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int mom_gpu_memset(void *ptr, int value, size_t bytes);
+
+/* Elapsed time in ms between two marks recorded on the context's stream.
+ * mom_gpu_mark(ctx, slot) records slot 0..7; mom_gpu_elapsed waits. */
+int mom_gpu_mark(mom_gpu_ctx_t *ctx, int slot);
+int mom_gpu_elapsed(mom_gpu_ctx_t *ctx, int slot0, int slot1, float *ms);
+
+/* ---- tracer advection ------------------------------------------------ */
+
+/* Replaces horz_advect_tracer, ocean_tracer_advect.F90:1898-2083, for the
+ * MDFL_SWEBY (advect_tracer_mdfl_sweby :3806-4066, sweby_limiter=1),
+ * DST_LINEAR (the same routine with sweby_limiter=0) and QUICKER
+ * (horz_advect_tracer_quicker :2538-2653) schemes, with
+ * advect_sweby_all=.false.:
+ *   wrk1 = 0 on the data domain; wrk1(comp) = -scheme(...);
+ *   th_tendency(comp) += wrk1.
+ * T_taum1 = Tracer%field(:,:,:,taum1); T_tau = Tracer%field(:,:,:,tau)
+ * (QUICKER only); rho_dzt_tau = Thickness%rho_dzt(:,:,:,tau) (MDFL only);
+ * tmask_limit = Tracer%tmask_limit (QUICKER with limit_with_upwind only).
+ * wrk1 and flux_x/flux_y/flux_z (the module diagnostics arrays, data
+ * domain) may be NULL. */
+int mom_gpu_horz_advect_tracer(mom_gpu_ctx_t *ctx, int scheme,
+    const double *T_taum1, const double *T_tau,
+    const double *uhrho_et, const double *vhrho_nt, const double *wrho_bt,
+    const double *rho_dzt_tau, const double *tmask_limit,
+    int limit_with_upwind, double dtime,
+    double *th_tendency, double *wrk1,
+    double *flux_x, double *flux_y, double *flux_z);
+
+/* Replaces vert_advect_tracer, ocean_tracer_advect.F90:2095-2226: QUICKER
+ * (vert_advect_tracer_quicker :2981-3031; tmask_limit is always read, :3006)
+ * adds its tendency to th_tendency; the three-dimensional MDFL schemes are
+ * a no-op here. */
+int mom_gpu_vert_advect_tracer(mom_gpu_ctx_t *ctx, int scheme,
+    const double *T_taum1, const double *T_tau, const double *wrho_bt,
+    const double *tmask_limit, double *th_tendency, double *wrk1,
+    double *flux_z);
+
+/* Replaces advect_tracer_sweby_all, ocean_tracer_advect.F90:4104-4511
+ * (advect_sweby_all=.true.): all ntr tracers in one call, with that
+ * routine's own rounding order; th_tendency[n](comp) += wrk1(n).
+ * wrk1 may be NULL, or hold NULL entries. */
+int mom_gpu_advect_sweby_all(mom_gpu_ctx_t *ctx, int ntr,
+    const double *const *T_taum1, const double *uhrho_et,
+    const double *vhrho_nt, const double *wrho_bt, const double *rho_dzt_tau,
+    double dtime, double *const *th_tendency, double *const *wrk1);
+
+/* Replaces compute_tmask_limit, ocean_tracer.F90:2788-2892.
+ * index_temp/index_salt are 0-based positions in the tables (-1: absent).
+ * Writes (isc-1:iec, jsc-1:jec, 1:nk) of each tmask_limit, like the
+ * reference; other points are left untouched. */
+int mom_gpu_compute_tmask_limit(mom_gpu_ctx_t *ctx, int ntr,
+    const double *const *T_tau, const double *min_limit,
+    const double *max_limit, int index_temp, int index_salt, int ts_same,
+    double *const *tmask_limit);
+
+/* Replaces the explicit update of update_ocean_tracer, ocean_tracer.F90:
+ * 2341-2350: T_taup1 = (rho_dzt_taum1*T_taum1 + dtime*th_tendency)*rho_dztr
+ * on the compute domain. */
+int mom_gpu_update_tracer_explicit(mom_gpu_ctx_t *ctx, const double *T_taum1,
+    const double *th_tendency, const double *rho_dzt_taum1,
+    const double *rho_dztr, double dtime, double *T_taup1);
+
+/* ---- implicit vertical diffusion -------------------------------------- */
+
+/* Replaces invtri(z, topbc, botbc, dcb, tdt, kmz, mask, dh, dhw, aidif, nk),
+ * ocean_core/ocean_util.F90:153-222 (nk from the context). */
+int mom_gpu_invtri(mom_gpu_ctx_t *ctx, double *z, const double *topbc,
+    const double *botbc, const double *dcb, double tdt, const int *kmz,
+    const double *mask, const double *dh, const double *dhw, double aidif);
+
+/* Replaces the tracer loop of vert_diffuse_implicit,
+ * ocean_param/vertical/ocean_vert_mix.F90:3426-3513: for each tracer
+ * dcb = rho0*diff_cbt(:,:,:,nmix) + K33_implicit(n) (nmix = 2 for
+ * index_salt, else 1; diff_cbt holds both planes back to back) and invtri
+ * with kmz = Grd%kmt, mask = Grd%tmask, dh = rho_dzt_taup1, dhw = dzwt.
+ * K33_implicit may be NULL or hold NULL entries (zero).  Tracers that share
+ * nmix and have no K33 share one factorisation (same arithmetic). */
+int mom_gpu_vert_diffuse_implicit(mom_gpu_ctx_t *ctx, int ntr,
+    double *const *T_taup1, const double *const *stf,
+    const double *const *btf, const double *diff_cbt,
+    const double *const *K33_implicit, int index_salt, double rho0,
+    const double *rho_dzt_taup1, const double *dzwt, double dtime_t,
+    double aidif);
+
+/* ---- barotropic -------------------------------------------------------- */
+
+/* Replaces update_ocean_barotropic (ocean_barotropic.F90:2763-2894) for the
+ * B-grid depth-based case: pred_corr_tropic_depth_bgrid (:3225-3579, with
+ * barotropic_halo=1 and use_legacy_barotropic_halos=.true., the defaults;
+ * no tides, geoid, friction, smoothing or OBC) and the time-averaging tail
+ * (:2804-2811, :2846-2888).  udrho_* and forcing_bt hold the two velocity
+ * components back to back.  conv_rho_ud_t (= -DIV_UD of the averaged
+ * transport) may be NULL.  Supports cyclic_y = 0 only.
+ * Precondition (what mpp_update_domains leaves in MOM on a cyclic-x
+ * domain): halo column isd equals column iec and ied equals isc, on every
+ * row, for the state and for the 2-D inputs. */
+int mom_gpu_update_ocean_barotropic(mom_gpu_ctx_t *ctx,
+    const mom_gpu_bt_params_t *p, const double *eta_t_tau,
+    const double *eta_t_bar_tau, const double *udrho_tau,
+    const double *forcing_bt, const double *thicku, const double *patm,
+    const double *pme, const double *river, const double *source,
+    const double *eta_smooth, const double *rho_g,
+    double *eta_t_bar_taup1, double *udrho_taup1, double *conv_rho_ud_t);
+
+/* ---- diagnostics -------------------------------------------------------- */
+
+/* Masked tracer content sum(T*rho_dzt*dat*tmask) over the compute domain,
+ * the quantity of tracer_integrals (ocean_diag/ocean_tracer_diag.F90:
+ * 2648-2661), accumulated with compensated (double-double) sums.  Waits
+ * for the stream and writes the result to a HOST double. */
+int mom_gpu_tracer_content(mom_gpu_ctx_t *ctx, const double *T,
+    const double *rho_dzt, double *content);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MOM5_GPU_H */

@@ -1,0 +1,101 @@
+"""ctypes binding of libmom5_gpu.so (the C-ABI in include/mom5_gpu.h).
+
+The library is built in-tree by ``python -m mom5_amd.build``.  Loading fails
+loudly when it is missing: there is no CPU fallback anywhere in mom5_amd.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmom5_gpu.so")
+
+MOM_GPU_OK = 0
+MOM_ADVECT_QUICKER = 5
+MOM_ADVECT_MDFL_SWEBY = 9
+MOM_ADVECT_DST_LINEAR = 10
+
+_vp = C.c_void_p
+_dp = C.c_void_p
+
+
+class Domain(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("isd", "ied", "jsd", "jed", "isc", "iec", "jsc", "jec",
+                                        "nk", "cyclic_x", "cyclic_y")]
+
+
+class Grid(C.Structure):
+    _fields_ = [(n, _vp) for n in ("dxt", "dyt", "dat", "datr", "dxte", "dyte", "dxtn", "dytn",
+                                    "dxu", "dyu", "dxur", "dyur", "f", "kmt", "kmu", "tmask",
+                                    "umask", "dzt")]
+
+
+class BtParams(C.Structure):
+    _fields_ = [("nts", C.c_int), ("dtbt", C.c_double), ("pred_corr_gamma", C.c_double),
+                ("rho0", C.c_double), ("stepping_a", C.c_int), ("initsum_with_bar", C.c_int)]
+
+
+# name -> (restype, argtypes)
+_PROTOS = {
+    "mom_gpu_create": (C.c_int, [C.POINTER(Domain), C.POINTER(Grid), C.POINTER(_vp)]),
+    "mom_gpu_destroy": (C.c_int, [_vp]),
+    "mom_gpu_sync": (C.c_int, [_vp]),
+    "mom_gpu_last_error": (C.c_char_p, [_vp]),
+    "mom_gpu_malloc": (C.c_int, [C.POINTER(_vp), C.c_size_t]),
+    "mom_gpu_free": (C.c_int, [_vp]),
+    "mom_gpu_memcpy_htod": (C.c_int, [_vp, _vp, C.c_size_t]),
+    "mom_gpu_memcpy_dtoh": (C.c_int, [_vp, _vp, C.c_size_t]),
+    "mom_gpu_memset": (C.c_int, [_vp, C.c_int, C.c_size_t]),
+    "mom_gpu_mark": (C.c_int, [_vp, C.c_int]),
+    "mom_gpu_elapsed": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_float)]),
+    "mom_gpu_horz_advect_tracer": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                             C.c_int, C.c_double, _dp, _dp, _dp, _dp, _dp]),
+    "mom_gpu_vert_advect_tracer": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "mom_gpu_advect_sweby_all": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), _dp, _dp, _dp, _dp,
+                                           C.c_double, C.POINTER(_vp), C.POINTER(_vp)]),
+    "mom_gpu_compute_tmask_limit": (C.c_int, [_vp, C.c_int, C.POINTER(_vp),
+                                              C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                              C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
+    "mom_gpu_update_tracer_explicit": (C.c_int, [_vp, _dp, _dp, _dp, _dp, C.c_double, _dp]),
+    "mom_gpu_invtri": (C.c_int, [_vp, _dp, _dp, _dp, _dp, C.c_double, _dp, _dp, _dp, _dp,
+                                 C.c_double]),
+    "mom_gpu_vert_diffuse_implicit": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(_vp),
+                                                C.POINTER(_vp), _dp, C.POINTER(_vp), C.c_int,
+                                                C.c_double, _dp, _dp, C.c_double, C.c_double]),
+    "mom_gpu_update_ocean_barotropic": (C.c_int, [_vp, C.POINTER(BtParams), _dp, _dp, _dp, _dp,
+                                                  _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                                  _dp]),
+    "mom_gpu_tracer_content": (C.c_int, [_vp, _dp, _dp, C.POINTER(C.c_double)]),
+}
+
+EXPORTED = tuple(_PROTOS)
+
+_lib = None
+
+
+class MomGpuError(RuntimeError):
+    pass
+
+
+def load(path: str | None = None):
+    """Load (once) and return the library; raise if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise MomGpuError(f"{p} not built: run `python -m mom5_amd.build` (no CPU fallback)")
+    lib = C.CDLL(p)
+    for name, (res, args) in _PROTOS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int, ctx=None, what: str = ""):
+    if rc != MOM_GPU_OK:
+        msg = load().mom_gpu_last_error(ctx)
+        raise MomGpuError(f"{what} failed ({rc}): {msg.decode(errors='replace') if msg else ''}")

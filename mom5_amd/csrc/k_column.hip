@@ -1,0 +1,427 @@
+// Column and point-wise kernels: invtri and the tracer loop of
+// vert_diffuse_implicit, compute_tmask_limit, the explicit tracer update,
+// the masked tracer content and the grid mask check.
+#include "mom_internal.h"
+
+namespace {
+
+constexpr double EPS_TRI = 1.0e-30;   // ocean_util.F90:175
+
+// ---- invtri, ocean_util.F90:153-222 -------------------------------------
+// One thread per column, i fastest across the wavefront so every level is
+// one coalesced row.  The forward sweep writes z(k) in place and e(k) to a
+// work array; the back substitution re-reads both.
+__global__ void k_invtri(KDom d, double *__restrict__ z, const double *__restrict__ topbc,
+                         const double *__restrict__ botbc, const double *__restrict__ dcb,
+                         double tdt, const int *__restrict__ kmz,
+                         const double *__restrict__ mask, const double *__restrict__ dh,
+                         const double *__restrict__ dhw, double aidif,
+                         double *__restrict__ ework) {
+  const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gj = blockIdx.y;
+  if (gi >= d.ni) return;
+  const int nk = d.nk, P2 = d.P2;
+  const int o = d2(d, gi, gj);
+  const double tdt_aidif = tdt * aidif;
+  const int kb = max(2, kmz[o]);
+  double cprev = 0.0, bet = 0.0, zprev = 0.0;
+  for (int k = 1; k <= nk; ++k) {
+    const int km1 = max(1, k - 1), kp1 = min(k + 1, nk);
+    const int ok = o + P2 * (k - 1);
+    const double dhk = dh[ok];
+    const double factu = tdt_aidif / (dhw[o + P2 * (k - 1)] * dhk);
+    const double factl = tdt_aidif / (dhw[o + P2 * k] * dhk);
+    const double mkk = mask[ok];
+    double a = -dcb[o + P2 * (km1 - 1)] * factu * mkk;
+    double c = -dcb[ok] * factl * mask[o + P2 * (kp1 - 1)];
+    if (k == 1) a = 0.0;
+    if (k == nk) c = 0.0;
+    const double b = 1.0 - a - c;
+    double f;
+    if (k == 1)
+      f = z[ok] + topbc[o] * tdt_aidif * mkk / dhk;
+    else if (k == kb)
+      f = z[ok] - botbc[o] * tdt_aidif * mkk / dhk;
+    else
+      f = z[ok] * mkk;
+    double zk;
+    if (k == 1) {
+      bet = mkk / (b + EPS_TRI);
+      zk = f * bet;
+    } else {
+      const double e = cprev * bet;
+      bet = mkk / (b - a * e + EPS_TRI);
+      zk = (f - a * zprev) * bet;
+      ework[ok] = e;
+    }
+    z[ok] = zk;
+    zprev = zk;
+    cprev = c;
+  }
+  double znext = zprev;
+  for (int k = nk - 1; k >= 1; --k) {
+    const int ok = o + P2 * (k - 1);
+    const double zk = z[ok] - ework[ok + P2] * znext;
+    z[ok] = zk;
+    znext = zk;
+  }
+}
+
+// ---- vert_diffuse_implicit, ocean_vert_mix.F90:3462-3491 -----------------
+// dcb = rho0*diff_cbt(nmix) + K33 formed on the fly; up to NT tracers that
+// share the same dcb share one factorisation (e, bet), identical arithmetic
+// to a per-tracer invtri.
+constexpr int NT = 8;
+struct TriGroup {
+  double *z[NT];
+  const double *stf[NT];
+  const double *btf[NT];
+  const double *k33;   // NULL: zero (only for a group of one tracer)
+  int n;
+};
+
+__global__ void k_vdiff(KDom d, TriGroup g, const double *__restrict__ dcbase, double rho0,
+                        const int *__restrict__ kmt, const double *__restrict__ dh,
+                        const double *__restrict__ dhw, double tdt, double aidif,
+                        double *__restrict__ ework) {
+  const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gj = blockIdx.y;
+  if (gi >= d.ni) return;
+  const int nk = d.nk, P2 = d.P2;
+  const int o = d2(d, gi, gj);
+  const double tdt_aidif = tdt * aidif;
+  const int kmo = kmt[o];
+  const int kb = max(2, kmo);
+  double cprev = 0.0, bet = 0.0;
+  double zprev[NT];
+#pragma unroll
+  for (int n = 0; n < NT; ++n) zprev[n] = 0.0;
+  double dcb_km1 = 0.0;
+  for (int k = 1; k <= nk; ++k) {
+    const int kp1 = min(k + 1, nk);
+    const int ok = o + P2 * (k - 1);
+    const double dcb_k = rho0 * dcbase[ok] + (g.k33 ? g.k33[ok] : 0.0);
+    if (k == 1) dcb_km1 = dcb_k;
+    const double dhk = dh[ok];
+    const double factu = tdt_aidif / (dhw[o + P2 * (k - 1)] * dhk);
+    const double factl = tdt_aidif / (dhw[o + P2 * k] * dhk);
+    const double mkk = (k <= kmo) ? 1.0 : 0.0;
+    const double mkp1 = (kp1 <= kmo) ? 1.0 : 0.0;
+    double a = -dcb_km1 * factu * mkk;
+    double c = -dcb_k * factl * mkp1;
+    if (k == 1) a = 0.0;
+    if (k == nk) c = 0.0;
+    const double b = 1.0 - a - c;
+    double e = 0.0;
+    if (k == 1) {
+      bet = mkk / (b + EPS_TRI);
+    } else {
+      e = cprev * bet;
+      bet = mkk / (b - a * e + EPS_TRI);
+      ework[ok] = e;
+    }
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+      if (n < g.n) {
+        double *z = g.z[n];
+        double f;
+        if (k == 1)
+          f = z[ok] + g.stf[n][o] * tdt_aidif * mkk / dhk;
+        else if (k == kb)
+          f = z[ok] - g.btf[n][o] * tdt_aidif * mkk / dhk;
+        else
+          f = z[ok] * mkk;
+        const double zk = (k == 1) ? f * bet : (f - a * zprev[n]) * bet;
+        z[ok] = zk;
+        zprev[n] = zk;
+      }
+    }
+    cprev = c;
+    dcb_km1 = dcb_k;
+  }
+  for (int k = nk - 1; k >= 1; --k) {
+    const int ok = o + P2 * (k - 1);
+    const double e1 = ework[ok + P2];
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+      if (n < g.n) {
+        double *z = g.z[n];
+        const double zk = z[ok] - e1 * zprev[n];
+        z[ok] = zk;
+        zprev[n] = zk;
+      }
+    }
+  }
+}
+
+// ---- compute_tmask_limit, ocean_tracer.F90:2788-2892 -----------------------
+constexpr int NL = 32;
+struct LimTable {
+  const double *T[NL];
+  double *lim[NL];
+  double mn[NL], mx[NL];
+  int n;
+};
+
+__global__ void k_tmask_limit(KDom d, LimTable t, int it, int is, int ts_same,
+                              const int *__restrict__ kmt) {
+  const int ii = blockIdx.x * blockDim.x + threadIdx.x;   // gi = ii - 1
+  const int gj = blockIdx.y - 1;
+  const int k = blockIdx.z + 1;
+  const int gi = ii - 1;
+  if (gi > d.ni - 1) return;
+  const int o = d2(d, gi, gj), P2 = d.P2, ok = o + P2 * (k - 1);
+  const double m0 = (k <= kmt[o]) ? 1.0 : 0.0;
+  const double me = (k <= kmt[o + 1]) ? 1.0 : 0.0;
+  const double mn_ = (k <= kmt[o + d.NI]) ? 1.0 : 0.0;
+  const double mb = (k < d.nk && k + 1 <= kmt[o]) ? 1.0 : 0.0;
+  double flagT = 0.0, flagS = 0.0;
+  for (int n = 0; n < t.n; ++n) {
+    const double *T = t.T[n];
+    const double lo = t.mn[n], hi = t.mx[n];
+    double L = 0.0;
+    double x = T[ok];
+    if (m0 == 1.0 && (x < lo || x > hi)) L = m0;
+    x = T[ok + 1];
+    if (me == 1.0 && (x < lo || x > hi)) L = m0;
+    x = T[ok + d.NI];
+    if (mn_ == 1.0 && (x < lo || x > hi)) L = m0;
+    if (k < d.nk) {
+      x = T[ok + P2];
+      if (mb == 1.0 && (x < lo || x > hi)) L = m0;
+    }
+    if (n == it) flagT = L;
+    if (n == is) flagS = L;
+    t.lim[n][ok] = L;
+  }
+  if (ts_same && it >= 0 && is >= 0 && (flagT == 1.0 || flagS == 1.0)) {
+    t.lim[it][ok] = 1.0;
+    t.lim[is][ok] = 1.0;
+  }
+}
+
+// ---- explicit update, ocean_tracer.F90:2341-2350 ---------------------------
+__global__ void k_explicit(KDom d, const double *__restrict__ Tm1, const double *__restrict__ th,
+                           const double *__restrict__ rdz, const double *__restrict__ rdzr,
+                           double dtime, double *__restrict__ Tp1) {
+  const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gj = blockIdx.y;
+  const int k = blockIdx.z;
+  if (gi >= d.ni) return;
+  const int o = d2(d, gi, gj) + d.P2 * k;
+  Tp1[o] = (rdz[o] * Tm1[o] + dtime * th[o]) * rdzr[o];
+}
+
+// ---- masked content with double-double accumulation -------------------------
+struct DD {
+  double hi, lo;
+};
+__device__ inline DD dd_add(DD a, double b) {
+  const double s = a.hi + b;
+  const double bb = s - a.hi;
+  const double err = (a.hi - (s - bb)) + (b - bb);
+  const double lo = a.lo + err;
+  const double hi = s + lo;
+  return {hi, lo - (hi - s)};
+}
+__device__ inline DD dd_add2(DD a, DD b) {
+  DD r = dd_add(a, b.hi);
+  return dd_add(r, b.lo);
+}
+
+constexpr int RB = 256;
+__global__ __launch_bounds__(RB) void k_content(KDom d, const double *__restrict__ T,
+                                                const double *__restrict__ rho,
+                                                const double *__restrict__ dat,
+                                                const int *__restrict__ kmt,
+                                                double *__restrict__ part) {
+  __shared__ double sh[RB], sl[RB];
+  DD acc{0.0, 0.0};
+  const int ncol = d.ni * d.nj;
+  for (int c = blockIdx.x * RB + threadIdx.x; c < ncol; c += gridDim.x * RB) {
+    const int gi = c % d.ni, gj = c / d.ni;
+    const int o = d2(d, gi, gj);
+    const int km = kmt[o];
+    const double a = dat[o];
+    for (int k = 1; k <= km && k <= d.nk; ++k) {
+      const int ok = o + d.P2 * (k - 1);
+      acc = dd_add(acc, T[ok] * (rho[ok] * a * 1.0));
+    }
+  }
+  sh[threadIdx.x] = acc.hi;
+  sl[threadIdx.x] = acc.lo;
+  __syncthreads();
+  for (int s = RB / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      DD x{sh[threadIdx.x], sl[threadIdx.x]}, y{sh[threadIdx.x + s], sl[threadIdx.x + s]};
+      x = dd_add2(x, y);
+      sh[threadIdx.x] = x.hi;
+      sl[threadIdx.x] = x.lo;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sh[0];
+    part[2 * blockIdx.x + 1] = sl[0];
+  }
+}
+
+__global__ __launch_bounds__(RB) void k_content_final(const double *__restrict__ part, int n,
+                                                      double *__restrict__ out) {
+  __shared__ double sh[RB], sl[RB];
+  DD acc{0.0, 0.0};
+  for (int b = threadIdx.x; b < n; b += RB) acc = dd_add2(acc, DD{part[2 * b], part[2 * b + 1]});
+  sh[threadIdx.x] = acc.hi;
+  sl[threadIdx.x] = acc.lo;
+  __syncthreads();
+  for (int s = RB / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      DD x{sh[threadIdx.x], sl[threadIdx.x]}, y{sh[threadIdx.x + s], sl[threadIdx.x + s]};
+      x = dd_add2(x, y);
+      sh[threadIdx.x] = x.hi;
+      sl[threadIdx.x] = x.lo;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sh[0] + sl[0];
+}
+
+// ---- grid mask consistency: tmask == (k<=kmt), umask == (k<=kmu) ------------
+// One count per data-domain column; the host adds them up.
+__global__ void k_check_masks(KDom d, const double *__restrict__ tmask,
+                              const double *__restrict__ umask, const int *__restrict__ kmt,
+                              const int *__restrict__ kmu, int *__restrict__ bad) {
+  const int ii = blockIdx.x * blockDim.x + threadIdx.x;
+  const int jj = blockIdx.y;
+  if (ii >= d.NI) return;
+  const int o = ii + d.NI * jj;
+  int mine = 0;
+  for (int k = 1; k <= d.nk; ++k) {
+    const double t = tmask[o + d.P2 * (k - 1)], u = umask[o + d.P2 * (k - 1)];
+    if (t != ((k <= kmt[o]) ? 1.0 : 0.0)) ++mine;
+    if (u != ((k <= kmu[o]) ? 1.0 : 0.0)) ++mine;
+  }
+  bad[o] = mine;
+}
+
+}  // namespace
+
+int launch_invtri(mom_gpu_ctx *ctx, double *z, const double *topbc, const double *botbc,
+                  const double *dcb, double tdt, const int *kmz, const double *mask,
+                  const double *dh, const double *dhw, double aidif) {
+  const KDom &d = ctx->kd;
+  hipLaunchKernelGGL(k_invtri, dim3((d.ni + 63) / 64, d.nj), dim3(64), 0, ctx->stream, d, z,
+                     topbc, botbc, dcb, tdt, kmz, mask, dh, dhw, aidif, ctx->tri_e);
+  MOM_LAUNCHED(ctx, "k_invtri");
+  return MOM_GPU_OK;
+}
+
+int launch_vdiff_implicit(mom_gpu_ctx *ctx, int ntr, double *const *T, const double *const *stf,
+                          const double *const *btf, const double *diff_cbt,
+                          const double *const *K33, int index_salt, double rho0,
+                          const double *dh, const double *dhw, double dt, double aidif) {
+  const KDom &d = ctx->kd;
+  const size_t p3 = (size_t)d.P2 * d.nk;
+  // groups: tracers with their own K33 go alone; the others are grouped by
+  // nmix in chunks of NT.
+  for (int nmix = 1; nmix <= 2; ++nmix) {
+    TriGroup g{};
+    g.n = 0;
+    auto flush = [&](const double *k33) -> int {
+      if (g.n == 0) return MOM_GPU_OK;
+      g.k33 = k33;
+      hipLaunchKernelGGL(k_vdiff, dim3((d.ni + 63) / 64, d.nj), dim3(64), 0, ctx->stream, d, g,
+                         diff_cbt + p3 * (nmix - 1), rho0, ctx->kmt, dh, dhw, dt, aidif,
+                         ctx->tri_e);
+      MOM_LAUNCHED(ctx, "k_vdiff");
+      g.n = 0;
+      return MOM_GPU_OK;
+    };
+    for (int n = 0; n < ntr; ++n) {
+      const int nm = (n == index_salt) ? 2 : 1;
+      if (nm != nmix) continue;
+      const double *k33 = (K33 && K33[n]) ? K33[n] : nullptr;
+      if (k33) {
+        int rc = flush(nullptr);
+        if (rc) return rc;
+        g.z[0] = T[n]; g.stf[0] = stf[n]; g.btf[0] = btf[n]; g.n = 1;
+        rc = flush(k33);
+        if (rc) return rc;
+        continue;
+      }
+      g.z[g.n] = T[n]; g.stf[g.n] = stf[n]; g.btf[g.n] = btf[n]; ++g.n;
+      if (g.n == NT) {
+        int rc = flush(nullptr);
+        if (rc) return rc;
+      }
+    }
+    int rc = flush(nullptr);
+    if (rc) return rc;
+  }
+  return MOM_GPU_OK;
+}
+
+int launch_tmask_limit(mom_gpu_ctx *ctx, int ntr, const double *const *T, const double *mn,
+                       const double *mx, int it, int is, int ts_same, double *const *lim) {
+  const KDom &d = ctx->kd;
+  if (ntr > NL) return mom_fail(ctx, MOM_GPU_ERR_ARG, "compute_tmask_limit: at most %d tracers", NL);
+  LimTable t{};
+  t.n = ntr;
+  for (int n = 0; n < ntr; ++n) {
+    t.T[n] = T[n]; t.lim[n] = lim[n]; t.mn[n] = mn[n]; t.mx[n] = mx[n];
+  }
+  // i in isc-1..iec, j in jsc-1..jec
+  hipLaunchKernelGGL(k_tmask_limit, dim3((d.ni + 1 + 63) / 64, d.nj + 1, d.nk), dim3(64), 0,
+                     ctx->stream, d, t, it, is, ts_same, ctx->kmt);
+  MOM_LAUNCHED(ctx, "k_tmask_limit");
+  return MOM_GPU_OK;
+}
+
+int launch_explicit(mom_gpu_ctx *ctx, const double *Tm1, const double *th, const double *rdz,
+                    const double *rdzr, double dtime, double *Tp1) {
+  const KDom &d = ctx->kd;
+  hipLaunchKernelGGL(k_explicit, dim3((d.ni + 255) / 256, d.nj, d.nk), dim3(256), 0, ctx->stream,
+                     d, Tm1, th, rdz, rdzr, dtime, Tp1);
+  MOM_LAUNCHED(ctx, "k_explicit");
+  return MOM_GPU_OK;
+}
+
+int launch_content(mom_gpu_ctx *ctx, const double *T, const double *rho, double *out) {
+  const KDom &d = ctx->kd;
+  const int nb = ctx->red_n;
+  hipLaunchKernelGGL(k_content, dim3(nb), dim3(RB), 0, ctx->stream, d, T, rho, ctx->dat,
+                     ctx->kmt, ctx->red);
+  MOM_LAUNCHED(ctx, "k_content");
+  hipLaunchKernelGGL(k_content_final, dim3(1), dim3(RB), 0, ctx->stream, ctx->red, nb,
+                     ctx->red + 2 * nb);
+  MOM_LAUNCHED(ctx, "k_content_final");
+  MOM_HIP(ctx, hipMemcpyAsync(out, ctx->red + 2 * nb, sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->stream));
+  MOM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return MOM_GPU_OK;
+}
+
+int launch_check_masks(mom_gpu_ctx *ctx, const double *tmask, const double *umask,
+                       long long *bad) {
+  const KDom &d = ctx->kd;
+  const size_t n = (size_t)d.P2;
+  int *dev = nullptr;
+  MOM_HIP(ctx, hipMalloc(&dev, n * sizeof(int)));
+  hipLaunchKernelGGL(k_check_masks, dim3((d.NI + 127) / 128, d.NJ), dim3(128), 0, ctx->stream, d,
+                     tmask, umask, ctx->kmt, ctx->kmu, dev);
+  hipError_t le = hipGetLastError();
+  int *host = (int *)malloc(n * sizeof(int));
+  hipError_t e1 = hipMemcpyAsync(host, dev, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
+  hipError_t e2 = hipStreamSynchronize(ctx->stream);
+  hipFree(dev);
+  if (le != hipSuccess || e1 != hipSuccess || e2 != hipSuccess) {
+    free(host);
+    return mom_fail(ctx, MOM_GPU_ERR_HIP, "mask check failed: %s",
+                    hipGetErrorString(le != hipSuccess ? le : (e1 != hipSuccess ? e1 : e2)));
+  }
+  long long s = 0;
+  for (size_t q = 0; q < n; ++q) s += host[q];
+  free(host);
+  *bad = s;
+  return MOM_GPU_OK;
+}

@@ -1,0 +1,308 @@
+// Fused MDFL-Sweby / DST-linear tracer advection for gfx950.
+//
+// One work-group owns a tile of TX x TY columns and marches down the k
+// levels.  At each level it
+//   Z  computes the provisional tracer of the z-sweep on the tile plus a
+//      two-cell halo (ocean_tracer_advect.F90:3843-3909); the T window
+//      T(k-1..k+2), the flux through the cell top and w(k-1) stay in
+//      registers from one level to the next;
+//   X  forms the flux through each cell's eastern face from wavefront
+//      shuffles (one region row = one 64-lane wavefront) and the x-sweep
+//      update (:3916-3969) -- no LDS and no barrier;
+//   Y  stages the x-swept tracer in LDS, forms the northern-face fluxes and
+//      finishes with the y-sweep, the divergence-correction removal and the
+//      tendency (:3974-4056), accumulated into th_tendency.
+// The halo columns/rows are recomputed by each tile with the same
+// arithmetic as the tile that owns them, which is what the two
+// mpp_update_domains calls of the reference (:3912, :3970) deliver.
+//
+// FLAVOR 0 rounds like advect_tracer_mdfl_sweby (per-tracer routine, with a
+// sweby_limiter argument); FLAVOR 1 like advect_tracer_sweby_all (:4104-4511).
+#include "mom_internal.h"
+
+namespace {
+
+constexpr int RX = 64;            // region width (one wavefront)
+constexpr int TX = RX - 4;        // interior tile width
+constexpr int NW = 8;             // wavefronts per work-group
+constexpr double ONESIXTH = 1.0 / 6.0;   // ocean_parameters.F90:92
+constexpr double EPS30 = 1.0e-30;
+
+__device__ inline double mk(int k, int kmt) { return (k <= kmt) ? 1.0 : 0.0; }
+
+template <int FLAVOR>
+__device__ inline double psi(double d0, double d1, double theta, double cfl,
+                             double lim) {
+  if (FLAVOR == 0) {
+    const double p = d0 + d1 * theta;
+    return p * (1.0 - lim) +
+           fmx(0.0, fmn(fmn(1.0, d0 + d1 * theta), (1.0 - cfl) / (EPS30 + cfl) * theta)) * lim;
+  } else {
+    return fmx(0.0, fmn(fmn(1.0, d0 + d1 * theta), (1.0 - cfl) / (EPS30 + cfl) * theta));
+  }
+}
+
+// Flux through one face: Tpos is the tracer upstream for a positive mass
+// flux, Tneg for a negative one; ma, mb the two masks in reference order.
+template <int FLAVOR>
+__device__ inline double face_flux(double Rjm, double Rj, double Rjp,
+                                   double massflux, double cfl, double Tpos,
+                                   double Tneg, double ma, double mb,
+                                   double lim) {
+  const double d0 = (2.0 - cfl) * (1.0 - cfl) * ONESIXTH;
+  const double d1 = (1.0 - cfl * cfl) * ONESIXTH;
+  const double thetaP = Rjm / (EPS30 + Rj);
+  const double thetaM = Rjp / (EPS30 + Rj);
+  const double psiP = psi<FLAVOR>(d0, d1, thetaP, cfl, lim);
+  const double psiM = psi<FLAVOR>(d0, d1, thetaM, cfl, lim);
+  return 0.5 * ((massflux + fabs(massflux)) * (Tpos + psiP * Rj) +
+                (massflux - fabs(massflux)) * (Tneg - psiM * Rj)) * ma * mb;
+}
+
+template <int FLAVOR, int RPW>
+__global__ __launch_bounds__(RX * NW) void k_mdfl(
+    KDom d, const double *__restrict__ T, const double *__restrict__ U,
+    const double *__restrict__ V, const double *__restrict__ W,
+    const double *__restrict__ R, const int *__restrict__ kmt,
+    const double *__restrict__ dat, const double *__restrict__ datr,
+    const double *__restrict__ dxte, const double *__restrict__ dyte,
+    const double *__restrict__ dxtn, const double *__restrict__ dytn,
+    double dtime, double lim, double *__restrict__ th,
+    double *__restrict__ wrk1, double *__restrict__ fxo,
+    double *__restrict__ fyo, double *__restrict__ fzo, int ntx, int ntiles,
+    int per_xcd) {
+  constexpr int RY = NW * RPW;
+  constexpr int TY = RY - 4;
+  __shared__ double s_tx[RY][RX];
+  __shared__ double s_fy[RY][RX];
+  __shared__ int s_km[RY][RX];
+
+  // Blocks are dealt to the 8 XCDs round-robin; give each XCD a contiguous
+  // run of tiles so that halo re-reads hit its own L2.
+  const int b = blockIdx.x;
+  const int tile = (b & 7) * per_xcd + (b >> 3);
+  if (tile >= ntiles) return;
+  const int tix = tile % ntx, tiy = tile / ntx;
+  const int gi0 = tix * TX, gj0 = tiy * TY;
+
+  const int lane = threadIdx.x & (RX - 1);
+  const int wave = threadIdx.x / RX;
+  const int nk = d.nk, P2 = d.P2;
+  const int gi = gi0 - 2 + lane;
+
+  int si = gi;
+  bool colok = true;
+  if (si < 0 || si >= d.ni) {
+    if (d.cyclic_x) si = wrapi(si, d.ni); else colok = false;
+  }
+
+  int km[RPW], offz[RPW], gjr[RPW], sjr[RPW];
+  bool ok[RPW], rowok[RPW];
+  double t0[RPW], t1[RPW], t2[RPW], t3[RPW], ftp[RPW], wkm1[RPW];
+  double datv[RPW], datrv[RPW];
+#pragma unroll
+  for (int q = 0; q < RPW; ++q) {
+    const int ry = wave + NW * q;
+    const int gj = gj0 - 2 + ry;
+    int sj = gj;
+    bool rok = true;
+    if (sj < 0 || sj >= d.nj) {
+      if (d.cyclic_y) sj = wrapi(sj, d.nj); else rok = false;
+    }
+    gjr[q] = gj;
+    sjr[q] = sj;
+    rowok[q] = rok;
+    ok[q] = colok && rok;
+    offz[q] = ok[q] ? d2(d, si, sj) : 0;
+    km[q] = ok[q] ? kmt[offz[q]] : 0;
+    s_km[ry][lane] = km[q];
+    datv[q] = ok[q] ? dat[offz[q]] : 0.0;
+    datrv[q] = ok[q] ? datr[offz[q]] : 0.0;
+    ftp[q] = 0.0;
+    wkm1[q] = 0.0;
+    if (ok[q]) {
+      const int o = offz[q];
+      t0[q] = T[o];
+      t1[q] = t0[q];
+      t2[q] = T[o + P2 * (min(2, nk) - 1)];
+      t3[q] = T[o + P2 * (min(3, nk) - 1)];
+    } else {
+      t0[q] = t1[q] = t2[q] = t3[q] = 0.0;
+    }
+  }
+  __syncthreads();
+
+  for (int k = 1; k <= nk; ++k) {
+    const int k0 = k - 1;
+    const int kp1 = min(k + 1, nk), kp2 = min(k + 2, nk), km1 = max(k - 1, 1);
+    double tz[RPW], rhov[RPW], wv[RPW], fbtv[RPW], txv[RPW], mfe[RPW], mfw[RPW], fxv[RPW];
+
+    // ---- Z: flux through the bottom face and the z-sweep update ----------
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+      tz[q] = 0.0; rhov[q] = 1.0; wv[q] = 0.0; fbtv[q] = 0.0;
+      if (ok[q]) {
+        const int o = offz[q];
+        const double w = W[o + P2 * k];
+        const double r = R[o + P2 * k0];
+        const double mkm1 = mk(km1, km[q]), mk0 = mk(k, km[q]);
+        const double mkp1 = mk(kp1, km[q]), mkp2 = mk(kp2, km[q]);
+        const double Rjp = (t0[q] - t1[q]) * mkm1 * mk0;
+        const double Rj = (t1[q] - t2[q]) * mk0 * mkp1;
+        const double Rjm = (t2[q] - t3[q]) * mkp1 * mkp2;
+        const double massflux = datv[q] * w;
+        const double cfl = fabs(w * dtime / r);
+        const double fbt = face_flux<FLAVOR>(Rjm, Rj, Rjp, massflux, cfl, t2[q], t1[q],
+                                             mkp1, mk0, lim);
+        if (FLAVOR == 0) {
+          tz[q] = t1[q] + dtime / r * (datrv[q] * (fbt - ftp[q]) + t1[q] * (wkm1[q] - w));
+        } else {
+          const double wk = datrv[q] * (fbt - ftp[q]) + t1[q] * (wkm1[q] - w);
+          tz[q] = t1[q] + wk * dtime / r;
+        }
+        rhov[q] = r; wv[q] = w; fbtv[q] = fbt;
+      }
+    }
+
+    // ---- X: eastern-face flux (shuffles) and the x-sweep update ----------
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+      const int ry = wave + NW * q;
+      const double tzm1 = __shfl(tz[q], lane - 1);
+      const double tzp1 = __shfl(tz[q], lane + 1);
+      const double tzp2 = __shfl(tz[q], lane + 2);
+      const int kmm1 = __shfl(km[q], lane - 1);
+      const int kmp1 = __shfl(km[q], lane + 1);
+      const int kmp2 = __shfl(km[q], lane + 2);
+      double fx = 0.0, mf = 0.0;
+      if (rowok[q] && lane >= 1 && lane <= RX - 3 && gi >= -1 && gi <= d.ni - 1) {
+        const int of = d2(d, gi, sjr[q]);
+        const double u = U[of + P2 * k0];
+        mf = dyte[of] * u;
+        const double rw = R[of + P2 * k0], re = R[of + 1 + P2 * k0];
+        const double cfl = fabs(u * dtime * 2.0 / ((rw + re) * dxte[of]));
+        const double mi = mk(k, km[q]), mim1 = mk(k, kmm1);
+        const double mip1 = mk(k, kmp1), mip2 = mk(k, kmp2);
+        const double Rjp = (tzp2 - tzp1) * mip2 * mip1;
+        const double Rj = (tzp1 - tz[q]) * mip1 * mi;
+        const double Rjm = (tz[q] - tzm1) * mi * mim1;
+        fx = face_flux<FLAVOR>(Rjm, Rj, Rjp, mf, cfl, tz[q], tzp1, mi, mip1, lim);
+      }
+      const double fxw = __shfl(fx, lane - 1);
+      const double mw = __shfl(mf, lane - 1);
+      double tx = 0.0;
+      if (rowok[q] && lane >= 2 && lane < RX - 2 && gi < d.ni) {
+        const double mi = mk(k, km[q]);
+        if (FLAVOR == 0) {
+          tx = tz[q] + dtime * mi * datrv[q] / rhov[q] * (fxw - fx + t1[q] * (mf - mw));
+        } else {
+          const double wk = mi * datrv[q] * (fxw - fx + t1[q] * (mf - mw));
+          tx = tz[q] + wk * dtime / rhov[q];
+        }
+      }
+      s_tx[ry][lane] = tx;
+      txv[q] = tx; mfe[q] = mf; mfw[q] = mw; fxv[q] = fx;
+    }
+    __syncthreads();
+
+    // ---- Y: northern-face flux from the staged x-swept tracer -------------
+    double fyv[RPW];
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+      const int ry = wave + NW * q;
+      const int gj = gjr[q];
+      double fy = 0.0;
+      if (ry >= 1 && ry <= RY - 3 && lane >= 2 && lane < RX - 2 && gi < d.ni &&
+          gj >= -1 && gj <= d.nj - 1) {
+        const int of = d2(d, gi, gj);
+        const double v = V[of + P2 * k0];
+        const double mf = dxtn[of] * v;
+        const double rs = R[of + P2 * k0], rn = R[of + d.NI + P2 * k0];
+        const double cfl = fabs(v * dtime * 2.0 / ((rs + rn) * dytn[of]));
+        const double tjm1 = s_tx[ry - 1][lane], tj = s_tx[ry][lane];
+        const double tjp1 = s_tx[ry + 1][lane], tjp2 = s_tx[ry + 2][lane];
+        const double mjm1 = mk(k, s_km[ry - 1][lane]), mj = mk(k, s_km[ry][lane]);
+        const double mjp1 = mk(k, s_km[ry + 1][lane]), mjp2 = mk(k, s_km[ry + 2][lane]);
+        const double Rjp = (tjp2 - tjp1) * mjp2 * mjp1;
+        const double Rj = (tjp1 - tj) * mjp1 * mj;
+        const double Rjm = (tj - tjm1) * mj * mjm1;
+        fy = face_flux<FLAVOR>(Rjm, Rj, Rjp, mf, cfl, tj, tjp1, mj, mjp1, lim);
+      }
+      s_fy[ry][lane] = fy;
+      fyv[q] = fy;
+    }
+    __syncthreads();
+
+    // ---- final: y-sweep update, correction removal, tendency -------------
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+      const int ry = wave + NW * q;
+      const int gj = gjr[q];
+      const bool cell = ry >= 2 && ry < RY - 2 && lane >= 2 && lane < RX - 2 &&
+                        gi < d.ni && gj < d.nj;
+      if (cell) {
+        const double fys = s_fy[ry - 1][lane], fyn = fyv[q];
+        const double mi = mk(k, km[q]);
+        const double r = rhov[q], Tc = t1[q];
+        double w1;
+        if (FLAVOR == 0) {
+          double tr = txv[q] + dtime * mi * datrv[q] / r * (fys - fyn);
+          tr = tr + dtime * Tc / r * (wv[q] - wkm1[q] + datrv[q] * (mfw[q] - mfe[q]));
+          const double adv = -(r * (tr - Tc) / dtime * mi);
+          w1 = -adv;
+        } else {
+          const double wk = mi * datrv[q] * (fys - fyn) +
+                            Tc * (wv[q] - wkm1[q] + datrv[q] * (mfw[q] - mfe[q]));
+          const double tr = txv[q] + wk * dtime / r;
+          w1 = r * (tr - Tc) / dtime * mi;
+        }
+        const int o = offz[q] + P2 * k0;
+        th[o] = th[o] + w1;
+        if (wrk1) wrk1[o] = w1;
+        if (fzo) fzo[o] = fbtv[q];
+        if (fxo) fxo[o] = fxv[q];
+        if (fyo) fyo[o] = fyn;
+      }
+      // the western/southern boundary faces belong to the first tile
+      if (fxo && tix == 0 && lane == 1 && ry >= 2 && ry < RY - 2 && gj < d.nj)
+        fxo[d2(d, -1, gj) + P2 * k0] = fxv[q];
+      if (fyo && tiy == 0 && ry == 1 && lane >= 2 && lane < RX - 2 && gi < d.ni)
+        fyo[d2(d, gi, -1) + P2 * k0] = fyv[q];
+    }
+
+    // ---- advance the k window -------------------------------------------
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+      ftp[q] = fbtv[q];
+      wkm1[q] = wv[q];
+      t0[q] = t1[q]; t1[q] = t2[q]; t2[q] = t3[q];
+      if (ok[q] && k < nk) t3[q] = T[offz[q] + P2 * (min(k + 3, nk) - 1)];
+    }
+  }
+}
+
+}  // namespace
+
+int launch_mdfl(mom_gpu_ctx *ctx, int flavor, const double *T,
+                const double *u, const double *v, const double *w,
+                const double *rho, double dtime, double lim, double *th,
+                double *wrk1, double *fx, double *fy, double *fz) {
+  constexpr int RPW = 4;
+  constexpr int TY = NW * RPW - 4;
+  const KDom &d = ctx->kd;
+  const int ntx = (d.ni + TX - 1) / TX;
+  const int nty = (d.nj + TY - 1) / TY;
+  const int ntiles = ntx * nty;
+  const int per = (ntiles + 7) / 8;
+  const dim3 grid(8 * per), block(RX * NW);
+  if (flavor == 0)
+    hipLaunchKernelGGL((k_mdfl<0, RPW>), grid, block, 0, ctx->stream, d, T, u, v, w, rho,
+                       ctx->kmt, ctx->dat, ctx->datr, ctx->dxte, ctx->dyte, ctx->dxtn,
+                       ctx->dytn, dtime, lim, th, wrk1, fx, fy, fz, ntx, ntiles, per);
+  else
+    hipLaunchKernelGGL((k_mdfl<1, RPW>), grid, block, 0, ctx->stream, d, T, u, v, w, rho,
+                       ctx->kmt, ctx->dat, ctx->datr, ctx->dxte, ctx->dyte, ctx->dxtn,
+                       ctx->dytn, dtime, lim, th, wrk1, fx, fy, fz, ntx, ntiles, per);
+  MOM_LAUNCHED(ctx, "k_mdfl");
+  return MOM_GPU_OK;
+}

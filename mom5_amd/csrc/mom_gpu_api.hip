@@ -1,0 +1,348 @@
+// C-ABI entry points of libmom5_gpu (declared in include/mom5_gpu.h):
+// argument checking, context lifetime and dispatch to the kernels.
+#include "mom_internal.h"
+
+#include <cstdlib>
+#include <cstring>
+
+static thread_local char g_err[512] = "";
+
+int mom_fail(mom_gpu_ctx *ctx, int code, const char *fmt, ...) {
+  char *buf = ctx ? ctx->err : g_err;
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, 512, fmt, ap);
+  va_end(ap);
+  if (ctx) std::snprintf(g_err, sizeof g_err, "%s", ctx->err);
+  return code;
+}
+
+#define REQUIRE(ctx, cond, ...)                                     \
+  do {                                                              \
+    if (!(cond)) return mom_fail((ctx), MOM_GPU_ERR_ARG, __VA_ARGS__); \
+  } while (0)
+
+extern "C" {
+
+const char *mom_gpu_last_error(const mom_gpu_ctx_t *ctx) { return ctx ? ctx->err : g_err; }
+
+int mom_gpu_malloc(void **ptr, size_t bytes) {
+  if (!ptr) return mom_fail(nullptr, MOM_GPU_ERR_ARG, "mom_gpu_malloc: ptr is NULL");
+  hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
+  if (e != hipSuccess)
+    return mom_fail(nullptr, MOM_GPU_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes,
+                    hipGetErrorString(e));
+  return MOM_GPU_OK;
+}
+
+int mom_gpu_free(void *ptr) {
+  hipError_t e = hipFree(ptr);
+  if (e != hipSuccess) return mom_fail(nullptr, MOM_GPU_ERR_HIP, "hipFree: %s", hipGetErrorString(e));
+  return MOM_GPU_OK;
+}
+
+int mom_gpu_memcpy_htod(void *dst, const void *src, size_t bytes) {
+  hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return mom_fail(nullptr, MOM_GPU_ERR_HIP, "hipMemcpy H2D: %s", hipGetErrorString(e));
+  return MOM_GPU_OK;
+}
+
+int mom_gpu_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
+  hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return mom_fail(nullptr, MOM_GPU_ERR_HIP, "hipMemcpy D2H: %s", hipGetErrorString(e));
+  return MOM_GPU_OK;
+}
+
+int mom_gpu_memset(void *ptr, int value, size_t bytes) {
+  hipError_t e = hipMemset(ptr, value, bytes);
+  if (e != hipSuccess) return mom_fail(nullptr, MOM_GPU_ERR_HIP, "hipMemset: %s", hipGetErrorString(e));
+  return MOM_GPU_OK;
+}
+
+static void free_ctx(mom_gpu_ctx *c) {
+  if (!c) return;
+  double **ds[] = {&c->dxt, &c->dyt, &c->dat, &c->datr, &c->dxte, &c->dyte, &c->dxtn, &c->dytn,
+                   &c->dxu, &c->dyu, &c->dxur, &c->dyur, &c->f, &c->dzt, &c->qw, &c->qzw,
+                   &c->bt, &c->tri_e, &c->red};
+  for (double **p : ds)
+    if (*p) hipFree(*p);
+  if (c->kmt) hipFree(c->kmt);
+  if (c->kmu) hipFree(c->kmu);
+  for (int i = 0; i < 8; ++i)
+    if (c->ev[i]) hipEventDestroy(c->ev[i]);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int mom_gpu_create(const mom_gpu_domain_t *dom, const mom_gpu_grid_t *grid, mom_gpu_ctx_t **out) {
+  if (!dom || !grid || !out) return mom_fail(nullptr, MOM_GPU_ERR_ARG, "mom_gpu_create: NULL argument");
+  *out = nullptr;
+  const int ni = dom->iec - dom->isc + 1, nj = dom->jec - dom->jsc + 1;
+  if (ni < 1 || nj < 1 || dom->nk < 2)
+    return mom_fail(nullptr, MOM_GPU_ERR_ARG, "mom_gpu_create: empty domain (ni=%d nj=%d nk=%d; nk>=2 required)",
+                    ni, nj, dom->nk);
+  if (dom->isd != dom->isc - 1 || dom->ied != dom->iec + 1 || dom->jsd != dom->jsc - 1 ||
+      dom->jed != dom->jec + 1)
+    return mom_fail(nullptr, MOM_GPU_ERR_ARG,
+                    "mom_gpu_create: only a one-cell halo is supported (isd=isc-1, ied=iec+1, ...)");
+  const long long cells = (long long)(ni + 2) * (nj + 2) * (dom->nk + 1);
+  if (cells >= (1LL << 31))
+    return mom_fail(nullptr, MOM_GPU_ERR_ARG, "mom_gpu_create: domain too large for 32-bit offsets");
+  const double *need[] = {grid->dxt, grid->dyt, grid->dat, grid->datr, grid->dxte, grid->dyte,
+                          grid->dxtn, grid->dytn, grid->dxu, grid->dyu, grid->dxur, grid->dyur,
+                          grid->f, grid->tmask, grid->umask, grid->dzt};
+  for (const double *p : need)
+    if (!p) return mom_fail(nullptr, MOM_GPU_ERR_ARG, "mom_gpu_create: a grid field is NULL");
+  if (!grid->kmt || !grid->kmu) return mom_fail(nullptr, MOM_GPU_ERR_ARG, "mom_gpu_create: kmt/kmu NULL");
+
+  mom_gpu_ctx *c = new mom_gpu_ctx();
+  std::memset(c, 0, sizeof *c);
+  c->dom = *dom;
+  KDom &k = c->kd;
+  k.ni = ni; k.nj = nj; k.nk = dom->nk;
+  k.NI = ni + 2; k.NJ = nj + 2;
+  k.P2 = k.NI * k.NJ;
+  k.cyclic_x = dom->cyclic_x ? 1 : 0;
+  k.cyclic_y = dom->cyclic_y ? 1 : 0;
+  const size_t P2 = (size_t)k.P2, nk = (size_t)k.nk;
+
+#define CK(call)                                                                      \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess) {                                                           \
+      int rc_ = mom_fail(nullptr, MOM_GPU_ERR_HIP, "mom_gpu_create: %s: %s", #call,   \
+                         hipGetErrorString(e_));                                      \
+      free_ctx(c);                                                                    \
+      return rc_;                                                                     \
+    }                                                                                 \
+  } while (0)
+  CK(hipGetDevice(&c->device));
+  CK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  for (int i = 0; i < 8; ++i) CK(hipEventCreate(&c->ev[i]));
+  struct {
+    double **dst;
+    const double *src;
+  } f2[] = {{&c->dxt, grid->dxt}, {&c->dyt, grid->dyt}, {&c->dat, grid->dat}, {&c->datr, grid->datr},
+            {&c->dxte, grid->dxte}, {&c->dyte, grid->dyte}, {&c->dxtn, grid->dxtn}, {&c->dytn, grid->dytn},
+            {&c->dxu, grid->dxu}, {&c->dyu, grid->dyu}, {&c->dxur, grid->dxur}, {&c->dyur, grid->dyur},
+            {&c->f, grid->f}};
+  for (auto &e : f2) {
+    CK(hipMalloc(e.dst, P2 * sizeof(double)));
+    CK(hipMemcpyAsync(*e.dst, e.src, P2 * sizeof(double), hipMemcpyDefault, c->stream));
+  }
+  CK(hipMalloc(&c->kmt, P2 * sizeof(int)));
+  CK(hipMemcpyAsync(c->kmt, grid->kmt, P2 * sizeof(int), hipMemcpyDefault, c->stream));
+  CK(hipMalloc(&c->kmu, P2 * sizeof(int)));
+  CK(hipMemcpyAsync(c->kmu, grid->kmu, P2 * sizeof(int), hipMemcpyDefault, c->stream));
+  CK(hipMalloc(&c->dzt, nk * sizeof(double)));
+  CK(hipMemcpyAsync(c->dzt, grid->dzt, nk * sizeof(double), hipMemcpyDefault, c->stream));
+  CK(hipMalloc(&c->qw, 16 * P2 * sizeof(double)));
+  CK(hipMalloc(&c->qzw, 8 * nk * sizeof(double)));
+  CK(hipMalloc(&c->bt, 9 * P2 * sizeof(double)));
+  CK(hipMalloc(&c->tri_e, P2 * nk * sizeof(double)));
+  c->red_n = 1024;
+  CK(hipMalloc(&c->red, (2 * (size_t)c->red_n + 1) * sizeof(double)));
+#undef CK
+
+  long long bad = 0;
+  int rc = launch_check_masks(c, grid->tmask, grid->umask, &bad);
+  if (rc == MOM_GPU_OK && bad != 0)
+    rc = mom_fail(c, MOM_GPU_ERR_GRID,
+                  "mom_gpu_create: %lld mask entries disagree with kmt/kmu "
+                  "(expected tmask = k<=kmt, umask = k<=kmu, ocean_grids.F90:1374-1386)", bad);
+  if (rc == MOM_GPU_OK) rc = launch_quicker_init(c);
+  if (rc == MOM_GPU_OK) {
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) rc = mom_fail(c, MOM_GPU_ERR_HIP, "mom_gpu_create: %s", hipGetErrorString(e));
+  }
+  if (rc != MOM_GPU_OK) {
+    std::snprintf(g_err, sizeof g_err, "%s", c->err);
+    free_ctx(c);
+    return rc;
+  }
+  *out = c;
+  return MOM_GPU_OK;
+}
+
+int mom_gpu_destroy(mom_gpu_ctx_t *ctx) {
+  if (!ctx) return MOM_GPU_OK;
+  hipStreamSynchronize(ctx->stream);
+  free_ctx(ctx);
+  return MOM_GPU_OK;
+}
+
+int mom_gpu_sync(mom_gpu_ctx_t *ctx) {
+  REQUIRE(nullptr, ctx, "mom_gpu_sync: NULL context");
+  MOM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return MOM_GPU_OK;
+}
+
+int mom_gpu_mark(mom_gpu_ctx_t *ctx, int slot) {
+  REQUIRE(nullptr, ctx, "mom_gpu_mark: NULL context");
+  REQUIRE(ctx, slot >= 0 && slot < 8, "mom_gpu_mark: slot %d out of range", slot);
+  MOM_HIP(ctx, hipEventRecord(ctx->ev[slot], ctx->stream));
+  return MOM_GPU_OK;
+}
+
+int mom_gpu_elapsed(mom_gpu_ctx_t *ctx, int s0, int s1, float *ms) {
+  REQUIRE(nullptr, ctx, "mom_gpu_elapsed: NULL context");
+  REQUIRE(ctx, s0 >= 0 && s0 < 8 && s1 >= 0 && s1 < 8 && ms, "mom_gpu_elapsed: bad slots");
+  MOM_HIP(ctx, hipEventSynchronize(ctx->ev[s1]));
+  MOM_HIP(ctx, hipEventElapsedTime(ms, ctx->ev[s0], ctx->ev[s1]));
+  return MOM_GPU_OK;
+}
+
+static size_t p3bytes(const mom_gpu_ctx *c) { return (size_t)c->kd.P2 * c->kd.nk * sizeof(double); }
+
+int mom_gpu_horz_advect_tracer(mom_gpu_ctx_t *ctx, int scheme, const double *T_taum1,
+                               const double *T_tau, const double *uhrho_et,
+                               const double *vhrho_nt, const double *wrho_bt,
+                               const double *rho_dzt_tau, const double *tmask_limit,
+                               int limit_with_upwind, double dtime, double *th_tendency,
+                               double *wrk1, double *flux_x, double *flux_y, double *flux_z) {
+  REQUIRE(nullptr, ctx, "mom_gpu_horz_advect_tracer: NULL context");
+  REQUIRE(ctx, T_taum1 && uhrho_et && vhrho_nt && th_tendency,
+          "mom_gpu_horz_advect_tracer: NULL required array");
+  const size_t b3 = p3bytes(ctx);
+  if (wrk1) MOM_HIP(ctx, hipMemsetAsync(wrk1, 0, b3, ctx->stream));
+  if (flux_x) MOM_HIP(ctx, hipMemsetAsync(flux_x, 0, b3, ctx->stream));
+  if (flux_y) MOM_HIP(ctx, hipMemsetAsync(flux_y, 0, b3, ctx->stream));
+  switch (scheme) {
+    case MOM_ADVECT_MDFL_SWEBY:
+    case MOM_ADVECT_DST_LINEAR:
+      REQUIRE(ctx, wrho_bt && rho_dzt_tau, "horz_advect_tracer(MDFL): wrho_bt/rho_dzt required");
+      REQUIRE(ctx, dtime > 0.0, "horz_advect_tracer(MDFL): dtime must be positive");
+      return launch_mdfl(ctx, 0, T_taum1, uhrho_et, vhrho_nt, wrho_bt, rho_dzt_tau, dtime,
+                         scheme == MOM_ADVECT_MDFL_SWEBY ? 1.0 : 0.0, th_tendency, wrk1, flux_x,
+                         flux_y, flux_z);
+    case MOM_ADVECT_QUICKER:
+      REQUIRE(ctx, T_tau, "horz_advect_tracer(QUICKER): T_tau required");
+      REQUIRE(ctx, !limit_with_upwind || tmask_limit,
+              "horz_advect_tracer(QUICKER): limit_with_upwind needs tmask_limit");
+      return launch_quicker_horz(ctx, T_taum1, T_tau, uhrho_et, vhrho_nt, tmask_limit,
+                                 limit_with_upwind, th_tendency, wrk1, flux_x, flux_y);
+    default:
+      return mom_fail(ctx, MOM_GPU_ERR_ARG,
+                      "==>Error from ocean_tracer_advect_mod (horz_advect_tracer): chose invalid "
+                      "horz advection scheme (%d)", scheme);
+  }
+}
+
+int mom_gpu_vert_advect_tracer(mom_gpu_ctx_t *ctx, int scheme, const double *T_taum1,
+                               const double *T_tau, const double *wrho_bt,
+                               const double *tmask_limit, double *th_tendency, double *wrk1,
+                               double *flux_z) {
+  REQUIRE(nullptr, ctx, "mom_gpu_vert_advect_tracer: NULL context");
+  REQUIRE(ctx, th_tendency, "mom_gpu_vert_advect_tracer: th_tendency required");
+  switch (scheme) {
+    case MOM_ADVECT_MDFL_SWEBY:
+    case MOM_ADVECT_DST_LINEAR:
+      return MOM_GPU_OK;   // three-dimensional schemes: done in horz_advect_tracer
+    case MOM_ADVECT_QUICKER:
+      REQUIRE(ctx, T_taum1 && T_tau && wrho_bt, "vert_advect_tracer(QUICKER): NULL required array");
+      if (wrk1) MOM_HIP(ctx, hipMemsetAsync(wrk1, 0, p3bytes(ctx), ctx->stream));
+      return launch_quicker_vert(ctx, T_taum1, T_tau, wrho_bt, tmask_limit, th_tendency, wrk1,
+                                 flux_z);
+    default:
+      return mom_fail(ctx, MOM_GPU_ERR_ARG,
+                      "==>Error from ocean_tracer_advect_mod (vert_advect_tracer): invalid "
+                      "advection scheme chosen (%d)", scheme);
+  }
+}
+
+int mom_gpu_advect_sweby_all(mom_gpu_ctx_t *ctx, int ntr, const double *const *T_taum1,
+                             const double *uhrho_et, const double *vhrho_nt,
+                             const double *wrho_bt, const double *rho_dzt_tau, double dtime,
+                             double *const *th_tendency, double *const *wrk1) {
+  REQUIRE(nullptr, ctx, "mom_gpu_advect_sweby_all: NULL context");
+  REQUIRE(ctx, ntr >= 0 && T_taum1 && th_tendency && uhrho_et && vhrho_nt && wrho_bt && rho_dzt_tau,
+          "mom_gpu_advect_sweby_all: NULL required array");
+  REQUIRE(ctx, dtime > 0.0, "mom_gpu_advect_sweby_all: dtime must be positive");
+  for (int n = 0; n < ntr; ++n) {
+    REQUIRE(ctx, T_taum1[n] && th_tendency[n], "mom_gpu_advect_sweby_all: tracer %d NULL", n);
+    double *w1 = wrk1 ? wrk1[n] : nullptr;
+    if (w1) MOM_HIP(ctx, hipMemsetAsync(w1, 0, p3bytes(ctx), ctx->stream));
+    int rc = launch_mdfl(ctx, 1, T_taum1[n], uhrho_et, vhrho_nt, wrho_bt, rho_dzt_tau, dtime, 1.0,
+                         th_tendency[n], w1, nullptr, nullptr, nullptr);
+    if (rc) return rc;
+  }
+  return MOM_GPU_OK;
+}
+
+int mom_gpu_compute_tmask_limit(mom_gpu_ctx_t *ctx, int ntr, const double *const *T_tau,
+                                const double *min_limit, const double *max_limit, int index_temp,
+                                int index_salt, int ts_same, double *const *tmask_limit) {
+  REQUIRE(nullptr, ctx, "mom_gpu_compute_tmask_limit: NULL context");
+  REQUIRE(ctx, ntr >= 0 && T_tau && min_limit && max_limit && tmask_limit,
+          "mom_gpu_compute_tmask_limit: NULL argument");
+  REQUIRE(ctx, index_temp < ntr && index_salt < ntr, "mom_gpu_compute_tmask_limit: bad index");
+  for (int n = 0; n < ntr; ++n)
+    REQUIRE(ctx, T_tau[n] && tmask_limit[n], "mom_gpu_compute_tmask_limit: tracer %d NULL", n);
+  if (ntr == 0) return MOM_GPU_OK;
+  return launch_tmask_limit(ctx, ntr, T_tau, min_limit, max_limit, index_temp, index_salt,
+                            ts_same, tmask_limit);
+}
+
+int mom_gpu_update_tracer_explicit(mom_gpu_ctx_t *ctx, const double *T_taum1,
+                                   const double *th_tendency, const double *rho_dzt_taum1,
+                                   const double *rho_dztr, double dtime, double *T_taup1) {
+  REQUIRE(nullptr, ctx, "mom_gpu_update_tracer_explicit: NULL context");
+  REQUIRE(ctx, T_taum1 && th_tendency && rho_dzt_taum1 && rho_dztr && T_taup1,
+          "mom_gpu_update_tracer_explicit: NULL argument");
+  return launch_explicit(ctx, T_taum1, th_tendency, rho_dzt_taum1, rho_dztr, dtime, T_taup1);
+}
+
+int mom_gpu_invtri(mom_gpu_ctx_t *ctx, double *z, const double *topbc, const double *botbc,
+                   const double *dcb, double tdt, const int *kmz, const double *mask,
+                   const double *dh, const double *dhw, double aidif) {
+  REQUIRE(nullptr, ctx, "mom_gpu_invtri: NULL context");
+  REQUIRE(ctx, z && topbc && botbc && dcb && kmz && mask && dh && dhw, "mom_gpu_invtri: NULL argument");
+  return launch_invtri(ctx, z, topbc, botbc, dcb, tdt, kmz, mask, dh, dhw, aidif);
+}
+
+int mom_gpu_vert_diffuse_implicit(mom_gpu_ctx_t *ctx, int ntr, double *const *T_taup1,
+                                  const double *const *stf, const double *const *btf,
+                                  const double *diff_cbt, const double *const *K33_implicit,
+                                  int index_salt, double rho0, const double *rho_dzt_taup1,
+                                  const double *dzwt, double dtime_t, double aidif) {
+  REQUIRE(nullptr, ctx, "mom_gpu_vert_diffuse_implicit: NULL context");
+  REQUIRE(ctx, ntr >= 0 && T_taup1 && stf && btf && diff_cbt && rho_dzt_taup1 && dzwt,
+          "mom_gpu_vert_diffuse_implicit: NULL argument");
+  for (int n = 0; n < ntr; ++n)
+    REQUIRE(ctx, T_taup1[n] && stf[n] && btf[n], "mom_gpu_vert_diffuse_implicit: tracer %d NULL", n);
+  if (aidif == 0.0 || ntr == 0) return MOM_GPU_OK;   // ocean_vert_mix.F90:3446
+  return launch_vdiff_implicit(ctx, ntr, T_taup1, stf, btf, diff_cbt, K33_implicit, index_salt,
+                               rho0, rho_dzt_taup1, dzwt, dtime_t, aidif);
+}
+
+int mom_gpu_update_ocean_barotropic(mom_gpu_ctx_t *ctx, const mom_gpu_bt_params_t *p,
+                                    const double *eta_t_tau, const double *eta_t_bar_tau,
+                                    const double *udrho_tau, const double *forcing_bt,
+                                    const double *thicku, const double *patm, const double *pme,
+                                    const double *river, const double *source,
+                                    const double *eta_smooth, const double *rho_g,
+                                    double *eta_t_bar_taup1, double *udrho_taup1,
+                                    double *conv_rho_ud_t) {
+  REQUIRE(nullptr, ctx, "mom_gpu_update_ocean_barotropic: NULL context");
+  REQUIRE(ctx, p, "mom_gpu_update_ocean_barotropic: NULL params");
+  REQUIRE(ctx, eta_t_bar_tau && udrho_tau && forcing_bt && thicku && patm && pme && river &&
+                   source && eta_smooth && rho_g && eta_t_bar_taup1 && udrho_taup1,
+          "mom_gpu_update_ocean_barotropic: NULL argument");
+  REQUIRE(ctx, p->initsum_with_bar || eta_t_tau,
+          "mom_gpu_update_ocean_barotropic: eta_t_tau required when initsum_with_bar is false");
+  REQUIRE(ctx, p->nts >= 1, "mom_gpu_update_ocean_barotropic: nts must be >= 1");
+  REQUIRE(ctx, p->dtbt > 0.0 && p->rho0 > 0.0, "mom_gpu_update_ocean_barotropic: bad dtbt/rho0");
+  REQUIRE(ctx, !ctx->kd.cyclic_y, "mom_gpu_update_ocean_barotropic: cyclic_y is not supported");
+  return launch_barotropic(ctx, p, eta_t_tau ? eta_t_tau : eta_t_bar_tau, eta_t_bar_tau, udrho_tau,
+                           forcing_bt, thicku, patm, pme, river, source, eta_smooth, rho_g,
+                           eta_t_bar_taup1, udrho_taup1, conv_rho_ud_t);
+}
+
+int mom_gpu_tracer_content(mom_gpu_ctx_t *ctx, const double *T, const double *rho_dzt,
+                           double *content) {
+  REQUIRE(nullptr, ctx, "mom_gpu_tracer_content: NULL context");
+  REQUIRE(ctx, T && rho_dzt && content, "mom_gpu_tracer_content: NULL argument");
+  return launch_content(ctx, T, rho_dzt, content);
+}
+
+}  // extern "C"

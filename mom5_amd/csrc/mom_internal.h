@@ -1,0 +1,129 @@
+// Internal declarations shared by the HIP translation units of libmom5_gpu.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+
+#include "mom5_gpu.h"
+
+// Compact domain description passed to kernels by value.  Compute-domain
+// indices are 0-based (gi in [0, ni), gj in [0, nj)); the data domain has a
+// one-cell halo, so the data-domain element of (gi, gj) is
+// (gi+1) + NI*(gj+1) and gi = -1 / gi = ni address the halo columns.
+struct KDom {
+  int ni, nj, nk;
+  int NI, NJ;
+  int P2;              // NI*NJ, stride of one k level
+  int cyclic_x, cyclic_y;
+};
+
+__host__ __device__ inline int d2(const KDom &d, int gi, int gj) {
+  return (gi + 1) + d.NI * (gj + 1);
+}
+
+// Wrap of a compute index on a cyclic axis.
+__host__ __device__ inline int wrapi(int g, int n) {
+  g %= n;
+  return g < 0 ? g + n : g;
+}
+
+// Fortran MIN/MAX on non-NaN data.  The ternary form pins which operand
+// is returned for +0/-0 so that the result matches the CPU restatement.
+__host__ __device__ inline double fmn(double a, double b) { return (b < a) ? b : a; }
+__host__ __device__ inline double fmx(double a, double b) { return (b > a) ? b : a; }
+__host__ __device__ inline float fmnf(float a, float b) { return (b < a) ? b : a; }
+__host__ __device__ inline float fmxf(float a, float b) { return (b > a) ? b : a; }
+
+struct mom_gpu_ctx {
+  mom_gpu_domain_t dom;
+  KDom kd;
+  int device;
+  hipStream_t stream;
+  hipEvent_t ev[8];
+  // device copies of the grid (owned)
+  double *dxt, *dyt, *dat, *datr, *dxte, *dyte, *dxtn, *dytn;
+  double *dxu, *dyu, *dxur, *dyur, *f;
+  int *kmt, *kmu;
+  double *dzt;
+  // Quicker weights (quicker_init): 16 data-domain planes
+  // quick_x(2) quick_y(2) curv_xp(3) curv_xn(3) curv_yp(3) curv_yn(3),
+  // and quick_z(nk,2) curv_zp(nk,3) curv_zn(nk,3).
+  double *qw;
+  double *qzw;
+  // barotropic work: eta_bt(2 levels), udrho_bt(2 comps x 2 levels),
+  // steady forcing, cori1, cori2
+  double *bt;
+  double bt_cori_dtbt;
+  // invtri work (e coefficients), one 3-D plane set
+  double *tri_e;
+  // reduction partials
+  double *red;
+  int red_n;
+  // staging for small per-call tables
+  char err[512];
+};
+
+// Error handling: every entry point returns through these.
+int mom_fail(mom_gpu_ctx *ctx, int code, const char *fmt, ...);
+
+#define MOM_HIP(ctx, call)                                                  \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess)                                                   \
+      return mom_fail((ctx), MOM_GPU_ERR_HIP, "%s failed: %s", #call,       \
+                      hipGetErrorString(e_));                               \
+  } while (0)
+
+#define MOM_LAUNCHED(ctx, name)                                             \
+  do {                                                                      \
+    hipError_t e_ = hipGetLastError();                                      \
+    if (e_ != hipSuccess)                                                   \
+      return mom_fail((ctx), MOM_GPU_ERR_HIP, "launch of %s failed: %s",    \
+                      (name), hipGetErrorString(e_));                       \
+  } while (0)
+
+// Kernel launchers implemented in the k_*.hip translation units.
+int launch_mdfl(mom_gpu_ctx *ctx, int flavor, const double *T,
+                const double *u, const double *v, const double *w,
+                const double *rho, double dtime, double lim, double *th,
+                double *wrk1, double *fx, double *fy, double *fz);
+int launch_quicker_init(mom_gpu_ctx *ctx);
+int launch_quicker_horz(mom_gpu_ctx *ctx, const double *Tm1, const double *Tt,
+                        const double *u, const double *v,
+                        const double *tlim, int limit_with_upwind,
+                        double *th, double *wrk1, double *fx, double *fy);
+int launch_quicker_vert(mom_gpu_ctx *ctx, const double *Tm1, const double *Tt,
+                        const double *w, const double *tlim, double *th,
+                        double *wrk1, double *fz);
+int launch_tmask_limit(mom_gpu_ctx *ctx, int ntr, const double *const *T,
+                       const double *mn, const double *mx, int it, int is,
+                       int ts_same, double *const *lim);
+int launch_explicit(mom_gpu_ctx *ctx, const double *Tm1, const double *th,
+                    const double *rdz, const double *rdzr, double dtime,
+                    double *Tp1);
+int launch_invtri(mom_gpu_ctx *ctx, double *z, const double *topbc,
+                  const double *botbc, const double *dcb, double tdt,
+                  const int *kmz, const double *mask, const double *dh,
+                  const double *dhw, double aidif);
+int launch_vdiff_implicit(mom_gpu_ctx *ctx, int ntr, double *const *T,
+                          const double *const *stf, const double *const *btf,
+                          const double *diff_cbt,
+                          const double *const *K33, int index_salt,
+                          double rho0, const double *dh, const double *dhw,
+                          double dt, double aidif);
+int launch_barotropic(mom_gpu_ctx *ctx, const mom_gpu_bt_params_t *p,
+                      const double *eta_t, const double *eta_bar_tau,
+                      const double *ud_tau, const double *forcing,
+                      const double *thicku, const double *patm,
+                      const double *pme, const double *river,
+                      const double *source, const double *eta_smooth,
+                      const double *rho_g, double *eta_bar, double *ud,
+                      double *conv);
+int launch_content(mom_gpu_ctx *ctx, const double *T, const double *rho,
+                   double *out);
+int launch_check_masks(mom_gpu_ctx *ctx, const double *tmask,
+                       const double *umask, long long *bad);

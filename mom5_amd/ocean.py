@@ -1,0 +1,245 @@
+"""Host-side mirror of MOM5's hot-path interface over the MI355X kernels.
+
+Method names and argument meaning follow the reference routines
+(``src/mom5`` of mom-ocean/MOM5):
+
+==============================  ============================================
+``horz_advect_tracer``          ocean_tracers/ocean_tracer_advect.F90:1898
+``vert_advect_tracer``          ocean_tracers/ocean_tracer_advect.F90:2095
+``advect_tracer_sweby_all``     ocean_tracers/ocean_tracer_advect.F90:4104
+``compute_tmask_limit``         ocean_tracers/ocean_tracer.F90:2788
+``update_tracer_explicit``      ocean_tracers/ocean_tracer.F90:2341-2350
+``invtri``                      ocean_core/ocean_util.F90:153
+``vert_diffuse_implicit``       ocean_param/vertical/ocean_vert_mix.F90:3426
+``update_ocean_barotropic``     ocean_core/ocean_barotropic.F90:2763
+``update_ocean_tracer``         ocean_tracers/ocean_tracer.F90:2243 (order)
+==============================  ============================================
+
+Every array argument is a :class:`DeviceArray` (or a raw device address);
+errors raise :class:`MomGpuError` with the library's message, as the
+reference's ``mpp_error(FATAL, ...)`` would stop the run.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import BtParams, Domain, Grid, MomGpuError, check, load
+
+ADVECT_QUICKER = _lib.MOM_ADVECT_QUICKER
+ADVECT_MDFL_SWEBY = _lib.MOM_ADVECT_MDFL_SWEBY
+ADVECT_DST_LINEAR = _lib.MOM_ADVECT_DST_LINEAR
+SCHEMES = {"quicker": ADVECT_QUICKER, "mdfl_sweby": ADVECT_MDFL_SWEBY,
+           "dst_linear": ADVECT_DST_LINEAR}
+
+
+class DeviceArray:
+    """A device allocation holding one field in its Fortran memory image."""
+
+    def __init__(self, shape, dtype=np.float64):
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        check(load().mom_gpu_malloc(C.byref(p), self.nbytes), None, "mom_gpu_malloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, arr):
+        arr = np.ascontiguousarray(arr)
+        d = cls(arr.shape, arr.dtype)
+        d.upload(arr)
+        return d
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr, dtype=self.dtype)
+        assert arr.nbytes == self.nbytes, (arr.shape, self.shape)
+        check(load().mom_gpu_memcpy_htod(self.ptr, arr.ctypes.data, self.nbytes), None, "H2D")
+
+    def to_host(self):
+        out = np.empty(self.shape, self.dtype)
+        check(load().mom_gpu_memcpy_dtoh(out.ctypes.data, self.ptr, self.nbytes), None, "D2H")
+        return out
+
+    def zero(self):
+        check(load().mom_gpu_memset(self.ptr, 0, self.nbytes), None, "memset")
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            load().mom_gpu_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _p(a):
+    if a is None:
+        return None
+    if isinstance(a, DeviceArray):
+        return a.ptr
+    return int(a)
+
+
+def _ptab(arrs):
+    tab = (C.c_void_p * max(len(arrs), 1))()
+    for n, a in enumerate(arrs):
+        tab[n] = _p(a)
+    return tab
+
+
+class OceanGPU:
+    """One MOM domain (global or regional) resident on the GPU.
+
+    ``dom`` holds the Fortran bounds (isd..jed, isc..jec, nk, cyclic_x,
+    cyclic_y); ``grid`` the host arrays of ocean_grid_type fields named as in
+    MOM (dxt, dyt, dat, datr, dxte, dyte, dxtn, dytn, dxu, dyu, dxur, dyur,
+    f, kmt, kmu, tmask, umask, dzt).  The grid is copied to the device.
+    """
+
+    GRID_FIELDS = ("dxt", "dyt", "dat", "datr", "dxte", "dyte", "dxtn", "dytn", "dxu", "dyu",
+                   "dxur", "dyur", "f", "kmt", "kmu", "tmask", "umask", "dzt")
+
+    def __init__(self, dom: dict, grid: dict):
+        self.L = load()
+        self.dom = dict(dom)
+        self.ni = dom["iec"] - dom["isc"] + 1
+        self.nj = dom["jec"] - dom["jsc"] + 1
+        self.nk = dom["nk"]
+        self.shape2 = (self.nj + 2, self.ni + 2)
+        self.shape3 = (self.nk,) + self.shape2
+        staged = {}
+        for name in self.GRID_FIELDS:
+            dt = np.int32 if name in ("kmt", "kmu") else np.float64
+            staged[name] = DeviceArray.from_host(np.asarray(grid[name], dtype=dt))
+        g = Grid(**{k: v.ptr for k, v in staged.items()})
+        self._dom_c = Domain(*[int(dom[n]) for n, _ in Domain._fields_])
+        ctx = C.c_void_p()
+        check(self.L.mom_gpu_create(C.byref(self._dom_c), C.byref(g), C.byref(ctx)), None,
+              "mom_gpu_create")
+        self.ctx = ctx.value
+        # the context copied what it needs
+        for v in staged.values():
+            v.free()
+        self.grid_host = grid
+
+    # ---- helpers ---------------------------------------------------------
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.L.mom_gpu_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        check(rc, self.ctx, what)
+
+    def sync(self):
+        self._chk(self.L.mom_gpu_sync(self.ctx), "sync")
+
+    def mark(self, slot):
+        self._chk(self.L.mom_gpu_mark(self.ctx, slot), "mark")
+
+    def elapsed_ms(self, s0, s1):
+        ms = C.c_float()
+        self._chk(self.L.mom_gpu_elapsed(self.ctx, s0, s1, C.byref(ms)), "elapsed")
+        return float(ms.value)
+
+    def field3(self, value=None, dtype=np.float64):
+        a = DeviceArray(self.shape3, dtype)
+        if value is None:
+            a.zero()
+        else:
+            a.upload(value)
+        return a
+
+    # ---- tracer advection ------------------------------------------------
+    def horz_advect_tracer(self, scheme, T_taum1, th_tendency, *, uhrho_et, vhrho_nt,
+                           wrho_bt=None, rho_dzt_tau=None, T_tau=None, dtime=0.0,
+                           tmask_limit=None, limit_with_upwind=False, wrk1=None,
+                           flux_x=None, flux_y=None, flux_z=None):
+        sch = SCHEMES.get(scheme, scheme)
+        self._chk(self.L.mom_gpu_horz_advect_tracer(
+            self.ctx, int(sch), _p(T_taum1), _p(T_tau), _p(uhrho_et), _p(vhrho_nt), _p(wrho_bt),
+            _p(rho_dzt_tau), _p(tmask_limit), int(bool(limit_with_upwind)), float(dtime),
+            _p(th_tendency), _p(wrk1), _p(flux_x), _p(flux_y), _p(flux_z)), "horz_advect_tracer")
+
+    def vert_advect_tracer(self, scheme, T_taum1, th_tendency, *, wrho_bt=None, T_tau=None,
+                           tmask_limit=None, wrk1=None, flux_z=None):
+        sch = SCHEMES.get(scheme, scheme)
+        self._chk(self.L.mom_gpu_vert_advect_tracer(
+            self.ctx, int(sch), _p(T_taum1), _p(T_tau), _p(wrho_bt), _p(tmask_limit),
+            _p(th_tendency), _p(wrk1), _p(flux_z)), "vert_advect_tracer")
+
+    def advect_tracer_sweby_all(self, T_taum1s, th_tendencies, *, uhrho_et, vhrho_nt, wrho_bt,
+                                rho_dzt_tau, dtime, wrk1s=None):
+        n = len(T_taum1s)
+        w = _ptab(wrk1s) if wrk1s is not None else None
+        self._chk(self.L.mom_gpu_advect_sweby_all(
+            self.ctx, n, _ptab(T_taum1s), _p(uhrho_et), _p(vhrho_nt), _p(wrho_bt),
+            _p(rho_dzt_tau), float(dtime), _ptab(th_tendencies), w), "advect_sweby_all")
+
+    def compute_tmask_limit(self, T_taus, min_limits, max_limits, tmask_limits, index_temp=0,
+                            index_salt=1, ts_same=True):
+        n = len(T_taus)
+        mn = (C.c_double * n)(*[float(x) for x in min_limits])
+        mx = (C.c_double * n)(*[float(x) for x in max_limits])
+        self._chk(self.L.mom_gpu_compute_tmask_limit(
+            self.ctx, n, _ptab(T_taus), mn, mx, int(index_temp), int(index_salt), int(ts_same),
+            _ptab(tmask_limits)), "compute_tmask_limit")
+
+    def update_tracer_explicit(self, T_taum1, th_tendency, rho_dzt_taum1, rho_dztr, dtime,
+                               T_taup1):
+        self._chk(self.L.mom_gpu_update_tracer_explicit(
+            self.ctx, _p(T_taum1), _p(th_tendency), _p(rho_dzt_taum1), _p(rho_dztr),
+            float(dtime), _p(T_taup1)), "update_tracer_explicit")
+
+    # ---- vertical diffusion ------------------------------------------------
+    def invtri(self, z, topbc, botbc, dcb, tdt, kmz, mask, dh, dhw, aidif=1.0):
+        self._chk(self.L.mom_gpu_invtri(self.ctx, _p(z), _p(topbc), _p(botbc), _p(dcb),
+                                        float(tdt), _p(kmz), _p(mask), _p(dh), _p(dhw),
+                                        float(aidif)), "invtri")
+
+    def vert_diffuse_implicit(self, T_taup1s, stfs, btfs, diff_cbt, rho_dzt_taup1, dzwt,
+                              dtime_t, *, index_salt=1, aidif=1.0, rho0=1035.0, K33s=None):
+        n = len(T_taup1s)
+        k33 = _ptab(K33s) if K33s is not None else None
+        self._chk(self.L.mom_gpu_vert_diffuse_implicit(
+            self.ctx, n, _ptab(T_taup1s), _ptab(stfs), _ptab(btfs), _p(diff_cbt), k33,
+            int(index_salt), float(rho0), _p(rho_dzt_taup1), _p(dzwt), float(dtime_t),
+            float(aidif)), "vert_diffuse_implicit")
+
+    # ---- barotropic ----------------------------------------------------------
+    def update_ocean_barotropic(self, *, nts, dtbt, eta_t_bar_tau, udrho_tau, forcing_bt, thicku,
+                                patm, pme, river, source, eta_smooth, rho_g, eta_t_bar_taup1,
+                                udrho_taup1, conv_rho_ud_t=None, eta_t_tau=None,
+                                pred_corr_gamma=0.2, rho0=1035.0, stepping_a=False,
+                                initsum_with_bar=True):
+        p = BtParams(nts=int(nts), dtbt=float(dtbt), pred_corr_gamma=float(pred_corr_gamma),
+                     rho0=float(rho0), stepping_a=int(bool(stepping_a)),
+                     initsum_with_bar=int(bool(initsum_with_bar)))
+        self._chk(self.L.mom_gpu_update_ocean_barotropic(
+            self.ctx, C.byref(p), _p(eta_t_tau), _p(eta_t_bar_tau), _p(udrho_tau),
+            _p(forcing_bt), _p(thicku), _p(patm), _p(pme), _p(river), _p(source),
+            _p(eta_smooth), _p(rho_g), _p(eta_t_bar_taup1), _p(udrho_taup1),
+            _p(conv_rho_ud_t)), "update_ocean_barotropic")
+
+    # ---- diagnostics ----------------------------------------------------------
+    def tracer_content(self, T, rho_dzt):
+        out = C.c_double()
+        self._chk(self.L.mom_gpu_tracer_content(self.ctx, _p(T), _p(rho_dzt), C.byref(out)),
+                  "tracer_content")
+        return out.value
+
+
+__all__ = ["DeviceArray", "OceanGPU", "MomGpuError", "ADVECT_QUICKER", "ADVECT_MDFL_SWEBY",
+           "ADVECT_DST_LINEAR", "SCHEMES"]

@@ -1,0 +1,335 @@
+"""Seeded synthetic ocean states on a spherical latitude-longitude B-grid.
+
+The fields follow the meaning MOM5 gives them (names are MOM's):
+
+* grid metrics as in ``ocean_core/ocean_grids.F90`` -- ``dat = dxt*dyt``
+  (:844), reciprocals (:956-974), masks from ``kmt``/``kmu`` (:1374-1410),
+  ``kmu = min`` of the four surrounding ``kmt`` (``ocean_topog.F90:337``),
+  Coriolis ``f = 2*omega*sin(lat_u)`` (``ocean_coriolis.F90:269``);
+* vertical grid ``dzt``, ``zw``, ``zt``, ``dzw`` as ``ocean_grids.F90:1342-1358``;
+* transports ``uhrho_et``/``vhrho_nt`` (mass flux per unit face length) from a
+  horizontal streamfunction plus an x-z overturning, and ``wrho_bt`` from
+  continuity in the form of ``ocean_advection_velocity.F90:635-641``.
+
+The horizontal streamfunction alone is non-divergent layer by layer, so its
+``wrho_bt`` would vanish; the overturning part has zero vertical integral on
+every face, so it drives vertical motion while keeping ``wrho_bt(kmt) = 0``.
+
+Arrays are numpy arrays whose C order equals the Fortran memory image:
+2-D ``(NJ, NI)``, 3-D ``(nk, NJ, NI)``, 0:nk arrays ``(nk+1, NJ, NI)``, with
+``NI = ni + 2`` and ``NJ = nj + 2`` (one halo cell, MOM's default).  On a
+cyclic-x domain every halo column (corner rows included) holds the wrap of
+the opposite compute column; closed boundaries hold land.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+RADIUS = 6371.0e3      # shared/constants/constants.F90:72
+OMEGA = 7.292e-5       # :75
+GRAV = 9.80            # ocean_core/ocean_parameters.F90:176
+RHO0 = 1035.0          # :194
+EPSLN = 1.0e-40        # shared/constants/constants.F90:123
+
+
+@dataclasses.dataclass
+class Case:
+    ni: int
+    nj: int
+    nk: int
+    cyclic_x: bool
+    dt: float                 # baroclinic tracer time step (dtime_t)
+    dtbt: float               # barotropic time step
+    nts: int                  # barotropic subcycles
+    grid: dict                # 2-D metrics, kmt/kmu, 3-D masks, dzt, dzw
+    fields: dict              # 3-D/2-D state and forcing
+    tracers: list             # list of (name, array) in prognostic order
+    info: dict
+
+    @property
+    def dom(self):
+        return dict(isd=0, ied=self.ni + 1, jsd=0, jed=self.nj + 1,
+                    isc=1, iec=self.ni, jsc=1, jec=self.nj, nk=self.nk,
+                    cyclic_x=int(self.cyclic_x), cyclic_y=0)
+
+
+def _wrap_x(a, cyclic):
+    """Fill halo columns: cyclic wrap (whole column) or closed (zero)."""
+    if cyclic:
+        a[..., 0] = a[..., -2]
+        a[..., -1] = a[..., 1]
+    else:
+        a[..., 0] = 0
+        a[..., -1] = 0
+    return a
+
+
+def _smooth_field(rng, ni, nj, nmodes, kmax, cyclic):
+    """Sum of random Fourier modes on the compute domain (nj, ni)."""
+    x = (np.arange(ni) + 0.5) / ni
+    y = (np.arange(nj) + 0.5) / nj
+    out = np.zeros((nj, ni))
+    for _ in range(nmodes):
+        kx = int(rng.integers(1, kmax + 1)) if cyclic else float(rng.uniform(0.5, kmax))
+        ky = float(rng.uniform(0.5, kmax))
+        ph1, ph2 = rng.uniform(0, 2 * math.pi, 2)
+        amp = rng.uniform(0.5, 1.0) / (kx * kx + ky * ky) ** 0.5
+        out += amp * np.outer(np.cos(2 * math.pi * ky * y + ph2),
+                              np.cos(2 * math.pi * kx * x + ph1))
+    return out
+
+
+def vertical_grid(nk, dz_top=10.0, dz_bot=500.0):
+    """Geometric layer thicknesses and the derived MOM vertical metrics."""
+    if nk == 1:
+        dzt = np.array([dz_top])
+    else:
+        r = (dz_bot / dz_top) ** (1.0 / (nk - 1))
+        dzt = dz_top * r ** np.arange(nk)
+    zw = np.cumsum(dzt)
+    zt = zw - 0.5 * dzt
+    dzw = np.empty(nk + 1)
+    dzw[0] = zt[0]
+    dzw[1:nk] = zt[1:] - zt[:-1]
+    dzw[nk] = zw[-1] - zt[-1]
+    return dzt, zw, zt, dzw
+
+
+def make_case(ni, nj, nk, *, lat_s=-10.0, lat_n=80.0, lon_span=360.0,
+              cyclic_x=True, land_frac=0.25, seed=0x4D4F4D35, n_passive=0,
+              cfl=0.3, nts=80, barotropic=True, passive_dtype=np.float64,
+              stf_scale=0.0, pme_scale=0.0):
+    """Build one seeded synthetic case (see module docstring)."""
+    rng = np.random.default_rng(seed)
+    NI, NJ = ni + 2, nj + 2
+    dlon = lon_span / ni
+    dlat = (lat_n - lat_s) / nj
+    ii = np.arange(NI, dtype=np.float64)          # Fortran i = 0..ni+1
+    jj = np.arange(NJ, dtype=np.float64)
+    lat_t = np.deg2rad(lat_s + (jj - 0.5) * dlat)
+    lat_u = np.deg2rad(lat_s + jj * dlat)
+    dlon_r = math.radians(dlon)
+    dlat_r = math.radians(dlat)
+    ones = np.ones((NJ, NI))
+    dxt = (RADIUS * np.cos(lat_t) * dlon_r)[:, None] * ones
+    dyt = RADIUS * dlat_r * ones
+    dxu = (RADIUS * np.cos(lat_u) * dlon_r)[:, None] * ones
+    dyu = RADIUS * dlat_r * ones
+    dxte = dxt.copy()                    # T(i) -> T(i+1) along the T row
+    dyte = dyt.copy()                    # east face length of the T cell
+    dxtn = dxu.copy()                    # north face length (at lat_u)
+    dytn = dyt.copy()                    # T(j) -> T(j+1)
+    dat = dxt * dyt
+    datr = 1.0 / (dat + EPSLN)
+    dxur = 1.0 / (dxu + EPSLN)
+    dyur = 1.0 / (dyu + EPSLN)
+    f = (2.0 * OMEGA * np.sin(lat_u))[:, None] * ones
+
+    dzt, zw, zt, dzw = vertical_grid(nk)
+
+    # bathymetry: smooth field thresholded to the land fraction
+    bf = _smooth_field(rng, ni, nj, 12, 6, cyclic_x)
+    q = np.quantile(bf, land_frac)
+    h = np.clip((bf - q) / (bf.max() - q + 1e-300), 0.0, 1.0)
+    depth = zw[-1] * np.clip(0.2 + 1.6 * h, 0.0, 1.0)
+    kmt_c = np.searchsorted(zw, depth * (1 - 1e-12), side="left") + 1
+    kmt_c = np.minimum(kmt_c, nk)
+    kmt_c[bf <= q] = 0
+    if not cyclic_x:                      # closed basin walls
+        kmt_c[:, 0] = 0
+        kmt_c[:, -1] = 0
+    kmt = np.zeros((NJ, NI), dtype=np.int32)
+    kmt[1:-1, 1:-1] = kmt_c
+    _wrap_x(kmt, cyclic_x)
+    kmu = np.zeros((NJ, NI), dtype=np.int32)
+    kmu[:, :-1][:-1] = np.minimum.reduce([kmt[:-1, :-1], kmt[:-1, 1:],
+                                          kmt[1:, :-1], kmt[1:, 1:]])
+    if cyclic_x:
+        kmu[:, -1] = kmu[:, 1]
+    kk = np.arange(1, nk + 1)[:, None, None]
+    tmask = (kk <= kmt[None]).astype(np.float64)
+    umask = (kk <= kmu[None]).astype(np.float64)
+
+    # horizontal streamfunction on U points (m^2/s per unit depth, unscaled)
+    psi_c = _smooth_field(rng, ni, nj, 8, 4, cyclic_x)
+    psi = np.zeros((NJ, NI))
+    psi[1:-1, 1:-1] = psi_c
+    _wrap_x(psi, cyclic_x)
+    decay = np.exp(-zt / 1000.0)
+    psik = psi[None] * decay[:, None, None] * umask          # (nk, NJ, NI)
+
+    # overturning streamfunction on x faces, zero at surface and face bottom
+    ov_c = _smooth_field(rng, ni, nj, 6, 3, cyclic_x)
+    ov = np.zeros((NJ, NI))
+    ov[1:-1, 1:-1] = ov_c
+    _wrap_x(ov, cyclic_x)
+    kface = np.zeros((NJ, NI), dtype=np.int32)
+    kface[:, :-1] = np.minimum(kmt[:, :-1], kmt[:, 1:])
+    if cyclic_x:
+        kface[:, -1] = kface[:, 1]
+    hface = np.where(kface > 0, zw[np.maximum(kface, 1) - 1], 0.0)
+    zint = np.concatenate([[0.0], zw])                          # interfaces 0..nk
+    phi = np.zeros((nk + 1, NJ, NI))
+    for kk_ in range(nk + 1):
+        inside = kk_ < kface
+        phi[kk_] = np.where(inside, ov * np.sin(math.pi * zint[kk_] / np.maximum(hface, 1.0)), 0.0)
+
+    # mass transports through T-cell faces (kg/s), unit amplitude
+    dyte_u = np.zeros((nk, NJ, NI))
+    dyte_u[:, 1:, :] = RHO0 * dzt[:, None, None] * (psik[:, 1:, :] - psik[:, :-1, :])
+    m_ov = RHO0 * (phi[:-1] - phi[1:])
+    dyte_u += m_ov * (0.3 * np.abs(dyte_u).max() / max(np.abs(m_ov).max(), 1e-300))
+    dxtn_v = np.zeros((nk, NJ, NI))
+    dxtn_v[:, :, 1:] = -RHO0 * dzt[:, None, None] * (psik[:, :, 1:] - psik[:, :, :-1])
+    # closed rows / walls carry no flow; wrap halo columns
+    dyte_u *= tmask * np.concatenate([tmask[:, :, 1:], np.zeros((nk, NJ, 1))], axis=2)
+    dxtn_v *= tmask * np.concatenate([tmask[:, 1:, :], np.zeros((nk, 1, NI))], axis=1)
+    dyte_u[:, 0, :] = 0.0
+    dyte_u[:, -1, :] = 0.0
+    dxtn_v[:, -1, :] = 0.0
+    _wrap_x(dyte_u, cyclic_x)
+    _wrap_x(dxtn_v, cyclic_x)
+
+    rho_dzt = np.broadcast_to(RHO0 * dzt[:, None, None], (nk, NJ, NI)).copy()
+    dzwt = np.broadcast_to(dzw[:, None, None], (nk + 1, NJ, NI)).copy()
+    hu = np.zeros((NJ, NI))
+    for k in range(nk):
+        hu += dzt[k] * umask[k]
+
+    # choose the time steps: barotropic stability fixes dt, the velocity
+    # amplitude is then scaled to the requested CFL; without a barotropic
+    # part the velocity scale is fixed and dt follows from the CFL.
+    wet_u = umask[0] > 0
+    if barotropic and wet_u.any():
+        cg = np.sqrt(GRAV * np.maximum(hu, 1.0))
+        dmin = np.minimum(dxu, dyu)
+        dtbt = 0.35 * float(np.min(dmin[wet_u] / cg[wet_u]))
+        dtbt = float(np.floor(dtbt * 64.0) / 64.0)
+        dt = dtbt * nts / 2.0
+    else:
+        dtbt = 0.0
+        dt = None
+
+    def cfl_max(scale, dtt):
+        u = scale * dyte_u / dyte[None]
+        v = scale * dxtn_v / dxtn[None]
+        cx = np.abs(u[:, :, :-1] * dtt * 2.0 / ((rho_dzt[:, :, :-1] + rho_dzt[:, :, 1:]) * dxte[None, :, :-1]))
+        cy = np.abs(v[:, :-1, :] * dtt * 2.0 / ((rho_dzt[:, :-1, :] + rho_dzt[:, 1:, :]) * dytn[None, :-1, :]))
+        w = _continuity(scale * dyte_u, scale * dxtn_v, tmask, datr)
+        cz = np.abs(w[1:] * dtt / rho_dzt)
+        return max(cx.max(), cy.max(), cz.max())
+
+    # reference velocity ~0.2 m/s at the surface for unit amplitude
+    base = 0.2 / max(np.abs(dyte_u[0] / dyte / (RHO0 * dzt[0])).max(), 1e-30)
+    if dt is None:
+        c1 = cfl_max(base, 1.0)
+        dt = cfl / c1
+        scale = base
+    else:
+        c1 = cfl_max(1.0, dt)
+        scale = cfl / c1
+    uhrho_et = scale * dyte_u / dyte[None]
+    vhrho_nt = scale * dxtn_v / dxtn[None]
+    wrho = _continuity(uhrho_et * dyte[None], vhrho_nt * dxtn[None], tmask, datr)
+    _wrap_x(wrho, cyclic_x)
+
+    # tracers
+    tr = []
+    noise = rng.standard_normal((nk, nj, ni))
+    T = np.zeros((nk, NJ, NI))
+    T[:, 1:-1, 1:-1] = 2.0 + 25.0 * np.exp(-zt / 800.0)[:, None, None] + 0.5 * noise
+    T *= tmask
+    _wrap_x(T, cyclic_x)
+    noise = rng.standard_normal((nk, nj, ni))
+    S = np.zeros((nk, NJ, NI))
+    S[:, 1:-1, 1:-1] = 35.0 + 0.2 * noise
+    S *= tmask
+    _wrap_x(S, cyclic_x)
+    tr.append(("temp", T))
+    tr.append(("salt", S))
+    for n in range(n_passive):
+        noise = rng.standard_normal((nk, nj, ni))
+        P = np.zeros((nk, NJ, NI))
+        P[:, 1:-1, 1:-1] = np.exp(math.log(1e-3) + 1.5 * noise)
+        P *= tmask
+        _wrap_x(P, cyclic_x)
+        tr.append((f"passive{n + 1:02d}", P.astype(passive_dtype)))
+
+    # vertical mixing coefficients, log-uniform in [1e-5, 1e-2] m^2/s
+    diff_cbt = np.zeros((2, nk, NJ, NI))
+    diff_cbt[:, :, 1:-1, 1:-1] = 10.0 ** rng.uniform(-5.0, -2.0, (2, nk, nj, ni))
+    _wrap_x(diff_cbt, cyclic_x)
+    stf = []
+    btf = []
+    for _ in tr:
+        s = np.zeros((NJ, NI))
+        b = np.zeros((NJ, NI))
+        if stf_scale:
+            s[1:-1, 1:-1] = stf_scale * rng.standard_normal((nj, ni))
+            b[1:-1, 1:-1] = stf_scale * rng.standard_normal((nj, ni))
+            s *= tmask[0]
+            b *= (kmt > 0)
+            _wrap_x(s, cyclic_x)
+            _wrap_x(b, cyclic_x)
+        stf.append(s)
+        btf.append(b)
+
+    # barotropic state and forcing
+    eta = np.zeros((NJ, NI))
+    eta[1:-1, 1:-1] = 0.1 * rng.standard_normal((nj, ni))
+    eta *= tmask[0]
+    _wrap_x(eta, cyclic_x)
+    forcing_bt = np.zeros((2, NJ, NI))
+    forcing_bt[:, 1:-1, 1:-1] = rng.standard_normal((2, nj, ni))
+    forcing_bt *= umask[0][None]
+    _wrap_x(forcing_bt, cyclic_x)
+    pme = np.zeros((NJ, NI))
+    if pme_scale:
+        pme[1:-1, 1:-1] = pme_scale * rng.standard_normal((nj, ni))
+        pme *= tmask[0]
+        _wrap_x(pme, cyclic_x)
+    zeros2 = np.zeros((NJ, NI))
+    grid = dict(dxt=dxt, dyt=dyt, dat=dat, datr=datr, dxte=dxte, dyte=dyte,
+                dxtn=dxtn, dytn=dytn, dxu=dxu, dyu=dyu, dxur=dxur, dyur=dyur,
+                f=f, kmt=kmt, kmu=kmu, tmask=tmask, umask=umask, dzt=dzt,
+                dzw=dzw, zw=zw, zt=zt)
+    fields = dict(uhrho_et=uhrho_et, vhrho_nt=vhrho_nt, wrho_bt=wrho,
+                  rho_dzt=rho_dzt, rho_dztr=1.0 / (rho_dzt + EPSLN),
+                  dzwt=dzwt, diff_cbt=diff_cbt, stf=stf, btf=btf,
+                  eta_t=eta.copy(), eta_t_bar=eta.copy(),
+                  udrho=np.zeros((2, NJ, NI)), forcing_bt=forcing_bt,
+                  thicku=hu, patm=zeros2.copy(), pme=pme, river=zeros2.copy(),
+                  source=zeros2.copy(), eta_smooth=zeros2.copy(),
+                  rho_g=np.full((NJ, NI), GRAV * RHO0))
+    info = dict(max_cfl=cfl, land_frac=float((kmt_c == 0).mean()), seed=seed)
+    return Case(ni=ni, nj=nj, nk=nk, cyclic_x=cyclic_x, dt=float(dt),
+                dtbt=float(dtbt), nts=int(nts), grid=grid, fields=fields,
+                tracers=tr, info=info)
+
+
+def _continuity(mx, my, tmask, datr):
+    """wrho_bt(k) = (div_h(k) + wrho_bt(k-1)) * tmask(k), wrho_bt(0) = 0,
+    with div_h from the face mass transports mx = dyte*uhrho_et and
+    my = dxtn*vhrho_nt (ocean_advection_velocity.F90:635-641 form)."""
+    nk = tmask.shape[0]
+    w = np.zeros((nk + 1,) + tmask.shape[1:])
+    div = np.zeros(tmask.shape)
+    div[:, :, 1:] += mx[:, :, 1:] - mx[:, :, :-1]
+    div[:, 1:, :] += my[:, 1:, :] - my[:, :-1, :]
+    div *= datr[None] * tmask
+    for k in range(nk):
+        w[k + 1] = (div[k] + w[k]) * tmask[k]
+    w[:, 0, :] = 0.0
+    w[:, -1, :] = 0.0
+    return w
+
+
+CONFIGS = {
+    # 90 x 60 x 25, advection only, MDFL-Sweby on T and S
+    0: dict(ni=90, nj=60, nk=25, land_frac=0.25, barotropic=False),
+    # 1440 x 1080 x 50, T,S (MDFL) + 1 Quicker tracer, 80 subcycles
+    1: dict(ni=1440, nj=1080, nk=50, land_frac=0.30, nts=80, n_passive=1),
+}
