@@ -1,0 +1,38 @@
+"""Hand-built domains for the closed-form and invariant checks."""
+import types
+
+import numpy as np
+
+
+def uniform_case(ni, nj, nk, *, cyclic_x=True, dx=1.0, dy=1.0, dz=1.0, rho=1.0, f=0.0,
+                 kmt=None, land_rows=True):
+    """A Cartesian box: every metric constant, every interior column kmt=nk
+    (unless given); closed rows j=0 and j=nj+1 are land."""
+    NI, NJ = ni + 2, nj + 2
+    one = np.ones((NJ, NI))
+    if kmt is None:
+        kmt = np.zeros((NJ, NI), dtype=np.int32)
+        kmt[1:-1, 1:-1] = nk
+        if cyclic_x:
+            kmt[:, 0] = kmt[:, -2]
+            kmt[:, -1] = kmt[:, 1]
+    kmt = np.ascontiguousarray(kmt, dtype=np.int32)
+    kmu = np.zeros_like(kmt)
+    kmu[:-1, :-1] = np.minimum.reduce([kmt[:-1, :-1], kmt[:-1, 1:], kmt[1:, :-1], kmt[1:, 1:]])
+    if cyclic_x:
+        kmu[:, -1] = kmu[:, 1]
+    kk = np.arange(1, nk + 1)[:, None, None]
+    tmask = (kk <= kmt[None]).astype(np.float64)
+    umask = (kk <= kmu[None]).astype(np.float64)
+    grid = dict(dxt=dx * one, dyt=dy * one, dat=dx * dy * one, datr=one / (dx * dy),
+                dxte=dx * one, dyte=dy * one, dxtn=dx * one, dytn=dy * one, dxu=dx * one,
+                dyu=dy * one, dxur=one / dx, dyur=one / dy, f=f * one, kmt=kmt, kmu=kmu,
+                tmask=tmask, umask=umask, dzt=dz * np.ones(nk))
+    z3 = np.zeros((nk, NJ, NI))
+    fields = dict(uhrho_et=z3.copy(), vhrho_nt=z3.copy(), wrho_bt=np.zeros((nk + 1, NJ, NI)),
+                  rho_dzt=rho * np.ones((nk, NJ, NI)), rho_dztr=np.ones((nk, NJ, NI)) / rho,
+                  dzwt=dz * np.ones((nk + 1, NJ, NI)), diff_cbt=np.zeros((2, nk, NJ, NI)))
+    c = types.SimpleNamespace(ni=ni, nj=nj, nk=nk, cyclic_x=cyclic_x, grid=grid, fields=fields,
+                              dom=dict(isd=0, ied=ni + 1, jsd=0, jed=nj + 1, isc=1, iec=ni,
+                                       jsc=1, jec=nj, nk=nk, cyclic_x=int(cyclic_x), cyclic_y=0))
+    return c

@@ -1,0 +1,207 @@
+"""Pins the MDFL-Sweby / DST-linear restatement (ocean_tracer_advect.F90:
+3806-4066, 4104-4511) with properties the mathematics fixes, not by
+re-deriving its formula:
+
+* zero velocity gives a zero tendency; constant tracer with w from
+  continuity gives a zero tendency (flux form + divergence correction);
+* the masked domain integral of dat*tendency vanishes (conservation);
+* at CFL = 1 every limiter coefficient vanishes and a sweep is an exact
+  one-cell shift (x, y and z);
+* DST-linear (limiter 0) is the third-order direct space-time scheme, exact
+  for cell averages of quadratics; with the Sweby limiter the same holds on
+  smooth monotone data, while rough data stays within its bounds (TVD);
+* advect_tracer_sweby_all differs from the per-tracer routine only in
+  rounding order.
+"""
+import numpy as np
+import pytest
+
+from _cases import uniform_case
+
+
+@pytest.fixture(scope="module")
+def orc(oracle_lib):
+    return oracle_lib
+
+
+def _synth(**kw):
+    from mom5_amd.synth import make_case
+    return make_case(**kw)
+
+
+@pytest.fixture(scope="module")
+def cfg0():
+    return _synth(ni=90, nj=60, nk=25, land_frac=0.25, barotropic=False)
+
+
+def test_zero_velocity_zero_tendency(orc, cfg0):
+    o = orc.Oracle(cfg0)
+    f = dict(cfg0.fields)
+    for k in ("uhrho_et", "vhrho_nt", "wrho_bt"):
+        f[k] = np.zeros_like(f[k])
+    for lim in (0.0, 1.0):
+        adv = o.advect_mdfl_sweby(cfg0.tracers[0][1], cfg0.dt, lim=lim, fields=f)
+        assert np.all(adv == 0.0)
+
+
+def test_constancy_with_w_from_continuity(orc, cfg0):
+    o = orc.Oracle(cfg0)
+    const = 7.25 * cfg0.grid["tmask"]
+    adv_T = o.advect_mdfl_sweby(cfg0.tracers[0][1], cfg0.dt)
+    scale = np.abs(adv_T).max()
+    for lim in (0.0, 1.0):
+        adv = o.advect_mdfl_sweby(const, cfg0.dt, lim=lim)
+        assert np.abs(adv).max() <= 1e-12 * scale * 7.25
+
+
+def test_constancy_detects_a_wrong_correction_sign(orc, cfg0):
+    """The same check fails when w is not the continuity field."""
+    o = orc.Oracle(cfg0)
+    f = dict(cfg0.fields)
+    f["wrho_bt"] = -cfg0.fields["wrho_bt"]
+    const = 7.25 * cfg0.grid["tmask"]
+    adv = o.advect_mdfl_sweby(const, cfg0.dt, fields=f)
+    scale = np.abs(o.advect_mdfl_sweby(cfg0.tracers[0][1], cfg0.dt)).max()
+    assert np.abs(adv).max() > 1e-6 * scale
+
+
+@pytest.mark.parametrize("lim", [0.0, 1.0])
+def test_conservation(orc, cfg0, lim):
+    o = orc.Oracle(cfg0)
+    dat = cfg0.grid["dat"][1:-1, 1:-1]
+    for _, T in cfg0.tracers:
+        adv = o.advect_mdfl_sweby(T, cfg0.dt, lim=lim)
+        tot = (adv * dat[None]).sum()
+        assert abs(tot) <= 1e-12 * np.abs(adv * dat[None]).sum()
+
+
+def test_conservation_through_explicit_update(orc, cfg0):
+    o = orc.Oracle(cfg0)
+    T = cfg0.tracers[0][1]
+    th = np.zeros_like(T)
+    o.horz_advect_mdfl(T, th, cfg0.dt)
+    Tn = o.update_explicit(T, th, cfg0.dt)
+    c0, c1 = o.tracer_content(T), o.tracer_content(Tn)
+    assert abs(c1 - c0) <= 1e-13 * abs(c0)
+
+
+def _shift_case(axis, lim):
+    ni, nj, nk = 16, 12, 6
+    c = uniform_case(ni, nj, nk)
+    rng = np.random.default_rng(3)
+    T = np.zeros((nk, nj + 2, ni + 2))
+    T[:, 1:-1, 1:-1] = 1.0 + rng.random((nk, nj, ni))     # values in [1,2): exact differences
+    T[:, :, 0] = T[:, :, -2]
+    T[:, :, -1] = T[:, :, 1]
+    f = c.fields
+    if axis == "x":
+        f["uhrho_et"][:] = 1.0            # cfl = |u*dt*2/((1+1)*1)| = 1
+    elif axis == "y":
+        f["vhrho_nt"][:, 1:-2, :] = 1.0   # interior north faces; walls closed
+    else:
+        f["wrho_bt"][1:nk] = -1.0         # downward through interior faces
+    return c, T
+
+
+@pytest.mark.parametrize("lim", [0.0, 1.0])
+def test_cfl_one_is_an_exact_shift_x(orc, lim):
+    c, T = _shift_case("x", lim)
+    o = orc.Oracle(c)
+    adv = o.advect_mdfl_sweby(T, 1.0, lim=lim)
+    Tn = T[:, 1:-1, 1:-1] - adv            # rho = dt = 1
+    assert np.array_equal(Tn, np.roll(T[:, 1:-1, 1:-1], 1, axis=2))
+
+
+@pytest.mark.parametrize("lim", [0.0, 1.0])
+def test_cfl_one_is_an_exact_shift_y(orc, lim):
+    c, T = _shift_case("y", lim)
+    o = orc.Oracle(c)
+    adv = o.advect_mdfl_sweby(T, 1.0, lim=lim)
+    Tn = T[:, 1:-1, 1:-1] - adv
+    Ti = T[:, 1:-1, 1:-1]
+    assert np.array_equal(Tn[:, 1:-1, :], Ti[:, :-2, :])   # interior rows: T(j-1)
+    assert np.array_equal(Tn[:, 0, :], np.zeros_like(Tn[:, 0, :]))
+
+
+@pytest.mark.parametrize("lim", [0.0, 1.0])
+def test_cfl_one_is_an_exact_shift_z(orc, lim):
+    c, T = _shift_case("z", lim)
+    o = orc.Oracle(c)
+    adv = o.advect_mdfl_sweby(T, 1.0, lim=lim)
+    Tn = T[:, 1:-1, 1:-1] - adv
+    Ti = T[:, 1:-1, 1:-1]
+    assert np.array_equal(Tn[1:-1], Ti[:-2])     # interior levels: T(k-1)
+    assert np.array_equal(Tn[0], np.zeros_like(Tn[0]))
+
+
+def _quadratic_case(cfl, x0):
+    ni, nj, nk = 40, 4, 3
+    c = uniform_case(ni, nj, nk)
+    c.fields["uhrho_et"][:] = cfl
+    xr = x0 + np.arange(ni + 2, dtype=np.float64)        # right edge of cell i is xr[i]
+    p = lambda x: 0.5 * x ** 3 / 3.0 + 0.25 * x ** 2 - 3.0 * x   # antiderivative of q
+    avg = p(xr) - p(xr - 1.0)                            # cell averages of q = x^2/2 + x/2 - 3
+    T = np.broadcast_to(avg, (nk, nj + 2, ni + 2)).copy()
+    exact = (p(xr - cfl) - p(xr - 1.0 - cfl))
+    return c, T, exact
+
+
+@pytest.mark.parametrize("cfl", [0.25, 0.5, 0.8125])
+def test_dst_linear_exact_for_quadratics(orc, cfl):
+    c, T, exact = _quadratic_case(cfl, 10.0)
+    o = orc.Oracle(c)
+    adv = o.advect_mdfl_sweby(T, 1.0, lim=0.0)
+    Tn = T[:, 1:-1, 1:-1] - adv
+    sl = slice(4, -4)          # away from the cyclic seam of the non-periodic profile
+    ref = np.broadcast_to(exact[1:-1][sl], Tn[:, :, sl].shape)
+    assert np.allclose(Tn[:, :, sl], ref, rtol=1e-13, atol=0)
+
+
+def test_dst_linear_error_is_not_zero_for_a_wrong_scheme(orc):
+    """Upwind (d0=d1=0) would fail the same quadratic check: the check has teeth."""
+    c, T, exact = _quadratic_case(0.25, 10.0)
+    Ti = T[0, 1, 1:-1]
+    upwind = Ti - 0.25 * (Ti - np.roll(Ti, 1))
+    assert not np.allclose(upwind[4:-4], exact[1:-1][4:-4], rtol=1e-6)
+
+
+def test_sweby_exact_on_smooth_monotone_quadratics(orc):
+    c, T, exact = _quadratic_case(0.5, 10.0)
+    o = orc.Oracle(c)
+    adv = o.advect_mdfl_sweby(T, 1.0, lim=1.0)
+    Tn = T[:, 1:-1, 1:-1] - adv
+    sl = slice(4, -4)
+    assert np.allclose(Tn[:, :, sl], np.broadcast_to(exact[1:-1][sl], Tn[:, :, sl].shape),
+                       rtol=1e-13, atol=0)
+
+
+def test_sweby_is_tvd_dst_is_not(orc):
+    ni, nj, nk = 64, 3, 3
+    c = uniform_case(ni, nj, nk)
+    c.fields["uhrho_et"][:] = 0.7
+    rng = np.random.default_rng(9)
+    T = np.zeros((nk, nj + 2, ni + 2))
+    T[:, 1:-1, 1:-1] = (rng.random(ni) > 0.5).astype(float)[None, None, :] * 2.0 + 1.0
+    T[:, :, 0] = T[:, :, -2]
+    T[:, :, -1] = T[:, :, 1]
+    o = orc.Oracle(c)
+    Ti = T[:, 1:-1, 1:-1]
+    tn_s = Ti - o.advect_mdfl_sweby(T, 1.0, lim=1.0)
+    tn_d = Ti - o.advect_mdfl_sweby(T, 1.0, lim=0.0)
+    assert tn_s.min() >= Ti.min() - 1e-14 and tn_s.max() <= Ti.max() + 1e-14
+    assert tn_d.min() < Ti.min() - 1e-3 or tn_d.max() > Ti.max() + 1e-3
+
+
+def test_sweby_all_matches_per_tracer_to_rounding(orc, cfg0):
+    o = orc.Oracle(cfg0)
+    Ts = [t for _, t in cfg0.tracers]
+    th_all = [np.zeros_like(t) for t in Ts]
+    o.sweby_all(Ts, th_all, cfg0.dt)
+    for n, T in enumerate(Ts):
+        th = np.zeros_like(T)
+        o.horz_advect_mdfl(T, th, cfg0.dt, lim=1.0)
+        # the tendency is rho*(T_new - T)/dt: its rounding scale is rho*|T|/dt
+        scale = np.abs(cfg0.fields["rho_dzt"] * T).max() / cfg0.dt
+        diff = np.abs(th_all[n] - th).max()
+        assert diff <= 1e-13 * scale
+        assert diff > 0.0, "the two routines round differently (MOM doc claims bitwise)"
