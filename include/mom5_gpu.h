@@ -98,6 +98,9 @@ int mom_gpu_free(void *ptr);
 int mom_gpu_memcpy_htod(void *dst, const void *src, size_t bytes);
 int mom_gpu_memcpy_dtoh(void *dst, const void *src, size_t bytes);
 int mom_gpu_memset(void *ptr, int value, size_t bytes);
+/* Fill n doubles with value on the context's stream (a kernel, so it is
+ * ordered with the other calls on ctx and may be captured in a graph). */
+int mom_gpu_fill(mom_gpu_ctx_t *ctx, double *ptr, double value, size_t n);
 
 /* Elapsed time in ms between two marks recorded on the context's stream.
  * mom_gpu_mark(ctx, slot) records slot 0..7; mom_gpu_elapsed waits. */

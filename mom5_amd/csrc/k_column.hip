@@ -304,7 +304,22 @@ __global__ void k_check_masks(KDom d, const double *__restrict__ tmask,
   bad[o] = mine;
 }
 
+__global__ void k_fill(double *__restrict__ p, double v, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
 }  // namespace
+
+int launch_fill(mom_gpu_ctx *ctx, double *p, double v, size_t n) {
+  if (n == 0) return MOM_GPU_OK;
+  const size_t nb = (n + 255) / 256;
+  const unsigned grid = (unsigned)(nb < 65536 ? nb : 65536);
+  hipLaunchKernelGGL(k_fill, dim3(grid), dim3(256), 0, ctx->stream, p, v, n);
+  MOM_LAUNCHED(ctx, "k_fill");
+  return MOM_GPU_OK;
+}
 
 int launch_invtri(mom_gpu_ctx *ctx, double *z, const double *topbc, const double *botbc,
                   const double *dcb, double tdt, const int *kmz, const double *mask,

@@ -54,9 +54,18 @@ int mom_gpu_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
 }
 
 int mom_gpu_memset(void *ptr, int value, size_t bytes) {
+  // null-stream memset, completed before returning so that work later
+  // enqueued on any context stream sees it
   hipError_t e = hipMemset(ptr, value, bytes);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) return mom_fail(nullptr, MOM_GPU_ERR_HIP, "hipMemset: %s", hipGetErrorString(e));
   return MOM_GPU_OK;
+}
+
+int mom_gpu_fill(mom_gpu_ctx_t *ctx, double *ptr, double value, size_t n) {
+  REQUIRE(nullptr, ctx, "mom_gpu_fill: NULL context");
+  REQUIRE(ctx, ptr || n == 0, "mom_gpu_fill: NULL pointer");
+  return launch_fill(ctx, ptr, value, n);
 }
 
 static void free_ctx(mom_gpu_ctx *c) {
@@ -204,9 +213,11 @@ int mom_gpu_horz_advect_tracer(mom_gpu_ctx_t *ctx, int scheme, const double *T_t
   REQUIRE(ctx, T_taum1 && uhrho_et && vhrho_nt && th_tendency,
           "mom_gpu_horz_advect_tracer: NULL required array");
   const size_t b3 = p3bytes(ctx);
-  if (wrk1) MOM_HIP(ctx, hipMemsetAsync(wrk1, 0, b3, ctx->stream));
-  if (flux_x) MOM_HIP(ctx, hipMemsetAsync(flux_x, 0, b3, ctx->stream));
-  if (flux_y) MOM_HIP(ctx, hipMemsetAsync(flux_y, 0, b3, ctx->stream));
+  const size_t n3 = b3 / sizeof(double);
+  int rc0 = MOM_GPU_OK;
+  if (wrk1 && (rc0 = launch_fill(ctx, wrk1, 0.0, n3))) return rc0;
+  if (flux_x && (rc0 = launch_fill(ctx, flux_x, 0.0, n3))) return rc0;
+  if (flux_y && (rc0 = launch_fill(ctx, flux_y, 0.0, n3))) return rc0;
   switch (scheme) {
     case MOM_ADVECT_MDFL_SWEBY:
     case MOM_ADVECT_DST_LINEAR:
@@ -240,7 +251,10 @@ int mom_gpu_vert_advect_tracer(mom_gpu_ctx_t *ctx, int scheme, const double *T_t
       return MOM_GPU_OK;   // three-dimensional schemes: done in horz_advect_tracer
     case MOM_ADVECT_QUICKER:
       REQUIRE(ctx, T_taum1 && T_tau && wrho_bt, "vert_advect_tracer(QUICKER): NULL required array");
-      if (wrk1) MOM_HIP(ctx, hipMemsetAsync(wrk1, 0, p3bytes(ctx), ctx->stream));
+      if (wrk1) {
+        int rc = launch_fill(ctx, wrk1, 0.0, p3bytes(ctx) / sizeof(double));
+        if (rc) return rc;
+      }
       return launch_quicker_vert(ctx, T_taum1, T_tau, wrho_bt, tmask_limit, th_tendency, wrk1,
                                  flux_z);
     default:
@@ -261,7 +275,10 @@ int mom_gpu_advect_sweby_all(mom_gpu_ctx_t *ctx, int ntr, const double *const *T
   for (int n = 0; n < ntr; ++n) {
     REQUIRE(ctx, T_taum1[n] && th_tendency[n], "mom_gpu_advect_sweby_all: tracer %d NULL", n);
     double *w1 = wrk1 ? wrk1[n] : nullptr;
-    if (w1) MOM_HIP(ctx, hipMemsetAsync(w1, 0, p3bytes(ctx), ctx->stream));
+    if (w1) {
+      int rc0 = launch_fill(ctx, w1, 0.0, p3bytes(ctx) / sizeof(double));
+      if (rc0) return rc0;
+    }
     int rc = launch_mdfl(ctx, 1, T_taum1[n], uhrho_et, vhrho_nt, wrho_bt, rho_dzt_tau, dtime, 1.0,
                          th_tendency[n], w1, nullptr, nullptr, nullptr);
     if (rc) return rc;

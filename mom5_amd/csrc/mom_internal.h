@@ -125,5 +125,6 @@ int launch_barotropic(mom_gpu_ctx *ctx, const mom_gpu_bt_params_t *p,
                       double *conv);
 int launch_content(mom_gpu_ctx *ctx, const double *T, const double *rho,
                    double *out);
+int launch_fill(mom_gpu_ctx *ctx, double *p, double v, size_t n);
 int launch_check_masks(mom_gpu_ctx *ctx, const double *tmask,
                        const double *umask, long long *bad);

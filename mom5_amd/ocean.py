@@ -154,6 +154,11 @@ class OceanGPU:
         self._chk(self.L.mom_gpu_elapsed(self.ctx, s0, s1, C.byref(ms)), "elapsed")
         return float(ms.value)
 
+    def fill(self, arr, value=0.0):
+        """Fill a float64 device array on the context's stream."""
+        n = arr.nbytes // 8
+        self._chk(self.L.mom_gpu_fill(self.ctx, _p(arr), float(value), n), "fill")
+
     def field3(self, value=None, dtype=np.float64):
         a = DeviceArray(self.shape3, dtype)
         if value is None:
