@@ -102,6 +102,15 @@ int mom_gpu_memset(void *ptr, int value, size_t bytes);
  * ordered with the other calls on ctx and may be captured in a graph). */
 int mom_gpu_fill(mom_gpu_ctx_t *ctx, double *ptr, double value, size_t n);
 
+/* Launch-shape knobs, set per context (the autotuner drives these; results
+ * do not depend on them).  Knobs:
+ *   "barotropic_subcycles_per_launch"  0: three launches per subcycle;
+ *                                      n>0: n subcycles per launch (1..9)
+ *   "mdfl_rows_per_wave"               2 or 4: tile height of the MDFL kernel
+ * Unknown knobs and out-of-range values return MOM_GPU_ERR_ARG. */
+int mom_gpu_set_tuning(mom_gpu_ctx_t *ctx, const char *knob, int value);
+int mom_gpu_get_tuning(const mom_gpu_ctx_t *ctx, const char *knob, int *value);
+
 /* Elapsed time in ms between two marks recorded on the context's stream.
  * mom_gpu_mark(ctx, slot) records slot 0..7; mom_gpu_elapsed waits. */
 int mom_gpu_mark(mom_gpu_ctx_t *ctx, int slot);

@@ -48,6 +48,8 @@ _PROTOS = {
     "mom_gpu_memcpy_dtoh": (C.c_int, [_vp, _vp, C.c_size_t]),
     "mom_gpu_memset": (C.c_int, [_vp, C.c_int, C.c_size_t]),
     "mom_gpu_fill": (C.c_int, [_vp, _vp, C.c_double, C.c_size_t]),
+    "mom_gpu_set_tuning": (C.c_int, [_vp, C.c_char_p, C.c_int]),
+    "mom_gpu_get_tuning": (C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_int)]),
     "mom_gpu_mark": (C.c_int, [_vp, C.c_int]),
     "mom_gpu_elapsed": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_float)]),
     "mom_gpu_horz_advect_tracer": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp,

@@ -68,6 +68,44 @@ def build(verbose=False, jobs=None):
     return LIB
 
 
+FORTRAN_DIR = os.path.join(HERE, "fortran")
+DRIVER = os.path.join(HERE, "ocean_gpu_driver")
+
+
+def flang():
+    for cand in (os.environ.get("FLANG"), "/opt/rocm/bin/amdflang", shutil.which("flang")):
+        if cand and os.path.exists(cand):
+            return cand
+    return None
+
+
+def build_fortran(verbose=False):
+    """Compile the ISO_C_BINDING module and the Fortran host driver with
+    flang (-fdefault-real-8 -fdefault-double-8 as MOM is built) and link them
+    against libmom5_gpu.so (found next to the executable at run time)."""
+    fc = flang()
+    if fc is None:
+        raise RuntimeError("flang not found (set FLANG)")
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    flags = ["-O2", "-fdefault-real-8", "-fdefault-double-8", "-module-dir", objdir, "-I", objdir]
+    mod_o = os.path.join(objdir, "mom_gpu_mod.o")
+    drv_o = os.path.join(objdir, "ocean_gpu_driver.o")
+    for src, obj in (("mom_gpu_mod.F90", mod_o), ("ocean_gpu_driver.F90", drv_o)):
+        cmd = [fc, *flags, "-c", os.path.join(FORTRAN_DIR, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+    tmp = DRIVER + ".tmp"
+    cmd = [fc, drv_o, mod_o, "-o", tmp, "-L", HERE, "-lmom5_gpu", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, DRIVER)
+    return DRIVER
+
+
 if __name__ == "__main__":
     build(verbose="-v" in sys.argv)
     print(LIB)
+    print(build_fortran(verbose="-v" in sys.argv))

@@ -9,6 +9,8 @@
 // compute rows take the value computed for the interior column they mirror.
 #include "mom_internal.h"
 
+#include <algorithm>
+
 namespace {
 
 struct BtBuf {
@@ -174,13 +176,252 @@ __global__ void k_bt_conv(KDom d, const double *__restrict__ ud, const double *_
   conv[ii + d.NI * jj] = -div_ud(d, ud, ud + d.P2, dxu, dyu, datr, ii, jj);
 }
 
+// ---- temporally blocked subcycles -----------------------------------------
+// One work-group owns a BRY x BRX region of the data domain and advances it
+// nsub subcycles with the state (eta, U) and the accumulators in registers
+// and only the exchanged quantities (U*dyu, U*dxu, surface pressure) in
+// LDS.  Each subcycle invalidates two rows/columns on the low side and one on
+// the high side of the region, so after nsub subcycles the interior
+// (BRY-3*nsub) x (BRX-3*nsub) is exact; the rest is recomputed by the
+// neighbouring groups.  On a cyclic-x domain the region addresses the
+// periodic images of the compute columns: with the inputs' halo columns
+// holding the wrap (what mpp_update_domains leaves), this reproduces the
+// reference's evolution of every point, the halo columns of the compute
+// rows excepted -- those are rewritten by the tail's halo update anyway.
+constexpr int BRX = 64;          // region columns (one wavefront per row)
+constexpr int BRY = 32;          // region rows
+constexpr int BPT = 2;           // region rows per wavefront
+constexpr int BNT = BRX * BRY / BPT;
+
+struct BtIn {
+  const double *rho_g, *patm, *thicku, *forcing, *f, *pme, *river, *source, *eta_smooth;
+  const double *dxu, *dyu, *datr, *dxur, *dyur;
+  const int *kmt, *kmu;
+};
+
+__global__ __launch_bounds__(BNT) void k_bt_block(
+    KDom d, BtIn in, const double *__restrict__ eta_src, const double *__restrict__ ud_src,
+    double *__restrict__ eta_dst, double *__restrict__ ud_dst, double *__restrict__ eb,
+    double *__restrict__ ub, int s0, int nsub, int nts, int step_a, double dtbt, double cp,
+    double cc, int xbase, int xcount, int ntx, int ntiles) {
+  __shared__ double s_ps[BRY][BRX];
+  __shared__ double s_uh[BRY][BRX];
+  __shared__ double s_vh[BRY][BRX];
+  const int tile = blockIdx.x;
+  if (tile >= ntiles) return;
+  const int TX = BRX - 3 * nsub, TY = BRY - 3 * nsub;
+  const int tix = tile % ntx, tiy = tile / ntx;
+  const int x_int0 = xbase + tix * TX, y_int0 = tiy * TY;
+  const int lane = threadIdx.x % BRX, wave = threadIdx.x / BRX;
+  const int P2 = d.P2;
+  const bool cyc = d.cyclic_x != 0;
+
+  int rr[BPT], off[BPT], vj[BPT];
+  bool present[BPT], interior[BPT], div0[BPT], grad0[BPT];
+  double eta[BPT], u1[BPT], u2[BPT], ebv[BPT], ub1[BPT], ub2[BPT];
+  double tm[BPT], sf[BPT], datr[BPT], dyu[BPT], dxu[BPT], rho_g[BPT], patm[BPT], thk[BPT];
+  double F1[BPT], F2[BPT], fc[BPT], cori1[BPT], cori2[BPT], dxur[BPT], dyur[BPT], um[BPT];
+  const int vi = x_int0 - 2 * nsub + lane;
+  int col = vi;
+  bool colok = true;
+  if (cyc) col = 1 + wrapi(vi - 1, d.ni);
+  else colok = vi >= 0 && vi < d.NI;
+#pragma unroll
+  for (int q = 0; q < BPT; ++q) {
+    const int r = wave + (BNT / BRX) * q;
+    rr[q] = r;
+    const int j = y_int0 - 2 * nsub + r;
+    vj[q] = j;
+    present[q] = colok && j >= 0 && j < d.NJ;
+    interior[q] = present[q] && lane >= 2 * nsub && lane < 2 * nsub + TX &&
+                  vi < xbase + xcount && r >= 2 * nsub && r < 2 * nsub + TY;
+    div0[q] = (j == 0) || (!cyc && vi == 0);
+    grad0[q] = (j == d.NJ - 1) || (!cyc && vi == d.NI - 1);
+    const int o = present[q] ? col + d.NI * j : 0;
+    off[q] = o;
+    if (present[q]) {
+      tm[q] = (1 <= in.kmt[o]) ? 1.0 : 0.0;
+      sf[q] = tm[q] * (in.pme[o] + in.river[o] + in.source[o] - in.eta_smooth[o]);
+      datr[q] = in.datr[o]; dyu[q] = in.dyu[o]; dxu[q] = in.dxu[o];
+      rho_g[q] = in.rho_g[o]; patm[q] = in.patm[o]; thk[q] = in.thicku[o];
+      F1[q] = in.forcing[o]; F2[q] = in.forcing[o + P2]; fc[q] = in.f[o];
+      cori1[q] = 0.5 * dtbt * fc[q];
+      cori2[q] = 1.0 / (1.0 + cori1[q] * cori1[q]);
+      dxur[q] = in.dxur[o]; dyur[q] = in.dyur[o];
+      um[q] = (1 <= in.kmu[o]) ? 1.0 : 0.0;
+      eta[q] = eta_src[o]; u1[q] = ud_src[o]; u2[q] = ud_src[o + P2];
+      if (interior[q]) { ebv[q] = eb[o]; ub1[q] = ub[o]; ub2[q] = ub[o + P2]; }
+      else { ebv[q] = ub1[q] = ub2[q] = 0.0; }
+    } else {
+      tm[q] = sf[q] = datr[q] = dyu[q] = dxu[q] = rho_g[q] = patm[q] = thk[q] = 0.0;
+      F1[q] = F2[q] = fc[q] = cori1[q] = cori2[q] = dxur[q] = dyur[q] = um[q] = 0.0;
+      eta[q] = u1[q] = u2[q] = ebv[q] = ub1[q] = ub2[q] = 0.0;
+    }
+    s_uh[r][lane] = u1[q] * dyu[q];
+    s_vh[r][lane] = u2[q] * dxu[q];
+  }
+  __syncthreads();
+
+  const int lm = max(lane - 1, 0), lp = min(lane + 1, BRX - 1);
+  for (int t = 0; t < nsub; ++t) {
+    const int itime = s0 + t;
+    // A: predictor and surface pressure
+#pragma unroll
+    for (int q = 0; q < BPT; ++q) {
+      const int r = rr[q], rm = max(r - 1, 0);
+      double div = 0.0;
+      if (!div0[q]) {
+        const double uh_bay = 0.5 * (s_uh[r][lane] + s_uh[rm][lane]);
+        const double vh_bax = 0.5 * (s_vh[r][lane] + s_vh[r][lm]);
+        const double uhim_bay = 0.5 * (s_uh[r][lm] + s_uh[rm][lm]);
+        const double vhjm_bax = 0.5 * (s_vh[rm][lane] + s_vh[rm][lm]);
+        div = ((uh_bay - uhim_bay) + (vh_bax - vhjm_bax)) * datr[q];
+      }
+      const double tmp = -div * tm[q];
+      const double es = eta[q] + cp * (tmp + sf[q]);
+      s_ps[r][lane] = rho_g[q] * es + patm[q];
+    }
+    __syncthreads();
+    // B: pressure gradient and momentum with implicit Coriolis
+#pragma unroll
+    for (int q = 0; q < BPT; ++q) {
+      const int r = rr[q], rp = min(r + 1, BRY - 1);
+      double g1 = 0.0, g2 = 0.0;
+      if (!grad0[q]) {
+        const double p = s_ps[r][lane], pe = s_ps[r][lp];
+        const double pn = s_ps[rp][lane], pne = s_ps[rp][lp];
+        const double p_fay = 0.5 * (p + pn);
+        const double p_fax = 0.5 * (p + pe);
+        const double pip_fay = 0.5 * (pe + pne);
+        const double pjp_fax = 0.5 * (pn + pne);
+        g1 = (pip_fay - p_fay) * dxur[q] * um[q];
+        g2 = (pjp_fax - p_fax) * dyur[q] * um[q];
+      }
+      const double pf1 = -(thk[q] * g1);
+      const double pf2 = -(thk[q] * g2);
+      const double t1 = u1[q] + dtbt * (0.5 * fc[q] * u2[q] + pf1 + F1[q] + 0.0);
+      const double t2 = u2[q] + dtbt * (-(0.5 * fc[q] * u1[q]) + pf2 + F2[q] + 0.0);
+      u1[q] = cori2[q] * (t1 + cori1[q] * t2);
+      u2[q] = cori2[q] * (t2 - cori1[q] * t1);
+      // phase A's reads of s_uh/s_vh finished before the barrier above
+      s_uh[r][lane] = u1[q] * dyu[q];
+      s_vh[r][lane] = u2[q] * dxu[q];
+    }
+    __syncthreads();
+    // C: corrector and time-average accumulation
+    const double wgt = (double)(nts - itime + 1);
+#pragma unroll
+    for (int q = 0; q < BPT; ++q) {
+      const int r = rr[q], rm = max(r - 1, 0);
+      double div = 0.0;
+      if (!div0[q]) {
+        const double uh_bay = 0.5 * (s_uh[r][lane] + s_uh[rm][lane]);
+        const double vh_bax = 0.5 * (s_vh[r][lane] + s_vh[r][lm]);
+        const double uhim_bay = 0.5 * (s_uh[r][lm] + s_uh[rm][lm]);
+        const double vhjm_bax = 0.5 * (s_vh[rm][lane] + s_vh[rm][lm]);
+        div = ((uh_bay - uhim_bay) + (vh_bax - vhjm_bax)) * datr[q];
+      }
+      const double tmp = -div * tm[q];
+      eta[q] = eta[q] + cc * (tmp + sf[q]);
+      ebv[q] = ebv[q] + eta[q];
+      if (step_a) {
+        ub1[q] = ub1[q] + u1[q];
+        ub2[q] = ub2[q] + u2[q];
+      } else {
+        ub1[q] = ub1[q] + wgt * u1[q];
+        ub2[q] = ub2[q] + wgt * u2[q];
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < BPT; ++q) {
+    if (interior[q]) {
+      const int o = off[q];
+      eta_dst[o] = eta[q];
+      ud_dst[o] = u1[q];
+      ud_dst[o + P2] = u2[q];
+      eb[o] = ebv[q];
+      ub[o] = ub1[q];
+      ub[o + P2] = ub2[q];
+    }
+  }
+}
+
+__global__ void k_bt_acc_init(KDom d, int initsum, const double *__restrict__ eta_t,
+                              const double *__restrict__ eta_bar_tau, double *__restrict__ eta_bar,
+                              double *__restrict__ ud) {
+  const int ii = blockIdx.x * blockDim.x + threadIdx.x, jj = blockIdx.y;
+  if (ii >= d.NI) return;
+  const int o = ii + d.NI * jj;
+  eta_bar[o] = initsum ? eta_bar_tau[o] : eta_t[o];
+  ud[o] = 0.0;
+  ud[o + d.P2] = 0.0;
+}
+
 }  // namespace
+
+int launch_barotropic_blocked(mom_gpu_ctx *ctx, const mom_gpu_bt_params_t *p,
+                              const double *eta_t, const double *eta_bar_tau,
+                              const double *ud_tau, const double *forcing, const double *thicku,
+                              const double *patm, const double *pme, const double *river,
+                              const double *source, const double *eta_smooth,
+                              const double *rho_g, double *eta_bar, double *ud, double *conv,
+                              int hsub) {
+  const KDom &d = ctx->kd;
+  const size_t P2 = (size_t)d.P2;
+  hipStream_t s = ctx->stream;
+  const double rho0r = 1.0 / p->rho0;
+  const double c_pred = p->dtbt * p->pred_corr_gamma * rho0r;   // dtbt_gamma_rho0r, :1196
+  const double c_corr = p->dtbt * rho0r;                        // dtbt_rho0r, :1197
+  const dim3 grid((d.NI + 255) / 256, d.NJ), block(256);
+  hipLaunchKernelGGL(k_bt_acc_init, grid, block, 0, s, d, p->initsum_with_bar, eta_t,
+                     eta_bar_tau, eta_bar, ud);
+  MOM_LAUNCHED(ctx, "k_bt_acc_init");
+  BtIn in{rho_g, patm, thicku, forcing, ctx->f, pme, river, source, eta_smooth,
+          ctx->dxu, ctx->dyu, ctx->datr, ctx->dxur, ctx->dyur, ctx->kmt, ctx->kmu};
+  const int xbase = d.cyclic_x ? 1 : 0, xcount = d.cyclic_x ? d.ni : d.NI;
+  double *eta_buf[2] = {ctx->bt, ctx->bt + P2};
+  double *ud_buf[2] = {ctx->bt + 2 * P2, ctx->bt + 4 * P2};
+  const double *eta_src = eta_bar_tau, *ud_src = ud_tau;
+  int which = 0;
+  for (int s0 = 1; s0 <= p->nts; s0 += hsub) {
+    const int nsub = std::min(hsub, p->nts - s0 + 1);
+    const int TX = BRX - 3 * nsub, TY = BRY - 3 * nsub;
+    const int ntx = (xcount + TX - 1) / TX, nty = (d.NJ + TY - 1) / TY;
+    hipLaunchKernelGGL(k_bt_block, dim3(ntx * nty), dim3(BNT), 0, s, d, in, eta_src, ud_src,
+                       eta_buf[which], ud_buf[which], eta_bar, ud, s0, nsub, p->nts,
+                       p->stepping_a, p->dtbt, c_pred, c_corr, xbase, xcount, ntx, ntx * nty);
+    MOM_LAUNCHED(ctx, "k_bt_block");
+    eta_src = eta_buf[which];
+    ud_src = ud_buf[which];
+    which ^= 1;
+  }
+  const double nts = (double)p->nts;
+  const double rntsp1 = 1.0 / (nts + 1.0);
+  const double rnts = p->stepping_a ? 1.0 / nts : 2.0 / (nts * (double)(p->nts + 1));
+  hipLaunchKernelGGL(k_bt_scale, grid, block, 0, s, d, rntsp1, rnts, eta_bar, ud);
+  const dim3 wg((d.NJ + 255) / 256), wb(256);
+  if (d.cyclic_x) {
+    hipLaunchKernelGGL(k_bt_wrap, wg, wb, 0, s, d, eta_bar, 1);
+    hipLaunchKernelGGL(k_bt_wrap, wg, wb, 0, s, d, ud, 2);
+  }
+  if (conv) {
+    hipLaunchKernelGGL(k_bt_conv, grid, block, 0, s, d, ud, ctx->dxu, ctx->dyu, ctx->datr, conv);
+    if (d.cyclic_x) hipLaunchKernelGGL(k_bt_wrap, wg, wb, 0, s, d, conv, 1);
+  }
+  MOM_LAUNCHED(ctx, "k_bt_tail");
+  return MOM_GPU_OK;
+}
 
 int launch_barotropic(mom_gpu_ctx *ctx, const mom_gpu_bt_params_t *p, const double *eta_t,
                       const double *eta_bar_tau, const double *ud_tau, const double *forcing,
                       const double *thicku, const double *patm, const double *pme,
                       const double *river, const double *source, const double *eta_smooth,
                       const double *rho_g, double *eta_bar, double *ud, double *conv) {
+  if (ctx->tune.bt_subcycles_per_launch > 0)
+    return launch_barotropic_blocked(ctx, p, eta_t, eta_bar_tau, ud_tau, forcing, thicku, patm,
+                                     pme, river, source, eta_smooth, rho_g, eta_bar, ud, conv,
+                                     ctx->tune.bt_subcycles_per_launch);
   const KDom &d = ctx->kd;
   const size_t P2 = (size_t)d.P2;
   BtBuf b;

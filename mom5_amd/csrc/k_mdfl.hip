@@ -283,26 +283,34 @@ __global__ __launch_bounds__(RX * NW) void k_mdfl(
 
 }  // namespace
 
-int launch_mdfl(mom_gpu_ctx *ctx, int flavor, const double *T,
-                const double *u, const double *v, const double *w,
-                const double *rho, double dtime, double lim, double *th,
-                double *wrk1, double *fx, double *fy, double *fz) {
-  constexpr int RPW = 4;
+template <int FLAVOR, int RPW>
+static void launch_mdfl_t(mom_gpu_ctx *ctx, const double *T, const double *u, const double *v,
+                          const double *w, const double *rho, double dtime, double lim,
+                          double *th, double *wrk1, double *fx, double *fy, double *fz) {
   constexpr int TY = NW * RPW - 4;
   const KDom &d = ctx->kd;
   const int ntx = (d.ni + TX - 1) / TX;
   const int nty = (d.nj + TY - 1) / TY;
   const int ntiles = ntx * nty;
   const int per = (ntiles + 7) / 8;
-  const dim3 grid(8 * per), block(RX * NW);
-  if (flavor == 0)
-    hipLaunchKernelGGL((k_mdfl<0, RPW>), grid, block, 0, ctx->stream, d, T, u, v, w, rho,
-                       ctx->kmt, ctx->dat, ctx->datr, ctx->dxte, ctx->dyte, ctx->dxtn,
-                       ctx->dytn, dtime, lim, th, wrk1, fx, fy, fz, ntx, ntiles, per);
+  hipLaunchKernelGGL((k_mdfl<FLAVOR, RPW>), dim3(8 * per), dim3(RX * NW), 0, ctx->stream, d, T,
+                     u, v, w, rho, ctx->kmt, ctx->dat, ctx->datr, ctx->dxte, ctx->dyte,
+                     ctx->dxtn, ctx->dytn, dtime, lim, th, wrk1, fx, fy, fz, ntx, ntiles, per);
+}
+
+int launch_mdfl(mom_gpu_ctx *ctx, int flavor, const double *T,
+                const double *u, const double *v, const double *w,
+                const double *rho, double dtime, double lim, double *th,
+                double *wrk1, double *fx, double *fy, double *fz) {
+  const int rpw = ctx->tune.mdfl_rows_per_wave;
+  if (flavor == 0 && rpw == 2)
+    launch_mdfl_t<0, 2>(ctx, T, u, v, w, rho, dtime, lim, th, wrk1, fx, fy, fz);
+  else if (flavor == 0)
+    launch_mdfl_t<0, 4>(ctx, T, u, v, w, rho, dtime, lim, th, wrk1, fx, fy, fz);
+  else if (rpw == 2)
+    launch_mdfl_t<1, 2>(ctx, T, u, v, w, rho, dtime, lim, th, wrk1, fx, fy, fz);
   else
-    hipLaunchKernelGGL((k_mdfl<1, RPW>), grid, block, 0, ctx->stream, d, T, u, v, w, rho,
-                       ctx->kmt, ctx->dat, ctx->datr, ctx->dxte, ctx->dyte, ctx->dxtn,
-                       ctx->dytn, dtime, lim, th, wrk1, fx, fy, fz, ntx, ntiles, per);
+    launch_mdfl_t<1, 4>(ctx, T, u, v, w, rho, dtime, lim, th, wrk1, fx, fy, fz);
   MOM_LAUNCHED(ctx, "k_mdfl");
   return MOM_GPU_OK;
 }

@@ -62,6 +62,42 @@ int mom_gpu_memset(void *ptr, int value, size_t bytes) {
   return MOM_GPU_OK;
 }
 
+static int *tuning_slot(mom_gpu_ctx *ctx, const char *knob, int *lo, int *hi) {
+  if (!knob) return nullptr;
+  if (!std::strcmp(knob, "barotropic_subcycles_per_launch")) {
+    *lo = 0; *hi = 9;
+    return &ctx->tune.bt_subcycles_per_launch;
+  }
+  if (!std::strcmp(knob, "mdfl_rows_per_wave")) {
+    *lo = 2; *hi = 4;
+    return &ctx->tune.mdfl_rows_per_wave;
+  }
+  return nullptr;
+}
+
+int mom_gpu_set_tuning(mom_gpu_ctx_t *ctx, const char *knob, int value) {
+  REQUIRE(nullptr, ctx, "mom_gpu_set_tuning: NULL context");
+  int lo = 0, hi = 0;
+  int *slot = tuning_slot(ctx, knob, &lo, &hi);
+  REQUIRE(ctx, slot, "mom_gpu_set_tuning: unknown knob '%s'", knob ? knob : "(null)");
+  REQUIRE(ctx, value >= lo && value <= hi, "mom_gpu_set_tuning: %s=%d outside [%d,%d]", knob,
+          value, lo, hi);
+  REQUIRE(ctx, slot != &ctx->tune.mdfl_rows_per_wave || value == 2 || value == 4,
+          "mom_gpu_set_tuning: mdfl_rows_per_wave must be 2 or 4");
+  *slot = value;
+  return MOM_GPU_OK;
+}
+
+int mom_gpu_get_tuning(const mom_gpu_ctx_t *ctx, const char *knob, int *value) {
+  REQUIRE(nullptr, ctx && value, "mom_gpu_get_tuning: NULL argument");
+  int lo = 0, hi = 0;
+  int *slot = tuning_slot(const_cast<mom_gpu_ctx *>(ctx), knob, &lo, &hi);
+  if (!slot) return mom_fail(const_cast<mom_gpu_ctx *>(ctx), MOM_GPU_ERR_ARG,
+                             "mom_gpu_get_tuning: unknown knob '%s'", knob ? knob : "(null)");
+  *value = *slot;
+  return MOM_GPU_OK;
+}
+
 int mom_gpu_fill(mom_gpu_ctx_t *ctx, double *ptr, double value, size_t n) {
   REQUIRE(nullptr, ctx, "mom_gpu_fill: NULL context");
   REQUIRE(ctx, ptr || n == 0, "mom_gpu_fill: NULL pointer");
@@ -113,6 +149,8 @@ int mom_gpu_create(const mom_gpu_domain_t *dom, const mom_gpu_grid_t *grid, mom_
   k.P2 = k.NI * k.NJ;
   k.cyclic_x = dom->cyclic_x ? 1 : 0;
   k.cyclic_y = dom->cyclic_y ? 1 : 0;
+  c->tune.bt_subcycles_per_launch = 4;
+  c->tune.mdfl_rows_per_wave = 4;
   const size_t P2 = (size_t)k.P2, nk = (size_t)k.nk;
 
 #define CK(call)                                                                      \

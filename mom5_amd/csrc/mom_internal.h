@@ -63,7 +63,11 @@ struct mom_gpu_ctx {
   // reduction partials
   double *red;
   int red_n;
-  // staging for small per-call tables
+  // launch-shape knobs (mom_gpu_set_tuning)
+  struct {
+    int bt_subcycles_per_launch;   // 0: one launch per phase per subcycle
+    int mdfl_rows_per_wave;        // 2 or 4 region rows per wavefront
+  } tune;
   char err[512];
 };
 
@@ -123,6 +127,13 @@ int launch_barotropic(mom_gpu_ctx *ctx, const mom_gpu_bt_params_t *p,
                       const double *source, const double *eta_smooth,
                       const double *rho_g, double *eta_bar, double *ud,
                       double *conv);
+int launch_barotropic_blocked(mom_gpu_ctx *ctx, const mom_gpu_bt_params_t *p,
+                              const double *eta_t, const double *eta_bar_tau,
+                              const double *ud_tau, const double *forcing, const double *thicku,
+                              const double *patm, const double *pme, const double *river,
+                              const double *source, const double *eta_smooth,
+                              const double *rho_g, double *eta_bar, double *ud, double *conv,
+                              int hsub);
 int launch_content(mom_gpu_ctx *ctx, const double *T, const double *rho,
                    double *out);
 int launch_fill(mom_gpu_ctx *ctx, double *p, double v, size_t n);

@@ -154,6 +154,14 @@ class OceanGPU:
         self._chk(self.L.mom_gpu_elapsed(self.ctx, s0, s1, C.byref(ms)), "elapsed")
         return float(ms.value)
 
+    def set_tuning(self, knob, value):
+        self._chk(self.L.mom_gpu_set_tuning(self.ctx, knob.encode(), int(value)), "set_tuning")
+
+    def get_tuning(self, knob):
+        v = C.c_int()
+        self._chk(self.L.mom_gpu_get_tuning(self.ctx, knob.encode(), C.byref(v)), "get_tuning")
+        return v.value
+
     def fill(self, arr, value=0.0):
         """Fill a float64 device array on the context's stream."""
         n = arr.nbytes // 8

@@ -46,10 +46,12 @@ def same(a, b):
                              f"(max abs diff {np.nanmax(np.abs(a - b)):.3e})")
 
 
+@pytest.mark.parametrize("rows_per_wave", [4, 2])
 @pytest.mark.parametrize("scheme,lim", [("mdfl_sweby", 1.0), ("dst_linear", 0.0)])
-def test_horz_advect_mdfl(setup, scheme, lim):
+def test_horz_advect_mdfl(setup, scheme, lim, rows_per_wave):
     from mom5_amd import DeviceArray
     case, orc, gpu, dev = setup
+    gpu.set_tuning("mdfl_rows_per_wave", rows_per_wave)
     rng = np.random.default_rng(5)
     for name, T in case.tracers[:2]:
         th0 = rng.standard_normal(orc.shape3)
@@ -206,12 +208,14 @@ def test_vert_diffuse_implicit(setup):
         same(dTs[n].to_host()[:, 1:-1, 1:-1], ref[n][:, 1:-1, 1:-1])
 
 
+@pytest.mark.parametrize("per_launch", [0, 1, 4, 7])
 @pytest.mark.parametrize("stepping_a,initsum", [(False, True), (True, False)])
-def test_update_ocean_barotropic(setup, stepping_a, initsum):
+def test_update_ocean_barotropic(setup, stepping_a, initsum, per_launch):
     from mom5_amd import DeviceArray
     case, orc, gpu, dev = setup
     if case.dtbt == 0.0:
         pytest.skip("advection-only case")
+    gpu.set_tuning("barotropic_subcycles_per_launch", per_launch)
     nts = case.nts
     eta_ref, ud_ref, conv_ref = orc.update_ocean_barotropic(nts, case.dtbt, stepping_a=stepping_a,
                                                             initsum_with_bar=initsum)
