@@ -107,6 +107,9 @@ int mom_gpu_fill(mom_gpu_ctx_t *ctx, double *ptr, double value, size_t n);
  *   "barotropic_subcycles_per_launch"  0: three launches per subcycle;
  *                                      n>0: n subcycles per launch (1..9)
  *   "mdfl_rows_per_wave"               2 or 4: tile height of the MDFL kernel
+ *   "step_group_f64"                   1 or 2 FP64 tracers per batched pass
+ *   "step_group_f32"                   1, 2 or 4 FP32 tracers per batched pass
+ *   "step_rows_per_wave"               2 or 4: tile height of the batched pass
  * Unknown knobs and out-of-range values return MOM_GPU_ERR_ARG. */
 int mom_gpu_set_tuning(mom_gpu_ctx_t *ctx, const char *knob, int value);
 int mom_gpu_get_tuning(const mom_gpu_ctx_t *ctx, const char *knob, int *value);
@@ -213,6 +216,44 @@ int mom_gpu_update_ocean_barotropic(mom_gpu_ctx_t *ctx,
     const double *pme, const double *river, const double *source,
     const double *eta_smooth, const double *rho_g,
     double *eta_t_bar_taup1, double *udrho_taup1, double *conv_rho_ud_t);
+
+/* ---- batched tracer step ------------------------------------------------ */
+
+/* All MDFL-Sweby tracers of one baroclinic step: replaces the tracer loop of
+ * update_ocean_tracer (ocean_tracers/ocean_tracer.F90:2295-2350: advection
+ * into th_tendency and the explicit update to taup1) followed by
+ * vert_diffuse_implicit (:2432, ocean_param/vertical/ocean_vert_mix.F90:
+ * 3426-3513), for tracers whose horizontal scheme is MDFL_SWEBY.
+ *
+ * Tracers are advanced in groups that share one pass over the shared
+ * fields.  dtype[n] = 8: FP64 storage, reference operation order --
+ * T_taup1 is bit-for-bit what mom_gpu_horz_advect_tracer (into a zeroed
+ * th_tendency plus th_tendency[n]), mom_gpu_update_tracer_explicit and
+ * mom_gpu_vert_diffuse_implicit produce.  dtype[n] = 4: FP32 storage and
+ * FP32 advection arithmetic with the division-free limiter; the implicit
+ * vertical solve stays FP64 (the forward-sweep values are stored in FP32).
+ *
+ * Optional pointers: th_tendency (table or entries; other tendency terms),
+ * rho_dzt_taum1 (default rho_dzt_tau, two-level stepping), rho_dztr
+ * (default 1/(rho_dzt_tau+epsln), ocean_thickness.F90:1567),
+ * rho_dzt_taup1 (default rho_dzt_tau), stf/btf (NULL: no implicit vertical
+ * diffusion). */
+typedef struct {
+  int ntr;
+  const void *const *T_taum1;       /* host table of device pointers */
+  void *const *T_taup1;
+  const int *dtype;                 /* host array: 8 (FP64) or 4 (FP32) */
+  const double *const *th_tendency;
+  const double *const *stf;
+  const double *const *btf;
+  int index_salt;                   /* 0-based, -1: none */
+  const double *uhrho_et, *vhrho_nt, *wrho_bt, *rho_dzt_tau;
+  const double *rho_dzt_taum1, *rho_dztr, *rho_dzt_taup1;
+  const double *dzwt, *diff_cbt;
+  double rho0, dtime, aidif;
+} mom_gpu_tracer_step_t;
+
+int mom_gpu_step_tracers(mom_gpu_ctx_t *ctx, const mom_gpu_tracer_step_t *s);
 
 /* ---- diagnostics -------------------------------------------------------- */
 

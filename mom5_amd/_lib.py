@@ -36,6 +36,16 @@ class BtParams(C.Structure):
                 ("rho0", C.c_double), ("stepping_a", C.c_int), ("initsum_with_bar", C.c_int)]
 
 
+class TracerStep(C.Structure):
+    _fields_ = [("ntr", C.c_int), ("T_taum1", C.POINTER(_vp)), ("T_taup1", C.POINTER(_vp)),
+                ("dtype", C.POINTER(C.c_int)), ("th_tendency", C.POINTER(_vp)),
+                ("stf", C.POINTER(_vp)), ("btf", C.POINTER(_vp)), ("index_salt", C.c_int),
+                ("uhrho_et", _vp), ("vhrho_nt", _vp), ("wrho_bt", _vp), ("rho_dzt_tau", _vp),
+                ("rho_dzt_taum1", _vp), ("rho_dztr", _vp), ("rho_dzt_taup1", _vp),
+                ("dzwt", _vp), ("diff_cbt", _vp), ("rho0", C.c_double), ("dtime", C.c_double),
+                ("aidif", C.c_double)]
+
+
 # name -> (restype, argtypes)
 _PROTOS = {
     "mom_gpu_create": (C.c_int, [C.POINTER(Domain), C.POINTER(Grid), C.POINTER(_vp)]),
@@ -69,6 +79,7 @@ _PROTOS = {
     "mom_gpu_update_ocean_barotropic": (C.c_int, [_vp, C.POINTER(BtParams), _dp, _dp, _dp, _dp,
                                                   _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                                   _dp]),
+    "mom_gpu_step_tracers": (C.c_int, [_vp, C.POINTER(TracerStep)]),
     "mom_gpu_tracer_content": (C.c_int, [_vp, _dp, _dp, C.POINTER(C.c_double)]),
 }
 

@@ -17,7 +17,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmom5_gpu.so")
-SOURCES = ["mom_gpu_api.hip", "k_mdfl.hip", "k_quicker.hip", "k_column.hip", "k_barotropic.hip"]
+SOURCES = ["mom_gpu_api.hip", "k_mdfl.hip", "k_quicker.hip", "k_column.hip", "k_barotropic.hip",
+           "k_step.hip"]
 ARCH = os.environ.get("MOM5_OFFLOAD_ARCH", "gfx950")
 
 

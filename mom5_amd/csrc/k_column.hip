@@ -70,17 +70,21 @@ __global__ void k_invtri(KDom d, double *__restrict__ z, const double *__restric
 // ---- vert_diffuse_implicit, ocean_vert_mix.F90:3462-3491 -----------------
 // dcb = rho0*diff_cbt(nmix) + K33 formed on the fly; up to NT tracers that
 // share the same dcb share one factorisation (e, bet), identical arithmetic
-// to a per-tracer invtri.
+// to a per-tracer invtri.  The arithmetic is FP64 for every tracer; an FP32
+// tracer (TT = float) is read and written in single precision, including
+// the forward-sweep values that the back substitution re-reads.
 constexpr int NT = 8;
+template <typename TT>
 struct TriGroup {
-  double *z[NT];
+  TT *z[NT];
   const double *stf[NT];
   const double *btf[NT];
   const double *k33;   // NULL: zero (only for a group of one tracer)
   int n;
 };
 
-__global__ void k_vdiff(KDom d, TriGroup g, const double *__restrict__ dcbase, double rho0,
+template <typename TT>
+__global__ void k_vdiff(KDom d, TriGroup<TT> g, const double *__restrict__ dcbase, double rho0,
                         const int *__restrict__ kmt, const double *__restrict__ dh,
                         const double *__restrict__ dhw, double tdt, double aidif,
                         double *__restrict__ ework) {
@@ -123,16 +127,17 @@ __global__ void k_vdiff(KDom d, TriGroup g, const double *__restrict__ dcbase, d
 #pragma unroll
     for (int n = 0; n < NT; ++n) {
       if (n < g.n) {
-        double *z = g.z[n];
+        TT *z = g.z[n];
+        const double zo = (double)z[ok];
         double f;
         if (k == 1)
-          f = z[ok] + g.stf[n][o] * tdt_aidif * mkk / dhk;
+          f = zo + g.stf[n][o] * tdt_aidif * mkk / dhk;
         else if (k == kb)
-          f = z[ok] - g.btf[n][o] * tdt_aidif * mkk / dhk;
+          f = zo - g.btf[n][o] * tdt_aidif * mkk / dhk;
         else
-          f = z[ok] * mkk;
+          f = zo * mkk;
         const double zk = (k == 1) ? f * bet : (f - a * zprev[n]) * bet;
-        z[ok] = zk;
+        z[ok] = (TT)zk;
         zprev[n] = zk;
       }
     }
@@ -145,9 +150,9 @@ __global__ void k_vdiff(KDom d, TriGroup g, const double *__restrict__ dcbase, d
 #pragma unroll
     for (int n = 0; n < NT; ++n) {
       if (n < g.n) {
-        double *z = g.z[n];
-        const double zk = z[ok] - e1 * zprev[n];
-        z[ok] = zk;
+        TT *z = g.z[n];
+        const double zk = (double)z[ok] - e1 * zprev[n];
+        z[ok] = (TT)zk;
         zprev[n] = zk;
       }
     }
@@ -331,40 +336,42 @@ int launch_invtri(mom_gpu_ctx *ctx, double *z, const double *topbc, const double
   return MOM_GPU_OK;
 }
 
-int launch_vdiff_implicit(mom_gpu_ctx *ctx, int ntr, double *const *T, const double *const *stf,
-                          const double *const *btf, const double *diff_cbt,
-                          const double *const *K33, int index_salt, double rho0,
-                          const double *dh, const double *dhw, double dt, double aidif) {
+template <typename TT>
+static int vdiff_groups(mom_gpu_ctx *ctx, int ntr, void *const *T, const int *dtype, int want,
+                        const double *const *stf, const double *const *btf,
+                        const double *diff_cbt, const double *const *K33, int index_salt,
+                        double rho0, const double *dh, const double *dhw, double dt,
+                        double aidif) {
   const KDom &d = ctx->kd;
   const size_t p3 = (size_t)d.P2 * d.nk;
-  // groups: tracers with their own K33 go alone; the others are grouped by
-  // nmix in chunks of NT.
+  // tracers with their own K33 go alone; the others are grouped by nmix
   for (int nmix = 1; nmix <= 2; ++nmix) {
-    TriGroup g{};
+    TriGroup<TT> g{};
     g.n = 0;
     auto flush = [&](const double *k33) -> int {
       if (g.n == 0) return MOM_GPU_OK;
       g.k33 = k33;
-      hipLaunchKernelGGL(k_vdiff, dim3((d.ni + 63) / 64, d.nj), dim3(64), 0, ctx->stream, d, g,
-                         diff_cbt + p3 * (nmix - 1), rho0, ctx->kmt, dh, dhw, dt, aidif,
+      hipLaunchKernelGGL(k_vdiff<TT>, dim3((d.ni + 63) / 64, d.nj), dim3(64), 0, ctx->stream, d,
+                         g, diff_cbt + p3 * (nmix - 1), rho0, ctx->kmt, dh, dhw, dt, aidif,
                          ctx->tri_e);
       MOM_LAUNCHED(ctx, "k_vdiff");
       g.n = 0;
       return MOM_GPU_OK;
     };
     for (int n = 0; n < ntr; ++n) {
+      if ((dtype ? dtype[n] : 8) != want) continue;
       const int nm = (n == index_salt) ? 2 : 1;
       if (nm != nmix) continue;
       const double *k33 = (K33 && K33[n]) ? K33[n] : nullptr;
       if (k33) {
         int rc = flush(nullptr);
         if (rc) return rc;
-        g.z[0] = T[n]; g.stf[0] = stf[n]; g.btf[0] = btf[n]; g.n = 1;
+        g.z[0] = (TT *)T[n]; g.stf[0] = stf[n]; g.btf[0] = btf[n]; g.n = 1;
         rc = flush(k33);
         if (rc) return rc;
         continue;
       }
-      g.z[g.n] = T[n]; g.stf[g.n] = stf[n]; g.btf[g.n] = btf[n]; ++g.n;
+      g.z[g.n] = (TT *)T[n]; g.stf[g.n] = stf[n]; g.btf[g.n] = btf[n]; ++g.n;
       if (g.n == NT) {
         int rc = flush(nullptr);
         if (rc) return rc;
@@ -374,6 +381,25 @@ int launch_vdiff_implicit(mom_gpu_ctx *ctx, int ntr, double *const *T, const dou
     if (rc) return rc;
   }
   return MOM_GPU_OK;
+}
+
+int launch_vdiff_implicit(mom_gpu_ctx *ctx, int ntr, double *const *T, const double *const *stf,
+                          const double *const *btf, const double *diff_cbt,
+                          const double *const *K33, int index_salt, double rho0,
+                          const double *dh, const double *dhw, double dt, double aidif) {
+  return vdiff_groups<double>(ctx, ntr, (void *const *)T, nullptr, 8, stf, btf, diff_cbt, K33,
+                              index_salt, rho0, dh, dhw, dt, aidif);
+}
+
+int launch_vdiff_mixed(mom_gpu_ctx *ctx, int ntr, void *const *T, const int *dtype,
+                       const double *const *stf, const double *const *btf,
+                       const double *diff_cbt, int index_salt, double rho0, const double *dh,
+                       const double *dhw, double dt, double aidif) {
+  int rc = vdiff_groups<double>(ctx, ntr, T, dtype, 8, stf, btf, diff_cbt, nullptr, index_salt,
+                                rho0, dh, dhw, dt, aidif);
+  if (rc) return rc;
+  return vdiff_groups<float>(ctx, ntr, T, dtype, 4, stf, btf, diff_cbt, nullptr, index_salt,
+                             rho0, dh, dhw, dt, aidif);
 }
 
 int launch_tmask_limit(mom_gpu_ctx *ctx, int ntr, const double *const *T, const double *mn,

@@ -1,0 +1,421 @@
+// Batched tracer step: MDFL-Sweby advection of a group of tracers and the
+// explicit update (ocean_tracer.F90:2341-2350) in one k-marching pass.
+//
+// The work-group layout is that of k_mdfl (64-wide region rows, one row per
+// wavefront, 2-cell halo, x sweep through shuffles, y sweep through LDS).
+// The tracer-independent terms of every face (mass flux, CFL and the
+// limiter coefficients d0, d1, (1-c)/(eps+c)) and of every update (dt/rho,
+// dt*tmask*datr/rho, the divergence brackets) are formed once per point and
+// level and reused for the G tracers of the group.
+//
+// FP64 groups keep the reference's operation order for every tracer-
+// dependent operation, so their T(taup1) equals, bit for bit, the per-tracer
+// path (horz_advect_tracer into a zeroed th_tendency, then the explicit
+// update).  FP32 groups store and compute the tracer in single precision and
+// use the division-free form of the limited flux:
+//   psi*Rj = max(0, min(Rj, d0*Rj + d1*Rjm, k*Rjm))   for Rj >= 0,
+//   psi*Rj = min(0, max(Rj, d0*Rj + d1*Rjm, k*Rjm))   for Rj < 0,
+// which is the limiter of ocean_tracer_advect.F90:3877-3885 multiplied
+// through by Rj (the 1e-30 guard is below FP32 resolution).
+#include "mom_internal.h"
+
+namespace {
+
+constexpr int RX = 64;
+constexpr int TX = RX - 4;
+constexpr int NW = 8;
+constexpr double ONESIXTH = 1.0 / 6.0;
+constexpr double EPS30 = 1.0e-30;
+constexpr double EPSLN = 1.0e-40;   // shared/constants/constants.F90:123
+
+template <typename TT>
+__device__ inline TT mk(int k, int kmt) { return (k <= kmt) ? TT(1) : TT(0); }
+
+// tracer-independent limiter coefficients of one face
+template <typename TT>
+struct FaceC {
+  TT M;      // mass flux through the face
+  TT d0, d1, kk;
+};
+
+__device__ inline FaceC<double> face_coef(double massflux, double cfl) {
+  FaceC<double> c;
+  c.M = massflux;
+  c.d0 = (2.0 - cfl) * (1.0 - cfl) * ONESIXTH;
+  c.d1 = (1.0 - cfl * cfl) * ONESIXTH;
+  c.kk = (1.0 - cfl) / (EPS30 + cfl);
+  return c;
+}
+
+// reference-order limited flux (FP64), lim = 1
+__device__ inline double flux_exact(const FaceC<double> &c, double Rjm, double Rj, double Rjp,
+                                    double Tpos, double Tneg, double ma, double mb) {
+  const double thetaP = Rjm / (EPS30 + Rj);
+  const double thetaM = Rjp / (EPS30 + Rj);
+  const double pP = c.d0 + c.d1 * thetaP;
+  const double pM = c.d0 + c.d1 * thetaM;
+  // psi*(1-lim) + max(...)*lim with lim = 1, as in the per-tracer routine
+  const double psiP = pP * 0.0 + fmx(0.0, fmn(fmn(1.0, c.d0 + c.d1 * thetaP), c.kk * thetaP)) * 1.0;
+  const double psiM = pM * 0.0 + fmx(0.0, fmn(fmn(1.0, c.d0 + c.d1 * thetaM), c.kk * thetaM)) * 1.0;
+  return 0.5 * ((c.M + fabs(c.M)) * (Tpos + psiP * Rj) + (c.M - fabs(c.M)) * (Tneg - psiM * Rj)) *
+         ma * mb;
+}
+
+// division-free limited flux (FP32)
+__device__ inline float psi_r(const FaceC<float> &c, float a, float r) {
+  const float lin = c.d0 * a + c.d1 * r;
+  const float kr = c.kk * r;
+  return (a >= 0.0f) ? fmxf(0.0f, fmnf(fmnf(a, lin), kr)) : fmnf(0.0f, fmxf(fmxf(a, lin), kr));
+}
+__device__ inline float flux_fast(const FaceC<float> &c, float Rjm, float Rj, float Rjp, float Tpos,
+                                  float Tneg, float ma, float mb) {
+  const float pP = psi_r(c, Rj, Rjm);
+  const float pM = psi_r(c, Rj, Rjp);
+  return 0.5f * ((c.M + fabsf(c.M)) * (Tpos + pP) + (c.M - fabsf(c.M)) * (Tneg - pM)) * ma * mb;
+}
+
+template <typename TT>
+struct StepTab {
+  const TT *T[8];      // tracer at taum1 (= tau)
+  TT *out[8];          // tracer at taup1
+  const double *th[8]; // other th_tendency contributions (NULL: none)
+};
+
+template <typename TT, int G, int RPW>
+__global__ __launch_bounds__(RX * NW) void k_step(
+    KDom d, StepTab<TT> tab, const double *__restrict__ U, const double *__restrict__ V,
+    const double *__restrict__ W, const double *__restrict__ R, const double *__restrict__ Rm1,
+    const double *__restrict__ Rr, const int *__restrict__ kmt, const double *__restrict__ dat,
+    const double *__restrict__ datr, const double *__restrict__ dxte,
+    const double *__restrict__ dyte, const double *__restrict__ dxtn,
+    const double *__restrict__ dytn, double dtime, int ntx, int ntiles, int per_xcd) {
+  constexpr bool EXACT = sizeof(TT) == 8;
+  constexpr int RY = NW * RPW;
+  __shared__ TT s_tx[G][RY][RX];
+  __shared__ TT s_fy[G][RY][RX];
+  __shared__ int s_km[RY][RX];
+
+  const int b = blockIdx.x;
+  const int tile = (b & 7) * per_xcd + (b >> 3);
+  if (tile >= ntiles) return;
+  constexpr int TY = RY - 4;
+  const int tix = tile % ntx, tiy = tile / ntx;
+  const int gi0 = tix * TX, gj0 = tiy * TY;
+  const int lane = threadIdx.x & (RX - 1);
+  const int wave = threadIdx.x / RX;
+  const int nk = d.nk, P2 = d.P2;
+  const int gi = gi0 - 2 + lane;
+
+  int si = gi;
+  bool colok = true;
+  if (si < 0 || si >= d.ni) {
+    if (d.cyclic_x) si = wrapi(si, d.ni); else colok = false;
+  }
+  int km[RPW], offz[RPW], gjr[RPW], sjr[RPW];
+  bool ok[RPW], rowok[RPW];
+  double datv[RPW], datrv[RPW], wkm1[RPW];
+  TT t0[G][RPW], t1[G][RPW], t2[G][RPW], t3[G][RPW], ftp[G][RPW];
+#pragma unroll
+  for (int q = 0; q < RPW; ++q) {
+    const int ry = wave + NW * q;
+    const int gj = gj0 - 2 + ry;
+    int sj = gj;
+    bool rok = true;
+    if (sj < 0 || sj >= d.nj) {
+      if (d.cyclic_y) sj = wrapi(sj, d.nj); else rok = false;
+    }
+    gjr[q] = gj; sjr[q] = sj; rowok[q] = rok;
+    ok[q] = colok && rok;
+    offz[q] = ok[q] ? d2(d, si, sj) : 0;
+    km[q] = ok[q] ? kmt[offz[q]] : 0;
+    s_km[ry][lane] = km[q];
+    datv[q] = ok[q] ? dat[offz[q]] : 0.0;
+    datrv[q] = ok[q] ? datr[offz[q]] : 0.0;
+    wkm1[q] = 0.0;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      ftp[g][q] = TT(0);
+      if (ok[q]) {
+        const TT *T = tab.T[g];
+        const int o = offz[q];
+        t0[g][q] = T[o];
+        t1[g][q] = t0[g][q];
+        t2[g][q] = T[o + P2 * (min(2, nk) - 1)];
+        t3[g][q] = T[o + P2 * (min(3, nk) - 1)];
+      } else {
+        t0[g][q] = t1[g][q] = t2[g][q] = t3[g][q] = TT(0);
+      }
+    }
+  }
+  __syncthreads();
+
+  for (int k = 1; k <= nk; ++k) {
+    const int k0 = k - 1;
+    const int kp1 = min(k + 1, nk), kp2 = min(k + 2, nk), km1 = max(k - 1, 1);
+    double rhov[RPW], wv[RPW];
+    TT tz[G][RPW], fbtv[G][RPW], txv[G][RPW], fyv[G][RPW];
+    TT mfe[RPW], mfw[RPW], xfac[RPW];
+
+    // ---- Z ----------------------------------------------------------------
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+      rhov[q] = 1.0; wv[q] = 0.0;
+      if (ok[q]) {
+        const int o = offz[q];
+        const double w = W[o + P2 * k];
+        const double r = R[o + P2 * k0];
+        rhov[q] = r; wv[q] = w;
+        const FaceC<double> cz = face_coef(datv[q] * w, fabs(w * dtime / r));
+        const double dtr = dtime / r;
+        const TT mkm1 = mk<TT>(km1, km[q]), mk0 = mk<TT>(k, km[q]);
+        const TT mkp1 = mk<TT>(kp1, km[q]), mkp2 = mk<TT>(kp2, km[q]);
+        if constexpr (EXACT) {
+          const double wdiff = wkm1[q] - w;
+#pragma unroll
+          for (int g = 0; g < G; ++g) {
+            const double Rjp = (t0[g][q] - t1[g][q]) * mkm1 * mk0;
+            const double Rj = (t1[g][q] - t2[g][q]) * mk0 * mkp1;
+            const double Rjm = (t2[g][q] - t3[g][q]) * mkp1 * mkp2;
+            const double fbt = flux_exact(cz, Rjm, Rj, Rjp, t2[g][q], t1[g][q], mkp1, mk0);
+            tz[g][q] = t1[g][q] + dtr * (datrv[q] * (fbt - ftp[g][q]) + t1[g][q] * wdiff);
+            fbtv[g][q] = fbt;
+          }
+        } else {
+          const FaceC<float> c{(float)cz.M, (float)cz.d0, (float)cz.d1, (float)cz.kk};
+          const float dtrf = (float)dtr, datrf = (float)datrv[q], wdiff = (float)(wkm1[q] - w);
+#pragma unroll
+          for (int g = 0; g < G; ++g) {
+            const float Rjp = (t0[g][q] - t1[g][q]) * mkm1 * mk0;
+            const float Rj = (t1[g][q] - t2[g][q]) * mk0 * mkp1;
+            const float Rjm = (t2[g][q] - t3[g][q]) * mkp1 * mkp2;
+            const float fbt = flux_fast(c, Rjm, Rj, Rjp, t2[g][q], t1[g][q], mkp1, mk0);
+            tz[g][q] = t1[g][q] + dtrf * (datrf * (fbt - ftp[g][q]) + t1[g][q] * wdiff);
+            fbtv[g][q] = fbt;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int g = 0; g < G; ++g) { tz[g][q] = TT(0); fbtv[g][q] = TT(0); }
+      }
+    }
+
+    // ---- X ----------------------------------------------------------------
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+      const int ry = wave + NW * q;
+      const int kmm1 = __shfl(km[q], lane - 1);
+      const int kmp1 = __shfl(km[q], lane + 1);
+      const int kmp2 = __shfl(km[q], lane + 2);
+      const bool face = rowok[q] && lane >= 1 && lane <= RX - 3 && gi >= -1 && gi <= d.ni - 1;
+      FaceC<double> cx{0.0, 0.0, 0.0, 0.0};
+      if (face) {
+        const int of = d2(d, gi, sjr[q]);
+        const double u = U[of + P2 * k0];
+        const double rw = R[of + P2 * k0], re = R[of + 1 + P2 * k0];
+        cx = face_coef(dyte[of] * u, fabs(u * dtime * 2.0 / ((rw + re) * dxte[of])));
+      }
+      const TT mi = mk<TT>(k, km[q]), mim1 = mk<TT>(k, kmm1);
+      const TT mip1 = mk<TT>(k, kmp1), mip2 = mk<TT>(k, kmp2);
+      const double mw64 = __shfl(cx.M, lane - 1);
+      const bool cell = rowok[q] && lane >= 2 && lane < RX - 2 && gi < d.ni;
+      // dtime*tmask*datr/rho (x and y sweeps)
+      const double xf64 = dtime * (double)mi * datrv[q] / rhov[q];
+      mfe[q] = (TT)cx.M;
+      mfw[q] = (TT)mw64;
+      xfac[q] = (TT)xf64;
+      const FaceC<float> cxf{(float)cx.M, (float)cx.d0, (float)cx.d1, (float)cx.kk};
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const TT tzm1 = __shfl(tz[g][q], lane - 1);
+        const TT tzp1 = __shfl(tz[g][q], lane + 1);
+        const TT tzp2 = __shfl(tz[g][q], lane + 2);
+        TT fx = TT(0);
+        if (face) {
+          const TT Rjp = (tzp2 - tzp1) * mip2 * mip1;
+          const TT Rj = (tzp1 - tz[g][q]) * mip1 * mi;
+          const TT Rjm = (tz[g][q] - tzm1) * mi * mim1;
+          if constexpr (EXACT)
+            fx = flux_exact(cx, Rjm, Rj, Rjp, tz[g][q], tzp1, mi, mip1);
+          else
+            fx = flux_fast(cxf, Rjm, Rj, Rjp, tz[g][q], tzp1, mi, mip1);
+        }
+        const TT fxw = __shfl(fx, lane - 1);
+        TT tx = TT(0);
+        if (cell) {
+          if constexpr (EXACT)
+            tx = tz[g][q] + xf64 * (fxw - fx + t1[g][q] * (cx.M - mw64));
+          else
+            tx = tz[g][q] + xfac[q] * (fxw - fx + t1[g][q] * (mfe[q] - mfw[q]));
+        }
+        s_tx[g][ry][lane] = tx;
+        txv[g][q] = tx;
+      }
+    }
+    __syncthreads();
+
+    // ---- Y ----------------------------------------------------------------
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+      const int ry = wave + NW * q;
+      const int gj = gjr[q];
+      const bool face = ry >= 1 && ry <= RY - 3 && lane >= 2 && lane < RX - 2 && gi < d.ni &&
+                        gj >= -1 && gj <= d.nj - 1;
+      FaceC<double> cy{0.0, 0.0, 0.0, 0.0};
+      if (face) {
+        const int of = d2(d, gi, gj);
+        const double v = V[of + P2 * k0];
+        const double rs = R[of + P2 * k0], rn = R[of + d.NI + P2 * k0];
+        cy = face_coef(dxtn[of] * v, fabs(v * dtime * 2.0 / ((rs + rn) * dytn[of])));
+      }
+      const TT mjm1 = mk<TT>(k, s_km[max(ry - 1, 0)][lane]), mj = mk<TT>(k, s_km[ry][lane]);
+      const TT mjp1 = mk<TT>(k, s_km[min(ry + 1, RY - 1)][lane]);
+      const TT mjp2 = mk<TT>(k, s_km[min(ry + 2, RY - 1)][lane]);
+      const FaceC<float> cyf{(float)cy.M, (float)cy.d0, (float)cy.d1, (float)cy.kk};
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        TT fy = TT(0);
+        if (face) {
+          const TT tjm1 = s_tx[g][ry - 1][lane], tj = s_tx[g][ry][lane];
+          const TT tjp1 = s_tx[g][ry + 1][lane], tjp2 = s_tx[g][ry + 2][lane];
+          const TT Rjp = (tjp2 - tjp1) * mjp2 * mjp1;
+          const TT Rj = (tjp1 - tj) * mjp1 * mj;
+          const TT Rjm = (tj - tjm1) * mj * mjm1;
+          if constexpr (EXACT)
+            fy = flux_exact(cy, Rjm, Rj, Rjp, tj, tjp1, mj, mjp1);
+          else
+            fy = flux_fast(cyf, Rjm, Rj, Rjp, tj, tjp1, mj, mjp1);
+        }
+        s_fy[g][ry][lane] = fy;
+        fyv[g][q] = fy;
+      }
+    }
+    __syncthreads();
+
+    // ---- final: y update, correction removal, explicit update --------------
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+      const int ry = wave + NW * q;
+      const int gj = gjr[q];
+      const bool cell = ry >= 2 && ry < RY - 2 && lane >= 2 && lane < RX - 2 && gi < d.ni &&
+                        gj < d.nj;
+      if (cell) {
+        const int o = offz[q] + P2 * k0;
+        const double r = rhov[q];
+        const double mi = mk<double>(k, km[q]);
+        const double xf64 = dtime * mi * datrv[q] / r;
+        const double brk = wv[q] - wkm1[q] + datrv[q] * ((double)mfw[q] - (double)mfe[q]);
+        const double rm1 = Rm1[o];
+        const double rr = Rr ? Rr[o] : 1.0 / (R[o] + EPSLN);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          const double thin = tab.th[g] ? tab.th[g][o] : 0.0;
+          if constexpr (EXACT) {
+            const double Tc = t1[g][q];
+            double tr = txv[g][q] + xf64 * (s_fy[g][ry - 1][lane] - fyv[g][q]);
+            tr = tr + dtime * Tc / r * brk;
+            const double adv = -(r * (tr - Tc) / dtime * mi);
+            const double th = thin + (-adv);
+            tab.out[g][o] = (rm1 * Tc + dtime * th) * rr;
+          } else {
+            const float Tc = t1[g][q];
+            float tr = txv[g][q] + xfac[q] * (s_fy[g][ry - 1][lane] - fyv[g][q]);
+            tr = tr + (float)(dtime / r) * Tc * (float)brk;
+            const float th = (float)thin + (float)(r / dtime) * (tr - Tc) * (float)mi;
+            tab.out[g][o] = ((float)rm1 * Tc + (float)dtime * th) * (float)rr;
+          }
+        }
+      }
+    }
+
+    // ---- advance the k window -----------------------------------------------
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+      wkm1[q] = wv[q];
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        ftp[g][q] = fbtv[g][q];
+        t0[g][q] = t1[g][q]; t1[g][q] = t2[g][q]; t2[g][q] = t3[g][q];
+        if (ok[q] && k < nk) t3[g][q] = tab.T[g][offz[q] + P2 * (min(k + 3, nk) - 1)];
+      }
+    }
+  }
+}
+
+template <typename TT, int G, int RPW>
+void launch_group(mom_gpu_ctx *ctx, const StepTab<TT> &tab, const double *u, const double *v,
+                  const double *w, const double *rho, const double *rm1, const double *rr,
+                  double dtime) {
+  constexpr int TY = NW * RPW - 4;
+  const KDom &d = ctx->kd;
+  const int ntx = (d.ni + TX - 1) / TX;
+  const int nty = (d.nj + TY - 1) / TY;
+  const int ntiles = ntx * nty;
+  const int per = (ntiles + 7) / 8;
+  hipLaunchKernelGGL((k_step<TT, G, RPW>), dim3(8 * per), dim3(RX * NW), 0, ctx->stream, d, tab,
+                     u, v, w, rho, rm1, rr, ctx->kmt, ctx->dat, ctx->datr, ctx->dxte, ctx->dyte,
+                     ctx->dxtn, ctx->dytn, dtime, ntx, ntiles, per);
+}
+
+
+// Launch one group of up to G tracers of type TT taken from list[0..cnt).
+template <typename TT>
+static void launch_some(mom_gpu_ctx *ctx, int rpw, const TT *const *Tin, TT *const *Tout,
+                        const double *const *thin, int cnt, const double *u, const double *v,
+                        const double *w, const double *rho, const double *rm1, const double *rr,
+                        double dtime) {
+  StepTab<TT> t{};
+  for (int i = 0; i < cnt; ++i) { t.T[i] = Tin[i]; t.out[i] = Tout[i]; t.th[i] = thin[i]; }
+  if (rpw == 2) {
+    if (cnt == 4) launch_group<TT, 4, 2>(ctx, t, u, v, w, rho, rm1, rr, dtime);
+    else if (cnt == 2) launch_group<TT, 2, 2>(ctx, t, u, v, w, rho, rm1, rr, dtime);
+    else launch_group<TT, 1, 2>(ctx, t, u, v, w, rho, rm1, rr, dtime);
+  } else {
+    if (cnt == 4) launch_group<TT, 4, 4>(ctx, t, u, v, w, rho, rm1, rr, dtime);
+    else if (cnt == 2) launch_group<TT, 2, 4>(ctx, t, u, v, w, rho, rm1, rr, dtime);
+    else launch_group<TT, 1, 4>(ctx, t, u, v, w, rho, rm1, rr, dtime);
+  }
+}
+
+template <typename TT>
+static void launch_all(mom_gpu_ctx *ctx, int gmax, int rpw, int ntr, const void *const *T_in,
+                       void *const *T_out, const int *dtype, const double *const *th_in,
+                       int want, const double *u, const double *v, const double *w,
+                       const double *rho, const double *rm1, const double *rr, double dtime) {
+  const TT *tin[4];
+  TT *tout[4];
+  const double *th[4];
+  int g = 0;
+  auto flush = [&]() {
+    int i = 0;
+    while (i < g) {                 // groups of 4, 2, 1 no larger than gmax
+      int c = (g - i >= 4 && gmax >= 4) ? 4 : (g - i >= 2 && gmax >= 2) ? 2 : 1;
+      launch_some<TT>(ctx, rpw, tin + i, tout + i, th + i, c, u, v, w, rho, rm1, rr, dtime);
+      i += c;
+    }
+    g = 0;
+  };
+  for (int n = 0; n < ntr; ++n) {
+    if (dtype[n] != want) continue;
+    tin[g] = (const TT *)T_in[n];
+    tout[g] = (TT *)T_out[n];
+    th[g] = th_in ? th_in[n] : nullptr;
+    if (++g == 4) flush();
+  }
+  flush();
+}
+
+}  // namespace
+
+// Advection + explicit update for ntr MDFL-Sweby tracers.  dtype[n]: 8 = FP64
+// (reference rounding), 4 = FP32 (single precision, division-free limiter).
+int launch_tracer_step(mom_gpu_ctx *ctx, int ntr, const void *const *T_in, void *const *T_out,
+                       const int *dtype, const double *const *th_in, const double *u,
+                       const double *v, const double *w, const double *rho,
+                       const double *rho_taum1, const double *rho_dztr, double dtime) {
+  launch_all<double>(ctx, ctx->tune.step_group_f64, ctx->tune.step_rows_per_wave, ntr, T_in,
+                     T_out, dtype, th_in, 8, u, v, w, rho, rho_taum1, rho_dztr, dtime);
+  launch_all<float>(ctx, ctx->tune.step_group_f32, ctx->tune.step_rows_per_wave, ntr, T_in,
+                    T_out, dtype, th_in, 4, u, v, w, rho, rho_taum1, rho_dztr, dtime);
+  MOM_LAUNCHED(ctx, "k_step");
+  return MOM_GPU_OK;
+}

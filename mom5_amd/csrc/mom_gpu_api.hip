@@ -72,6 +72,18 @@ static int *tuning_slot(mom_gpu_ctx *ctx, const char *knob, int *lo, int *hi) {
     *lo = 2; *hi = 4;
     return &ctx->tune.mdfl_rows_per_wave;
   }
+  if (!std::strcmp(knob, "step_group_f64")) {
+    *lo = 1; *hi = 2;
+    return &ctx->tune.step_group_f64;
+  }
+  if (!std::strcmp(knob, "step_group_f32")) {
+    *lo = 1; *hi = 4;
+    return &ctx->tune.step_group_f32;
+  }
+  if (!std::strcmp(knob, "step_rows_per_wave")) {
+    *lo = 2; *hi = 4;
+    return &ctx->tune.step_rows_per_wave;
+  }
   return nullptr;
 }
 
@@ -82,8 +94,11 @@ int mom_gpu_set_tuning(mom_gpu_ctx_t *ctx, const char *knob, int value) {
   REQUIRE(ctx, slot, "mom_gpu_set_tuning: unknown knob '%s'", knob ? knob : "(null)");
   REQUIRE(ctx, value >= lo && value <= hi, "mom_gpu_set_tuning: %s=%d outside [%d,%d]", knob,
           value, lo, hi);
-  REQUIRE(ctx, slot != &ctx->tune.mdfl_rows_per_wave || value == 2 || value == 4,
-          "mom_gpu_set_tuning: mdfl_rows_per_wave must be 2 or 4");
+  REQUIRE(ctx, (slot != &ctx->tune.mdfl_rows_per_wave && slot != &ctx->tune.step_rows_per_wave) ||
+                   value == 2 || value == 4,
+          "mom_gpu_set_tuning: %s must be 2 or 4", knob);
+  REQUIRE(ctx, slot != &ctx->tune.step_group_f32 || value != 3,
+          "mom_gpu_set_tuning: step_group_f32 must be 1, 2 or 4");
   *slot = value;
   return MOM_GPU_OK;
 }
@@ -151,6 +166,9 @@ int mom_gpu_create(const mom_gpu_domain_t *dom, const mom_gpu_grid_t *grid, mom_
   k.cyclic_y = dom->cyclic_y ? 1 : 0;
   c->tune.bt_subcycles_per_launch = 4;
   c->tune.mdfl_rows_per_wave = 4;
+  c->tune.step_group_f64 = 1;
+  c->tune.step_group_f32 = 2;
+  c->tune.step_rows_per_wave = 4;
   const size_t P2 = (size_t)k.P2, nk = (size_t)k.nk;
 
 #define CK(call)                                                                      \
@@ -391,6 +409,38 @@ int mom_gpu_update_ocean_barotropic(mom_gpu_ctx_t *ctx, const mom_gpu_bt_params_
   return launch_barotropic(ctx, p, eta_t_tau ? eta_t_tau : eta_t_bar_tau, eta_t_bar_tau, udrho_tau,
                            forcing_bt, thicku, patm, pme, river, source, eta_smooth, rho_g,
                            eta_t_bar_taup1, udrho_taup1, conv_rho_ud_t);
+}
+
+int mom_gpu_step_tracers(mom_gpu_ctx_t *ctx, const mom_gpu_tracer_step_t *s) {
+  REQUIRE(nullptr, ctx, "mom_gpu_step_tracers: NULL context");
+  REQUIRE(ctx, s, "mom_gpu_step_tracers: NULL descriptor");
+  REQUIRE(ctx, s->ntr >= 0 && (s->ntr == 0 || (s->T_taum1 && s->T_taup1 && s->dtype)),
+          "mom_gpu_step_tracers: NULL tracer tables");
+  REQUIRE(ctx, s->uhrho_et && s->vhrho_nt && s->wrho_bt && s->rho_dzt_tau,
+          "mom_gpu_step_tracers: NULL advection field");
+  REQUIRE(ctx, s->dtime > 0.0, "mom_gpu_step_tracers: dtime must be positive");
+  for (int n = 0; n < s->ntr; ++n) {
+    REQUIRE(ctx, s->dtype[n] == 8 || s->dtype[n] == 4, "mom_gpu_step_tracers: dtype[%d]=%d", n,
+            s->dtype[n]);
+    REQUIRE(ctx, s->T_taum1[n] && s->T_taup1[n], "mom_gpu_step_tracers: tracer %d NULL", n);
+    REQUIRE(ctx, s->T_taum1[n] != (const void *)s->T_taup1[n],
+            "mom_gpu_step_tracers: tracer %d: taum1 and taup1 must differ", n);
+  }
+  if (s->ntr == 0) return MOM_GPU_OK;
+  const double *rm1 = s->rho_dzt_taum1 ? s->rho_dzt_taum1 : s->rho_dzt_tau;
+  int rc = launch_tracer_step(ctx, s->ntr, s->T_taum1, s->T_taup1, s->dtype, s->th_tendency,
+                              s->uhrho_et, s->vhrho_nt, s->wrho_bt, s->rho_dzt_tau, rm1,
+                              s->rho_dztr, s->dtime);
+  if (rc) return rc;
+  if (s->stf && s->btf && s->aidif != 0.0) {
+    REQUIRE(ctx, s->dzwt && s->diff_cbt, "mom_gpu_step_tracers: dzwt/diff_cbt required");
+    for (int n = 0; n < s->ntr; ++n)
+      REQUIRE(ctx, s->stf[n] && s->btf[n], "mom_gpu_step_tracers: stf/btf %d NULL", n);
+    const double *dh = s->rho_dzt_taup1 ? s->rho_dzt_taup1 : s->rho_dzt_tau;
+    rc = launch_vdiff_mixed(ctx, s->ntr, s->T_taup1, s->dtype, s->stf, s->btf, s->diff_cbt,
+                            s->index_salt, s->rho0, dh, s->dzwt, s->dtime, s->aidif);
+  }
+  return rc;
 }
 
 int mom_gpu_tracer_content(mom_gpu_ctx_t *ctx, const double *T, const double *rho_dzt,

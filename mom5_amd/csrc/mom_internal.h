@@ -67,6 +67,9 @@ struct mom_gpu_ctx {
   struct {
     int bt_subcycles_per_launch;   // 0: one launch per phase per subcycle
     int mdfl_rows_per_wave;        // 2 or 4 region rows per wavefront
+    int step_group_f64;            // FP64 tracers per pass of the batched step (1, 2)
+    int step_group_f32;            // FP32 tracers per pass (1, 2, 4)
+    int step_rows_per_wave;        // 2 or 4
   } tune;
   char err[512];
 };
@@ -103,6 +106,14 @@ int launch_quicker_horz(mom_gpu_ctx *ctx, const double *Tm1, const double *Tt,
 int launch_quicker_vert(mom_gpu_ctx *ctx, const double *Tm1, const double *Tt,
                         const double *w, const double *tlim, double *th,
                         double *wrk1, double *fz);
+int launch_vdiff_mixed(mom_gpu_ctx *ctx, int ntr, void *const *T, const int *dtype,
+                       const double *const *stf, const double *const *btf,
+                       const double *diff_cbt, int index_salt, double rho0, const double *dh,
+                       const double *dhw, double dt, double aidif);
+int launch_tracer_step(mom_gpu_ctx *ctx, int ntr, const void *const *T_in, void *const *T_out,
+                       const int *dtype, const double *const *th_in, const double *u,
+                       const double *v, const double *w, const double *rho,
+                       const double *rho_taum1, const double *rho_dztr, double dtime);
 int launch_tmask_limit(mom_gpu_ctx *ctx, int ntr, const double *const *T,
                        const double *mn, const double *mx, int it, int is,
                        int ts_same, double *const *lim);

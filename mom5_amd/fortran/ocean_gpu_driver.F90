@@ -46,7 +46,7 @@ program ocean_gpu_driver
   type(c_ptr) :: d_eta(2), d_ud(2), d_conv, d_forc
   type(c_ptr), target :: d_T(ntr,2), d_th(ntr), d_lim(ntr), d_stf(ntr), d_btf(ntr)
   type(c_ptr), target :: tab_now(ntr), tab_nxt(ntr), tab_lim(ntr)
-  real(c_double) :: c0(ntr), c1(ntr), dt, dtbt, rel
+  real(c_double) :: c0(ntr), c1(ntr), dt, dtbt, rel, gk(nk), gmean
   integer :: i, j, k, n, step, now, nxt
   integer(c_int) :: rc
   integer(c_size_t) :: b2, b3, b3w, bi2
@@ -88,6 +88,14 @@ program ocean_gpu_driver
   fcor = 1.0d-4; f2 = dx
 
   ! ---- transports from a streamfunction at U points, w by continuity ----
+  ! overturning profile with zero thickness-weighted mean, so that the
+  ! column-integrated divergence vanishes and wrho_bt(nk) = 0
+  gmean = 0.0d0
+  do k = 1, nk
+     gk(k) = real(nk/2 - k, 8)
+     gmean = gmean + gk(k)*dzt(k)
+  enddo
+  gk = gk - gmean/sum(dzt)
   psi = 0.0d0
   do j = jsd, jed
      do i = isd, ied
@@ -99,7 +107,7 @@ program ocean_gpu_driver
      do j = jsc, jec
         do i = isd, iec
            uh(i,j,k) = rho0*dzt(k)*(psi(i,j)*umask(i,j,k) - psi(i,j-1)*umask(i,j-1,k))/dy &
-                + rho0*dzt(k)*1.0d-3*real(nk/2 - k, 8)*tmask(i,j,k)*tmask(i+1,j,k)*tmask(i,j,nk)*tmask(i+1,j,nk)
+                + rho0*dzt(k)*1.0d-3*gk(k)*tmask(i,j,k)*tmask(i+1,j,k)*tmask(i,j,nk)*tmask(i+1,j,nk)
         enddo
      enddo
      do j = jsd, jec
@@ -107,7 +115,9 @@ program ocean_gpu_driver
            vh(i,j,k) = -rho0*dzt(k)*(psi(i,j)*umask(i,j,k) - psi(i-1,j)*umask(i-1,j,k))/dx
         enddo
      enddo
-     uh(ied,:,k) = uh(isc,:,k); vh(isd,:,k) = vh(iec,:,k); vh(ied,:,k) = vh(isc,:,k)
+     ! halo columns hold the cyclic wrap, as mpp_update_domains leaves them
+     uh(isd,:,k) = uh(iec,:,k); uh(ied,:,k) = uh(isc,:,k)
+     vh(isd,:,k) = vh(iec,:,k); vh(ied,:,k) = vh(isc,:,k)
   enddo
   do k = 1, nk
      do j = jsc, jec

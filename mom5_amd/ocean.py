@@ -26,7 +26,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import BtParams, Domain, Grid, MomGpuError, check, load
+from ._lib import BtParams, Domain, Grid, MomGpuError, TracerStep, check, load
 
 ADVECT_QUICKER = _lib.MOM_ADVECT_QUICKER
 ADVECT_MDFL_SWEBY = _lib.MOM_ADVECT_MDFL_SWEBY
@@ -245,6 +245,34 @@ class OceanGPU:
             _p(forcing_bt), _p(thicku), _p(patm), _p(pme), _p(river), _p(source),
             _p(eta_smooth), _p(rho_g), _p(eta_t_bar_taup1), _p(udrho_taup1),
             _p(conv_rho_ud_t)), "update_ocean_barotropic")
+
+    # ---- batched tracer step -------------------------------------------------
+    def step_tracers(self, T_taum1s, T_taup1s, *, uhrho_et, vhrho_nt, wrho_bt, rho_dzt_tau,
+                     dtime, stfs=None, btfs=None, dzwt=None, diff_cbt=None, index_salt=1,
+                     rho0=1035.0, aidif=1.0, th_tendencies=None, rho_dzt_taum1=None,
+                     rho_dztr=None, rho_dzt_taup1=None):
+        """Advection (MDFL-Sweby), explicit update and implicit vertical
+        diffusion of every tracer; each tracer's dtype is its array's."""
+        n = len(T_taum1s)
+        dt = (C.c_int * max(n, 1))(*[8 if np.dtype(a.dtype) == np.float64 else 4
+                                     for a in T_taum1s])
+        keep = []
+
+        def tab(xs):
+            if xs is None:
+                return None
+            t = _ptab(xs)
+            keep.append(t)
+            return C.cast(t, C.POINTER(C.c_void_p))
+
+        s = TracerStep(ntr=n, T_taum1=tab(T_taum1s), T_taup1=tab(T_taup1s), dtype=dt,
+                       th_tendency=tab(th_tendencies), stf=tab(stfs), btf=tab(btfs),
+                       index_salt=int(index_salt), uhrho_et=_p(uhrho_et), vhrho_nt=_p(vhrho_nt),
+                       wrho_bt=_p(wrho_bt), rho_dzt_tau=_p(rho_dzt_tau),
+                       rho_dzt_taum1=_p(rho_dzt_taum1), rho_dztr=_p(rho_dztr),
+                       rho_dzt_taup1=_p(rho_dzt_taup1), dzwt=_p(dzwt), diff_cbt=_p(diff_cbt),
+                       rho0=float(rho0), dtime=float(dtime), aidif=float(aidif))
+        self._chk(self.L.mom_gpu_step_tracers(self.ctx, C.byref(s)), "step_tracers")
 
     # ---- diagnostics ----------------------------------------------------------
     def tracer_content(self, T, rho_dzt):
