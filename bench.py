@@ -1,18 +1,28 @@
 #!/usr/bin/env python3
 """Throughput of one baroclinic step of the MOM5 hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 3|1|0]
 
 A step is the reference's order (ocean_model.F90:1441-2108, the part this
-build implements): compute_tmask_limit; per tracer horz/vert advection into
-th_tendency and the explicit update; vert_diffuse_implicit (invtri) for all
-tracers; the barotropic subcycle with its time-averaging tail.  Inputs are
-seeded synthetic fields resident in HBM before timing starts.
+build implements): the tracer update (advection into th_tendency, explicit
+update, vert_diffuse_implicit/invtri) for every tracer, then the barotropic
+subcycle with its time-averaging tail.  Inputs are seeded synthetic fields
+resident in HBM before timing starts.
 
-Prints one JSON line: tracer-cell-updates/s for the whole step (value), the
-roofline of the dominant kernel (MDFL-Sweby advection, HIP-event timed on the
-context's stream), and a CPU baseline: the CPU restatement timed on a bounded
-sample on the host (single thread).
+--config 3 (default): 3600 x 2700 x 75, T and S in FP64 plus 10 passive
+tracers in FP32, 120 barotropic subcycles, advanced by the batched step
+(mom_gpu_step_tracers + update_ocean_barotropic).  The same case is then run
+through the plain per-tracer FP64 path for the same number of steps: that run
+gives the FP64 rate the batched step is compared with, the bitwise check of
+T and S, and the FP64 reference for each passive tracer's error.  Masked
+tracer content (compensated FP64 sums) is reported before and after.
+
+--config 1: 1440 x 1080 x 50 plain FP64 path (T, S MDFL-Sweby, one Quicker
+tracer, 80 subcycles).  --config 0: 90 x 60 x 25, MDFL only.
+
+Prints one JSON line: tracer-cell-updates/s of the step (value), the roofline
+of the dominant kernel (HIP-event timed on the context's stream) and a CPU
+baseline (the CPU restatement on a bounded sample, single thread).
 """
 from __future__ import annotations
 
@@ -30,6 +40,9 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
+    3: dict(name="cfg3_mixed_3600x2700x75", ni=3600, nj=2700, nk=75, land_frac=0.30,
+            nts=120, n_passive=10, mixed=True, steps=18, warmup=2,
+            schemes=["mdfl_sweby"] * 12, sample=dict(ni=240, nj=180)),
     1: dict(name="cfg1_plain_fp64_1440x1080x50", ni=1440, nj=1080, nk=50, land_frac=0.30,
             nts=80, n_passive=1, schemes=["mdfl_sweby", "mdfl_sweby", "quicker"],
             sample=dict(ni=480, nj=360)),
@@ -94,34 +107,200 @@ def cpu_baseline(w, steps=8):
                        f"({sum(times):.1f} s timed)")
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", type=int, default=1, choices=sorted(WORKLOADS))
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU decomposition is a design, not an implementation.", file=sys.stderr)
-        sys.exit(2)
+def log(msg):
+    print(f"# {msg}", file=sys.stderr, flush=True)
 
-    from mom5_amd.step import PlainStep
-    w = WORKLOADS[args.config]
+
+def step_kernel_bytes(ni, nj, nk, elem, group):
+    """Algorithmic bytes of one fused tracer-step launch (advection +
+    explicit update of `group` tracers): per cell uhrho_et, vhrho_nt,
+    wrho_bt, rho_dzt, rho_dztr read (8 B each) and each tracer read and
+    written once (elem B each way); per column the FP64 metrics and kmt."""
+    return ni * nj * nk * (40 + 2 * elem * group) + ni * nj * 52
+
+
+def mixed_step_bytes(ni, nj, nk, n64, n32):
+    """Algorithmic bytes of one whole mixed step: the shared 3-D fields
+    (uhrho_et, vhrho_nt, wrho_bt, rho_dzt, rho_dztr, dzwt, two diff_cbt
+    planes) read once, every tracer read once and written once."""
+    return ni * nj * nk * (64 + 16 * n64 + 8 * n32)
+
+
+def time_launches(g, fn, reps=10):
+    fn()
+    g.sync()
+    g.mark(2)
+    for _ in range(reps):
+        fn()
+    g.mark(3)
+    return g.elapsed_ms(2, 3) / reps
+
+
+def run_mixed(args, w):
+    import dataclasses
+    import gc
+    from mom5_amd import DeviceArray
+    from mom5_amd.step import BatchedStep, PlainStep
+    from mom5_amd.synth import make_case_device
+    steps = w["steps"] if args.steps is None else args.steps
+    warm = w["warmup"] if args.warmup is None else args.warmup
+    ni, nj, nk = w["ni"], w["nj"], w["nk"]
+    if args.shape:
+        ni, nj, nk = (int(x) for x in args.shape.split("x"))
+    cells = ni * nj * nk
     t0 = time.time()
-    case = make(w)
-    print(f"# generated {w['name']} in {time.time() - t0:.1f}s (dt={case.dt:.3f}s, "
-          f"dtbt={case.dtbt:.4f}s, nts={case.nts})", file=sys.stderr, flush=True)
-    st = PlainStep(case, schemes=w["schemes"])
-    g = st.gpu
-    st.run(args.warmup)
+    case = make_case_device(ni, nj, nk, land_frac=w["land_frac"], nts=w["nts"],
+                            n_passive=w["n_passive"], passive_dtype=np.float32)
+    g = case.info["gpu"]
+    log(f"generated {ni}x{nj}x{nk} on the device in {time.time() - t0:.1f}s "
+        f"(dt={case.dt:.3f}s, dtbt={case.dtbt:.4f}s, nts={case.nts})")
+    bs = BatchedStep(case, gpu=g)
+    ntr, names = bs.ntr, bs.names
+    rho = bs.dev["rho_dzt"]
+    tuned = None
+    if not args.no_autotune:
+        from mom5_amd.tune import autotune
+        t0 = time.time()
+        tuned = autotune(bs)
+        log(f"autotuned in {time.time() - t0:.1f}s: " + "; ".join(
+            f"{k} {v['best']} {v['best_ms']:.3f} ms (default {v['default_ms']:.3f})"
+            for k, v in tuned.items()))
+    c0 = [g.tracer_content(bs.tracer(n), rho) for n in range(ntr)]
+    bs.run(warm)
     g.sync()
     g.mark(0)
-    st.run(args.steps)
+    bs.run(steps)
     g.mark(1)
-    ms_total = g.elapsed_ms(0, 1)
-    ms = ms_total / args.steps
+    ms = g.elapsed_ms(0, 1) / steps
+    value = ntr * cells / (ms / 1e3)
+    c1 = [g.tracer_content(bs.tracer(n), rho) for n in range(ntr)]
+    log(f"batched mixed step {ms:.2f} ms ({value:.3e} tracer-cell-updates/s)")
+
+    # the fused tracer-step kernels, each timed alone on the context's stream
+    d = bs.dev
+    kw = dict(uhrho_et=d["uhrho_et"], vhrho_nt=d["vhrho_nt"], wrho_bt=d["wrho_bt"],
+              rho_dzt_tau=d["rho_dzt"], rho_dztr=d["rho_dztr"], dtime=case.dt)
+    is64 = [bs.tracer(n).element_size() == 8 for n in range(ntr)]
+    n64 = sum(is64)
+    n32 = ntr - n64
+    cand = []
+    for want64, elem in ((True, 8), (False, 4)):
+        grp = g.get_tuning("step_group_f64" if want64 else "step_group_f32")
+        idx = [n for n in range(ntr) if is64[n] == want64][:grp]
+        if len(idx) < grp:
+            continue
+        src = [bs.tracer(n) for n in idx]
+        dst = [bs.T[n][1 - bs.cur] for n in idx]
+        k_ms = time_launches(g, lambda: g.step_tracers(src, dst, **kw))
+        nl = -(-(n64 if want64 else n32) // grp)
+        cand.append(dict(kernel=f"k_step<{'double' if want64 else 'float'},G={grp}> "
+                                f"(fused MDFL-Sweby advection + explicit update)",
+                         kernel_ms=k_ms, bytes_per_launch=step_kernel_bytes(ni, nj, nk, elem, grp),
+                         launches_per_step=nl))
+        log(f"{cand[-1]['kernel']}: {k_ms:.3f} ms/launch x {nl} per step")
+        del src, dst
+    dom = max(cand, key=lambda c: c["kernel_ms"] * c["launches_per_step"])
+    achieved = dom["bytes_per_launch"] / (dom["kernel_ms"] / 1e3) / 1e9
+    roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, **{
+                    k: (round(v, 4) if isinstance(v, float) else v) for k, v in dom.items()})
+    step_bytes = mixed_step_bytes(ni, nj, nk, n64, n32)
+    step_frac = step_bytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS
+
+    # keep the batched results (FP64 on the device, FP32 on the host), free the rest
+    fin64 = {n: bs.tracer(n) for n in range(ntr) if is64[n]}
+    fin32 = {n: bs.tracer(n).to_host() for n in range(ntr) if not is64[n]}
+    del bs, cand
+    case.tracers = []
+    gc.collect()
+
+    # the plain per-tracer FP64 path from the same initial state
+    pcase = dataclasses.replace(case, tracers=[
+        (names[n], case.info["tracer"](n, np.float64)) for n in range(ntr)])
+    plain = PlainStep(pcase, schemes=["mdfl_sweby"] * ntr, gpu=g)
+    del pcase
+    if not args.no_autotune:
+        from mom5_amd.tune import autotune
+        autotune(plain, barotropic=False)
+    pc0 = [g.tracer_content(plain.tracer(n), rho) for n in range(ntr)]
+    plain.run(warm)
+    g.sync()
+    g.mark(0)
+    plain.run(steps)
+    g.mark(1)
+    ms_plain = g.elapsed_ms(0, 1) / steps
+    value_plain = ntr * cells / (ms_plain / 1e3)
+    pc1 = [g.tracer_content(plain.tracer(n), rho) for n in range(ntr)]
+    log(f"plain FP64 step {ms_plain:.2f} ms ({value_plain:.3e} tracer-cell-updates/s)")
+    ref = {n: plain.tracer(n) for n in range(ntr)}
+    del plain
+    gc.collect()
+
+    errors = {}
+    for n in range(ntr):
+        if is64[n]:
+            e = g.tracer_error(fin64[n], ref[n])
+            errors[names[n]] = dict(cells_differ_from_plain_fp64=e["cells_differ"],
+                                    cells=e["cells"])
+            continue
+        a = DeviceArray.from_host(fin32[n])
+        e = g.tracer_error(a, ref[n])
+        e2 = g.tracer_error(a, ref[n], rel_floor=1e-6 * e["max_ref"])
+        errors[names[n]] = dict(max_rel_normwise=e["normwise"], rms_rel=e["rms_rel"],
+                                max_rel_cell_where_ref_ge_1em6_max=e2["max_rel_cell"])
+        del a
+    drift = {names[n]: dict(mixed=abs(c1[n] - c0[n]) / abs(c0[n]),
+                            plain_fp64=abs(pc1[n] - pc0[n]) / abs(pc0[n])) for n in range(ntr)}
+    worst = max((e["max_rel_normwise"] for e in errors.values() if "max_rel_normwise" in e),
+                default=None)
+    ts_equal = all(e.get("cells_differ_from_plain_fp64", 0) == 0 for e in errors.values())
+    log(f"T/S bitwise equal to plain FP64: {ts_equal}; worst passive normwise error {worst}")
+    extra = dict(autotune=_tune_summary(tuned), plain_fp64_value=value_plain, plain_fp64_ms_per_step=ms_plain,
+                 speedup_vs_plain_fp64=value / value_plain, step_algorithmic_bytes=step_bytes,
+                 step_hbm_frac=round(step_frac, 4), ts_bitwise_equal_plain_fp64=ts_equal,
+                 passive_max_rel_err_normwise=worst, errors_after_steps=warm + steps,
+                 tracer_errors=errors, content_drift_rel=drift)
+    name = w["name"] if not args.shape else f"{w['name']}@{args.shape}"
+    cfg = {"workload": name, "grid": [ni, nj, nk], "tracers": ntr,
+           "tracer_dtypes": {"f64": n64, "f32": n32}, "schemes": "mdfl_sweby (all)",
+           "barotropic_nts": case.nts, "path": "batched mixed (mom_gpu_step_tracers)",
+           "tuning": {k: g.get_tuning(k) for k in TUNING_KEYS}}
+    return dict(value=value, ms=ms, steps=steps, warmup=warm, dtype="f64+f32", cfg=cfg,
+                roof=roof, extra=extra, ntr=ntr)
+
+
+def _tune_summary(rep):
+    if not rep:
+        return None
+    return {k: dict(best=v["best"], best_ms=round(v["best_ms"], 4),
+                    default=v["default"], default_ms=round(v["default_ms"], 4))
+            for k, v in rep.items()}
+
+
+TUNING_KEYS = ("barotropic_subcycles_per_launch", "mdfl_rows_per_wave", "step_group_f64",
+               "step_group_f32", "step_rows_per_wave", "step_rows_per_wave_f32")
+
+
+def run_plain(args, w):
+    from mom5_amd.step import PlainStep
+    steps = 20 if args.steps is None else args.steps
+    warm = 3 if args.warmup is None else args.warmup
+    t0 = time.time()
+    case = make(w)
+    log(f"generated {w['name']} in {time.time() - t0:.1f}s (dt={case.dt:.3f}s, "
+        f"dtbt={case.dtbt:.4f}s, nts={case.nts})")
+    st = PlainStep(case, schemes=w["schemes"])
+    g = st.gpu
+    tuned = None
+    if not args.no_autotune:
+        from mom5_amd.tune import autotune
+        tuned = autotune(st)
+    st.run(warm)
+    g.sync()
+    g.mark(0)
+    st.run(steps)
+    g.mark(1)
+    ms = g.elapsed_ms(0, 1) / steps
     ntr = st.ntr
     cells = case.ni * case.nj * case.nk
     value = ntr * cells / (ms / 1e3)
@@ -130,38 +309,95 @@ def main():
     d = st.dev
     th = g.field3()
     T0 = st.tracer(0)
-    reps = 10
-    g.horz_advect_tracer("mdfl_sweby", T0, th, uhrho_et=d["uhrho_et"], vhrho_nt=d["vhrho_nt"],
-                         wrho_bt=d["wrho_bt"], rho_dzt_tau=d["rho_dzt"], dtime=case.dt)
-    g.sync()
-    g.mark(2)
-    for _ in range(reps):
-        g.horz_advect_tracer("mdfl_sweby", T0, th, uhrho_et=d["uhrho_et"],
-                             vhrho_nt=d["vhrho_nt"], wrho_bt=d["wrho_bt"],
-                             rho_dzt_tau=d["rho_dzt"], dtime=case.dt)
-    g.mark(3)
-    k_ms = g.elapsed_ms(2, 3) / reps
+    k_ms = time_launches(g, lambda: g.horz_advect_tracer(
+        "mdfl_sweby", T0, th, uhrho_et=d["uhrho_et"], vhrho_nt=d["vhrho_nt"],
+        wrho_bt=d["wrho_bt"], rho_dzt_tau=d["rho_dzt"], dtime=case.dt))
     kb = mdfl_bytes_per_launch(case.ni, case.nj, case.nk)
     achieved = kb / (k_ms / 1e3) / 1e9
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
                 kernel="k_mdfl (MDFL-Sweby advection, one tracer)",
                 kernel_ms=round(k_ms, 4), bytes_per_launch=kb)
-    print(f"# step {ms:.3f} ms; mdfl launch {k_ms:.4f} ms", file=sys.stderr, flush=True)
+    log(f"step {ms:.3f} ms; mdfl launch {k_ms:.4f} ms")
+    cfg = {"workload": w["name"], "grid": [case.ni, case.nj, case.nk], "tracers": ntr,
+           "schemes": w["schemes"], "barotropic_nts": case.nts, "path": "plain per-tracer FP64",
+           "tuning": {k: g.get_tuning(k) for k in TUNING_KEYS}}
+    return dict(value=value, ms=ms, steps=steps, warmup=warm, dtype="f64", cfg=cfg, roof=roof,
+                extra=dict(autotune=_tune_summary(tuned)), ntr=ntr)
+
+
+def profile_kernels(args, w):
+    """Launch only the kernels whose HBM traffic is measured with
+    rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE), three dispatches each:
+    k_copy of one 3-D FP64 field (the calibration: reads and writes
+    exactly its size) and one pass of each fused tracer-step group."""
+    from mom5_amd import DeviceArray
+    from mom5_amd.synth import make_case_device
+    ni, nj, nk = w["ni"], w["nj"], w["nk"]
+    if args.shape:
+        ni, nj, nk = (int(x) for x in args.shape.split("x"))
+    case = make_case_device(ni, nj, nk, land_frac=w["land_frac"], nts=w["nts"],
+                            n_passive=w["n_passive"], passive_dtype=np.float32)
+    g = case.info["gpu"]
+    f = case.fields
+    kw = dict(uhrho_et=f["uhrho_et"], vhrho_nt=f["vhrho_nt"], wrho_bt=f["wrho_bt"],
+              rho_dzt_tau=f["rho_dzt"], rho_dztr=f["rho_dztr"], dtime=case.dt)
+    src = f["rho_dzt"]
+    dst = DeviceArray(src.shape)
+    for _ in range(3):
+        g.copy(dst, src)
+    g.sync()
+    log(f"k_copy x3: {src.nbytes} bytes read and written per dispatch")
+    trs = [t for _, t in case.tracers]
+    for size, knob in ((8, "step_group_f64"), (4, "step_group_f32")):
+        grp = g.get_tuning(knob)
+        sel = [t for t in trs if t.element_size() == size][:grp]
+        out = [DeviceArray(t.shape, t.dtype) for t in sel]
+        for _ in range(3):
+            g.step_tracers(sel, out, **kw)
+        g.sync()
+        log(f"k_step<{'double' if size == 8 else 'float'},G={grp}> x3: algorithmic "
+            f"{step_kernel_bytes(ni, nj, nk, size, grp)} bytes per dispatch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default 18 for config 3, 20 otherwise)")
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed steps first (default 2 for config 3, 3 otherwise)")
+    ap.add_argument("--config", type=int, default=3, choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-kernels", action="store_true",
+                    help="only launch the kernels measured with rocprofv3 --pmc (mixed workload)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes instead of timing the candidates")
+    ap.add_argument("--shape", default=None,
+                    help="NIxNJxNK: another grid for the mixed workload (quick runs)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU decomposition is a design, not an implementation.", file=sys.stderr)
+        sys.exit(2)
+
+    w = WORKLOADS[args.config]
+    if args.profile_kernels:
+        profile_kernels(args, w)
+        return
+    r = run_mixed(args, w) if w.get("mixed") else run_plain(args, w)
 
     cpu = None
     if not args.no_cpu_baseline:
-        print("# timing the CPU baseline sample", file=sys.stderr, flush=True)
+        log("timing the CPU baseline sample")
         cpu = cpu_baseline(w)
 
-    line = {"metric": "tracer-cell-updates/s", "value": value,
-            "unit": "tracer-cell-updates/s", "n_gpus": 1, "steps": args.steps,
-            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
-            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-            "config": {"workload": w["name"], "grid": [case.ni, case.nj, case.nk],
-                       "tracers": ntr, "schemes": w["schemes"], "barotropic_nts": case.nts,
-                       "path": "plain per-tracer FP64"},
-            "roofline": roof, "cpu_baseline": cpu}
+    line = {"metric": "tracer-cell-updates/s", "value": r["value"],
+            "unit": "tracer-cell-updates/s", "n_gpus": 1, "steps": r["steps"],
+            "warmup": r["warmup"], "ms_per_step": r["ms"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": r["dtype"], "data": "synthetic",
+            "config": r["cfg"], "roofline": r["roof"], "cpu_baseline": cpu}
+    line.update(r["extra"])
     print(json.dumps(line), flush=True)
 
 

@@ -55,7 +55,9 @@ typedef struct {
 
 /* Grid fields (ocean_core/ocean_types.F90:747-920, ocean_grid_type) on the
  * data domain.  tmask/umask are (isd:ied,jsd:jed,nk); dzt is (1:nk).
- * mom_gpu_create() copies what it needs; the caller may free these after. */
+ * mom_gpu_create() copies what it needs; the caller may free these after.
+ * tmask/umask may both be NULL: the masks are then taken as k<=kmt and
+ * k<=kmu (their definition, ocean_grids.F90:1374-1386) without a check. */
 typedef struct {
   const double *dxt, *dyt, *dat, *datr;
   const double *dxte, *dyte, *dxtn, *dytn;
@@ -92,24 +94,31 @@ int mom_gpu_sync(mom_gpu_ctx_t *ctx);
 const char *mom_gpu_last_error(const mom_gpu_ctx_t *ctx);
 
 /* Device memory helpers for hosts without a GPU allocator (the Fortran
- * host keeps the returned addresses as type(c_ptr)). */
+ * host keeps the returned addresses as type(c_ptr)).  The copies first wait
+ * for all work queued on the device, on every context's stream. */
 int mom_gpu_malloc(void **ptr, size_t bytes);
 int mom_gpu_free(void *ptr);
 int mom_gpu_memcpy_htod(void *dst, const void *src, size_t bytes);
 int mom_gpu_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int mom_gpu_memcpy_dtod(void *dst, const void *src, size_t bytes);
 int mom_gpu_memset(void *ptr, int value, size_t bytes);
 /* Fill n doubles with value on the context's stream (a kernel, so it is
  * ordered with the other calls on ctx and may be captured in a graph). */
 int mom_gpu_fill(mom_gpu_ctx_t *ctx, double *ptr, double value, size_t n);
+/* Copy n doubles device to device on the context's stream (a kernel that
+ * reads and writes each byte once; also the reference point for measured
+ * HBM traffic). */
+int mom_gpu_copy(mom_gpu_ctx_t *ctx, double *dst, const double *src, size_t n);
 
-/* Launch-shape knobs, set per context (the autotuner drives these; results
- * do not depend on them).  Knobs:
+/* Launch-shape knobs, set per context (the autotuner, mom5_amd/tune.py,
+ * drives these; results do not depend on them, bit for bit).  Knobs:
  *   "barotropic_subcycles_per_launch"  0: three launches per subcycle;
  *                                      n>0: n subcycles per launch (1..9)
  *   "mdfl_rows_per_wave"               2 or 4: tile height of the MDFL kernel
  *   "step_group_f64"                   1 or 2 FP64 tracers per batched pass
  *   "step_group_f32"                   1, 2 or 4 FP32 tracers per batched pass
- *   "step_rows_per_wave"               2 or 4: tile height of the batched pass
+ *   "step_rows_per_wave"               2 or 4: tile height of the FP64 passes
+ *   "step_rows_per_wave_f32"           2 or 4: tile height of the FP32 passes
  * Unknown knobs and out-of-range values return MOM_GPU_ERR_ARG. */
 int mom_gpu_set_tuning(mom_gpu_ctx_t *ctx, const char *knob, int value);
 int mom_gpu_get_tuning(const mom_gpu_ctx_t *ctx, const char *knob, int *value);
@@ -263,6 +272,20 @@ int mom_gpu_step_tracers(mom_gpu_ctx_t *ctx, const mom_gpu_tracer_step_t *s);
  * for the stream and writes the result to a HOST double. */
 int mom_gpu_tracer_content(mom_gpu_ctx_t *ctx, const double *T,
     const double *rho_dzt, double *content);
+
+/* The same sum for a tracer stored in FP32 (each value widened to FP64,
+ * products and sums in FP64 with the same compensation). */
+int mom_gpu_tracer_content_f32(mom_gpu_ctx_t *ctx, const float *T,
+    const double *rho_dzt, double *content);
+
+/* Error of a tracer a (dtype 8: FP64, 4: FP32) against an FP64 result b
+ * over the wet compute cells (k <= kmt).  No reference counterpart: the
+ * report of each result's error against an FP64 run.  Writes 7 doubles to
+ * the HOST array out: max|a-b|, max|b|, sum (a-b)^2, sum b^2, number of
+ * cells, number of cells whose FP64 bit patterns differ, and
+ * max |a-b|/|b| over cells with |b| >= rel_floor.  Waits for the stream. */
+int mom_gpu_tracer_error(mom_gpu_ctx_t *ctx, const void *a, int dtype,
+    const double *b, double rel_floor, double *out);
 
 #ifdef __cplusplus
 }

@@ -56,6 +56,8 @@ _PROTOS = {
     "mom_gpu_free": (C.c_int, [_vp]),
     "mom_gpu_memcpy_htod": (C.c_int, [_vp, _vp, C.c_size_t]),
     "mom_gpu_memcpy_dtoh": (C.c_int, [_vp, _vp, C.c_size_t]),
+    "mom_gpu_memcpy_dtod": (C.c_int, [_vp, _vp, C.c_size_t]),
+    "mom_gpu_copy": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
     "mom_gpu_memset": (C.c_int, [_vp, C.c_int, C.c_size_t]),
     "mom_gpu_fill": (C.c_int, [_vp, _vp, C.c_double, C.c_size_t]),
     "mom_gpu_set_tuning": (C.c_int, [_vp, C.c_char_p, C.c_int]),
@@ -80,6 +82,20 @@ _PROTOS = {
                                                   _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                                   _dp]),
     "mom_gpu_step_tracers": (C.c_int, [_vp, C.POINTER(TracerStep)]),
+    "mom_synth_tracer": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_ulonglong, _vp]),
+    "mom_synth_loguniform": (C.c_int, [_vp, _vp, C.c_int, C.c_ulonglong, C.c_double,
+                                       C.c_double]),
+    "mom_synth_profile": (C.c_int, [_vp, _vp, _vp, C.c_int]),
+    "mom_synth_transports_unit": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_double, _vp, _vp,
+                                            _vp]),
+    "mom_synth_maxabs": (C.c_int, [_vp, _vp, C.c_size_t, C.POINTER(C.c_double)]),
+    "mom_synth_transports_combine": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double]),
+    "mom_synth_cfl": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_double, _vp,
+                                C.POINTER(C.c_double)]),
+    "mom_synth_transports_finish": (C.c_int, [_vp, _vp, _vp, C.c_double, _vp]),
+    "mom_gpu_tracer_error": (C.c_int, [_vp, _vp, C.c_int, _vp, C.c_double,
+                                       C.POINTER(C.c_double)]),
+    "mom_gpu_tracer_content_f32": (C.c_int, [_vp, _vp, _vp, C.POINTER(C.c_double)]),
     "mom_gpu_tracer_content": (C.c_int, [_vp, _dp, _dp, C.POINTER(C.c_double)]),
 }
 

@@ -3,6 +3,8 @@
 // the masked tracer content and the grid mask check.
 #include "mom_internal.h"
 
+#include <vector>
+
 namespace {
 
 constexpr double EPS_TRI = 1.0e-30;   // ocean_util.F90:175
@@ -235,7 +237,8 @@ __device__ inline DD dd_add2(DD a, DD b) {
 }
 
 constexpr int RB = 256;
-__global__ __launch_bounds__(RB) void k_content(KDom d, const double *__restrict__ T,
+template <typename TT>
+__global__ __launch_bounds__(RB) void k_content(KDom d, const TT *__restrict__ T,
                                                 const double *__restrict__ rho,
                                                 const double *__restrict__ dat,
                                                 const int *__restrict__ kmt,
@@ -250,7 +253,7 @@ __global__ __launch_bounds__(RB) void k_content(KDom d, const double *__restrict
     const double a = dat[o];
     for (int k = 1; k <= km && k <= d.nk; ++k) {
       const int ok = o + d.P2 * (k - 1);
-      acc = dd_add(acc, T[ok] * (rho[ok] * a * 1.0));
+      acc = dd_add(acc, (double)T[ok] * (rho[ok] * a * 1.0));
     }
   }
   sh[threadIdx.x] = acc.hi;
@@ -291,6 +294,49 @@ __global__ __launch_bounds__(RB) void k_content_final(const double *__restrict__
   if (threadIdx.x == 0) out[0] = sh[0] + sl[0];
 }
 
+// ---- error of a tracer against an FP64 reference (wet compute cells) -------
+// Per block: max|a-b|, max|b|, sum (a-b)^2, sum b^2, cells, cells whose bits
+// differ, max |a-b|/|b| where |b| >= floor.
+constexpr int NERR = 7;
+template <typename TT>
+__global__ __launch_bounds__(RB) void k_tracer_error(KDom d, const TT *__restrict__ a,
+                                                     const double *__restrict__ b,
+                                                     const int *__restrict__ kmt, double floor,
+                                                     double *__restrict__ part) {
+  __shared__ double sh[NERR][RB];
+  double v[NERR] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const int ncol = d.ni * d.nj;
+  for (int c = blockIdx.x * RB + threadIdx.x; c < ncol; c += gridDim.x * RB) {
+    const int gi = c % d.ni, gj = c / d.ni;
+    const int o = d2(d, gi, gj);
+    const int km = kmt[o] < d.nk ? kmt[o] : d.nk;
+    for (int k = 1; k <= km; ++k) {
+      const int ok = o + d.P2 * (k - 1);
+      const double x = (double)a[ok], y = b[ok];
+      const double e = fabs(x - y), ay = fabs(y);
+      v[0] = fmax(v[0], e);
+      v[1] = fmax(v[1], ay);
+      v[2] += e * e;
+      v[3] += y * y;
+      v[4] += 1.0;
+      v[5] += __double_as_longlong(x) != __double_as_longlong(y) ? 1.0 : 0.0;
+      if (ay >= floor && ay > 0.0) v[6] = fmax(v[6], e / ay);
+    }
+  }
+  for (int q = 0; q < NERR; ++q) sh[q][threadIdx.x] = v[q];
+  __syncthreads();
+  for (int s = RB / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      for (int q = 0; q < NERR; ++q) {
+        const double x = sh[q][threadIdx.x], y = sh[q][threadIdx.x + s];
+        sh[q][threadIdx.x] = (q == 0 || q == 1 || q == 6) ? fmax(x, y) : x + y;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < NERR) part[NERR * blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
+}
+
 // ---- grid mask consistency: tmask == (k<=kmt), umask == (k<=kmu) ------------
 // One count per data-domain column; the host adds them up.
 __global__ void k_check_masks(KDom d, const double *__restrict__ tmask,
@@ -315,7 +361,22 @@ __global__ void k_fill(double *__restrict__ p, double v, size_t n) {
     p[i] = v;
 }
 
+__global__ void k_copy(double *__restrict__ dst, const double *__restrict__ src, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 }  // namespace
+
+int launch_copy(mom_gpu_ctx *ctx, double *dst, const double *src, size_t n) {
+  if (n == 0) return MOM_GPU_OK;
+  const size_t nb = (n + 255) / 256;
+  const unsigned grid = (unsigned)(nb < 65536 ? nb : 65536);
+  hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, ctx->stream, dst, src, n);
+  MOM_LAUNCHED(ctx, "k_copy");
+  return MOM_GPU_OK;
+}
 
 int launch_fill(mom_gpu_ctx *ctx, double *p, double v, size_t n) {
   if (n == 0) return MOM_GPU_OK;
@@ -427,11 +488,39 @@ int launch_explicit(mom_gpu_ctx *ctx, const double *Tm1, const double *th, const
   return MOM_GPU_OK;
 }
 
-int launch_content(mom_gpu_ctx *ctx, const double *T, const double *rho, double *out) {
+int launch_tracer_error(mom_gpu_ctx *ctx, const void *a, int dtype, const double *b,
+                        double floor, double *out) {
   const KDom &d = ctx->kd;
   const int nb = ctx->red_n;
-  hipLaunchKernelGGL(k_content, dim3(nb), dim3(RB), 0, ctx->stream, d, T, rho, ctx->dat,
-                     ctx->kmt, ctx->red);
+  if (dtype == 4)
+    hipLaunchKernelGGL(k_tracer_error<float>, dim3(nb), dim3(RB), 0, ctx->stream, d,
+                       (const float *)a, b, ctx->kmt, floor, ctx->red);
+  else
+    hipLaunchKernelGGL(k_tracer_error<double>, dim3(nb), dim3(RB), 0, ctx->stream, d,
+                       (const double *)a, b, ctx->kmt, floor, ctx->red);
+  MOM_LAUNCHED(ctx, "k_tracer_error");
+  std::vector<double> part((size_t)NERR * nb);
+  MOM_HIP(ctx, hipMemcpyAsync(part.data(), ctx->red, sizeof(double) * part.size(),
+                              hipMemcpyDeviceToHost, ctx->stream));
+  MOM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (int q = 0; q < NERR; ++q) out[q] = 0.0;
+  for (int blk = 0; blk < nb; ++blk)
+    for (int q = 0; q < NERR; ++q) {
+      const double x = part[(size_t)NERR * blk + q];
+      out[q] = (q == 0 || q == 1 || q == 6) ? (x > out[q] ? x : out[q]) : out[q] + x;
+    }
+  return MOM_GPU_OK;
+}
+
+int launch_content(mom_gpu_ctx *ctx, const void *T, int dtype, const double *rho, double *out) {
+  const KDom &d = ctx->kd;
+  const int nb = ctx->red_n;
+  if (dtype == 4)
+    hipLaunchKernelGGL(k_content<float>, dim3(nb), dim3(RB), 0, ctx->stream, d,
+                       (const float *)T, rho, ctx->dat, ctx->kmt, ctx->red);
+  else
+    hipLaunchKernelGGL(k_content<double>, dim3(nb), dim3(RB), 0, ctx->stream, d,
+                       (const double *)T, rho, ctx->dat, ctx->kmt, ctx->red);
   MOM_LAUNCHED(ctx, "k_content");
   hipLaunchKernelGGL(k_content_final, dim3(1), dim3(RB), 0, ctx->stream, ctx->red, nb,
                      ctx->red + 2 * nb);

@@ -61,6 +61,18 @@ __device__ inline double flux_exact(const FaceC<double> &c, double Rjm, double R
          ma * mb;
 }
 
+// FP32 groups: the same coefficients in single precision, with the
+// hardware reciprocal (about 1 ulp) in place of the divisions
+__device__ inline float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline FaceC<float> face_coef_f(float massflux, float cfl) {
+  FaceC<float> c;
+  c.M = massflux;
+  c.d0 = (2.0f - cfl) * (1.0f - cfl) * (float)ONESIXTH;
+  c.d1 = (1.0f - cfl * cfl) * (float)ONESIXTH;
+  c.kk = (1.0f - cfl) * rcpf(1.0e-30f + cfl);
+  return c;
+}
+
 // division-free limited flux (FP32)
 __device__ inline float psi_r(const FaceC<float> &c, float a, float r) {
   const float lin = c.d0 * a + c.d1 * r;
@@ -152,24 +164,26 @@ __global__ __launch_bounds__(RX * NW) void k_step(
   for (int k = 1; k <= nk; ++k) {
     const int k0 = k - 1;
     const int kp1 = min(k + 1, nk), kp2 = min(k + 2, nk), km1 = max(k - 1, 1);
-    double rhov[RPW], wv[RPW];
+    double rhov[RPW], wv[RPW], xf64v[RPW];
+    float rinv[RPW];   // FP32 groups: about 1/rho_dzt
     TT tz[G][RPW], fbtv[G][RPW], txv[G][RPW], fyv[G][RPW];
+    TT dxf[G][RPW];    // FP32 groups: x flux difference of the cell
     TT mfe[RPW], mfw[RPW], xfac[RPW];
 
     // ---- Z ----------------------------------------------------------------
 #pragma unroll
     for (int q = 0; q < RPW; ++q) {
-      rhov[q] = 1.0; wv[q] = 0.0;
+      rhov[q] = 1.0; wv[q] = 0.0; rinv[q] = 1.0f;
       if (ok[q]) {
         const int o = offz[q];
         const double w = W[o + P2 * k];
         const double r = R[o + P2 * k0];
         rhov[q] = r; wv[q] = w;
-        const FaceC<double> cz = face_coef(datv[q] * w, fabs(w * dtime / r));
-        const double dtr = dtime / r;
         const TT mkm1 = mk<TT>(km1, km[q]), mk0 = mk<TT>(k, km[q]);
         const TT mkp1 = mk<TT>(kp1, km[q]), mkp2 = mk<TT>(kp2, km[q]);
         if constexpr (EXACT) {
+          const FaceC<double> cz = face_coef(datv[q] * w, fabs(w * dtime / r));
+          const double dtr = dtime / r;
           const double wdiff = wkm1[q] - w;
 #pragma unroll
           for (int g = 0; g < G; ++g) {
@@ -181,8 +195,12 @@ __global__ __launch_bounds__(RX * NW) void k_step(
             fbtv[g][q] = fbt;
           }
         } else {
-          const FaceC<float> c{(float)cz.M, (float)cz.d0, (float)cz.d1, (float)cz.kk};
-          const float dtrf = (float)dtr, datrf = (float)datrv[q], wdiff = (float)(wkm1[q] - w);
+          const float ri = rcpf((float)r);
+          rinv[q] = ri;
+          const float wf = (float)w;
+          const FaceC<float> c = face_coef_f((float)(datv[q] * w), fabsf(wf * (float)dtime * ri));
+          const float dtrf = (float)dtime * ri, datrf = (float)datrv[q];
+          const float wdiff = (float)(wkm1[q] - w);
 #pragma unroll
           for (int g = 0; g < G; ++g) {
             const float Rjp = (t0[g][q] - t1[g][q]) * mkm1 * mk0;
@@ -208,22 +226,35 @@ __global__ __launch_bounds__(RX * NW) void k_step(
       const int kmp2 = __shfl(km[q], lane + 2);
       const bool face = rowok[q] && lane >= 1 && lane <= RX - 3 && gi >= -1 && gi <= d.ni - 1;
       FaceC<double> cx{0.0, 0.0, 0.0, 0.0};
+      FaceC<float> cxf{0.0f, 0.0f, 0.0f, 0.0f};
       if (face) {
         const int of = d2(d, gi, sjr[q]);
         const double u = U[of + P2 * k0];
         const double rw = R[of + P2 * k0], re = R[of + 1 + P2 * k0];
-        cx = face_coef(dyte[of] * u, fabs(u * dtime * 2.0 / ((rw + re) * dxte[of])));
+        if constexpr (EXACT)
+          cx = face_coef(dyte[of] * u, fabs(u * dtime * 2.0 / ((rw + re) * dxte[of])));
+        else
+          cxf = face_coef_f((float)(dyte[of] * u),
+                            fabsf((float)u * (float)(dtime * 2.0) * rcpf((float)((rw + re) * dxte[of]))));
       }
       const TT mi = mk<TT>(k, km[q]), mim1 = mk<TT>(k, kmm1);
       const TT mip1 = mk<TT>(k, kmp1), mip2 = mk<TT>(k, kmp2);
-      const double mw64 = __shfl(cx.M, lane - 1);
       const bool cell = rowok[q] && lane >= 2 && lane < RX - 2 && gi < d.ni;
-      // dtime*tmask*datr/rho (x and y sweeps)
-      const double xf64 = dtime * (double)mi * datrv[q] / rhov[q];
-      mfe[q] = (TT)cx.M;
-      mfw[q] = (TT)mw64;
-      xfac[q] = (TT)xf64;
-      const FaceC<float> cxf{(float)cx.M, (float)cx.d0, (float)cx.d1, (float)cx.kk};
+      double mw64 = 0.0;
+      if constexpr (EXACT) {
+        mw64 = __shfl(cx.M, lane - 1);
+        // dtime*tmask*datr/rho (x and y sweeps)
+        const double xf64 = dtime * (double)mi * datrv[q] / rhov[q];
+        xf64v[q] = xf64;
+        mfe[q] = (TT)cx.M;
+        mfw[q] = (TT)mw64;
+        xfac[q] = (TT)xf64;
+      } else {
+        xf64v[q] = 0.0;
+        mfe[q] = cxf.M;
+        mfw[q] = __shfl(cxf.M, lane - 1);
+        xfac[q] = (float)dtime * mi * (float)datrv[q] * rinv[q];
+      }
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         const TT tzm1 = __shfl(tz[g][q], lane - 1);
@@ -240,10 +271,11 @@ __global__ __launch_bounds__(RX * NW) void k_step(
             fx = flux_fast(cxf, Rjm, Rj, Rjp, tz[g][q], tzp1, mi, mip1);
         }
         const TT fxw = __shfl(fx, lane - 1);
+        dxf[g][q] = fxw - fx;
         TT tx = TT(0);
         if (cell) {
           if constexpr (EXACT)
-            tx = tz[g][q] + xf64 * (fxw - fx + t1[g][q] * (cx.M - mw64));
+            tx = tz[g][q] + xf64v[q] * (fxw - fx + t1[g][q] * (cx.M - mw64));
           else
             tx = tz[g][q] + xfac[q] * (fxw - fx + t1[g][q] * (mfe[q] - mfw[q]));
         }
@@ -261,16 +293,20 @@ __global__ __launch_bounds__(RX * NW) void k_step(
       const bool face = ry >= 1 && ry <= RY - 3 && lane >= 2 && lane < RX - 2 && gi < d.ni &&
                         gj >= -1 && gj <= d.nj - 1;
       FaceC<double> cy{0.0, 0.0, 0.0, 0.0};
+      FaceC<float> cyf{0.0f, 0.0f, 0.0f, 0.0f};
       if (face) {
         const int of = d2(d, gi, gj);
         const double v = V[of + P2 * k0];
         const double rs = R[of + P2 * k0], rn = R[of + d.NI + P2 * k0];
-        cy = face_coef(dxtn[of] * v, fabs(v * dtime * 2.0 / ((rs + rn) * dytn[of])));
+        if constexpr (EXACT)
+          cy = face_coef(dxtn[of] * v, fabs(v * dtime * 2.0 / ((rs + rn) * dytn[of])));
+        else
+          cyf = face_coef_f((float)(dxtn[of] * v),
+                            fabsf((float)v * (float)(dtime * 2.0) * rcpf((float)((rs + rn) * dytn[of]))));
       }
       const TT mjm1 = mk<TT>(k, s_km[max(ry - 1, 0)][lane]), mj = mk<TT>(k, s_km[ry][lane]);
       const TT mjp1 = mk<TT>(k, s_km[min(ry + 1, RY - 1)][lane]);
       const TT mjp2 = mk<TT>(k, s_km[min(ry + 2, RY - 1)][lane]);
-      const FaceC<float> cyf{(float)cy.M, (float)cy.d0, (float)cy.d1, (float)cy.kk};
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         TT fy = TT(0);
@@ -302,26 +338,34 @@ __global__ __launch_bounds__(RX * NW) void k_step(
         const int o = offz[q] + P2 * k0;
         const double r = rhov[q];
         const double mi = mk<double>(k, km[q]);
-        const double xf64 = dtime * mi * datrv[q] / r;
         const double brk = wv[q] - wkm1[q] + datrv[q] * ((double)mfw[q] - (double)mfe[q]);
         const double rm1 = Rm1[o];
         const double rr = Rr ? Rr[o] : 1.0 / (R[o] + EPSLN);
+        // FP32: T(taup1) = T*(rho_taum1/rho_taup1) + dtime*datr/rho_taup1
+        //                  * (sum of the face flux differences) + th*dtime/rho_taup1.
+        // The update is rebuilt from the limited fluxes alone (flux form); the
+        // T*div terms that the sweeps add for the limiter and the reference
+        // removes at the end cancel in exact arithmetic but not in FP32.  The
+        // factors are formed in FP64 without division, so every cell's content
+        // weight rho*dat*factor is dtime to 1e-16 and the fluxes telescope:
+        // what is left is the unbiased rounding of the FP32 result.
+        const float a32 = (float)(rm1 * rr), c32 = (float)(dtime * rr);
+        const double f64 = dtime * rr * datrv[q] * mi;
 #pragma unroll
         for (int g = 0; g < G; ++g) {
           const double thin = tab.th[g] ? tab.th[g][o] : 0.0;
           if constexpr (EXACT) {
             const double Tc = t1[g][q];
-            double tr = txv[g][q] + xf64 * (s_fy[g][ry - 1][lane] - fyv[g][q]);
+            double tr = txv[g][q] + xf64v[q] * (s_fy[g][ry - 1][lane] - fyv[g][q]);
             tr = tr + dtime * Tc / r * brk;
             const double adv = -(r * (tr - Tc) / dtime * mi);
             const double th = thin + (-adv);
             tab.out[g][o] = (rm1 * Tc + dtime * th) * rr;
           } else {
             const float Tc = t1[g][q];
-            float tr = txv[g][q] + xfac[q] * (s_fy[g][ry - 1][lane] - fyv[g][q]);
-            tr = tr + (float)(dtime / r) * Tc * (float)brk;
-            const float th = (float)thin + (float)(r / dtime) * (tr - Tc) * (float)mi;
-            tab.out[g][o] = ((float)rm1 * Tc + (float)dtime * th) * (float)rr;
+            const float dflux = (fbtv[g][q] - ftp[g][q]) + dxf[g][q] +
+                                (s_fy[g][ry - 1][lane] - fyv[g][q]);
+            tab.out[g][o] = Tc * a32 + (float)(f64 * (double)dflux) + (float)thin * c32;
           }
         }
       }
@@ -414,7 +458,7 @@ int launch_tracer_step(mom_gpu_ctx *ctx, int ntr, const void *const *T_in, void 
                        const double *rho_taum1, const double *rho_dztr, double dtime) {
   launch_all<double>(ctx, ctx->tune.step_group_f64, ctx->tune.step_rows_per_wave, ntr, T_in,
                      T_out, dtype, th_in, 8, u, v, w, rho, rho_taum1, rho_dztr, dtime);
-  launch_all<float>(ctx, ctx->tune.step_group_f32, ctx->tune.step_rows_per_wave, ntr, T_in,
+  launch_all<float>(ctx, ctx->tune.step_group_f32, ctx->tune.step_rows_per_wave_f32, ntr, T_in,
                     T_out, dtype, th_in, 4, u, v, w, rho, rho_taum1, rho_dztr, dtime);
   MOM_LAUNCHED(ctx, "k_step");
   return MOM_GPU_OK;

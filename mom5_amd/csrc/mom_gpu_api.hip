@@ -41,15 +41,27 @@ int mom_gpu_free(void *ptr) {
   return MOM_GPU_OK;
 }
 
+// The copies wait for all work on the device first: contexts launch on
+// their own non-blocking streams, which the null stream does not wait for.
 int mom_gpu_memcpy_htod(void *dst, const void *src, size_t bytes) {
-  hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+  hipError_t e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
   if (e != hipSuccess) return mom_fail(nullptr, MOM_GPU_ERR_HIP, "hipMemcpy H2D: %s", hipGetErrorString(e));
   return MOM_GPU_OK;
 }
 
 int mom_gpu_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
-  hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+  hipError_t e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
   if (e != hipSuccess) return mom_fail(nullptr, MOM_GPU_ERR_HIP, "hipMemcpy D2H: %s", hipGetErrorString(e));
+  return MOM_GPU_OK;
+}
+
+int mom_gpu_memcpy_dtod(void *dst, const void *src, size_t bytes) {
+  hipError_t e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) return mom_fail(nullptr, MOM_GPU_ERR_HIP, "hipMemcpy D2D: %s", hipGetErrorString(e));
   return MOM_GPU_OK;
 }
 
@@ -84,6 +96,10 @@ static int *tuning_slot(mom_gpu_ctx *ctx, const char *knob, int *lo, int *hi) {
     *lo = 2; *hi = 4;
     return &ctx->tune.step_rows_per_wave;
   }
+  if (!std::strcmp(knob, "step_rows_per_wave_f32")) {
+    *lo = 2; *hi = 4;
+    return &ctx->tune.step_rows_per_wave_f32;
+  }
   return nullptr;
 }
 
@@ -94,7 +110,8 @@ int mom_gpu_set_tuning(mom_gpu_ctx_t *ctx, const char *knob, int value) {
   REQUIRE(ctx, slot, "mom_gpu_set_tuning: unknown knob '%s'", knob ? knob : "(null)");
   REQUIRE(ctx, value >= lo && value <= hi, "mom_gpu_set_tuning: %s=%d outside [%d,%d]", knob,
           value, lo, hi);
-  REQUIRE(ctx, (slot != &ctx->tune.mdfl_rows_per_wave && slot != &ctx->tune.step_rows_per_wave) ||
+  REQUIRE(ctx, (slot != &ctx->tune.mdfl_rows_per_wave && slot != &ctx->tune.step_rows_per_wave &&
+                slot != &ctx->tune.step_rows_per_wave_f32) ||
                    value == 2 || value == 4,
           "mom_gpu_set_tuning: %s must be 2 or 4", knob);
   REQUIRE(ctx, slot != &ctx->tune.step_group_f32 || value != 3,
@@ -117,6 +134,12 @@ int mom_gpu_fill(mom_gpu_ctx_t *ctx, double *ptr, double value, size_t n) {
   REQUIRE(nullptr, ctx, "mom_gpu_fill: NULL context");
   REQUIRE(ctx, ptr || n == 0, "mom_gpu_fill: NULL pointer");
   return launch_fill(ctx, ptr, value, n);
+}
+
+int mom_gpu_copy(mom_gpu_ctx_t *ctx, double *dst, const double *src, size_t n) {
+  REQUIRE(nullptr, ctx, "mom_gpu_copy: NULL context");
+  REQUIRE(ctx, (dst && src) || n == 0, "mom_gpu_copy: NULL pointer");
+  return launch_copy(ctx, dst, src, n);
 }
 
 static void free_ctx(mom_gpu_ctx *c) {
@@ -150,7 +173,7 @@ int mom_gpu_create(const mom_gpu_domain_t *dom, const mom_gpu_grid_t *grid, mom_
     return mom_fail(nullptr, MOM_GPU_ERR_ARG, "mom_gpu_create: domain too large for 32-bit offsets");
   const double *need[] = {grid->dxt, grid->dyt, grid->dat, grid->datr, grid->dxte, grid->dyte,
                           grid->dxtn, grid->dytn, grid->dxu, grid->dyu, grid->dxur, grid->dyur,
-                          grid->f, grid->tmask, grid->umask, grid->dzt};
+                          grid->f, grid->dzt};
   for (const double *p : need)
     if (!p) return mom_fail(nullptr, MOM_GPU_ERR_ARG, "mom_gpu_create: a grid field is NULL");
   if (!grid->kmt || !grid->kmu) return mom_fail(nullptr, MOM_GPU_ERR_ARG, "mom_gpu_create: kmt/kmu NULL");
@@ -166,9 +189,10 @@ int mom_gpu_create(const mom_gpu_domain_t *dom, const mom_gpu_grid_t *grid, mom_
   k.cyclic_y = dom->cyclic_y ? 1 : 0;
   c->tune.bt_subcycles_per_launch = 4;
   c->tune.mdfl_rows_per_wave = 4;
-  c->tune.step_group_f64 = 1;
+  c->tune.step_group_f64 = 2;
   c->tune.step_group_f32 = 2;
-  c->tune.step_rows_per_wave = 4;
+  c->tune.step_rows_per_wave = 2;
+  c->tune.step_rows_per_wave_f32 = 2;
   const size_t P2 = (size_t)k.P2, nk = (size_t)k.nk;
 
 #define CK(call)                                                                      \
@@ -206,11 +230,12 @@ int mom_gpu_create(const mom_gpu_domain_t *dom, const mom_gpu_grid_t *grid, mom_
   CK(hipMalloc(&c->bt, 9 * P2 * sizeof(double)));
   CK(hipMalloc(&c->tri_e, P2 * nk * sizeof(double)));
   c->red_n = 1024;
-  CK(hipMalloc(&c->red, (2 * (size_t)c->red_n + 1) * sizeof(double)));
+  CK(hipMalloc(&c->red, (8 * (size_t)c->red_n + 8) * sizeof(double)));
 #undef CK
 
   long long bad = 0;
-  int rc = launch_check_masks(c, grid->tmask, grid->umask, &bad);
+  int rc = MOM_GPU_OK;
+  if (grid->tmask && grid->umask) rc = launch_check_masks(c, grid->tmask, grid->umask, &bad);
   if (rc == MOM_GPU_OK && bad != 0)
     rc = mom_fail(c, MOM_GPU_ERR_GRID,
                   "mom_gpu_create: %lld mask entries disagree with kmt/kmu "
@@ -447,7 +472,22 @@ int mom_gpu_tracer_content(mom_gpu_ctx_t *ctx, const double *T, const double *rh
                            double *content) {
   REQUIRE(nullptr, ctx, "mom_gpu_tracer_content: NULL context");
   REQUIRE(ctx, T && rho_dzt && content, "mom_gpu_tracer_content: NULL argument");
-  return launch_content(ctx, T, rho_dzt, content);
+  return launch_content(ctx, T, 8, rho_dzt, content);
+}
+
+int mom_gpu_tracer_content_f32(mom_gpu_ctx_t *ctx, const float *T, const double *rho_dzt,
+                               double *content) {
+  REQUIRE(nullptr, ctx, "mom_gpu_tracer_content_f32: NULL context");
+  REQUIRE(ctx, T && rho_dzt && content, "mom_gpu_tracer_content_f32: NULL argument");
+  return launch_content(ctx, T, 4, rho_dzt, content);
+}
+
+int mom_gpu_tracer_error(mom_gpu_ctx_t *ctx, const void *a, int dtype, const double *b,
+                         double rel_floor, double *out) {
+  REQUIRE(nullptr, ctx, "mom_gpu_tracer_error: NULL context");
+  REQUIRE(ctx, a && b && out, "mom_gpu_tracer_error: NULL argument");
+  REQUIRE(ctx, dtype == 8 || dtype == 4, "mom_gpu_tracer_error: dtype must be 8 or 4");
+  return launch_tracer_error(ctx, a, dtype, b, rel_floor, out);
 }
 
 }  // extern "C"

@@ -69,7 +69,8 @@ struct mom_gpu_ctx {
     int mdfl_rows_per_wave;        // 2 or 4 region rows per wavefront
     int step_group_f64;            // FP64 tracers per pass of the batched step (1, 2)
     int step_group_f32;            // FP32 tracers per pass (1, 2, 4)
-    int step_rows_per_wave;        // 2 or 4
+    int step_rows_per_wave;        // 2 or 4 (FP64 groups)
+    int step_rows_per_wave_f32;    // 2 or 4 (FP32 groups)
   } tune;
   char err[512];
 };
@@ -145,8 +146,10 @@ int launch_barotropic_blocked(mom_gpu_ctx *ctx, const mom_gpu_bt_params_t *p,
                               const double *source, const double *eta_smooth,
                               const double *rho_g, double *eta_bar, double *ud, double *conv,
                               int hsub);
-int launch_content(mom_gpu_ctx *ctx, const double *T, const double *rho,
-                   double *out);
+int launch_content(mom_gpu_ctx *ctx, const void *T, int dtype, const double *rho, double *out);
+int launch_tracer_error(mom_gpu_ctx *ctx, const void *a, int dtype, const double *b,
+                        double floor, double *out);
 int launch_fill(mom_gpu_ctx *ctx, double *p, double v, size_t n);
+int launch_copy(mom_gpu_ctx *ctx, double *dst, const double *src, size_t n);
 int launch_check_masks(mom_gpu_ctx *ctx, const double *tmask,
                        const double *umask, long long *bad);

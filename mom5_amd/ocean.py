@@ -66,6 +66,15 @@ class DeviceArray:
     def zero(self):
         check(load().mom_gpu_memset(self.ptr, 0, self.nbytes), None, "memset")
 
+    def copy(self):
+        """A new device array with the same contents."""
+        d = DeviceArray(self.shape, self.dtype)
+        check(load().mom_gpu_memcpy_dtod(d.ptr, self.ptr, self.nbytes), None, "D2D")
+        return d
+
+    def element_size(self):
+        return self.dtype.itemsize
+
     def free(self):
         if getattr(self, "ptr", None):
             load().mom_gpu_free(self.ptr)
@@ -86,6 +95,10 @@ def _p(a):
     return int(a)
 
 
+def _elem_size(a):
+    return int(np.dtype(a.dtype).itemsize)
+
+
 def _ptab(arrs):
     tab = (C.c_void_p * max(len(arrs), 1))()
     for n, a in enumerate(arrs):
@@ -100,6 +113,8 @@ class OceanGPU:
     cyclic_y); ``grid`` the host arrays of ocean_grid_type fields named as in
     MOM (dxt, dyt, dat, datr, dxte, dyte, dxtn, dytn, dxu, dyu, dxur, dyur,
     f, kmt, kmu, tmask, umask, dzt).  The grid is copied to the device.
+    Entries may also be :class:`DeviceArray` (read in place), and tmask/umask
+    may be left out: the masks are then k <= kmt and k <= kmu.
     """
 
     GRID_FIELDS = ("dxt", "dyt", "dat", "datr", "dxte", "dyte", "dxtn", "dytn", "dxu", "dyu",
@@ -113,11 +128,18 @@ class OceanGPU:
         self.nk = dom["nk"]
         self.shape2 = (self.nj + 2, self.ni + 2)
         self.shape3 = (self.nk,) + self.shape2
-        staged = {}
+        staged, ptrs = {}, {}
         for name in self.GRID_FIELDS:
-            dt = np.int32 if name in ("kmt", "kmu") else np.float64
-            staged[name] = DeviceArray.from_host(np.asarray(grid[name], dtype=dt))
-        g = Grid(**{k: v.ptr for k, v in staged.items()})
+            a = grid.get(name)
+            if a is None and name in ("tmask", "umask"):
+                ptrs[name] = None
+            elif isinstance(a, DeviceArray):
+                ptrs[name] = a.ptr
+            else:
+                dt = np.int32 if name in ("kmt", "kmu") else np.float64
+                staged[name] = DeviceArray.from_host(np.asarray(a, dtype=dt))
+                ptrs[name] = staged[name].ptr
+        g = Grid(**ptrs)
         self._dom_c = Domain(*[int(dom[n]) for n, _ in Domain._fields_])
         ctx = C.c_void_p()
         check(self.L.mom_gpu_create(C.byref(self._dom_c), C.byref(g), C.byref(ctx)), None,
@@ -166,6 +188,12 @@ class OceanGPU:
         """Fill a float64 device array on the context's stream."""
         n = arr.nbytes // 8
         self._chk(self.L.mom_gpu_fill(self.ctx, _p(arr), float(value), n), "fill")
+
+    def copy(self, dst, src):
+        """Copy a float64 device array into another on the context's stream."""
+        n = src.nbytes // 8
+        assert dst.nbytes >= src.nbytes
+        self._chk(self.L.mom_gpu_copy(self.ctx, _p(dst), _p(src), n), "copy")
 
     def field3(self, value=None, dtype=np.float64):
         a = DeviceArray(self.shape3, dtype)
@@ -254,8 +282,7 @@ class OceanGPU:
         """Advection (MDFL-Sweby), explicit update and implicit vertical
         diffusion of every tracer; each tracer's dtype is its array's."""
         n = len(T_taum1s)
-        dt = (C.c_int * max(n, 1))(*[8 if np.dtype(a.dtype) == np.float64 else 4
-                                     for a in T_taum1s])
+        dt = (C.c_int * max(n, 1))(*[_elem_size(a) for a in T_taum1s])
         keep = []
 
         def tab(xs):
@@ -275,10 +302,28 @@ class OceanGPU:
         self._chk(self.L.mom_gpu_step_tracers(self.ctx, C.byref(s)), "step_tracers")
 
     # ---- diagnostics ----------------------------------------------------------
+    def tracer_error(self, a, b, rel_floor=0.0):
+        """Error of tracer a (FP64 or FP32) against the FP64 result b over
+        the wet compute cells: max_abs, max_ref, rms_rel (rms of the error
+        over rms of b), normwise (max_abs / max_ref), cells, cells_differ
+        (FP64 bit patterns), max_rel_cell (where |b| >= rel_floor)."""
+        out = (C.c_double * 7)()
+        self._chk(self.L.mom_gpu_tracer_error(self.ctx, _p(a), _elem_size(a), _p(b),
+                                              float(rel_floor), out), "tracer_error")
+        v = list(out)
+        return dict(max_abs=v[0], max_ref=v[1],
+                    rms_rel=(v[2] / v[3]) ** 0.5 if v[3] > 0 else 0.0,
+                    normwise=v[0] / v[1] if v[1] > 0 else 0.0,
+                    cells=int(v[4]), cells_differ=int(v[5]), max_rel_cell=v[6])
+
+
     def tracer_content(self, T, rho_dzt):
+        """Masked content sum(T*rho_dzt*dat) (compensated FP64 sums) of an
+        FP64 or FP32 tracer."""
         out = C.c_double()
-        self._chk(self.L.mom_gpu_tracer_content(self.ctx, _p(T), _p(rho_dzt), C.byref(out)),
-                  "tracer_content")
+        fn = (self.L.mom_gpu_tracer_content if _elem_size(T) == 8
+              else self.L.mom_gpu_tracer_content_f32)
+        self._chk(fn(self.ctx, _p(T), _p(rho_dzt), C.byref(out)), "tracer_content")
         return out.value
 
 

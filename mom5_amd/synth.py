@@ -98,12 +98,10 @@ def vertical_grid(nk, dz_top=10.0, dz_bot=500.0):
     return dzt, zw, zt, dzw
 
 
-def make_case(ni, nj, nk, *, lat_s=-10.0, lat_n=80.0, lon_span=360.0,
-              cyclic_x=True, land_frac=0.25, seed=0x4D4F4D35, n_passive=0,
-              cfl=0.3, nts=80, barotropic=True, passive_dtype=np.float64,
-              stf_scale=0.0, pme_scale=0.0):
-    """Build one seeded synthetic case (see module docstring)."""
-    rng = np.random.default_rng(seed)
+def _horizontal(rng, ni, nj, nk, lat_s, lat_n, lon_span, cyclic_x, land_frac):
+    """The 2-D part of a case: metrics, bathymetry (kmt, kmu), the unscaled
+    horizontal streamfunction psi and overturning amplitude ov, and the
+    x-face depth index kface with its depth hface."""
     NI, NJ = ni + 2, nj + 2
     dlon = lon_span / ni
     dlat = (lat_n - lat_s) / nj
@@ -149,17 +147,12 @@ def make_case(ni, nj, nk, *, lat_s=-10.0, lat_n=80.0, lon_span=360.0,
                                           kmt[1:, :-1], kmt[1:, 1:]])
     if cyclic_x:
         kmu[:, -1] = kmu[:, 1]
-    kk = np.arange(1, nk + 1)[:, None, None]
-    tmask = (kk <= kmt[None]).astype(np.float64)
-    umask = (kk <= kmu[None]).astype(np.float64)
 
     # horizontal streamfunction on U points (m^2/s per unit depth, unscaled)
     psi_c = _smooth_field(rng, ni, nj, 8, 4, cyclic_x)
     psi = np.zeros((NJ, NI))
     psi[1:-1, 1:-1] = psi_c
     _wrap_x(psi, cyclic_x)
-    decay = np.exp(-zt / 1000.0)
-    psik = psi[None] * decay[:, None, None] * umask          # (nk, NJ, NI)
 
     # overturning streamfunction on x faces, zero at surface and face bottom
     ov_c = _smooth_field(rng, ni, nj, 6, 3, cyclic_x)
@@ -171,6 +164,31 @@ def make_case(ni, nj, nk, *, lat_s=-10.0, lat_n=80.0, lon_span=360.0,
     if cyclic_x:
         kface[:, -1] = kface[:, 1]
     hface = np.where(kface > 0, zw[np.maximum(kface, 1) - 1], 0.0)
+    return dict(NI=NI, NJ=NJ, dxt=dxt, dyt=dyt, dat=dat, datr=datr, dxte=dxte, dyte=dyte,
+                dxtn=dxtn, dytn=dytn, dxu=dxu, dyu=dyu, dxur=dxur, dyur=dyur, f=f,
+                dzt=dzt, zw=zw, zt=zt, dzw=dzw, kmt_c=kmt_c, kmt=kmt, kmu=kmu,
+                psi=psi, ov=ov, kface=kface, hface=hface)
+
+
+def make_case(ni, nj, nk, *, lat_s=-10.0, lat_n=80.0, lon_span=360.0,
+              cyclic_x=True, land_frac=0.25, seed=0x4D4F4D35, n_passive=0,
+              cfl=0.3, nts=80, barotropic=True, passive_dtype=np.float64,
+              stf_scale=0.0, pme_scale=0.0):
+    """Build one seeded synthetic case (see module docstring)."""
+    rng = np.random.default_rng(seed)
+    h = _horizontal(rng, ni, nj, nk, lat_s, lat_n, lon_span, cyclic_x, land_frac)
+    NI, NJ = h["NI"], h["NJ"]
+    dxt, dyt, dat, datr = h["dxt"], h["dyt"], h["dat"], h["datr"]
+    dxte, dyte, dxtn, dytn = h["dxte"], h["dyte"], h["dxtn"], h["dytn"]
+    dxu, dyu, dxur, dyur, f = h["dxu"], h["dyu"], h["dxur"], h["dyur"], h["f"]
+    dzt, zw, zt, dzw = h["dzt"], h["zw"], h["zt"], h["dzw"]
+    kmt_c, kmt, kmu = h["kmt_c"], h["kmt"], h["kmu"]
+    psi, ov, kface, hface = h["psi"], h["ov"], h["kface"], h["hface"]
+    kk = np.arange(1, nk + 1)[:, None, None]
+    tmask = (kk <= kmt[None]).astype(np.float64)
+    umask = (kk <= kmu[None]).astype(np.float64)
+    decay = np.exp(-zt / 1000.0)
+    psik = psi[None] * decay[:, None, None] * umask          # (nk, NJ, NI)
     zint = np.concatenate([[0.0], zw])                          # interfaces 0..nk
     phi = np.zeros((nk + 1, NJ, NI))
     for kk_ in range(nk + 1):
@@ -310,6 +328,171 @@ def make_case(ni, nj, nk, *, lat_s=-10.0, lat_n=80.0, lon_span=360.0,
                 tracers=tr, info=info)
 
 
+def make_case_device(ni, nj, nk, *, lat_s=-10.0, lat_n=80.0, lon_span=360.0,
+                     cyclic_x=True, land_frac=0.25, seed=0x4D4F4D35, n_passive=0, cfl=0.3,
+                     nts=80, barotropic=True, passive_dtype=np.float32, stf_scale=0.0,
+                     pme_scale=0.0):
+    """The case of :func:`make_case` built on the device (``mom5_synth.h``).
+
+    For shapes whose 3-D fields would not fit the host comfortably (the
+    3600 x 2700 x 75 workload: 5.8 GB per FP64 field).  The 2-D part
+    (metrics, bathymetry, streamfunctions) is the same code as
+    :func:`make_case`; transports, ``wrho_bt`` and the CFL scaling repeat its
+    arithmetic on the device, so both agree to the rounding of ``sin``.  The
+    3-D random fields (tracers, diffusivities) come from a counter-based
+    hash instead of numpy's generator, so they differ from :func:`make_case`.
+
+    Returns a :class:`Case` whose 3-D fields and tracers are
+    :class:`~mom5_amd.ocean.DeviceArray` (2-D fields stay on the host) and
+    ``info["gpu"]``, the :class:`~mom5_amd.ocean.OceanGPU` context that built
+    them.  ``info["tracer"](n, dtype)`` regenerates tracer n's initial field
+    (FP64 or FP32), the same values every call.
+    """
+    import ctypes as C
+    from .ocean import DeviceArray, OceanGPU
+    rng = np.random.default_rng(seed)
+    h = _horizontal(rng, ni, nj, nk, lat_s, lat_n, lon_span, cyclic_x, land_frac)
+    NI, NJ = h["NI"], h["NJ"]
+    dzt, zw, zt, dzw = h["dzt"], h["zw"], h["zt"], h["dzw"]
+    kmt, kmu = h["kmt"], h["kmu"]
+    grid = {k: h[k] for k in ("dxt", "dyt", "dat", "datr", "dxte", "dyte", "dxtn", "dytn",
+                              "dxu", "dyu", "dxur", "dyur", "f", "kmt", "kmu", "dzt")}
+    dom = dict(isd=0, ied=ni + 1, jsd=0, jed=nj + 1, isc=1, iec=ni, jsc=1, jec=nj, nk=nk,
+               cyclic_x=int(cyclic_x), cyclic_y=0)
+    gpu = OceanGPU(dom, grid)
+    L, ctx = gpu.L, gpu.ctx
+
+    def chk(rc, what):
+        gpu._chk(rc, what)
+
+    def dptr(a):
+        return C.c_void_p(a.ctypes.data)
+
+    shape3 = (nk, NJ, NI)
+    uh = DeviceArray(shape3)
+    vh = DeviceArray(shape3)
+    wrho = DeviceArray((nk + 1, NJ, NI))
+    mov = DeviceArray(shape3)
+    psi, ov = DeviceArray.from_host(h["psi"]), DeviceArray.from_host(h["ov"])
+    hden = DeviceArray.from_host(np.maximum(h["hface"], 1.0))
+    kface = DeviceArray.from_host(h["kface"].astype(np.int32))
+    decay = np.exp(-zt / 1000.0)
+    zint = np.concatenate([[0.0], zw])
+    coef = RHO0 * dzt
+    prof = np.ascontiguousarray(np.concatenate([coef, decay, zint]))
+    chk(L.mom_synth_transports_unit(ctx, psi.ptr, ov.ptr, hden.ptr, kface.ptr, dptr(prof),
+                                    RHO0, uh.ptr, vh.ptr, mov.ptr), "synth transports")
+    m_u, m_ov = C.c_double(), C.c_double()
+    chk(L.mom_synth_maxabs(ctx, uh.ptr, uh.nbytes // 8, C.byref(m_u)), "synth maxabs")
+    chk(L.mom_synth_maxabs(ctx, mov.ptr, mov.nbytes // 8, C.byref(m_ov)), "synth maxabs")
+    fac = 0.3 * m_u.value / max(m_ov.value, 1e-300)
+    chk(L.mom_synth_transports_combine(ctx, uh.ptr, vh.ptr, mov.ptr, fac), "synth combine")
+    del mov, psi, ov, hden, kface
+
+    # time steps and velocity scale (make_case's rules)
+    hu = np.zeros((NJ, NI))
+    for k in range(nk):
+        hu += dzt[k] * (kmu >= k + 1)
+    wet_u = kmu >= 1
+    if barotropic and wet_u.any():
+        cg = np.sqrt(GRAV * np.maximum(hu, 1.0))
+        dmin = np.minimum(h["dxu"], h["dyu"])
+        dtbt = 0.35 * float(np.min(dmin[wet_u] / cg[wet_u]))
+        dtbt = float(np.floor(dtbt * 64.0) / 64.0)
+        dt = dtbt * nts / 2.0
+    else:
+        dtbt, dt = 0.0, None
+    colmax = DeviceArray((NJ, NI))
+    rdz = np.ascontiguousarray(coef)
+
+    def cfl_max(scale, dtt):
+        out = C.c_double()
+        chk(L.mom_synth_cfl(ctx, uh.ptr, vh.ptr, dptr(rdz), float(scale), float(dtt),
+                            colmax.ptr, C.byref(out)), "synth cfl")
+        return out.value
+
+    top = np.empty((NJ, NI))
+    chk(L.mom_gpu_memcpy_dtoh(dptr(top), uh.ptr, top.nbytes), "D2H")
+    base = 0.2 / max(np.abs(top / h["dyte"] / (RHO0 * dzt[0])).max(), 1e-30)
+    if dt is None:
+        c1 = cfl_max(base, 1.0)
+        dt = cfl / c1
+        scale = base
+    else:
+        c1 = cfl_max(1.0, dt)
+        scale = cfl / c1
+    chk(L.mom_synth_transports_finish(ctx, uh.ptr, vh.ptr, float(scale), wrho.ptr),
+        "synth finish")
+    del colmax
+
+    def profile(values, nlev):
+        a = DeviceArray((nlev, NJ, NI))
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        chk(L.mom_synth_profile(ctx, a.ptr, dptr(v), nlev), "synth profile")
+        return a
+
+    rho_dzt = profile(RHO0 * dzt, nk)
+    rho_dztr = profile(1.0 / (RHO0 * dzt + EPSLN), nk)
+    dzwt = profile(dzw, nk + 1)
+    kinds = [0, 1] + [2] * n_passive           # MOM_SYNTH_TEMP, _SALT, _PASSIVE
+    names = ["temp", "salt"] + [f"passive{n + 1:02d}" for n in range(n_passive)]
+    zt_c = np.ascontiguousarray(zt)
+
+    def tracer(n, dtype=np.float64):
+        a = DeviceArray(shape3, dtype)
+        chk(L.mom_synth_tracer(ctx, a.ptr, a.dtype.itemsize, kinds[n],
+                               (seed * 1000003 + n) & 0xFFFFFFFFFFFFFFFF, dptr(zt_c)),
+            "synth tracer")
+        return a
+
+    tr = [(names[n], tracer(n, np.float64 if n < 2 else passive_dtype))
+          for n in range(2 + n_passive)]
+    diff_cbt = DeviceArray((2, nk, NJ, NI))
+    chk(L.mom_synth_loguniform(ctx, diff_cbt.ptr, 2, (seed * 1000003 + 999) & 0xFFFFFFFFFFFFFFFF,
+                               1e-5, 1e-2), "synth diffusivity")
+
+    # 2-D forcing and barotropic state (host)
+    tm0 = (kmt >= 1).astype(np.float64)
+    um0 = wet_u.astype(np.float64)
+    stf, btf = [], []
+    for _ in tr:
+        s_ = np.zeros((NJ, NI))
+        b_ = np.zeros((NJ, NI))
+        if stf_scale:
+            s_[1:-1, 1:-1] = stf_scale * rng.standard_normal((nj, ni))
+            b_[1:-1, 1:-1] = stf_scale * rng.standard_normal((nj, ni))
+            s_ *= tm0
+            b_ *= (kmt > 0)
+            _wrap_x(s_, cyclic_x)
+            _wrap_x(b_, cyclic_x)
+        stf.append(s_)
+        btf.append(b_)
+    eta = np.zeros((NJ, NI))
+    eta[1:-1, 1:-1] = 0.1 * rng.standard_normal((nj, ni))
+    eta *= tm0
+    _wrap_x(eta, cyclic_x)
+    forcing_bt = np.zeros((2, NJ, NI))
+    forcing_bt[:, 1:-1, 1:-1] = rng.standard_normal((2, nj, ni))
+    forcing_bt *= um0[None]
+    _wrap_x(forcing_bt, cyclic_x)
+    pme = np.zeros((NJ, NI))
+    if pme_scale:
+        pme[1:-1, 1:-1] = pme_scale * rng.standard_normal((nj, ni))
+        pme *= tm0
+        _wrap_x(pme, cyclic_x)
+    z2 = np.zeros((NJ, NI))
+    grid.update(dzw=dzw, zw=zw, zt=zt)
+    fields = dict(uhrho_et=uh, vhrho_nt=vh, wrho_bt=wrho, rho_dzt=rho_dzt, rho_dztr=rho_dztr,
+                  dzwt=dzwt, diff_cbt=diff_cbt, stf=stf, btf=btf, eta_t=eta.copy(),
+                  eta_t_bar=eta.copy(), udrho=np.zeros((2, NJ, NI)), forcing_bt=forcing_bt,
+                  thicku=hu, patm=z2.copy(), pme=pme, river=z2.copy(), source=z2.copy(),
+                  eta_smooth=z2.copy(), rho_g=np.full((NJ, NI), GRAV * RHO0))
+    info = dict(max_cfl=cfl, land_frac=float((h["kmt_c"] == 0).mean()), seed=seed, gpu=gpu,
+                tracer=tracer)
+    return Case(ni=ni, nj=nj, nk=nk, cyclic_x=cyclic_x, dt=float(dt), dtbt=float(dtbt),
+                nts=int(nts), grid=grid, fields=fields, tracers=tr, info=info)
+
+
 def _continuity(mx, my, tmask, datr):
     """wrho_bt(k) = (div_h(k) + wrho_bt(k-1)) * tmask(k), wrho_bt(0) = 0,
     with div_h from the face mass transports mx = dyte*uhrho_et and
@@ -332,4 +515,8 @@ CONFIGS = {
     0: dict(ni=90, nj=60, nk=25, land_frac=0.25, barotropic=False),
     # 1440 x 1080 x 50, T,S (MDFL) + 1 Quicker tracer, 80 subcycles
     1: dict(ni=1440, nj=1080, nk=50, land_frac=0.30, nts=80, n_passive=1),
+    # 1440 x 1080 x 50, T,S + 30 passive (MDFL), FP64, batched
+    2: dict(ni=1440, nj=1080, nk=50, land_frac=0.30, nts=80, n_passive=30),
+    # 3600 x 2700 x 75, T,S FP64 + 10 FP32 passive, 120 subcycles (device-built)
+    3: dict(ni=3600, nj=2700, nk=75, land_frac=0.30, nts=120, n_passive=10),
 }

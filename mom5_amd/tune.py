@@ -1,0 +1,86 @@
+"""Autotuner for the launch-shape knobs of a context.
+
+The knobs (``mom_gpu_set_tuning``) change tile heights, how many tracers
+share one pass and how many barotropic subcycles one launch advances; they
+never change results (the GPU tests check bit-for-bit equality across
+them).  The best setting depends on the grid and the tracer mix, so the
+tuner times each candidate on the caller's own arrays, with HIP events on
+the context's stream, and keeps the fastest.
+
+    from mom5_amd.tune import autotune
+    report = autotune(step)            # a BatchedStep or PlainStep
+
+Each kernel family is tuned on its own (FP64 tracer passes, FP32 tracer
+passes, the barotropic subcycle, the MDFL kernel of the plain path), so the
+search is a sum, not a product, of the candidate counts.
+"""
+from __future__ import annotations
+
+import itertools
+
+
+def _time(gpu, fn, reps):
+    fn()                        # first launch of a new shape: code object load
+    gpu.sync()
+    gpu.mark(6)
+    for _ in range(reps):
+        fn()
+    gpu.mark(7)
+    return gpu.elapsed_ms(6, 7) / reps
+
+
+def _search(gpu, knobs, candidates, fn, reps):
+    saved = {k: gpu.get_tuning(k) for k in knobs}
+    results = []
+    for vals in candidates:
+        for k, v in zip(knobs, vals):
+            gpu.set_tuning(k, v)
+        results.append((_time(gpu, fn, reps), dict(zip(knobs, vals))))
+    best_ms, best = min(results, key=lambda r: r[0])
+    for k, v in best.items():
+        gpu.set_tuning(k, v)
+    default = next(ms for ms, c in results if c == saved)
+    return dict(best=best, best_ms=best_ms, default=saved, default_ms=default,
+                tried=[dict(ms=round(ms, 4), **c) for ms, c in results])
+
+
+def autotune(step, reps=3, barotropic=True):
+    """Tune the knobs that the step's kernels use; returns a report per
+    kernel family and leaves the context set to the fastest choices."""
+    from .step import BatchedStep
+    g, d, c = step.gpu, step.dev, step.case
+    report = {}
+    if isinstance(step, BatchedStep):
+        kw = dict(uhrho_et=d["uhrho_et"], vhrho_nt=d["vhrho_nt"], wrho_bt=d["wrho_bt"],
+                  rho_dzt_tau=d["rho_dzt"], rho_dztr=d["rho_dztr"], dtime=c.dt)
+        cur, nxt = step.cur, 1 - step.cur
+        for tag, size, gknob, rknob, groups in (
+                ("fp64", 8, "step_group_f64", "step_rows_per_wave", (1, 2)),
+                ("fp32", 4, "step_group_f32", "step_rows_per_wave_f32", (1, 2, 4))):
+            idx = [n for n in range(step.ntr) if step.tracer(n).element_size() == size]
+            if not idx:
+                continue
+            src = [step.T[n][cur] for n in idx]
+            dst = [step.T[n][nxt] for n in idx]
+            report[tag] = _search(g, (gknob, rknob), list(itertools.product(groups, (2, 4))),
+                                  lambda: g.step_tracers(src, dst, **kw), reps)
+    else:
+        th = step.th
+        T0 = step.tracer(0)
+        report["mdfl"] = _search(
+            g, ("mdfl_rows_per_wave",), [(2,), (4,)],
+            lambda: g.horz_advect_tracer("mdfl_sweby", T0, th, uhrho_et=d["uhrho_et"],
+                                         vhrho_nt=d["vhrho_nt"], wrho_bt=d["wrho_bt"],
+                                         rho_dzt_tau=d["rho_dzt"], dtime=c.dt), reps)
+    if barotropic and getattr(step, "barotropic", c.dtbt > 0.0) and c.dtbt > 0.0:
+        cur, nxt = step.cur, 1 - step.cur
+        report["barotropic"] = _search(
+            g, ("barotropic_subcycles_per_launch",), [(n,) for n in range(0, 10)],
+            lambda: g.update_ocean_barotropic(
+                nts=c.nts, dtbt=c.dtbt, eta_t_bar_tau=step.eta_bar[cur],
+                udrho_tau=step.udrho[cur], forcing_bt=d["forcing_bt"], thicku=d["thicku"],
+                patm=d["patm"], pme=d["pme"], river=d["river"], source=d["source"],
+                eta_smooth=d["eta_smooth"], rho_g=d["rho_g"],
+                eta_t_bar_taup1=step.eta_bar[nxt], udrho_taup1=step.udrho[nxt],
+                conv_rho_ud_t=step.conv), reps)
+    return report

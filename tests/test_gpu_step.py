@@ -93,7 +93,7 @@ def test_fp64_step_bitwise_equal_to_plain_path_and_oracle(setup, group, rows):
 def test_fp32_passive_error_vs_fp64(setup, group, rows):
     case, orc, gpu, dev = setup
     gpu.set_tuning("step_group_f32", group)
-    gpu.set_tuning("step_rows_per_wave", rows)
+    gpu.set_tuning("step_rows_per_wave_f32", rows)
     Ts = [t for _, t in case.tracers]
     dtypes = [np.float64, np.float64] + [np.float32] * (len(Ts) - 2)
     got = _run_step(case, gpu, dev, Ts, dtypes)
@@ -112,3 +112,20 @@ def test_fp32_passive_error_vs_fp64(setup, group, rows):
               f"median/p99/max {cell[0]:.2e}/{cell[1]:.2e}/{cell[2]:.2e}")
         assert normwise <= 1e-5
         assert rms <= 1e-6
+
+
+def test_fp32_results_independent_of_launch_shape(setup):
+    """Grouping and tile height change which threads do the work, not the
+    arithmetic of any tracer: FP32 results are bit-for-bit the same."""
+    case, orc, gpu, dev = setup
+    Ts = [t for _, t in case.tracers]
+    dtypes = [np.float64, np.float64] + [np.float32] * (len(Ts) - 2)
+    base = None
+    for group, rows in [(1, 2), (2, 2), (4, 2), (1, 4), (2, 4), (4, 4)]:
+        gpu.set_tuning("step_group_f32", group)
+        gpu.set_tuning("step_rows_per_wave_f32", rows)
+        got = [g[:, 1:-1, 1:-1] for g in _run_step(case, gpu, dev, Ts, dtypes)]
+        if base is None:
+            base = got
+        for n in range(len(Ts)):
+            assert np.array_equal(got[n], base[n]), (group, rows, n)
