@@ -161,6 +161,106 @@ __global__ void k_vdiff(KDom d, TriGroup<TT> g, const double *__restrict__ dcbas
   }
 }
 
+// FP32 storage: the same tridiagonal system solved for the increment,
+//   M dT = bflux - (M - I) T,   T(new) = (T + dT) * tmask,
+// with (M - I) T = a (T(k-1) - T(k)) + c (T(k+1) - T(k)) in FP64.  The forward
+// values dT' are small numbers kept in an FP32 scratch field, not near-copies
+// of T written over the tracer: rounding those (weak-diffusion rows of M are
+// close to the identity) is biased and drifts the tracer content, while the
+// rounding of dT' is not.  Arithmetic stays FP64.
+constexpr int NTF = 4;
+struct TriGroupF {
+  float *z[NTF];
+  const double *stf[NTF];
+  const double *btf[NTF];
+  int n;
+};
+
+__global__ void k_vdiff_inc(KDom d, TriGroupF g, const double *__restrict__ dcbase, double rho0,
+                            const int *__restrict__ kmt, const double *__restrict__ dh,
+                            const double *__restrict__ dhw, double tdt, double aidif,
+                            double *__restrict__ ework, float *__restrict__ scr) {
+  const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gj = blockIdx.y;
+  if (gi >= d.ni) return;
+  const int nk = d.nk, P2 = d.P2;
+  const size_t P3 = (size_t)P2 * nk;
+  const int o = d2(d, gi, gj);
+  const double tdt_aidif = tdt * aidif;
+  const int kmo = kmt[o];
+  const int kb = max(2, kmo);
+  double cprev = 0.0, bet = 0.0, dcb_km1 = 0.0;
+  double dprev[NTF], zkm1[NTF], zk[NTF];
+#pragma unroll
+  for (int n = 0; n < NTF; ++n) {
+    dprev[n] = 0.0;
+    zkm1[n] = 0.0;
+    zk[n] = (n < g.n) ? (double)g.z[n][o] : 0.0;
+  }
+  for (int k = 1; k <= nk; ++k) {
+    const int kp1 = min(k + 1, nk);
+    const int ok = o + P2 * (k - 1);
+    const double dcb_k = rho0 * dcbase[ok];
+    if (k == 1) dcb_km1 = dcb_k;
+    const double dhk = dh[ok];
+    const double factu = tdt_aidif / (dhw[o + P2 * (k - 1)] * dhk);
+    const double factl = tdt_aidif / (dhw[o + P2 * k] * dhk);
+    const double mkk = (k <= kmo) ? 1.0 : 0.0;
+    const double mkp1 = (kp1 <= kmo) ? 1.0 : 0.0;
+    double a = -dcb_km1 * factu * mkk;
+    double c = -dcb_k * factl * mkp1;
+    if (k == 1) a = 0.0;
+    if (k == nk) c = 0.0;
+    const double b = 1.0 - a - c;
+    if (k == 1) {
+      bet = mkk / (b + EPS_TRI);
+    } else {
+      const double e = cprev * bet;
+      bet = mkk / (b - a * e + EPS_TRI);
+      ework[ok] = e;
+    }
+#pragma unroll
+    for (int n = 0; n < NTF; ++n) {
+      if (n < g.n) {
+        const double zkp1 = (k < nk) ? (double)g.z[n][ok + P2] : 0.0;
+        double bfl = 0.0;
+        if (k == 1)
+          bfl = g.stf[n][o] * tdt_aidif * mkk / dhk;
+        else if (k == kb)
+          bfl = -(g.btf[n][o] * tdt_aidif * mkk / dhk);
+        const double rhs = bfl - (a * (zkm1[n] - zk[n]) + c * (zkp1 - zk[n]));
+        const double dk = (k == 1) ? rhs * bet : (rhs - a * dprev[n]) * bet;
+        scr[(size_t)n * P3 + ok] = (float)dk;
+        dprev[n] = dk;
+        zkm1[n] = zk[n];
+        zk[n] = zkp1;
+      }
+    }
+    cprev = c;
+    dcb_km1 = dcb_k;
+  }
+  {
+    const int ok = o + P2 * (nk - 1);
+    const double mk = (nk <= kmo) ? 1.0 : 0.0;
+#pragma unroll
+    for (int n = 0; n < NTF; ++n)
+      if (n < g.n) g.z[n][ok] = (float)(((double)g.z[n][ok] + dprev[n]) * mk);
+  }
+  for (int k = nk - 1; k >= 1; --k) {
+    const int ok = o + P2 * (k - 1);
+    const double e1 = ework[ok + P2];
+    const double mk = (k <= kmo) ? 1.0 : 0.0;
+#pragma unroll
+    for (int n = 0; n < NTF; ++n) {
+      if (n < g.n) {
+        const double dT = (double)scr[(size_t)n * P3 + ok] - e1 * dprev[n];
+        g.z[n][ok] = (float)(((double)g.z[n][ok] + dT) * mk);
+        dprev[n] = dT;
+      }
+    }
+  }
+}
+
 // ---- compute_tmask_limit, ocean_tracer.F90:2788-2892 -----------------------
 constexpr int NL = 32;
 struct LimTable {
@@ -459,8 +559,30 @@ int launch_vdiff_mixed(mom_gpu_ctx *ctx, int ntr, void *const *T, const int *dty
   int rc = vdiff_groups<double>(ctx, ntr, T, dtype, 8, stf, btf, diff_cbt, nullptr, index_salt,
                                 rho0, dh, dhw, dt, aidif);
   if (rc) return rc;
-  return vdiff_groups<float>(ctx, ntr, T, dtype, 4, stf, btf, diff_cbt, nullptr, index_salt,
-                             rho0, dh, dhw, dt, aidif);
+  const KDom &d = ctx->kd;
+  const size_t p3 = (size_t)d.P2 * d.nk;
+  bool any32 = false;
+  for (int n = 0; n < ntr; ++n) any32 = any32 || dtype[n] == 4;
+  if (!any32) return MOM_GPU_OK;
+  if (!ctx->vd_scr) MOM_HIP(ctx, hipMalloc((void **)&ctx->vd_scr, sizeof(float) * NTF * p3));
+  for (int nmix = 1; nmix <= 2; ++nmix) {
+    TriGroupF g{};
+    auto flush = [&]() {
+      if (g.n == 0) return;
+      hipLaunchKernelGGL(k_vdiff_inc, dim3((d.ni + 63) / 64, d.nj), dim3(64), 0, ctx->stream, d,
+                         g, diff_cbt + p3 * (nmix - 1), rho0, ctx->kmt, dh, dhw, dt, aidif,
+                         ctx->tri_e, ctx->vd_scr);
+      g.n = 0;
+    };
+    for (int n = 0; n < ntr; ++n) {
+      if (dtype[n] != 4 || ((n == index_salt) ? 2 : 1) != nmix) continue;
+      g.z[g.n] = (float *)T[n]; g.stf[g.n] = stf[n]; g.btf[g.n] = btf[n]; ++g.n;
+      if (g.n == NTF) flush();
+    }
+    flush();
+  }
+  MOM_LAUNCHED(ctx, "k_vdiff_inc");
+  return MOM_GPU_OK;
 }
 
 int launch_tmask_limit(mom_gpu_ctx *ctx, int ntr, const double *const *T, const double *mn,

@@ -151,6 +151,7 @@ static void free_ctx(mom_gpu_ctx *c) {
     if (*p) hipFree(*p);
   if (c->kmt) hipFree(c->kmt);
   if (c->kmu) hipFree(c->kmu);
+  if (c->vd_scr) hipFree(c->vd_scr);
   for (int i = 0; i < 8; ++i)
     if (c->ev[i]) hipEventDestroy(c->ev[i]);
   if (c->stream) hipStreamDestroy(c->stream);

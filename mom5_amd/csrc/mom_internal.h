@@ -60,6 +60,8 @@ struct mom_gpu_ctx {
   double bt_cori_dtbt;
   // invtri work (e coefficients), one 3-D plane set
   double *tri_e;
+  // FP32 implicit diffusion: forward increments of up to 4 tracers (lazy)
+  float *vd_scr;
   // reduction partials
   double *red;
   int red_n;
