@@ -119,11 +119,19 @@ def step_kernel_bytes(ni, nj, nk, elem, group):
     return ni * nj * nk * (40 + 2 * elem * group) + ni * nj * 52
 
 
-def mixed_step_bytes(ni, nj, nk, n64, n32):
-    """Algorithmic bytes of one whole mixed step: the shared 3-D fields
-    (uhrho_et, vhrho_nt, wrho_bt, rho_dzt, rho_dztr, dzwt, two diff_cbt
-    planes) read once, every tracer read once and written once."""
-    return ni * nj * nk * (64 + 16 * n64 + 8 * n32)
+def mixed_step_bytes(ni, nj, nk, n64, n32, barotropic=True):
+    """Algorithmic bytes of one whole step, each distinct input read once and
+    each output written once:
+      C*(sum over tracers of 2*s_n + 8*S3) + C2*(52 + 16*N_tr) [+ C2*156],
+    C = ni*nj*nk cells, C2 = ni*nj columns, s_n the tracer's storage size,
+    S3 = 8 shared FP64 3-D arrays passed (uhrho_et, vhrho_nt, wrho_bt,
+    rho_dzt, rho_dztr, dzwt, two diff_cbt planes), 52 B/column of metrics and
+    kmt, 16 B/column/tracer of stf and btf, 156 B/column of barotropic inputs
+    and outputs."""
+    C, C2 = ni * nj * nk, ni * nj
+    ntr = n64 + n32
+    return (C * (16 * n64 + 8 * n32 + 8 * 8) + C2 * (52 + 16 * ntr)
+            + (C2 * 156 if barotropic else 0))
 
 
 def time_launches(g, fn, reps=10):
@@ -134,6 +142,22 @@ def time_launches(g, fn, reps=10):
         fn()
     g.mark(3)
     return g.elapsed_ms(2, 3) / reps
+
+
+def measured_traffic(workload, key):
+    """HBM bytes per dispatch of kernel `key` on `workload`, from the
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs committed under profiles/
+    (scaled by the copy-kernel calibration recorded there); None if that
+    kernel and launch shape were not measured."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_cfg3_3600x2700x75.json")
+    if workload is None or not os.path.exists(path):
+        return None, None
+    with open(path) as fh:
+        rec = json.load(fh)
+    k = rec.get("kernels", {}).get(key)
+    if rec.get("workload") != workload or k is None:
+        return None, None
+    return k["traffic_bytes"], os.path.relpath(path, ROOT)
 
 
 def run_mixed(args, w):
@@ -186,6 +210,11 @@ def run_mixed(args, w):
     cand = []
     for want64, elem in ((True, 8), (False, 4)):
         grp = g.get_tuning("step_group_f64" if want64 else "step_group_f32")
+        rows = g.get_tuning("step_rows_per_wave" if want64 else "step_rows_per_wave_f32")
+        waves = 8 if want64 else g.get_tuning("step_waves_f32")
+        if not want64 and (waves == 16 and (rows != 2 or grp > 2)):
+            waves = 8
+        key = f"k_step<{'double' if want64 else 'float'}, {grp}, {rows}, {waves}>"
         idx = [n for n in range(ntr) if is64[n] == want64][:grp]
         if len(idx) < grp:
             continue
@@ -193,17 +222,21 @@ def run_mixed(args, w):
         dst = [bs.T[n][1 - bs.cur] for n in idx]
         k_ms = time_launches(g, lambda: g.step_tracers(src, dst, **kw))
         nl = -(-(n64 if want64 else n32) // grp)
-        cand.append(dict(kernel=f"k_step<{'double' if want64 else 'float'},G={grp}> "
-                                f"(fused MDFL-Sweby advection + explicit update)",
+        cand.append(dict(kernel=f"{key} (fused MDFL-Sweby advection + explicit update; "
+                                f"template args: type, tracers, rows per wave, waves)",
+                         key=key,
                          kernel_ms=k_ms, bytes_per_launch=step_kernel_bytes(ni, nj, nk, elem, grp),
                          launches_per_step=nl))
         log(f"{cand[-1]['kernel']}: {k_ms:.3f} ms/launch x {nl} per step")
         del src, dst
     dom = max(cand, key=lambda c: c["kernel_ms"] * c["launches_per_step"])
     achieved = dom["bytes_per_launch"] / (dom["kernel_ms"] / 1e3) / 1e9
+    traffic, tsrc = measured_traffic(w["name"] if not args.shape else None, dom.pop("key"))
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, **{
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, **{
                     k: (round(v, 4) if isinstance(v, float) else v) for k, v in dom.items()})
+    if tsrc:
+        roof["traffic_source"] = tsrc
     step_bytes = mixed_step_bytes(ni, nj, nk, n64, n32)
     step_frac = step_bytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS
 
@@ -278,7 +311,8 @@ def _tune_summary(rep):
 
 
 TUNING_KEYS = ("barotropic_subcycles_per_launch", "mdfl_rows_per_wave", "step_group_f64",
-               "step_group_f32", "step_rows_per_wave", "step_rows_per_wave_f32")
+               "step_group_f32", "step_rows_per_wave", "step_rows_per_wave_f32",
+               "step_waves_f32")
 
 
 def run_plain(args, w):

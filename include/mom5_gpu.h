@@ -119,6 +119,8 @@ int mom_gpu_copy(mom_gpu_ctx_t *ctx, double *dst, const double *src, size_t n);
  *   "step_group_f32"                   1, 2 or 4 FP32 tracers per batched pass
  *   "step_rows_per_wave"               2 or 4: tile height of the FP64 passes
  *   "step_rows_per_wave_f32"           2 or 4: tile height of the FP32 passes
+ *   "step_waves_f32"                   8 or 16 wavefronts per FP32 tile (16 is
+ *                                      used with 2 rows and 1-2 tracers a pass)
  * Unknown knobs and out-of-range values return MOM_GPU_ERR_ARG. */
 int mom_gpu_set_tuning(mom_gpu_ctx_t *ctx, const char *knob, int value);
 int mom_gpu_get_tuning(const mom_gpu_ctx_t *ctx, const char *knob, int *value);

@@ -23,7 +23,6 @@ namespace {
 
 constexpr int RX = 64;
 constexpr int TX = RX - 4;
-constexpr int NW = 8;
 constexpr double ONESIXTH = 1.0 / 6.0;
 constexpr double EPS30 = 1.0e-30;
 constexpr double EPSLN = 1.0e-40;   // shared/constants/constants.F90:123
@@ -93,7 +92,7 @@ struct StepTab {
   const double *th[8]; // other th_tendency contributions (NULL: none)
 };
 
-template <typename TT, int G, int RPW>
+template <typename TT, int G, int RPW, int NW>
 __global__ __launch_bounds__(RX * NW) void k_step(
     KDom d, StepTab<TT> tab, const double *__restrict__ U, const double *__restrict__ V,
     const double *__restrict__ W, const double *__restrict__ R, const double *__restrict__ Rm1,
@@ -385,7 +384,7 @@ __global__ __launch_bounds__(RX * NW) void k_step(
   }
 }
 
-template <typename TT, int G, int RPW>
+template <typename TT, int G, int RPW, int NW = 8>
 void launch_group(mom_gpu_ctx *ctx, const StepTab<TT> &tab, const double *u, const double *v,
                   const double *w, const double *rho, const double *rm1, const double *rr,
                   double dtime) {
@@ -395,7 +394,7 @@ void launch_group(mom_gpu_ctx *ctx, const StepTab<TT> &tab, const double *u, con
   const int nty = (d.nj + TY - 1) / TY;
   const int ntiles = ntx * nty;
   const int per = (ntiles + 7) / 8;
-  hipLaunchKernelGGL((k_step<TT, G, RPW>), dim3(8 * per), dim3(RX * NW), 0, ctx->stream, d, tab,
+  hipLaunchKernelGGL((k_step<TT, G, RPW, NW>), dim3(8 * per), dim3(RX * NW), 0, ctx->stream, d, tab,
                      u, v, w, rho, rm1, rr, ctx->kmt, ctx->dat, ctx->datr, ctx->dxte, ctx->dyte,
                      ctx->dxtn, ctx->dytn, dtime, ntx, ntiles, per);
 }
@@ -403,12 +402,21 @@ void launch_group(mom_gpu_ctx *ctx, const StepTab<TT> &tab, const double *u, con
 
 // Launch one group of up to G tracers of type TT taken from list[0..cnt).
 template <typename TT>
-static void launch_some(mom_gpu_ctx *ctx, int rpw, const TT *const *Tin, TT *const *Tout,
-                        const double *const *thin, int cnt, const double *u, const double *v,
-                        const double *w, const double *rho, const double *rm1, const double *rr,
-                        double dtime) {
+static void launch_some(mom_gpu_ctx *ctx, int rpw, int waves, const TT *const *Tin,
+                        TT *const *Tout, const double *const *thin, int cnt, const double *u,
+                        const double *v, const double *w, const double *rho, const double *rm1,
+                        const double *rr, double dtime) {
   StepTab<TT> t{};
   for (int i = 0; i < cnt; ++i) { t.T[i] = Tin[i]; t.out[i] = Tout[i]; t.th[i] = thin[i]; }
+  // 16-wave tiles (32 rows, 28 interior: less halo re-read) fit 128 VGPRs only
+  // for FP32 groups of one or two tracers at two rows per wave
+  if constexpr (sizeof(TT) == 4) {
+    if (waves == 16 && rpw == 2 && cnt <= 2) {
+      if (cnt == 2) launch_group<TT, 2, 2, 16>(ctx, t, u, v, w, rho, rm1, rr, dtime);
+      else launch_group<TT, 1, 2, 16>(ctx, t, u, v, w, rho, rm1, rr, dtime);
+      return;
+    }
+  }
   if (rpw == 2) {
     if (cnt == 4) launch_group<TT, 4, 2>(ctx, t, u, v, w, rho, rm1, rr, dtime);
     else if (cnt == 2) launch_group<TT, 2, 2>(ctx, t, u, v, w, rho, rm1, rr, dtime);
@@ -421,7 +429,8 @@ static void launch_some(mom_gpu_ctx *ctx, int rpw, const TT *const *Tin, TT *con
 }
 
 template <typename TT>
-static void launch_all(mom_gpu_ctx *ctx, int gmax, int rpw, int ntr, const void *const *T_in,
+static void launch_all(mom_gpu_ctx *ctx, int gmax, int rpw, int waves, int ntr,
+                       const void *const *T_in,
                        void *const *T_out, const int *dtype, const double *const *th_in,
                        int want, const double *u, const double *v, const double *w,
                        const double *rho, const double *rm1, const double *rr, double dtime) {
@@ -433,7 +442,7 @@ static void launch_all(mom_gpu_ctx *ctx, int gmax, int rpw, int ntr, const void 
     int i = 0;
     while (i < g) {                 // groups of 4, 2, 1 no larger than gmax
       int c = (g - i >= 4 && gmax >= 4) ? 4 : (g - i >= 2 && gmax >= 2) ? 2 : 1;
-      launch_some<TT>(ctx, rpw, tin + i, tout + i, th + i, c, u, v, w, rho, rm1, rr, dtime);
+      launch_some<TT>(ctx, rpw, waves, tin + i, tout + i, th + i, c, u, v, w, rho, rm1, rr, dtime);
       i += c;
     }
     g = 0;
@@ -456,10 +465,11 @@ int launch_tracer_step(mom_gpu_ctx *ctx, int ntr, const void *const *T_in, void 
                        const int *dtype, const double *const *th_in, const double *u,
                        const double *v, const double *w, const double *rho,
                        const double *rho_taum1, const double *rho_dztr, double dtime) {
-  launch_all<double>(ctx, ctx->tune.step_group_f64, ctx->tune.step_rows_per_wave, ntr, T_in,
+  launch_all<double>(ctx, ctx->tune.step_group_f64, ctx->tune.step_rows_per_wave, 8, ntr, T_in,
                      T_out, dtype, th_in, 8, u, v, w, rho, rho_taum1, rho_dztr, dtime);
-  launch_all<float>(ctx, ctx->tune.step_group_f32, ctx->tune.step_rows_per_wave_f32, ntr, T_in,
-                    T_out, dtype, th_in, 4, u, v, w, rho, rho_taum1, rho_dztr, dtime);
+  launch_all<float>(ctx, ctx->tune.step_group_f32, ctx->tune.step_rows_per_wave_f32,
+                    ctx->tune.step_waves_f32, ntr, T_in, T_out, dtype, th_in, 4, u, v, w, rho,
+                    rho_taum1, rho_dztr, dtime);
   MOM_LAUNCHED(ctx, "k_step");
   return MOM_GPU_OK;
 }

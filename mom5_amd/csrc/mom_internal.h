@@ -73,6 +73,7 @@ struct mom_gpu_ctx {
     int step_group_f32;            // FP32 tracers per pass (1, 2, 4)
     int step_rows_per_wave;        // 2 or 4 (FP64 groups)
     int step_rows_per_wave_f32;    // 2 or 4 (FP32 groups)
+    int step_waves_f32;            // 8 or 16 wavefronts per tile (FP32 groups)
   } tune;
   char err[512];
 };

@@ -54,16 +54,19 @@ def autotune(step, reps=3, barotropic=True):
         kw = dict(uhrho_et=d["uhrho_et"], vhrho_nt=d["vhrho_nt"], wrho_bt=d["wrho_bt"],
                   rho_dzt_tau=d["rho_dzt"], rho_dztr=d["rho_dztr"], dtime=c.dt)
         cur, nxt = step.cur, 1 - step.cur
-        for tag, size, gknob, rknob, groups in (
-                ("fp64", 8, "step_group_f64", "step_rows_per_wave", (1, 2)),
-                ("fp32", 4, "step_group_f32", "step_rows_per_wave_f32", (1, 2, 4))):
+        fp32 = [c_ + (8,) for c_ in itertools.product((1, 2, 4), (2, 4))] + \
+            [(1, 2, 16), (2, 2, 16)]
+        for tag, size, knobs, cands in (
+                ("fp64", 8, ("step_group_f64", "step_rows_per_wave"),
+                 list(itertools.product((1, 2), (2, 4)))),
+                ("fp32", 4, ("step_group_f32", "step_rows_per_wave_f32", "step_waves_f32"),
+                 fp32)):
             idx = [n for n in range(step.ntr) if step.tracer(n).element_size() == size]
             if not idx:
                 continue
             src = [step.T[n][cur] for n in idx]
             dst = [step.T[n][nxt] for n in idx]
-            report[tag] = _search(g, (gknob, rknob), list(itertools.product(groups, (2, 4))),
-                                  lambda: g.step_tracers(src, dst, **kw), reps)
+            report[tag] = _search(g, knobs, cands, lambda: g.step_tracers(src, dst, **kw), reps)
     else:
         th = step.th
         T0 = step.tracer(0)

@@ -121,11 +121,14 @@ def test_fp32_results_independent_of_launch_shape(setup):
     Ts = [t for _, t in case.tracers]
     dtypes = [np.float64, np.float64] + [np.float32] * (len(Ts) - 2)
     base = None
-    for group, rows in [(1, 2), (2, 2), (4, 2), (1, 4), (2, 4), (4, 4)]:
+    for group, rows, waves in [(1, 2, 8), (2, 2, 8), (4, 2, 8), (1, 4, 8), (2, 4, 8), (4, 4, 8),
+                               (1, 2, 16), (2, 2, 16)]:
         gpu.set_tuning("step_group_f32", group)
         gpu.set_tuning("step_rows_per_wave_f32", rows)
+        gpu.set_tuning("step_waves_f32", waves)
         got = [g[:, 1:-1, 1:-1] for g in _run_step(case, gpu, dev, Ts, dtypes)]
         if base is None:
             base = got
         for n in range(len(Ts)):
-            assert np.array_equal(got[n], base[n]), (group, rows, n)
+            assert np.array_equal(got[n], base[n]), (group, rows, waves, n)
+    gpu.set_tuning("step_waves_f32", 8)
