@@ -266,6 +266,23 @@ typedef struct {
 
 int mom_gpu_step_tracers(mom_gpu_ctx_t *ctx, const mom_gpu_tracer_step_t *s);
 
+/* ---- hipGraph capture of whole steps -------------------------------------
+ * No reference counterpart (MOM launches nothing): a caller captures the
+ * entry points of one or more steps once and replays them.  Between begin
+ * and end only kernel-launching entry points may be called on ctx
+ * (advection, explicit update, tmask limit, invtri / vert_diffuse_implicit,
+ * step_tracers, barotropic, fill, copy); copies, syncs, marks, content and
+ * error reports end the capture with an error.  Arguments (addresses,
+ * scalars, knobs) are frozen at capture: with two-level time stepping,
+ * capture an even number of steps so a replay leaves the time levels where
+ * it found them.  A replay runs asynchronously on ctx's stream. */
+typedef struct mom_gpu_graph mom_gpu_graph_t;
+int mom_gpu_graph_begin(mom_gpu_ctx_t *ctx);
+int mom_gpu_graph_end(mom_gpu_ctx_t *ctx, mom_gpu_graph_t **graph);
+int mom_gpu_graph_launch(mom_gpu_ctx_t *ctx, const mom_gpu_graph_t *graph);
+int mom_gpu_graph_nodes(const mom_gpu_graph_t *graph);
+int mom_gpu_graph_destroy(mom_gpu_graph_t *graph);
+
 /* ---- diagnostics -------------------------------------------------------- */
 
 /* Masked tracer content sum(T*rho_dzt*dat*tmask) over the compute domain,

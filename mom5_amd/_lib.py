@@ -58,6 +58,11 @@ _PROTOS = {
     "mom_gpu_memcpy_dtoh": (C.c_int, [_vp, _vp, C.c_size_t]),
     "mom_gpu_memcpy_dtod": (C.c_int, [_vp, _vp, C.c_size_t]),
     "mom_gpu_copy": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
+    "mom_gpu_graph_begin": (C.c_int, [_vp]),
+    "mom_gpu_graph_end": (C.c_int, [_vp, C.POINTER(C.c_void_p)]),
+    "mom_gpu_graph_launch": (C.c_int, [_vp, _vp]),
+    "mom_gpu_graph_nodes": (C.c_int, [_vp]),
+    "mom_gpu_graph_destroy": (C.c_int, [_vp]),
     "mom_gpu_memset": (C.c_int, [_vp, C.c_int, C.c_size_t]),
     "mom_gpu_fill": (C.c_int, [_vp, _vp, C.c_double, C.c_size_t]),
     "mom_gpu_set_tuning": (C.c_int, [_vp, C.c_char_p, C.c_int]),

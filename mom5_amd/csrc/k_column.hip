@@ -564,7 +564,15 @@ int launch_vdiff_mixed(mom_gpu_ctx *ctx, int ntr, void *const *T, const int *dty
   bool any32 = false;
   for (int n = 0; n < ntr; ++n) any32 = any32 || dtype[n] == 4;
   if (!any32) return MOM_GPU_OK;
-  if (!ctx->vd_scr) MOM_HIP(ctx, hipMalloc((void **)&ctx->vd_scr, sizeof(float) * NTF * p3));
+  if (!ctx->vd_scr) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    MOM_HIP(ctx, hipStreamIsCapturing(ctx->stream, &cs));
+    if (cs != hipStreamCaptureStatusNone)
+      return mom_fail(ctx, MOM_GPU_ERR_ARG,
+                      "step_tracers: the first step with FP32 tracers allocates its scratch; "
+                      "run one step before capturing a graph");
+    MOM_HIP(ctx, hipMalloc((void **)&ctx->vd_scr, sizeof(float) * NTF * p3));
+  }
   for (int nmix = 1; nmix <= 2; ++nmix) {
     TriGroupF g{};
     auto flush = [&]() {

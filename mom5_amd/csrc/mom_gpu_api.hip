@@ -290,6 +290,71 @@ int mom_gpu_elapsed(mom_gpu_ctx_t *ctx, int s0, int s1, float *ms) {
   return MOM_GPU_OK;
 }
 
+// ---- hipGraph capture of whole steps -----------------------------------------
+struct mom_gpu_graph {
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+  int nodes;
+};
+
+int mom_gpu_graph_begin(mom_gpu_ctx_t *ctx) {
+  REQUIRE(nullptr, ctx, "mom_gpu_graph_begin: NULL context");
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  MOM_HIP(ctx, hipStreamIsCapturing(ctx->stream, &st));
+  REQUIRE(ctx, st == hipStreamCaptureStatusNone, "mom_gpu_graph_begin: already capturing");
+  MOM_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  return MOM_GPU_OK;
+}
+
+int mom_gpu_graph_end(mom_gpu_ctx_t *ctx, mom_gpu_graph_t **out) {
+  REQUIRE(nullptr, ctx, "mom_gpu_graph_end: NULL context");
+  REQUIRE(ctx, out, "mom_gpu_graph_end: NULL output");
+  *out = nullptr;
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+  if (e != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    // a stream whose capture was invalidated may stay unusable: replace it
+    hipStream_t fresh = nullptr;
+    if (hipStreamCreateWithFlags(&fresh, hipStreamNonBlocking) == hipSuccess) {
+      (void)hipStreamDestroy(ctx->stream);
+      ctx->stream = fresh;
+    }
+    (void)hipGetLastError();
+    return mom_fail(ctx, MOM_GPU_ERR_HIP,
+                    "mom_gpu_graph_end: capture failed (%s); between begin and end only "
+                    "kernel-launching entry points may be called (no sync, copies, marks, "
+                    "content or first-use allocations)", hipGetErrorString(e));
+  }
+  size_t n = 0;
+  (void)hipGraphGetNodes(g, nullptr, &n);
+  hipGraphExec_t x = nullptr;
+  e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return mom_fail(ctx, MOM_GPU_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+  }
+  *out = new mom_gpu_graph{g, x, (int)n};
+  return MOM_GPU_OK;
+}
+
+int mom_gpu_graph_launch(mom_gpu_ctx_t *ctx, const mom_gpu_graph_t *graph) {
+  REQUIRE(nullptr, ctx, "mom_gpu_graph_launch: NULL context");
+  REQUIRE(ctx, graph, "mom_gpu_graph_launch: NULL graph");
+  MOM_HIP(ctx, hipGraphLaunch(graph->exec, ctx->stream));
+  return MOM_GPU_OK;
+}
+
+int mom_gpu_graph_nodes(const mom_gpu_graph_t *graph) { return graph ? graph->nodes : -1; }
+
+int mom_gpu_graph_destroy(mom_gpu_graph_t *graph) {
+  if (!graph) return MOM_GPU_OK;
+  (void)hipGraphExecDestroy(graph->exec);
+  (void)hipGraphDestroy(graph->graph);
+  delete graph;
+  return MOM_GPU_OK;
+}
+
 static size_t p3bytes(const mom_gpu_ctx *c) { return (size_t)c->kd.P2 * c->kd.nk * sizeof(double); }
 
 int mom_gpu_horz_advect_tracer(mom_gpu_ctx_t *ctx, int scheme, const double *T_taum1,

@@ -50,6 +50,19 @@ module mom_gpu_mod
      integer(c_int) :: initsum_with_bar
   end type mom_gpu_bt_params_t
 
+  ! the tracer loop of update_ocean_tracer + vert_diffuse_implicit
+  ! (ocean_tracers/ocean_tracer.F90:2295-2350,2432): host tables of device
+  ! addresses (c_loc of type(c_ptr) arrays), dtype(n) = 8 (FP64) or 4 (FP32)
+  type, bind(C), public :: mom_gpu_tracer_step_t
+     integer(c_int) :: ntr
+     type(c_ptr) :: T_taum1, T_taup1, dtype, th_tendency, stf, btf
+     integer(c_int) :: index_salt
+     type(c_ptr) :: uhrho_et, vhrho_nt, wrho_bt, rho_dzt_tau
+     type(c_ptr) :: rho_dzt_taum1, rho_dztr, rho_dzt_taup1
+     type(c_ptr) :: dzwt, diff_cbt
+     real(c_double) :: rho0, dtime, aidif
+  end type mom_gpu_tracer_step_t
+
   public :: mom_gpu_create, mom_gpu_destroy, mom_gpu_sync, mom_gpu_last_error_f
   public :: mom_gpu_malloc, mom_gpu_free, mom_gpu_memcpy_htod, mom_gpu_memcpy_dtoh
   public :: mom_gpu_fill, mom_gpu_set_tuning
@@ -58,6 +71,8 @@ module mom_gpu_mod
   public :: mom_gpu_update_tracer_explicit, mom_gpu_invtri
   public :: mom_gpu_vert_diffuse_implicit, mom_gpu_update_ocean_barotropic
   public :: mom_gpu_tracer_content, mom_gpu_check
+  public :: mom_gpu_step_tracers, mom_gpu_tracer_content_f32, mom_gpu_tracer_error
+  public :: mom_gpu_copy, mom_gpu_memcpy_dtod, mom_gpu_get_tuning
 
   interface
      integer(c_int) function mom_gpu_create(dom, grid, ctx) bind(C, name="mom_gpu_create")
@@ -220,6 +235,52 @@ module mom_gpu_mod
        type(c_ptr), value :: ctx, T, rho_dzt
        real(c_double), intent(out) :: content
      end function mom_gpu_tracer_content
+
+     ! the batched step (all MDFL-Sweby tracers, FP64 or FP32 storage each)
+     integer(c_int) function mom_gpu_step_tracers(ctx, s) bind(C, name="mom_gpu_step_tracers")
+       import :: c_int, c_ptr, mom_gpu_tracer_step_t
+       type(c_ptr), value :: ctx
+       type(mom_gpu_tracer_step_t), intent(in) :: s
+     end function mom_gpu_step_tracers
+
+     integer(c_int) function mom_gpu_tracer_content_f32(ctx, T, rho_dzt, content) &
+          bind(C, name="mom_gpu_tracer_content_f32")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx, T, rho_dzt
+       real(c_double), intent(out) :: content
+     end function mom_gpu_tracer_content_f32
+
+     ! error of a tracer against an FP64 run: out(1:7) = max|a-b|, max|b|,
+     ! sum (a-b)**2, sum b**2, cells, cells differing, max relative
+     integer(c_int) function mom_gpu_tracer_error(ctx, a, dtype, b, rel_floor, out) &
+          bind(C, name="mom_gpu_tracer_error")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx, a, b
+       integer(c_int), value :: dtype
+       real(c_double), value :: rel_floor
+       real(c_double), intent(out) :: out(7)
+     end function mom_gpu_tracer_error
+
+     integer(c_int) function mom_gpu_copy(ctx, dst, src, n) bind(C, name="mom_gpu_copy")
+       import :: c_int, c_ptr, c_size_t
+       type(c_ptr), value :: ctx, dst, src
+       integer(c_size_t), value :: n
+     end function mom_gpu_copy
+
+     integer(c_int) function mom_gpu_memcpy_dtod(dst, src, bytes) &
+          bind(C, name="mom_gpu_memcpy_dtod")
+       import :: c_int, c_ptr, c_size_t
+       type(c_ptr), value :: dst, src
+       integer(c_size_t), value :: bytes
+     end function mom_gpu_memcpy_dtod
+
+     integer(c_int) function mom_gpu_get_tuning(ctx, knob, value) &
+          bind(C, name="mom_gpu_get_tuning")
+       import :: c_int, c_ptr, c_char
+       type(c_ptr), value :: ctx
+       character(kind=c_char), intent(in) :: knob(*)
+       integer(c_int), intent(out) :: value
+     end function mom_gpu_get_tuning
 
      integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
        import :: c_size_t, c_ptr

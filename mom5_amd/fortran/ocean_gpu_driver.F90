@@ -46,6 +46,11 @@ program ocean_gpu_driver
   type(c_ptr) :: d_eta(2), d_ud(2), d_conv, d_forc
   type(c_ptr), target :: d_T(ntr,2), d_th(ntr), d_lim(ntr), d_stf(ntr), d_btf(ntr)
   type(c_ptr), target :: tab_now(ntr), tab_nxt(ntr), tab_lim(ntr)
+  type(c_ptr), target :: d_B(ntr,2), tab_in(ntr), tab_out(ntr)
+  integer(c_int), target :: dtypes(ntr)
+  type(mom_gpu_tracer_step_t) :: st
+  real(c_double) :: errv(7)
+  integer :: bnow, bnxt
   real(c_double) :: c0(ntr), c1(ntr), dt, dtbt, rel, gk(nk), gmean
   integer :: i, j, k, n, step, now, nxt
   integer(c_int) :: rc
@@ -235,6 +240,35 @@ program ocean_gpu_driver
         error stop 2
      endif
   enddo
+  ! ---- the same tracer steps through the batched entry point ----
+  do n = 1, ntr
+     d_B(n,1) = up(c_loc(tr(isd,jsd,1,n)), b3)
+     d_B(n,2) = up(c_loc(tr(isd,jsd,1,n)), b3)
+  enddo
+  dtypes = 8
+  bnow = 1
+  do step = 1, nsteps
+     bnxt = 3 - bnow
+     do n = 1, ntr
+        tab_in(n) = d_B(n,bnow); tab_out(n) = d_B(n,bnxt)
+     enddo
+     st = mom_gpu_tracer_step_t(ntr, c_loc(tab_in), c_loc(tab_out), c_loc(dtypes), c_null_ptr, &
+          c_loc(d_stf), c_loc(d_btf), 1, d_uh, d_vh, d_wb, d_rdz, c_null_ptr, d_rdzr, c_null_ptr, &
+          d_dzw, d_dcbt, rho0, dt, 1.0d0)
+     call mom_gpu_check(mom_gpu_step_tracers(ctx, st), ctx, 'step_tracers')
+     bnow = bnxt
+  enddo
+  do n = 1, ntr
+     call mom_gpu_check(mom_gpu_tracer_error(ctx, d_B(n,bnow), 8, d_T(n,now), 0.0d0, errv), ctx, &
+          'tracer_error')
+     write(*,'(a,i2,a,i0,a,i0,a)') 'tracer', n, ' batched vs per-tracer: ', nint(errv(6)), &
+          ' of ', nint(errv(5)), ' wet cells differ'
+     if (errv(6) /= 0.0d0) then
+        write(*,'(a)') 'FATAL: batched step differs from the per-tracer calls'
+        error stop 3
+     endif
+  enddo
+
   call mom_gpu_check(mom_gpu_destroy(ctx), c_null_ptr, 'destroy')
   write(*,'(a,i0,a)') 'ocean_gpu_driver: ', nsteps, ' steps done on the GPU, content conserved'
 

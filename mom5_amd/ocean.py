@@ -106,6 +106,28 @@ def _ptab(arrs):
     return tab
 
 
+class Graph:
+    """An instantiated hipGraph of captured entry-point calls."""
+
+    def __init__(self, gpu, handle):
+        self.gpu, self.h = gpu, handle
+        self.nodes = gpu.L.mom_gpu_graph_nodes(handle)
+
+    def launch(self):
+        self.gpu._chk(self.gpu.L.mom_gpu_graph_launch(self.gpu.ctx, self.h), "graph_launch")
+
+    def close(self):
+        if self.h:
+            self.gpu.L.mom_gpu_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class OceanGPU:
     """One MOM domain (global or regional) resident on the GPU.
 
@@ -300,6 +322,16 @@ class OceanGPU:
                        rho_dzt_taup1=_p(rho_dzt_taup1), dzwt=_p(dzwt), diff_cbt=_p(diff_cbt),
                        rho0=float(rho0), dtime=float(dtime), aidif=float(aidif))
         self._chk(self.L.mom_gpu_step_tracers(self.ctx, C.byref(s)), "step_tracers")
+
+    # ---- hipGraph capture --------------------------------------------------------
+    def graph_begin(self):
+        self._chk(self.L.mom_gpu_graph_begin(self.ctx), "graph_begin")
+
+    def graph_end(self):
+        """End the capture; returns a :class:`Graph` to replay."""
+        h = C.c_void_p()
+        self._chk(self.L.mom_gpu_graph_end(self.ctx, C.byref(h)), "graph_end")
+        return Graph(self, h.value)
 
     # ---- diagnostics ----------------------------------------------------------
     def tracer_error(self, a, b, rel_floor=0.0):

@@ -49,7 +49,31 @@ LIMITS = {"temp": (-2.0, 32.0), "salt": (1.0, 42.0)}
 PASSIVE_LIMITS = (-0.1, 1.0)
 
 
-class PlainStep:
+class _Captured:
+    """hipGraph capture of whole steps (shared by both step drivers)."""
+
+    def capture(self, nsteps=2):
+        """Capture `nsteps` steps (even: the time levels come back to where
+        they were) into a graph; a replay advances the state by nsteps."""
+        if nsteps % 2:
+            raise ValueError("capture an even number of steps (two-level time stepping)")
+        g = self.gpu
+        g.sync()
+        g.graph_begin()
+        try:
+            self.run(nsteps)
+        except Exception:
+            try:
+                g.graph_end()
+            except Exception:
+                pass
+            raise
+        graph = g.graph_end()
+        graph.steps = nsteps
+        return graph
+
+
+class PlainStep(_Captured):
     """Per-tracer FP64 path: one C-ABI call per reference routine."""
 
     def __init__(self, case, schemes=None, gpu: OceanGPU | None = None):
@@ -126,7 +150,7 @@ class PlainStep:
             self.cur = nxt
 
 
-class BatchedStep:
+class BatchedStep(_Captured):
     """All tracers MDFL-Sweby through ``mom_gpu_step_tracers``, then the
     barotropic subcycle.  Each tracer keeps the dtype of its initial array
     (FP64, or FP32 for passive tracers); fields may be host arrays (copied
