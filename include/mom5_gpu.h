@@ -266,6 +266,34 @@ typedef struct {
 
 int mom_gpu_step_tracers(mom_gpu_ctx_t *ctx, const mom_gpu_tracer_step_t *s);
 
+/* ---- ensembles of domains ------------------------------------------------
+ * No reference counterpart: several independent domains (each its own
+ * context, so its own grid, bathymetry, time step and subcycle count) are
+ * advanced together.  Each phase of a step is one launch over all domains:
+ * the tracer passes use a persistent grid whose work-groups take (domain,
+ * tile) items from a device-side queue, largest domain first; the implicit
+ * diffusion and each block of barotropic subcycles cover every domain in one
+ * grid.  Launches go to the stream of ctxs[0] (capture that context to graph
+ * a whole ensemble step); the member contexts must not run work of their own
+ * meanwhile.  Results are bit-for-bit those of the per-domain calls. */
+typedef struct mom_gpu_ensemble mom_gpu_ensemble_t;
+
+/* The 2-D barotropic fields of one domain (as mom_gpu_update_ocean_barotropic). */
+typedef struct {
+  const double *eta_t_tau, *eta_t_bar_tau, *udrho_tau, *forcing_bt, *thicku;
+  const double *patm, *pme, *river, *source, *eta_smooth, *rho_g;
+  double *eta_t_bar_taup1, *udrho_taup1, *conv_rho_ud_t;
+} mom_gpu_bt_fields_t;
+
+int mom_gpu_ensemble_create(int ndom, mom_gpu_ctx_t *const *ctxs, mom_gpu_ensemble_t **out);
+int mom_gpu_ensemble_destroy(mom_gpu_ensemble_t *ens);
+/* steps[d]: domain d's batched tracer step (see mom_gpu_step_tracers). */
+int mom_gpu_ensemble_step_tracers(mom_gpu_ensemble_t *ens, const mom_gpu_tracer_step_t *steps);
+/* p[d], f[d]: domain d's barotropic parameters and fields (ndom <= 64,
+ * cyclic_y = 0 for every domain). */
+int mom_gpu_ensemble_update_barotropic(mom_gpu_ensemble_t *ens, const mom_gpu_bt_params_t *p,
+                                       const mom_gpu_bt_fields_t *f);
+
 /* ---- hipGraph capture of whole steps -------------------------------------
  * No reference counterpart (MOM launches nothing): a caller captures the
  * entry points of one or more steps once and replays them.  Between begin

@@ -46,6 +46,12 @@ class TracerStep(C.Structure):
                 ("aidif", C.c_double)]
 
 
+class BtFields(C.Structure):
+    _fields_ = [(n, _vp) for n in ("eta_t_tau", "eta_t_bar_tau", "udrho_tau", "forcing_bt",
+                                    "thicku", "patm", "pme", "river", "source", "eta_smooth",
+                                    "rho_g", "eta_t_bar_taup1", "udrho_taup1", "conv_rho_ud_t")]
+
+
 # name -> (restype, argtypes)
 _PROTOS = {
     "mom_gpu_create": (C.c_int, [C.POINTER(Domain), C.POINTER(Grid), C.POINTER(_vp)]),
@@ -58,6 +64,11 @@ _PROTOS = {
     "mom_gpu_memcpy_dtoh": (C.c_int, [_vp, _vp, C.c_size_t]),
     "mom_gpu_memcpy_dtod": (C.c_int, [_vp, _vp, C.c_size_t]),
     "mom_gpu_copy": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
+    "mom_gpu_ensemble_create": (C.c_int, [C.c_int, C.POINTER(_vp), C.POINTER(_vp)]),
+    "mom_gpu_ensemble_destroy": (C.c_int, [_vp]),
+    "mom_gpu_ensemble_step_tracers": (C.c_int, [_vp, C.POINTER(TracerStep)]),
+    "mom_gpu_ensemble_update_barotropic": (C.c_int, [_vp, C.POINTER(BtParams),
+                                                     C.POINTER(BtFields)]),
     "mom_gpu_graph_begin": (C.c_int, [_vp]),
     "mom_gpu_graph_end": (C.c_int, [_vp, C.POINTER(C.c_void_p)]),
     "mom_gpu_graph_launch": (C.c_int, [_vp, _vp]),

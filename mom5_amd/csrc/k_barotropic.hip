@@ -10,6 +10,9 @@
 #include "mom_internal.h"
 
 #include <algorithm>
+#include <cstring>
+
+#include <algorithm>
 
 namespace {
 
@@ -199,16 +202,32 @@ struct BtIn {
   const int *kmt, *kmu;
 };
 
-__global__ __launch_bounds__(BNT) void k_bt_block(
-    KDom d, BtIn in, const double *__restrict__ eta_src, const double *__restrict__ ud_src,
-    double *__restrict__ eta_dst, double *__restrict__ ud_dst, double *__restrict__ eb,
-    double *__restrict__ ub, int s0, int nsub, int nts, int step_a, double dtbt, double cp,
-    double cc, int xbase, int xcount, int ntx, int ntiles) {
+// One launch of the blocked subcycle for one domain: nsub subcycles from s0.
+struct BtJob {
+  KDom d;
+  BtIn in;
+  const double *eta_src, *ud_src;
+  double *eta_dst, *ud_dst, *eb, *ub;
+  int s0, nsub, nts, step_a;
+  double dtbt, cp, cc;
+  int xbase, xcount, ntx, ntiles;
+};
+
+__device__ __forceinline__ void bt_tile(const BtJob &J, int tile) {
+  const KDom d = J.d;
+  const BtIn &in = J.in;
+  const double *__restrict__ eta_src = J.eta_src;
+  const double *__restrict__ ud_src = J.ud_src;
+  double *__restrict__ eta_dst = J.eta_dst;
+  double *__restrict__ ud_dst = J.ud_dst;
+  double *__restrict__ eb = J.eb;
+  double *__restrict__ ub = J.ub;
+  const int s0 = J.s0, nsub = J.nsub, nts = J.nts, step_a = J.step_a;
+  const double dtbt = J.dtbt, cp = J.cp, cc = J.cc;
+  const int xbase = J.xbase, xcount = J.xcount, ntx = J.ntx;
   __shared__ double s_ps[BRY][BRX];
   __shared__ double s_uh[BRY][BRX];
   __shared__ double s_vh[BRY][BRX];
-  const int tile = blockIdx.x;
-  if (tile >= ntiles) return;
   const int TX = BRX - 3 * nsub, TY = BRY - 3 * nsub;
   const int tix = tile % ntx, tiy = tile / ntx;
   const int x_int0 = xbase + tix * TX, y_int0 = tiy * TY;
@@ -347,6 +366,73 @@ __global__ __launch_bounds__(BNT) void k_bt_block(
   }
 }
 
+__global__ __launch_bounds__(BNT) void k_bt_block(BtJob J) {
+  if ((int)blockIdx.x >= J.ntiles) return;
+  bt_tile(J, blockIdx.x);
+}
+
+// Ensemble: launch ell of every domain that still has subcycles; the static
+// per-domain part is a device table, the per-launch tile prefix an argument.
+constexpr int BT_MAXDOM = 64;
+struct BtDom {
+  KDom d;
+  BtIn in;
+  const double *eta_bar_tau, *ud_tau;
+  double *eta_buf[2], *ud_buf[2];
+  double *eb, *ub;
+  int nts, step_a;
+  double dtbt, cp, cc;
+  int xbase, xcount;
+};
+struct BtLaunch {
+  int first[BT_MAXDOM + 1];   // first tile of each domain in this launch
+  int ndom, ell, hsub;
+};
+
+__device__ __host__ inline void bt_shape(const BtDom &D, int ell, int hsub, int &nsub, int &ntx,
+                                         int &ntiles) {
+  const int left = D.nts - ell * hsub;
+  nsub = left < hsub ? left : hsub;
+  if (nsub <= 0) { nsub = 0; ntx = 0; ntiles = 0; return; }
+  const int TX = BRX - 3 * nsub, TY = BRY - 3 * nsub;
+  ntx = (D.xcount + TX - 1) / TX;
+  ntiles = ntx * ((D.d.NJ + TY - 1) / TY);
+}
+
+__global__ __launch_bounds__(BNT) void k_bt_block_ens(const BtDom *__restrict__ doms, BtLaunch L) {
+  const int b = blockIdx.x;
+  if (b >= L.first[L.ndom]) return;
+  int lo = 0, hi = L.ndom - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (L.first[mid] <= b) lo = mid; else hi = mid - 1;
+  }
+  const BtDom &D = doms[lo];
+  BtJob J;
+  J.d = D.d;
+  J.in = D.in;
+  int nsub, ntx, ntiles;
+  bt_shape(D, L.ell, L.hsub, nsub, ntx, ntiles);
+  J.eta_src = L.ell == 0 ? D.eta_bar_tau : D.eta_buf[(L.ell - 1) & 1];
+  J.ud_src = L.ell == 0 ? D.ud_tau : D.ud_buf[(L.ell - 1) & 1];
+  J.eta_dst = D.eta_buf[L.ell & 1];
+  J.ud_dst = D.ud_buf[L.ell & 1];
+  J.eb = D.eb;
+  J.ub = D.ub;
+  J.s0 = L.ell * L.hsub + 1;
+  J.nsub = nsub;
+  J.nts = D.nts;
+  J.step_a = D.step_a;
+  J.dtbt = D.dtbt;
+  J.cp = D.cp;
+  J.cc = D.cc;
+  J.xbase = D.xbase;
+  J.xcount = D.xcount;
+  J.ntx = ntx;
+  J.ntiles = ntiles;
+  bt_tile(J, b - L.first[lo]);
+}
+
 __global__ void k_bt_acc_init(KDom d, int initsum, const double *__restrict__ eta_t,
                               const double *__restrict__ eta_bar_tau, double *__restrict__ eta_bar,
                               double *__restrict__ ud) {
@@ -356,6 +442,58 @@ __global__ void k_bt_acc_init(KDom d, int initsum, const double *__restrict__ et
   eta_bar[o] = initsum ? eta_bar_tau[o] : eta_t[o];
   ud[o] = 0.0;
   ud[o + d.P2] = 0.0;
+}
+
+// Ensemble tails: blockIdx.z is the domain.
+struct BtTail {
+  KDom d;
+  int initsum;
+  const double *eta_t, *eta_bar_tau;
+  double *eta_bar, *ud, *conv;
+  double rntsp1, rnts;
+  const double *dxu, *dyu, *datr;
+};
+
+__global__ void k_bt_acc_init_ens(const BtTail *__restrict__ T) {
+  const BtTail &t = T[blockIdx.z];
+  const int ii = blockIdx.x * blockDim.x + threadIdx.x, jj = blockIdx.y;
+  if (ii >= t.d.NI || jj >= t.d.NJ) return;
+  const int o = ii + t.d.NI * jj;
+  t.eta_bar[o] = t.initsum ? t.eta_bar_tau[o] : t.eta_t[o];
+  t.ud[o] = 0.0;
+  t.ud[o + t.d.P2] = 0.0;
+}
+
+__global__ void k_bt_scale_ens(const BtTail *__restrict__ T) {
+  const BtTail &t = T[blockIdx.z];
+  const int ii = blockIdx.x * blockDim.x + threadIdx.x, jj = blockIdx.y;
+  if (ii >= t.d.NI || jj >= t.d.NJ) return;
+  const int o = ii + t.d.NI * jj;
+  t.eta_bar[o] = t.rntsp1 * t.eta_bar[o];
+  t.ud[o] = t.rnts * t.ud[o];
+  t.ud[o + t.d.P2] = t.rnts * t.ud[o + t.d.P2];
+}
+
+// what: 0 = eta_bar and ud (after scaling), 1 = conv
+__global__ void k_bt_wrap_ens(const BtTail *__restrict__ T, int what) {
+  const BtTail &t = T[blockIdx.z];
+  const int jj = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  if (!t.d.cyclic_x || jj > t.d.NJ - 2) return;
+  if (what == 1 && !t.conv) return;
+  double *planes[3] = {what ? t.conv : t.eta_bar, t.ud, t.ud + t.d.P2};
+  const int np = what ? 1 : 3;
+  for (int c = 0; c < np; ++c) {
+    double *p = planes[c] + (size_t)t.d.NI * jj;
+    p[0] = p[t.d.NI - 2];
+    p[t.d.NI - 1] = p[1];
+  }
+}
+
+__global__ void k_bt_conv_ens(const BtTail *__restrict__ T) {
+  const BtTail &t = T[blockIdx.z];
+  const int ii = blockIdx.x * blockDim.x + threadIdx.x, jj = blockIdx.y;
+  if (!t.conv || ii >= t.d.NI || jj >= t.d.NJ) return;
+  t.conv[ii + t.d.NI * jj] = -div_ud(t.d, t.ud, t.ud + t.d.P2, t.dxu, t.dyu, t.datr, ii, jj);
 }
 
 }  // namespace
@@ -388,9 +526,9 @@ int launch_barotropic_blocked(mom_gpu_ctx *ctx, const mom_gpu_bt_params_t *p,
     const int nsub = std::min(hsub, p->nts - s0 + 1);
     const int TX = BRX - 3 * nsub, TY = BRY - 3 * nsub;
     const int ntx = (xcount + TX - 1) / TX, nty = (d.NJ + TY - 1) / TY;
-    hipLaunchKernelGGL(k_bt_block, dim3(ntx * nty), dim3(BNT), 0, s, d, in, eta_src, ud_src,
-                       eta_buf[which], ud_buf[which], eta_bar, ud, s0, nsub, p->nts,
-                       p->stepping_a, p->dtbt, c_pred, c_corr, xbase, xcount, ntx, ntx * nty);
+    const BtJob J{d, in, eta_src, ud_src, eta_buf[which], ud_buf[which], eta_bar, ud, s0, nsub,
+                  p->nts, p->stepping_a, p->dtbt, c_pred, c_corr, xbase, xcount, ntx, ntx * nty};
+    hipLaunchKernelGGL(k_bt_block, dim3(ntx * nty), dim3(BNT), 0, s, J);
     MOM_LAUNCHED(ctx, "k_bt_block");
     eta_src = eta_buf[which];
     ud_src = ud_buf[which];
@@ -465,5 +603,96 @@ int launch_barotropic(mom_gpu_ctx *ctx, const mom_gpu_bt_params_t *p, const doub
     if (d.cyclic_x) hipLaunchKernelGGL(k_bt_wrap, wg, wb, 0, s, d, conv, 1);
   }
   MOM_LAUNCHED(ctx, "k_bt_tail");
+  return MOM_GPU_OK;
+}
+
+// Barotropic subcycles of every domain of an ensemble: launch ell advances
+// hsub subcycles of each domain that has any left (its own nts, dtbt).
+int launch_barotropic_ensemble(mom_gpu_ensemble *e, const mom_gpu_bt_params_t *p,
+                               const mom_gpu_bt_fields_t *f) {
+  const int nd = (int)e->ctx.size();
+  if (nd > BT_MAXDOM) return mom_fail(e->lead, MOM_GPU_ERR_ARG, "ensemble: too many domains");
+  const int hsub = e->lead->tune.bt_subcycles_per_launch > 0 ? e->lead->tune.bt_subcycles_per_launch
+                                                             : 4;
+  std::vector<BtDom> doms(nd);
+  std::vector<BtTail> tails(nd);
+  std::memset((void *)doms.data(), 0, sizeof(BtDom) * nd);
+  std::memset((void *)tails.data(), 0, sizeof(BtTail) * nd);
+  int maxNI = 1, maxNJ = 1, nlaunch = 0;
+  for (int k = 0; k < nd; ++k) {
+    mom_gpu_ctx *ctx = e->ctx[k];
+    const KDom &d = ctx->kd;
+    const size_t P2 = (size_t)d.P2;
+    const mom_gpu_bt_fields_t &x = f[k];
+    const double rho0r = 1.0 / p[k].rho0;
+    BtDom &D = doms[k];
+    D.d = d;
+    D.in = BtIn{x.rho_g, x.patm, x.thicku, x.forcing_bt, ctx->f, x.pme, x.river, x.source,
+                x.eta_smooth, ctx->dxu, ctx->dyu, ctx->datr, ctx->dxur, ctx->dyur, ctx->kmt,
+                ctx->kmu};
+    D.eta_bar_tau = x.eta_t_bar_tau;
+    D.ud_tau = x.udrho_tau;
+    D.eta_buf[0] = ctx->bt;
+    D.eta_buf[1] = ctx->bt + P2;
+    D.ud_buf[0] = ctx->bt + 2 * P2;
+    D.ud_buf[1] = ctx->bt + 4 * P2;
+    D.eb = x.eta_t_bar_taup1;
+    D.ub = x.udrho_taup1;
+    D.nts = p[k].nts;
+    D.step_a = p[k].stepping_a;
+    D.dtbt = p[k].dtbt;
+    D.cp = p[k].dtbt * p[k].pred_corr_gamma * rho0r;   // dtbt_gamma_rho0r, :1196
+    D.cc = p[k].dtbt * rho0r;                          // dtbt_rho0r, :1197
+    D.xbase = d.cyclic_x ? 1 : 0;
+    D.xcount = d.cyclic_x ? d.ni : d.NI;
+    BtTail &T = tails[k];
+    T.d = d;
+    T.initsum = p[k].initsum_with_bar;
+    T.eta_t = x.eta_t_tau;
+    T.eta_bar_tau = x.eta_t_bar_tau;
+    T.eta_bar = x.eta_t_bar_taup1;
+    T.ud = x.udrho_taup1;
+    T.conv = x.conv_rho_ud_t;
+    const double nts = (double)p[k].nts;
+    T.rntsp1 = 1.0 / (nts + 1.0);
+    T.rnts = p[k].stepping_a ? 1.0 / nts : 2.0 / (nts * (double)(p[k].nts + 1));
+    T.dxu = ctx->dxu;
+    T.dyu = ctx->dyu;
+    T.datr = ctx->datr;
+    maxNI = std::max(maxNI, d.NI);
+    maxNJ = std::max(maxNJ, d.NJ);
+    nlaunch = std::max(nlaunch, (p[k].nts + hsub - 1) / hsub);
+  }
+  const BtDom *ddev = (const BtDom *)ens_table(e, doms.data(), sizeof(BtDom) * nd);
+  if (!ddev) return MOM_GPU_ERR_HIP;
+  const BtTail *tdev = (const BtTail *)ens_table(e, tails.data(), sizeof(BtTail) * nd);
+  if (!tdev) return MOM_GPU_ERR_HIP;
+  hipStream_t s = e->lead->stream;
+  const dim3 g2((maxNI + 255) / 256, maxNJ, nd), b2(256);
+  hipLaunchKernelGGL(k_bt_acc_init_ens, g2, b2, 0, s, tdev);
+  MOM_LAUNCHED(e->lead, "k_bt_acc_init_ens");
+  for (int ell = 0; ell < nlaunch; ++ell) {
+    BtLaunch L{};
+    L.ndom = nd;
+    L.ell = ell;
+    L.hsub = hsub;
+    int total = 0;
+    for (int k = 0; k < nd; ++k) {
+      int nsub, ntx, ntiles;
+      bt_shape(doms[k], ell, hsub, nsub, ntx, ntiles);
+      L.first[k] = total;
+      total += ntiles;
+    }
+    L.first[nd] = total;
+    if (total == 0) continue;
+    hipLaunchKernelGGL(k_bt_block_ens, dim3(total), dim3(BNT), 0, s, ddev, L);
+    MOM_LAUNCHED(e->lead, "k_bt_block_ens");
+  }
+  hipLaunchKernelGGL(k_bt_scale_ens, g2, b2, 0, s, tdev);
+  const dim3 gw((maxNJ + 255) / 256, 1, nd);
+  hipLaunchKernelGGL(k_bt_wrap_ens, gw, b2, 0, s, tdev, 0);
+  hipLaunchKernelGGL(k_bt_conv_ens, g2, b2, 0, s, tdev);
+  hipLaunchKernelGGL(k_bt_wrap_ens, gw, b2, 0, s, tdev, 1);
+  MOM_LAUNCHED(e->lead, "k_bt_tail_ens");
   return MOM_GPU_OK;
 }

@@ -3,6 +3,8 @@
 // the masked tracer content and the grid mask check.
 #include "mom_internal.h"
 
+#include <algorithm>
+#include <cstring>
 #include <vector>
 
 namespace {
@@ -85,14 +87,31 @@ struct TriGroup {
   int n;
 };
 
+// One group of one domain (the ensemble launches read these from a table).
 template <typename TT>
-__global__ void k_vdiff(KDom d, TriGroup<TT> g, const double *__restrict__ dcbase, double rho0,
-                        const int *__restrict__ kmt, const double *__restrict__ dh,
-                        const double *__restrict__ dhw, double tdt, double aidif,
-                        double *__restrict__ ework) {
-  const int gi = blockIdx.x * blockDim.x + threadIdx.x;
-  const int gj = blockIdx.y;
-  if (gi >= d.ni) return;
+struct VdJob {
+  KDom d;
+  TriGroup<TT> g;
+  const double *dcbase;
+  double rho0;
+  const int *kmt;
+  const double *dh, *dhw;
+  double tdt, aidif;
+  double *ework;
+  int nbx, nblocks;
+};
+
+template <typename TT>
+__device__ __forceinline__ void vdiff_col(const VdJob<TT> &J, int gi, int gj) {
+  const KDom d = J.d;
+  const TriGroup<TT> &g = J.g;
+  const double *__restrict__ dcbase = J.dcbase;
+  const double rho0 = J.rho0;
+  const int *__restrict__ kmt = J.kmt;
+  const double *__restrict__ dh = J.dh;
+  const double *__restrict__ dhw = J.dhw;
+  const double tdt = J.tdt, aidif = J.aidif;
+  double *__restrict__ ework = J.ework;
   const int nk = d.nk, P2 = d.P2;
   const int o = d2(d, gi, gj);
   const double tdt_aidif = tdt * aidif;
@@ -161,6 +180,13 @@ __global__ void k_vdiff(KDom d, TriGroup<TT> g, const double *__restrict__ dcbas
   }
 }
 
+template <typename TT>
+__global__ void k_vdiff(VdJob<TT> J) {
+  const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= J.d.ni) return;
+  vdiff_col(J, gi, blockIdx.y);
+}
+
 // FP32 storage: the same tridiagonal system solved for the increment,
 //   M dT = bflux - (M - I) T,   T(new) = (T + dT) * tmask,
 // with (M - I) T = a (T(k-1) - T(k)) + c (T(k+1) - T(k)) in FP64.  The forward
@@ -176,13 +202,30 @@ struct TriGroupF {
   int n;
 };
 
-__global__ void k_vdiff_inc(KDom d, TriGroupF g, const double *__restrict__ dcbase, double rho0,
-                            const int *__restrict__ kmt, const double *__restrict__ dh,
-                            const double *__restrict__ dhw, double tdt, double aidif,
-                            double *__restrict__ ework, float *__restrict__ scr) {
-  const int gi = blockIdx.x * blockDim.x + threadIdx.x;
-  const int gj = blockIdx.y;
-  if (gi >= d.ni) return;
+struct VdJobF {
+  KDom d;
+  TriGroupF g;
+  const double *dcbase;
+  double rho0;
+  const int *kmt;
+  const double *dh, *dhw;
+  double tdt, aidif;
+  double *ework;
+  float *scr;
+  int nbx, nblocks;
+};
+
+__device__ __forceinline__ void vdiff_inc_col(const VdJobF &J, int gi, int gj) {
+  const KDom d = J.d;
+  const TriGroupF &g = J.g;
+  const double *__restrict__ dcbase = J.dcbase;
+  const double rho0 = J.rho0;
+  const int *__restrict__ kmt = J.kmt;
+  const double *__restrict__ dh = J.dh;
+  const double *__restrict__ dhw = J.dhw;
+  const double tdt = J.tdt, aidif = J.aidif;
+  double *__restrict__ ework = J.ework;
+  float *__restrict__ scr = J.scr;
   const int nk = d.nk, P2 = d.P2;
   const size_t P3 = (size_t)P2 * nk;
   const int o = d2(d, gi, gj);
@@ -259,6 +302,54 @@ __global__ void k_vdiff_inc(KDom d, TriGroupF g, const double *__restrict__ dcba
       }
     }
   }
+}
+
+__global__ void k_vdiff_inc(VdJobF J) {
+  const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= J.d.ni) return;
+  vdiff_inc_col(J, gi, blockIdx.y);
+}
+
+// Ensemble: one group of every domain per launch (groups of one domain share
+// that domain's solver scratch, so they go in successive launches).
+constexpr int VD_MAXJ = 64;
+struct VdLaunch {
+  int first[VD_MAXJ + 1];
+  int njobs;
+};
+
+template <typename JOB>
+__device__ inline const JOB *vd_pick(const JOB *jobs, const VdLaunch &L, int &gi, int &gj) {
+  const int b = blockIdx.x;
+  if (b >= L.first[L.njobs]) return nullptr;
+  int lo = 0, hi = L.njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (L.first[mid] <= b) lo = mid; else hi = mid - 1;
+  }
+  const JOB &J = jobs[lo];
+  const int local = b - L.first[lo];
+  gi = (local % J.nbx) * 64 + threadIdx.x;
+  gj = local / J.nbx;
+  return gi < J.d.ni ? &J : nullptr;
+}
+
+__global__ __launch_bounds__(64) void k_vdiff_ens(const VdJob<double> *__restrict__ jobs,
+                                                  VdLaunch L) {
+  int gi, gj;
+  const VdJob<double> *J = vd_pick(jobs, L, gi, gj);
+  if (J) vdiff_col(*J, gi, gj);
+}
+
+__global__ __launch_bounds__(64) void k_vdiff_inc_ens(const VdJobF *__restrict__ jobs, VdLaunch L) {
+  int gi, gj;
+  const VdJobF *J = vd_pick(jobs, L, gi, gj);
+  if (J) vdiff_inc_col(*J, gi, gj);
+}
+
+__global__ void k_set_ints(int *__restrict__ p, int v, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
 }
 
 // ---- compute_tmask_limit, ocean_tracer.F90:2788-2892 -----------------------
@@ -512,9 +603,9 @@ static int vdiff_groups(mom_gpu_ctx *ctx, int ntr, void *const *T, const int *dt
     auto flush = [&](const double *k33) -> int {
       if (g.n == 0) return MOM_GPU_OK;
       g.k33 = k33;
-      hipLaunchKernelGGL(k_vdiff<TT>, dim3((d.ni + 63) / 64, d.nj), dim3(64), 0, ctx->stream, d,
-                         g, diff_cbt + p3 * (nmix - 1), rho0, ctx->kmt, dh, dhw, dt, aidif,
-                         ctx->tri_e);
+      VdJob<TT> J{d, g, diff_cbt + p3 * (nmix - 1), rho0, ctx->kmt, dh, dhw, dt, aidif,
+                  ctx->tri_e, 0, 0};
+      hipLaunchKernelGGL(k_vdiff<TT>, dim3((d.ni + 63) / 64, d.nj), dim3(64), 0, ctx->stream, J);
       MOM_LAUNCHED(ctx, "k_vdiff");
       g.n = 0;
       return MOM_GPU_OK;
@@ -577,9 +668,9 @@ int launch_vdiff_mixed(mom_gpu_ctx *ctx, int ntr, void *const *T, const int *dty
     TriGroupF g{};
     auto flush = [&]() {
       if (g.n == 0) return;
-      hipLaunchKernelGGL(k_vdiff_inc, dim3((d.ni + 63) / 64, d.nj), dim3(64), 0, ctx->stream, d,
-                         g, diff_cbt + p3 * (nmix - 1), rho0, ctx->kmt, dh, dhw, dt, aidif,
-                         ctx->tri_e, ctx->vd_scr);
+      VdJobF J{d, g, diff_cbt + p3 * (nmix - 1), rho0, ctx->kmt, dh, dhw, dt, aidif,
+               ctx->tri_e, ctx->vd_scr, 0, 0};
+      hipLaunchKernelGGL(k_vdiff_inc, dim3((d.ni + 63) / 64, d.nj), dim3(64), 0, ctx->stream, J);
       g.n = 0;
     };
     for (int n = 0; n < ntr; ++n) {
@@ -684,4 +775,109 @@ int launch_check_masks(mom_gpu_ctx *ctx, const double *tmask, const double *umas
   free(host);
   *bad = s;
   return MOM_GPU_OK;
+}
+
+int launch_set_ints(mom_gpu_ctx *ctx, int *p, int v, int n) {
+  hipLaunchKernelGGL(k_set_ints, dim3((n + 63) / 64), dim3(64), 0, ctx->stream, p, v, n);
+  MOM_LAUNCHED(ctx, "k_set_ints");
+  return MOM_GPU_OK;
+}
+
+// Implicit vertical diffusion of every domain of an ensemble: round r holds
+// the r-th solver group of each domain, so no two jobs of a launch share a
+// domain's scratch.
+template <typename JOB, typename KER>
+static int vd_rounds(mom_gpu_ensemble *e, std::vector<std::vector<JOB>> &groups, KER kernel,
+                     const char *what) {
+  size_t rounds = 0;
+  for (auto &g : groups) rounds = std::max(rounds, g.size());
+  for (size_t r = 0; r < rounds; ++r) {
+    std::vector<unsigned char> buf;
+    VdLaunch L{};
+    int total = 0, nj = 0;
+    for (auto &g : groups) {
+      if (g.size() <= r) continue;
+      const JOB &J = g[r];
+      const size_t at = buf.size();
+      buf.resize(at + sizeof(JOB));
+      std::memcpy(buf.data() + at, &J, sizeof(JOB));
+      L.first[nj++] = total;
+      total += J.nblocks;
+    }
+    L.first[nj] = total;
+    L.njobs = nj;
+    if (total == 0) continue;
+    const void *dev = ens_table(e, buf.data(), buf.size());
+    if (!dev) return MOM_GPU_ERR_HIP;
+    hipLaunchKernelGGL(kernel, dim3(total), dim3(64), 0, e->lead->stream, (const JOB *)dev, L);
+    MOM_LAUNCHED(e->lead, what);
+  }
+  return MOM_GPU_OK;
+}
+
+int launch_vdiff_ensemble(mom_gpu_ensemble *e, const mom_gpu_tracer_step_t *steps) {
+  const size_t nd = e->ctx.size();
+  if (nd > (size_t)VD_MAXJ) return mom_fail(e->lead, MOM_GPU_ERR_ARG, "ensemble: too many domains");
+  std::vector<std::vector<VdJob<double>>> g64(nd);
+  std::vector<std::vector<VdJobF>> g32(nd);
+  for (size_t dd = 0; dd < nd; ++dd) {
+    const mom_gpu_tracer_step_t &s = steps[dd];
+    if (!(s.stf && s.btf && s.aidif != 0.0) || s.ntr == 0) continue;
+    mom_gpu_ctx *ctx = e->ctx[dd];
+    const KDom &d = ctx->kd;
+    const size_t p3 = (size_t)d.P2 * d.nk;
+    const double *dh = s.rho_dzt_taup1 ? s.rho_dzt_taup1 : s.rho_dzt_tau;
+    const int nbx = (d.ni + 63) / 64;
+    bool any32 = false;
+    for (int n = 0; n < s.ntr; ++n) any32 = any32 || s.dtype[n] == 4;
+    if (any32 && !ctx->vd_scr) {
+      hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+      MOM_HIP(e->lead, hipStreamIsCapturing(e->lead->stream, &cs));
+      if (cs != hipStreamCaptureStatusNone)
+        return mom_fail(e->lead, MOM_GPU_ERR_ARG,
+                        "ensemble: the first step with FP32 tracers allocates solver scratch; "
+                        "run one step before capturing");
+      MOM_HIP(e->lead, hipMalloc((void **)&ctx->vd_scr, sizeof(float) * NTF * p3));
+    }
+    for (int nmix = 1; nmix <= 2; ++nmix) {
+      const double *dcb = s.diff_cbt + p3 * (nmix - 1);
+      VdJob<double> j64;
+      VdJobF j32;
+      std::memset((void *)&j64, 0, sizeof j64);
+      std::memset((void *)&j32, 0, sizeof j32);
+      auto flush64 = [&]() {
+        if (j64.g.n == 0) return;
+        j64.d = d; j64.dcbase = dcb; j64.rho0 = s.rho0; j64.kmt = ctx->kmt; j64.dh = dh;
+        j64.dhw = s.dzwt; j64.tdt = s.dtime; j64.aidif = s.aidif; j64.ework = ctx->tri_e;
+        j64.nbx = nbx; j64.nblocks = nbx * d.nj;
+        g64[dd].push_back(j64);
+        std::memset((void *)&j64, 0, sizeof j64);
+      };
+      auto flush32 = [&]() {
+        if (j32.g.n == 0) return;
+        j32.d = d; j32.dcbase = dcb; j32.rho0 = s.rho0; j32.kmt = ctx->kmt; j32.dh = dh;
+        j32.dhw = s.dzwt; j32.tdt = s.dtime; j32.aidif = s.aidif; j32.ework = ctx->tri_e;
+        j32.scr = ctx->vd_scr; j32.nbx = nbx; j32.nblocks = nbx * d.nj;
+        g32[dd].push_back(j32);
+        std::memset((void *)&j32, 0, sizeof j32);
+      };
+      for (int n = 0; n < s.ntr; ++n) {
+        if (((n == s.index_salt) ? 2 : 1) != nmix) continue;
+        if (s.dtype[n] == 8) {
+          TriGroup<double> &g = j64.g;
+          g.z[g.n] = (double *)s.T_taup1[n]; g.stf[g.n] = s.stf[n]; g.btf[g.n] = s.btf[n];
+          if (++g.n == NT) flush64();
+        } else {
+          TriGroupF &g = j32.g;
+          g.z[g.n] = (float *)s.T_taup1[n]; g.stf[g.n] = s.stf[n]; g.btf[g.n] = s.btf[n];
+          if (++g.n == NTF) flush32();
+        }
+      }
+      flush64();
+      flush32();
+    }
+  }
+  int rc = vd_rounds(e, g64, k_vdiff_ens, "k_vdiff_ens");
+  if (rc) return rc;
+  return vd_rounds(e, g32, k_vdiff_inc_ens, "k_vdiff_inc_ens");
 }

@@ -19,6 +19,9 @@
 // through by Rj (the 1e-30 guard is below FP32 resolution).
 #include "mom_internal.h"
 
+#include <algorithm>
+#include <cstring>
+
 namespace {
 
 constexpr int RX = 64;
@@ -92,23 +95,46 @@ struct StepTab {
   const double *th[8]; // other th_tendency contributions (NULL: none)
 };
 
+// Everything one pass of one domain needs: the batched kernel takes one
+// job by value; the ensemble kernel reads its jobs from a device table.
+template <typename TT>
+struct StepJob {
+  KDom d;
+  StepTab<TT> tab;
+  const double *U, *V, *W, *R, *Rm1, *Rr;
+  const int *kmt;
+  const double *dat, *datr, *dxte, *dyte, *dxtn, *dytn;
+  double dtime;
+  int ntx, ntiles;
+};
+
+// One 64-wide tile of one job, marched through all levels.  Every thread of
+// the block calls it with the same job and tile.
 template <typename TT, int G, int RPW, int NW>
-__global__ __launch_bounds__(RX * NW) void k_step(
-    KDom d, StepTab<TT> tab, const double *__restrict__ U, const double *__restrict__ V,
-    const double *__restrict__ W, const double *__restrict__ R, const double *__restrict__ Rm1,
-    const double *__restrict__ Rr, const int *__restrict__ kmt, const double *__restrict__ dat,
-    const double *__restrict__ datr, const double *__restrict__ dxte,
-    const double *__restrict__ dyte, const double *__restrict__ dxtn,
-    const double *__restrict__ dytn, double dtime, int ntx, int ntiles, int per_xcd) {
+__device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
+  const KDom d = J.d;
+  const StepTab<TT> &tab = J.tab;
+  const double *__restrict__ U = J.U;
+  const double *__restrict__ V = J.V;
+  const double *__restrict__ W = J.W;
+  const double *__restrict__ R = J.R;
+  const double *__restrict__ Rm1 = J.Rm1;
+  const double *__restrict__ Rr = J.Rr;
+  const int *__restrict__ kmt = J.kmt;
+  const double *__restrict__ dat = J.dat;
+  const double *__restrict__ datr = J.datr;
+  const double *__restrict__ dxte = J.dxte;
+  const double *__restrict__ dyte = J.dyte;
+  const double *__restrict__ dxtn = J.dxtn;
+  const double *__restrict__ dytn = J.dytn;
+  const double dtime = J.dtime;
+  const int ntx = J.ntx;
   constexpr bool EXACT = sizeof(TT) == 8;
   constexpr int RY = NW * RPW;
   __shared__ TT s_tx[G][RY][RX];
   __shared__ TT s_fy[G][RY][RX];
   __shared__ int s_km[RY][RX];
 
-  const int b = blockIdx.x;
-  const int tile = (b & 7) * per_xcd + (b >> 3);
-  if (tile >= ntiles) return;
   constexpr int TY = RY - 4;
   const int tix = tile % ntx, tiy = tile / ntx;
   const int gi0 = tix * TX, gj0 = tiy * TY;
@@ -384,6 +410,39 @@ __global__ __launch_bounds__(RX * NW) void k_step(
   }
 }
 
+
+template <typename TT, int G, int RPW, int NW>
+__global__ __launch_bounds__(RX * NW) void k_step(StepJob<TT> J, int per_xcd) {
+  const int b = blockIdx.x;
+  const int tile = (b & 7) * per_xcd + (b >> 3);   // XCD-aware: neighbours share an L2
+  if (tile >= J.ntiles) return;
+  step_tile<TT, G, RPW, NW>(J, tile);
+}
+
+// Ensemble pass: a persistent grid takes (job, tile) items from a device-side
+// counter in order (jobs sorted largest first) until none is left.  first[j]
+// is the index of job j's first item; total = sum of the jobs' tiles.  Every
+// block leaves the loop once the counter passes total.
+template <typename TT, int G, int RPW, int NW>
+__global__ __launch_bounds__(RX * NW) void k_step_queue(const StepJob<TT> *__restrict__ jobs,
+                                                        const int *__restrict__ first, int njobs,
+                                                        int total, int *__restrict__ counter) {
+  __shared__ int s_item;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int item = s_item;
+    if (item >= total) return;
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (first[mid] <= item) lo = mid; else hi = mid - 1;
+    }
+    step_tile<TT, G, RPW, NW>(jobs[lo], item - first[lo]);
+  }
+}
+
 template <typename TT, int G, int RPW, int NW = 8>
 void launch_group(mom_gpu_ctx *ctx, const StepTab<TT> &tab, const double *u, const double *v,
                   const double *w, const double *rho, const double *rm1, const double *rr,
@@ -394,9 +453,10 @@ void launch_group(mom_gpu_ctx *ctx, const StepTab<TT> &tab, const double *u, con
   const int nty = (d.nj + TY - 1) / TY;
   const int ntiles = ntx * nty;
   const int per = (ntiles + 7) / 8;
-  hipLaunchKernelGGL((k_step<TT, G, RPW, NW>), dim3(8 * per), dim3(RX * NW), 0, ctx->stream, d, tab,
-                     u, v, w, rho, rm1, rr, ctx->kmt, ctx->dat, ctx->datr, ctx->dxte, ctx->dyte,
-                     ctx->dxtn, ctx->dytn, dtime, ntx, ntiles, per);
+  const StepJob<TT> J{d, tab, u, v, w, rho, rm1, rr, ctx->kmt, ctx->dat, ctx->datr,
+                      ctx->dxte, ctx->dyte, ctx->dxtn, ctx->dytn, dtime, ntx, ntiles};
+  hipLaunchKernelGGL((k_step<TT, G, RPW, NW>), dim3(8 * per), dim3(RX * NW), 0, ctx->stream, J,
+                     per);
 }
 
 
@@ -457,6 +517,97 @@ static void launch_all(mom_gpu_ctx *ctx, int gmax, int rpw, int waves, int ntr,
   flush();
 }
 
+
+// ---- ensembles: one queue launch per (storage type, group size) -------------
+template <typename TT, int G, int RPW, int NW>
+static int queue_launch(mom_gpu_ensemble *e, std::vector<StepJob<TT>> &jobs, int slot) {
+  if (jobs.empty()) return MOM_GPU_OK;
+  // largest first; the table is compared byte for byte, so jobs are copied
+  // as bytes from their zero-initialised slots (no stray padding)
+  const size_t nj = jobs.size();
+  std::vector<size_t> order(nj);
+  for (size_t j = 0; j < nj; ++j) order[j] = j;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t a, size_t b) { return jobs[a].ntiles > jobs[b].ntiles; });
+  const size_t jbytes = (sizeof(StepJob<TT>) * nj + 255) / 256 * 256;
+  std::vector<unsigned char> buf(jbytes + sizeof(int) * nj, 0);
+  int *first = (int *)(buf.data() + jbytes);
+  int total = 0;
+  for (size_t j = 0; j < nj; ++j) {
+    std::memcpy(buf.data() + sizeof(StepJob<TT>) * j, &jobs[order[j]], sizeof(StepJob<TT>));
+    first[j] = total;
+    total += jobs[order[j]].ntiles;
+  }
+  const unsigned char *dev = (const unsigned char *)ens_table(e, buf.data(), buf.size());
+  if (!dev) return MOM_GPU_ERR_HIP;
+  // enough persistent groups to fill the device; late ones find the queue empty
+  const int grid = std::min(total, 2048);
+  hipLaunchKernelGGL((k_step_queue<TT, G, RPW, NW>), dim3(grid), dim3(RX * NW), 0,
+                     e->lead->stream, (const StepJob<TT> *)dev, (const int *)(dev + jbytes),
+                     (int)nj, total, e->counters + slot);
+  MOM_LAUNCHED(e->lead, "k_step_queue");
+  return MOM_GPU_OK;
+}
+
+template <typename TT>
+static int ensemble_type(mom_gpu_ensemble *e, const mom_gpu_tracer_step_t *steps, int want,
+                         int gmax, int rpw, int waves, int &slot) {
+  // jobs bucketed by group size: groups of 4, 2, 1 no larger than gmax, per domain
+  std::vector<StepJob<TT>> by[5];
+  for (size_t dd = 0; dd < e->ctx.size(); ++dd) {
+    mom_gpu_ctx *ctx = e->ctx[dd];
+    const mom_gpu_tracer_step_t &s = steps[dd];
+    const KDom &d = ctx->kd;
+    const double *rm1 = s.rho_dzt_taum1 ? s.rho_dzt_taum1 : s.rho_dzt_tau;
+    std::vector<int> idx;
+    for (int n = 0; n < s.ntr; ++n)
+      if (s.dtype[n] == want) idx.push_back(n);
+    size_t i = 0;
+    while (i < idx.size()) {
+      const size_t left = idx.size() - i;
+      const int c = (left >= 4 && gmax >= 4) ? 4 : (left >= 2 && gmax >= 2) ? 2 : 1;
+      StepTab<TT> t{};
+      for (int q = 0; q < c; ++q) {
+        const int n = idx[i + q];
+        t.T[q] = (const TT *)s.T_taum1[n];
+        t.out[q] = (TT *)s.T_taup1[n];
+        t.th[q] = s.th_tendency ? s.th_tendency[n] : nullptr;
+      }
+      const int w = (sizeof(TT) == 4 && waves == 16 && rpw == 2 && c <= 2) ? 16 : 8;
+      const int TY = w * rpw - 4;
+      const int ntx = (d.ni + TX - 1) / TX;
+      const int ntiles = ntx * ((d.nj + TY - 1) / TY);
+      by[c].emplace_back();
+      StepJob<TT> &J = by[c].back();
+      std::memset((void *)&J, 0, sizeof J);
+      J.d = d;
+      J.tab = t;
+      J.U = s.uhrho_et; J.V = s.vhrho_nt; J.W = s.wrho_bt; J.R = s.rho_dzt_tau; J.Rm1 = rm1;
+      J.Rr = s.rho_dztr;
+      J.kmt = ctx->kmt; J.dat = ctx->dat; J.datr = ctx->datr; J.dxte = ctx->dxte;
+      J.dyte = ctx->dyte; J.dxtn = ctx->dxtn; J.dytn = ctx->dytn;
+      J.dtime = s.dtime; J.ntx = ntx; J.ntiles = ntiles;
+      i += c;
+    }
+  }
+  int rc = MOM_GPU_OK;
+  const bool w16 = sizeof(TT) == 4 && waves == 16 && rpw == 2;
+  if (rpw == 2) {
+    rc = rc ? rc : queue_launch<TT, 4, 2, 8>(e, by[4], slot++);
+    if (w16) {
+      rc = rc ? rc : queue_launch<TT, 2, 2, 16>(e, by[2], slot++);
+      rc = rc ? rc : queue_launch<TT, 1, 2, 16>(e, by[1], slot++);
+    } else {
+      rc = rc ? rc : queue_launch<TT, 2, 2, 8>(e, by[2], slot++);
+      rc = rc ? rc : queue_launch<TT, 1, 2, 8>(e, by[1], slot++);
+    }
+  } else {
+    rc = rc ? rc : queue_launch<TT, 4, 4, 8>(e, by[4], slot++);
+    rc = rc ? rc : queue_launch<TT, 2, 4, 8>(e, by[2], slot++);
+    rc = rc ? rc : queue_launch<TT, 1, 4, 8>(e, by[1], slot++);
+  }
+  return rc;
+}
 }  // namespace
 
 // Advection + explicit update for ntr MDFL-Sweby tracers.  dtype[n]: 8 = FP64
@@ -472,4 +623,18 @@ int launch_tracer_step(mom_gpu_ctx *ctx, int ntr, const void *const *T_in, void 
                     rho_taum1, rho_dztr, dtime);
   MOM_LAUNCHED(ctx, "k_step");
   return MOM_GPU_OK;
+}
+
+// The tracer passes of every domain: one persistent queue launch per storage
+// type and group size (knobs from the lead context).
+int launch_step_ensemble(mom_gpu_ensemble *e, const mom_gpu_tracer_step_t *steps) {
+  mom_gpu_ctx *L = e->lead;
+  int rc = launch_set_ints(L, e->counters, 0, mom_gpu_ensemble::NCOUNT);
+  if (rc) return rc;
+  int slot = 0;
+  rc = ensemble_type<double>(e, steps, 8, L->tune.step_group_f64, L->tune.step_rows_per_wave, 8,
+                             slot);
+  if (rc) return rc;
+  return ensemble_type<float>(e, steps, 4, L->tune.step_group_f32, L->tune.step_rows_per_wave_f32,
+                              L->tune.step_waves_f32, slot);
 }

@@ -290,6 +290,99 @@ int mom_gpu_elapsed(mom_gpu_ctx_t *ctx, int s0, int s1, float *ms) {
   return MOM_GPU_OK;
 }
 
+// ---- ensembles -----------------------------------------------------------------
+}  // extern "C"
+
+const void *ens_table(mom_gpu_ensemble *e, const void *host, size_t bytes) {
+  for (auto &t : e->tables)
+    if (t.host.size() == bytes && std::memcmp(t.host.data(), host, bytes) == 0) return t.dev;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(e->lead->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) {
+    mom_fail(e->lead, MOM_GPU_ERR_ARG,
+             "ensemble: a new launch table while capturing; run one step of each time-level "
+             "parity before capturing");
+    return nullptr;
+  }
+  if (e->tables.size() >= 64) {             // keep the cache bounded
+    if (hipStreamSynchronize(e->lead->stream) != hipSuccess) return nullptr;
+    (void)hipFree(e->tables.front().dev);
+    e->tables.erase(e->tables.begin());
+  }
+  void *dev = nullptr;
+  hipError_t err = hipMalloc(&dev, bytes);
+  if (err == hipSuccess) err = hipMemcpy(dev, host, bytes, hipMemcpyHostToDevice);
+  if (err != hipSuccess) {
+    if (dev) (void)hipFree(dev);
+    mom_fail(e->lead, MOM_GPU_ERR_HIP, "ensemble table: %s", hipGetErrorString(err));
+    return nullptr;
+  }
+  const unsigned char *h = (const unsigned char *)host;
+  e->tables.push_back({std::vector<unsigned char>(h, h + bytes), dev});
+  return dev;
+}
+
+extern "C" {
+
+int mom_gpu_ensemble_create(int ndom, mom_gpu_ctx_t *const *ctxs, mom_gpu_ensemble_t **out) {
+  REQUIRE(nullptr, out && ctxs && ndom >= 1, "mom_gpu_ensemble_create: bad argument");
+  *out = nullptr;
+  REQUIRE(nullptr, ndom <= 64, "mom_gpu_ensemble_create: at most 64 domains (%d)", ndom);
+  for (int d = 0; d < ndom; ++d) {
+    REQUIRE(nullptr, ctxs[d], "mom_gpu_ensemble_create: NULL context %d", d);
+    REQUIRE(ctxs[0], ctxs[d]->device == ctxs[0]->device,
+            "mom_gpu_ensemble_create: contexts on different devices");
+  }
+  auto *e = new mom_gpu_ensemble();
+  e->ctx.assign(ctxs, ctxs + ndom);
+  e->lead = ctxs[0];
+  if (hipMalloc((void **)&e->counters, sizeof(int) * mom_gpu_ensemble::NCOUNT) != hipSuccess) {
+    delete e;
+    return mom_fail(ctxs[0], MOM_GPU_ERR_NOMEM, "mom_gpu_ensemble_create: counters");
+  }
+  *out = e;
+  return MOM_GPU_OK;
+}
+
+int mom_gpu_ensemble_destroy(mom_gpu_ensemble_t *e) {
+  if (!e) return MOM_GPU_OK;
+  (void)hipStreamSynchronize(e->lead->stream);
+  for (auto &t : e->tables) (void)hipFree(t.dev);
+  if (e->counters) (void)hipFree(e->counters);
+  delete e;
+  return MOM_GPU_OK;
+}
+
+static int check_step(mom_gpu_ctx *ctx, const mom_gpu_tracer_step_t *s, const char *who);
+
+int mom_gpu_ensemble_step_tracers(mom_gpu_ensemble_t *e, const mom_gpu_tracer_step_t *steps) {
+  REQUIRE(nullptr, e && steps, "mom_gpu_ensemble_step_tracers: NULL argument");
+  for (size_t d = 0; d < e->ctx.size(); ++d) {
+    const int rc = check_step(e->ctx[d], &steps[d], "mom_gpu_ensemble_step_tracers");
+    if (rc) return mom_fail(e->lead, rc, "domain %zu: %s", d, e->ctx[d]->err);
+  }
+  int rc = launch_step_ensemble(e, steps);
+  if (rc) return rc;
+  return launch_vdiff_ensemble(e, steps);
+}
+
+int mom_gpu_ensemble_update_barotropic(mom_gpu_ensemble_t *e, const mom_gpu_bt_params_t *p,
+                                       const mom_gpu_bt_fields_t *f) {
+  REQUIRE(nullptr, e && p && f, "mom_gpu_ensemble_update_barotropic: NULL argument");
+  for (size_t d = 0; d < e->ctx.size(); ++d) {
+    const mom_gpu_bt_fields_t &x = f[d];
+    REQUIRE(e->lead, p[d].nts >= 1 && p[d].dtbt > 0.0 && p[d].rho0 > 0.0,
+            "mom_gpu_ensemble_update_barotropic: domain %zu: bad parameters", d);
+    REQUIRE(e->lead, e->ctx[d]->kd.cyclic_y == 0,
+            "mom_gpu_ensemble_update_barotropic: domain %zu: cyclic_y unsupported", d);
+    REQUIRE(e->lead, x.eta_t_bar_tau && x.udrho_tau && x.forcing_bt && x.thicku && x.patm &&
+                         x.pme && x.river && x.source && x.eta_smooth && x.rho_g &&
+                         x.eta_t_bar_taup1 && x.udrho_taup1 &&
+                         (p[d].initsum_with_bar || x.eta_t_tau),
+            "mom_gpu_ensemble_update_barotropic: domain %zu: NULL required field", d);
+  }
+  return launch_barotropic_ensemble(e, p, f);
+}
+
 // ---- hipGraph capture of whole steps -----------------------------------------
 struct mom_gpu_graph {
   hipGraph_t graph;
@@ -509,31 +602,38 @@ int mom_gpu_update_ocean_barotropic(mom_gpu_ctx_t *ctx, const mom_gpu_bt_params_
                            eta_t_bar_taup1, udrho_taup1, conv_rho_ud_t);
 }
 
+static int check_step(mom_gpu_ctx *ctx, const mom_gpu_tracer_step_t *s, const char *who) {
+  REQUIRE(ctx, s, "%s: NULL descriptor", who);
+  REQUIRE(ctx, s->ntr >= 0 && (s->ntr == 0 || (s->T_taum1 && s->T_taup1 && s->dtype)),
+          "%s: NULL tracer tables", who);
+  REQUIRE(ctx, s->uhrho_et && s->vhrho_nt && s->wrho_bt && s->rho_dzt_tau,
+          "%s: NULL advection field", who);
+  REQUIRE(ctx, s->dtime > 0.0, "%s: dtime must be positive", who);
+  for (int n = 0; n < s->ntr; ++n) {
+    REQUIRE(ctx, s->dtype[n] == 8 || s->dtype[n] == 4, "%s: dtype[%d]=%d", who, n, s->dtype[n]);
+    REQUIRE(ctx, s->T_taum1[n] && s->T_taup1[n], "%s: tracer %d NULL", who, n);
+    REQUIRE(ctx, s->T_taum1[n] != (const void *)s->T_taup1[n],
+            "%s: tracer %d: taum1 and taup1 must differ", who, n);
+  }
+  if (s->stf && s->btf && s->aidif != 0.0) {
+    REQUIRE(ctx, s->dzwt && s->diff_cbt, "%s: dzwt/diff_cbt required", who);
+    for (int n = 0; n < s->ntr; ++n)
+      REQUIRE(ctx, s->stf[n] && s->btf[n], "%s: stf/btf %d NULL", who, n);
+  }
+  return MOM_GPU_OK;
+}
+
 int mom_gpu_step_tracers(mom_gpu_ctx_t *ctx, const mom_gpu_tracer_step_t *s) {
   REQUIRE(nullptr, ctx, "mom_gpu_step_tracers: NULL context");
-  REQUIRE(ctx, s, "mom_gpu_step_tracers: NULL descriptor");
-  REQUIRE(ctx, s->ntr >= 0 && (s->ntr == 0 || (s->T_taum1 && s->T_taup1 && s->dtype)),
-          "mom_gpu_step_tracers: NULL tracer tables");
-  REQUIRE(ctx, s->uhrho_et && s->vhrho_nt && s->wrho_bt && s->rho_dzt_tau,
-          "mom_gpu_step_tracers: NULL advection field");
-  REQUIRE(ctx, s->dtime > 0.0, "mom_gpu_step_tracers: dtime must be positive");
-  for (int n = 0; n < s->ntr; ++n) {
-    REQUIRE(ctx, s->dtype[n] == 8 || s->dtype[n] == 4, "mom_gpu_step_tracers: dtype[%d]=%d", n,
-            s->dtype[n]);
-    REQUIRE(ctx, s->T_taum1[n] && s->T_taup1[n], "mom_gpu_step_tracers: tracer %d NULL", n);
-    REQUIRE(ctx, s->T_taum1[n] != (const void *)s->T_taup1[n],
-            "mom_gpu_step_tracers: tracer %d: taum1 and taup1 must differ", n);
-  }
+  int rc = check_step(ctx, s, "mom_gpu_step_tracers");
+  if (rc) return rc;
   if (s->ntr == 0) return MOM_GPU_OK;
   const double *rm1 = s->rho_dzt_taum1 ? s->rho_dzt_taum1 : s->rho_dzt_tau;
-  int rc = launch_tracer_step(ctx, s->ntr, s->T_taum1, s->T_taup1, s->dtype, s->th_tendency,
-                              s->uhrho_et, s->vhrho_nt, s->wrho_bt, s->rho_dzt_tau, rm1,
-                              s->rho_dztr, s->dtime);
+  rc = launch_tracer_step(ctx, s->ntr, s->T_taum1, s->T_taup1, s->dtype, s->th_tendency,
+                          s->uhrho_et, s->vhrho_nt, s->wrho_bt, s->rho_dzt_tau, rm1,
+                          s->rho_dztr, s->dtime);
   if (rc) return rc;
   if (s->stf && s->btf && s->aidif != 0.0) {
-    REQUIRE(ctx, s->dzwt && s->diff_cbt, "mom_gpu_step_tracers: dzwt/diff_cbt required");
-    for (int n = 0; n < s->ntr; ++n)
-      REQUIRE(ctx, s->stf[n] && s->btf[n], "mom_gpu_step_tracers: stf/btf %d NULL", n);
     const double *dh = s->rho_dzt_taup1 ? s->rho_dzt_taup1 : s->rho_dzt_tau;
     rc = launch_vdiff_mixed(ctx, s->ntr, s->T_taup1, s->dtype, s->stf, s->btf, s->diff_cbt,
                             s->index_salt, s->rho0, dh, s->dzwt, s->dtime, s->aidif);

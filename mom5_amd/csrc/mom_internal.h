@@ -7,6 +7,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <cstdio>
+#include <vector>
 
 #include "mom5_gpu.h"
 
@@ -156,3 +157,26 @@ int launch_fill(mom_gpu_ctx *ctx, double *p, double v, size_t n);
 int launch_copy(mom_gpu_ctx *ctx, double *dst, const double *src, size_t n);
 int launch_check_masks(mom_gpu_ctx *ctx, const double *tmask,
                        const double *umask, long long *bad);
+
+// ---- ensembles of domains ---------------------------------------------------
+struct mom_gpu_ensemble {
+  std::vector<mom_gpu_ctx *> ctx;   // not owned; ctx[0] is the lead (stream, errors)
+  mom_gpu_ctx *lead = nullptr;
+  int *counters = nullptr;          // device work-queue counters
+  static constexpr int NCOUNT = 16;
+  struct Table {
+    std::vector<unsigned char> host;
+    void *dev;
+  };
+  std::vector<Table> tables;        // launch tables, shared by content
+};
+
+// Device copy of a launch table; identical tables (the two parities of
+// two-level time stepping) share one copy.  Uploading a new table is refused
+// while the lead stream is being captured.  nullptr on failure.
+const void *ens_table(mom_gpu_ensemble *e, const void *host, size_t bytes);
+int launch_step_ensemble(mom_gpu_ensemble *e, const mom_gpu_tracer_step_t *steps);
+int launch_vdiff_ensemble(mom_gpu_ensemble *e, const mom_gpu_tracer_step_t *steps);
+int launch_barotropic_ensemble(mom_gpu_ensemble *e, const mom_gpu_bt_params_t *p,
+                               const mom_gpu_bt_fields_t *f);
+int launch_set_ints(mom_gpu_ctx *ctx, int *p, int v, int n);

@@ -106,6 +106,33 @@ def _ptab(arrs):
     return tab
 
 
+def tracer_step_struct(T_taum1s, T_taup1s, *, uhrho_et, vhrho_nt, wrho_bt, rho_dzt_tau, dtime,
+                       stfs=None, btfs=None, dzwt=None, diff_cbt=None, index_salt=1, rho0=1035.0,
+                       aidif=1.0, th_tendencies=None, rho_dzt_taum1=None, rho_dztr=None,
+                       rho_dzt_taup1=None):
+    """The mom_gpu_tracer_step_t of one domain and the ctypes tables it
+    points to (keep both alive until the call returns)."""
+    n = len(T_taum1s)
+    dt = (C.c_int * max(n, 1))(*[_elem_size(a) for a in T_taum1s])
+    keep = [dt]
+
+    def tab(xs):
+        if xs is None:
+            return None
+        t = _ptab(xs)
+        keep.append(t)
+        return C.cast(t, C.POINTER(C.c_void_p))
+
+    s = TracerStep(ntr=n, T_taum1=tab(T_taum1s), T_taup1=tab(T_taup1s), dtype=dt,
+                   th_tendency=tab(th_tendencies), stf=tab(stfs), btf=tab(btfs),
+                   index_salt=int(index_salt), uhrho_et=_p(uhrho_et), vhrho_nt=_p(vhrho_nt),
+                   wrho_bt=_p(wrho_bt), rho_dzt_tau=_p(rho_dzt_tau),
+                   rho_dzt_taum1=_p(rho_dzt_taum1), rho_dztr=_p(rho_dztr),
+                   rho_dzt_taup1=_p(rho_dzt_taup1), dzwt=_p(dzwt), diff_cbt=_p(diff_cbt),
+                   rho0=float(rho0), dtime=float(dtime), aidif=float(aidif))
+    return s, keep
+
+
 class Graph:
     """An instantiated hipGraph of captured entry-point calls."""
 
@@ -303,24 +330,12 @@ class OceanGPU:
                      rho_dztr=None, rho_dzt_taup1=None):
         """Advection (MDFL-Sweby), explicit update and implicit vertical
         diffusion of every tracer; each tracer's dtype is its array's."""
-        n = len(T_taum1s)
-        dt = (C.c_int * max(n, 1))(*[_elem_size(a) for a in T_taum1s])
-        keep = []
-
-        def tab(xs):
-            if xs is None:
-                return None
-            t = _ptab(xs)
-            keep.append(t)
-            return C.cast(t, C.POINTER(C.c_void_p))
-
-        s = TracerStep(ntr=n, T_taum1=tab(T_taum1s), T_taup1=tab(T_taup1s), dtype=dt,
-                       th_tendency=tab(th_tendencies), stf=tab(stfs), btf=tab(btfs),
-                       index_salt=int(index_salt), uhrho_et=_p(uhrho_et), vhrho_nt=_p(vhrho_nt),
-                       wrho_bt=_p(wrho_bt), rho_dzt_tau=_p(rho_dzt_tau),
-                       rho_dzt_taum1=_p(rho_dzt_taum1), rho_dztr=_p(rho_dztr),
-                       rho_dzt_taup1=_p(rho_dzt_taup1), dzwt=_p(dzwt), diff_cbt=_p(diff_cbt),
-                       rho0=float(rho0), dtime=float(dtime), aidif=float(aidif))
+        s, keep = tracer_step_struct(T_taum1s, T_taup1s, uhrho_et=uhrho_et, vhrho_nt=vhrho_nt,
+                                     wrho_bt=wrho_bt, rho_dzt_tau=rho_dzt_tau, dtime=dtime,
+                                     stfs=stfs, btfs=btfs, dzwt=dzwt, diff_cbt=diff_cbt,
+                                     index_salt=index_salt, rho0=rho0, aidif=aidif,
+                                     th_tendencies=th_tendencies, rho_dzt_taum1=rho_dzt_taum1,
+                                     rho_dztr=rho_dztr, rho_dzt_taup1=rho_dzt_taup1)
         self._chk(self.L.mom_gpu_step_tracers(self.ctx, C.byref(s)), "step_tracers")
 
     # ---- hipGraph capture --------------------------------------------------------

@@ -188,17 +188,34 @@ class BatchedStep(_Captured):
         """Device array of tracer n at the current time level."""
         return self.T[n][self.cur]
 
+    def step_args(self):
+        """Keyword arguments of this step's tracer call at the current level."""
+        d, c = self.dev, self.case
+        now, nxt = self.cur, 1 - self.cur
+        return dict(T_taum1s=[p[now] for p in self.T], T_taup1s=[p[nxt] for p in self.T],
+                    uhrho_et=d["uhrho_et"], vhrho_nt=d["vhrho_nt"], wrho_bt=d["wrho_bt"],
+                    rho_dzt_tau=d["rho_dzt"], rho_dztr=d["rho_dztr"], dtime=c.dt,
+                    stfs=self.stf, btfs=self.btf, dzwt=d["dzwt"] if self.diffuse else None,
+                    diff_cbt=d["diff_cbt"] if self.diffuse else None,
+                    index_salt=self.index_salt)
+
+    def bt_args(self):
+        """Barotropic parameters and fields of this step at the current level."""
+        d, c = self.dev, self.case
+        now, nxt = self.cur, 1 - self.cur
+        return dict(nts=c.nts, dtbt=c.dtbt, eta_t_bar_tau=self.eta_bar[now],
+                    udrho_tau=self.udrho[now], forcing_bt=d["forcing_bt"], thicku=d["thicku"],
+                    patm=d["patm"], pme=d["pme"], river=d["river"], source=d["source"],
+                    eta_smooth=d["eta_smooth"], rho_g=d["rho_g"],
+                    eta_t_bar_taup1=self.eta_bar[nxt], udrho_taup1=self.udrho[nxt],
+                    conv_rho_ud_t=self.conv)
+
     def run(self, nsteps=1):
         g, d, c = self.gpu, self.dev, self.case
         for _ in range(nsteps):
             now, nxt = self.cur, 1 - self.cur
-            g.step_tracers([p[now] for p in self.T], [p[nxt] for p in self.T],
-                           uhrho_et=d["uhrho_et"], vhrho_nt=d["vhrho_nt"], wrho_bt=d["wrho_bt"],
-                           rho_dzt_tau=d["rho_dzt"], rho_dztr=d["rho_dztr"], dtime=c.dt,
-                           stfs=self.stf, btfs=self.btf,
-                           dzwt=d["dzwt"] if self.diffuse else None,
-                           diff_cbt=d["diff_cbt"] if self.diffuse else None,
-                           index_salt=self.index_salt)
+            a = self.step_args()
+            g.step_tracers(a.pop("T_taum1s"), a.pop("T_taup1s"), **a)
             if self.barotropic:
                 g.update_ocean_barotropic(
                     nts=c.nts, dtbt=c.dtbt, eta_t_bar_tau=self.eta_bar[now],
