@@ -46,6 +46,8 @@ WORKLOADS = {
     2: dict(name="cfg2_batched_fp64_1440x1080x50", ni=1440, nj=1080, nk=50, land_frac=0.30,
             nts=80, n_passive=30, mixed=True, passive_dtype=np.float64, steps=20, warmup=2,
             schemes=["mdfl_sweby"] * 32, sample=dict(ni=240, nj=180)),
+    4: dict(name="cfg4_ensemble_24_regional", ndom=24, n_passive=6, ensemble=True, steps=20,
+            warmup=2, sample=dict(ni=200, nj=150, nk=40, nts=80, regional=True)),
     1: dict(name="cfg1_plain_fp64_1440x1080x50", ni=1440, nj=1080, nk=50, land_frac=0.30,
             nts=80, n_passive=1, schemes=["mdfl_sweby", "mdfl_sweby", "quicker"],
             sample=dict(ni=480, nj=360)),
@@ -56,6 +58,11 @@ WORKLOADS = {
 
 def make(w, ni=None, nj=None, seed=0x4D4F4D35):
     from mom5_amd.synth import make_case
+    s = w.get("sample", {})
+    if s.get("regional"):              # a closed regional basin like the ensemble's members
+        return make_case(ni or s["ni"], nj or s["nj"], s["nk"], cyclic_x=False, lat_s=20.0,
+                         lat_n=20.0 + 0.1 * (nj or s["nj"]), lon_span=0.1 * (ni or s["ni"]),
+                         land_frac=0.25, nts=s["nts"], n_passive=w["n_passive"], seed=seed)
     return make_case(ni or w["ni"], nj or w["nj"], w["nk"], land_frac=w["land_frac"],
                      nts=w["nts"] or 80, n_passive=w["n_passive"], barotropic=w["nts"] > 0,
                      seed=seed)
@@ -69,7 +76,8 @@ def mdfl_bytes_per_launch(ni, nj, nk):
 
 
 def cpu_baseline(w, steps=8):
-    """The CPU restatement on a bounded sample of the same workload."""
+    """The CPU restatement on a bounded sample of the same workload: at least
+    `steps` steps after one warm-up and at least 10 s of timed work."""
     from oracle.oracle import Oracle
     s = w["sample"]
     case = make(w, ni=s["ni"], nj=s["nj"], seed=0x4D4F4D35 + 7)
@@ -85,23 +93,24 @@ def cpu_baseline(w, steps=8):
         out = []
         for n, T in enumerate(Ts):
             th = np.zeros_like(T)
-            if w["schemes"][n] == "quicker":
+            if (w.get("schemes") or ["mdfl_sweby"] * len(Ts))[n] == "quicker":
                 th[:, 1:-1, 1:-1] -= o.horz_quicker(T, T, lim[n])
                 th[:, 1:-1, 1:-1] -= o.vert_quicker(T, T, lim[n])
             else:
                 o.horz_advect_mdfl(T, th, case.dt, lim=1.0)
             out.append(o.update_explicit(T, th, case.dt))
         o.vert_diffuse_implicit(out, f["stf"], f["btf"], case.dt, index_salt=1)
-        if w["nts"]:
+        if case.dtbt > 0:
             o.update_ocean_barotropic(case.nts, case.dtbt)
         return out
 
     Ts = step(Ts)                       # warm-up
     times = []
-    for _ in range(steps):
+    while len(times) < steps or (sum(times) < 10.0 and len(times) < 64):
         t0 = time.perf_counter()
         Ts = step(Ts)
         times.append(time.perf_counter() - t0)
+    steps = len(times)
     dt = float(np.median(times))
     cells = case.ni * case.nj * case.nk * len(Ts)
     return dict(value=cells / dt, unit="tracer-cell-updates/s", cores=1, kind="port",
@@ -145,6 +154,96 @@ def time_launches(g, fn, reps=10):
         fn()
     g.mark(3)
     return g.elapsed_ms(2, 3) / reps
+
+
+def run_ensemble(args, w):
+    """Config 4: 24 seeded regional domains (T, S FP64 + 6 FP32 passive
+    each, own dt and subcycle count) advanced as one ensemble, timed from
+    graph replays, against the same domains one after another on the plain
+    per-tracer FP64 path."""
+    import dataclasses
+    import gc
+    from mom5_amd.ensemble import EnsembleStep, make_regional_ensemble, regional_shapes
+    from mom5_amd.step import BatchedStep, PlainStep
+    steps = w["steps"] if args.steps is None else args.steps
+    warm = w["warmup"] if args.warmup is None else args.warmup
+    shapes = regional_shapes(w["ndom"])
+    t0 = time.time()
+    cases = make_regional_ensemble(shapes, n_passive=w["n_passive"])
+    members = [BatchedStep(c, gpu=c.info["gpu"]) for c in cases]
+    ens = EnsembleStep(members)
+    g = ens.gpu
+    ncells = ens.cells()
+    ntc = ens.tracer_cells()
+    log(f"generated {len(cases)} regional domains ({ncells / 1e6:.1f} M cells, "
+        f"{ntc / 1e6:.0f} M tracer-cells) in {time.time() - t0:.1f}s")
+    ens.run(warm)
+    g.sync()
+    g.mark(0)
+    ens.run(2)
+    g.mark(1)
+    ms_eager = g.elapsed_ms(0, 1) / 2
+    graph = ens.capture(2)
+    g.mark(0)
+    for _ in range(steps // 2):
+        graph.launch()
+    if steps % 2:
+        ens.run(1)
+    g.mark(1)
+    ms = g.elapsed_ms(0, 1) / steps
+    nodes = graph.nodes
+    graph.close()
+    warm += 2
+    value = ntc / (ms / 1e3)
+    log(f"ensemble step {ms:.2f} ms from graph replays ({nodes} nodes per 2 steps; eager "
+        f"{ms_eager:.2f} ms) = {value:.3e} tracer-cell-updates/s")
+
+    # the same domains one after another on the plain FP64 path
+    total_plain_ms = 0.0
+    ts_differ = 0
+    worst = 0.0
+    for k, c in enumerate(cases):
+        m = members[k]
+        mg = c.info["gpu"]
+        pc = dataclasses.replace(c, tracers=[(nm, c.info["tracer"](i, np.float64))
+                                             for i, (nm, _) in enumerate(c.tracers)])
+        plain = PlainStep(pc, schemes=["mdfl_sweby"] * m.ntr, gpu=mg)
+        plain.run(warm)
+        mg.sync()
+        mg.mark(0)
+        plain.run(steps)
+        mg.mark(1)
+        total_plain_ms += mg.elapsed_ms(0, 1) / steps
+        for n in range(m.ntr):
+            e = mg.tracer_error(m.tracer(n), plain.tracer(n))
+            if n < 2:
+                ts_differ += e["cells_differ"]
+            else:
+                worst = max(worst, e["normwise"])
+        del plain, pc
+        gc.collect()
+    value_plain = ntc / (total_plain_ms / 1e3)
+    log(f"plain FP64, domains one after another: {total_plain_ms:.2f} ms per step "
+        f"({value_plain:.3e}); T/S cells differing {ts_differ}; worst passive error {worst:.2e}")
+    step_bytes = sum(mixed_step_bytes(c.ni, c.nj, c.nk, 2, len(c.tracers) - 2) for c in cases)
+    achieved = step_bytes / (ms / 1e3) / 1e9
+    roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                kernel="whole ensemble step (all phases)", bytes_per_launch=step_bytes)
+    extra = dict(timed_as="hipGraph replays of 2 captured ensemble steps",
+                 graph_nodes_per_2_steps=nodes, eager_ms_per_step=ms_eager,
+                 plain_fp64_sequential_value=value_plain,
+                 plain_fp64_sequential_ms_per_step=total_plain_ms,
+                 speedup_vs_plain_fp64_sequential=value / value_plain,
+                 ts_cells_differing_from_plain_fp64=ts_differ,
+                 passive_max_rel_err_normwise=worst, errors_after_steps=warm + steps)
+    cfg = {"workload": w["name"], "domains": len(cases),
+           "shapes": [[s["ni"], s["nj"], s["nk"], s["nts"]] for s in shapes],
+           "cells": ncells, "tracers_per_domain": {"f64": 2, "f32": w["n_passive"]},
+           "path": "ensemble (mom_gpu_ensemble_*: work-queue tracer passes)",
+           "tuning": {k: g.get_tuning(k) for k in TUNING_KEYS}}
+    return dict(value=value, ms=ms, steps=steps, warmup=warm, dtype="f64+f32", cfg=cfg,
+                roof=roof, extra=extra, ntr=None)
 
 
 def measured_traffic(workload, key):
@@ -437,8 +536,11 @@ def main():
     if args.profile_kernels:
         profile_kernels(args, w)
         return
-    r = run_mixed(args, w) if w.get("mixed") else run_plain(args, w)
+    if not w.get("ensemble"):
+        r = run_mixed(args, w) if w.get("mixed") else run_plain(args, w)
 
+    if w.get("ensemble") and not args.profile_kernels:
+        r = run_ensemble(args, w)
     cpu = None
     if not args.no_cpu_baseline:
         log("timing the CPU baseline sample")

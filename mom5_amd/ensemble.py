@@ -95,7 +95,7 @@ def regional_shapes(ndom=24, seed=0x4D4F4D35 + 4, nx=(200, 800), ny=(150, 600), 
         nk = int(rng.integers(nz[0], nz[1] + 1))
         ns = int(rng.integers(nts[0], nts[1] + 1))
         ns += ns % 2                      # even subcycle counts: dt = dtbt*nts/2
-        lat_s = float(rng.uniform(-60.0, 40.0))
+        lat_s = float(rng.uniform(-75.0, 75.0 - 0.1 * nj))   # stay clear of the poles
         out.append(dict(ni=ni, nj=nj, nk=nk, nts=ns, lat_s=lat_s,
                         lat_n=lat_s + 0.1 * nj, lon_span=0.1 * ni,
                         seed=int(rng.integers(1, 2 ** 31))))

@@ -103,6 +103,8 @@ def _horizontal(rng, ni, nj, nk, lat_s, lat_n, lon_span, cyclic_x, land_frac):
     horizontal streamfunction psi and overturning amplitude ov, and the
     x-face depth index kface with its depth hface."""
     NI, NJ = ni + 2, nj + 2
+    if not -90.0 < lat_s < lat_n < 90.0:
+        raise ValueError(f"latitudes must satisfy -90 < lat_s < lat_n < 90 ({lat_s}, {lat_n})")
     dlon = lon_span / ni
     dlat = (lat_n - lat_s) / nj
     ii = np.arange(NI, dtype=np.float64)          # Fortran i = 0..ni+1
