@@ -50,8 +50,10 @@ __device__ inline FaceC<double> face_coef(double massflux, double cfl) {
 }
 
 // reference-order limited flux (FP64), lim = 1
+// mab = ma*mb of the two cells of the face: for masks in {0, 1},
+// x*ma*mb == x*(ma*mb) bit for bit (signed zeros and NaN included)
 __device__ inline double flux_exact(const FaceC<double> &c, double Rjm, double Rj, double Rjp,
-                                    double Tpos, double Tneg, double ma, double mb) {
+                                    double Tpos, double Tneg, double mab) {
   const double thetaP = Rjm / (EPS30 + Rj);
   const double thetaM = Rjp / (EPS30 + Rj);
   const double pP = c.d0 + c.d1 * thetaP;
@@ -60,32 +62,35 @@ __device__ inline double flux_exact(const FaceC<double> &c, double Rjm, double R
   const double psiP = pP * 0.0 + fmx(0.0, fmn(fmn(1.0, c.d0 + c.d1 * thetaP), c.kk * thetaP)) * 1.0;
   const double psiM = pM * 0.0 + fmx(0.0, fmn(fmn(1.0, c.d0 + c.d1 * thetaM), c.kk * thetaM)) * 1.0;
   return 0.5 * ((c.M + fabs(c.M)) * (Tpos + psiP * Rj) + (c.M - fabs(c.M)) * (Tneg - psiM * Rj)) *
-         ma * mb;
+         mab;
 }
 
 // FP32 groups: the same coefficients in single precision, with the
 // hardware reciprocal (about 1 ulp) in place of the divisions
 __device__ inline float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
+// FP32 code uses explicit fmaf: the same fused operations in every kernel
+// that inlines it, so FP32 results never depend on the launch shape
+// (contraction left to the compiler could differ between instantiations).
 __device__ inline FaceC<float> face_coef_f(float massflux, float cfl) {
   FaceC<float> c;
   c.M = massflux;
   c.d0 = (2.0f - cfl) * (1.0f - cfl) * (float)ONESIXTH;
-  c.d1 = (1.0f - cfl * cfl) * (float)ONESIXTH;
+  c.d1 = fmaf(-cfl, cfl, 1.0f) * (float)ONESIXTH;
   c.kk = (1.0f - cfl) * rcpf(1.0e-30f + cfl);
   return c;
 }
 
 // division-free limited flux (FP32)
 __device__ inline float psi_r(const FaceC<float> &c, float a, float r) {
-  const float lin = c.d0 * a + c.d1 * r;
+  const float lin = fmaf(c.d0, a, c.d1 * r);
   const float kr = c.kk * r;
   return (a >= 0.0f) ? fmxf(0.0f, fmnf(fmnf(a, lin), kr)) : fmnf(0.0f, fmxf(fmxf(a, lin), kr));
 }
 __device__ inline float flux_fast(const FaceC<float> &c, float Rjm, float Rj, float Rjp, float Tpos,
-                                  float Tneg, float ma, float mb) {
+                                  float Tneg, float mab) {
   const float pP = psi_r(c, Rj, Rjm);
   const float pM = psi_r(c, Rj, Rjp);
-  return 0.5f * ((c.M + fabsf(c.M)) * (Tpos + pP) + (c.M - fabsf(c.M)) * (Tneg - pM)) * ma * mb;
+  return 0.5f * fmaf(c.M + fabsf(c.M), Tpos + pP, (c.M - fabsf(c.M)) * (Tneg - pM)) * mab;
 }
 
 template <typename TT>
@@ -206,16 +211,17 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
         rhov[q] = r; wv[q] = w;
         const TT mkm1 = mk<TT>(km1, km[q]), mk0 = mk<TT>(k, km[q]);
         const TT mkp1 = mk<TT>(kp1, km[q]), mkp2 = mk<TT>(kp2, km[q]);
+        const TT mA = mkm1 * mk0, mB = mk0 * mkp1, mC = mkp1 * mkp2;
         if constexpr (EXACT) {
           const FaceC<double> cz = face_coef(datv[q] * w, fabs(w * dtime / r));
           const double dtr = dtime / r;
           const double wdiff = wkm1[q] - w;
 #pragma unroll
           for (int g = 0; g < G; ++g) {
-            const double Rjp = (t0[g][q] - t1[g][q]) * mkm1 * mk0;
-            const double Rj = (t1[g][q] - t2[g][q]) * mk0 * mkp1;
-            const double Rjm = (t2[g][q] - t3[g][q]) * mkp1 * mkp2;
-            const double fbt = flux_exact(cz, Rjm, Rj, Rjp, t2[g][q], t1[g][q], mkp1, mk0);
+            const double Rjp = (t0[g][q] - t1[g][q]) * mA;
+            const double Rj = (t1[g][q] - t2[g][q]) * mB;
+            const double Rjm = (t2[g][q] - t3[g][q]) * mC;
+            const double fbt = flux_exact(cz, Rjm, Rj, Rjp, t2[g][q], t1[g][q], mB);
             tz[g][q] = t1[g][q] + dtr * (datrv[q] * (fbt - ftp[g][q]) + t1[g][q] * wdiff);
             fbtv[g][q] = fbt;
           }
@@ -223,16 +229,16 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
           const float ri = rcpf((float)r);
           rinv[q] = ri;
           const float wf = (float)w;
-          const FaceC<float> c = face_coef_f((float)(datv[q] * w), fabsf(wf * (float)dtime * ri));
+          const FaceC<float> c = face_coef_f((float)datv[q] * wf, fabsf(wf * (float)dtime * ri));
           const float dtrf = (float)dtime * ri, datrf = (float)datrv[q];
           const float wdiff = (float)(wkm1[q] - w);
 #pragma unroll
           for (int g = 0; g < G; ++g) {
-            const float Rjp = (t0[g][q] - t1[g][q]) * mkm1 * mk0;
-            const float Rj = (t1[g][q] - t2[g][q]) * mk0 * mkp1;
-            const float Rjm = (t2[g][q] - t3[g][q]) * mkp1 * mkp2;
-            const float fbt = flux_fast(c, Rjm, Rj, Rjp, t2[g][q], t1[g][q], mkp1, mk0);
-            tz[g][q] = t1[g][q] + dtrf * (datrf * (fbt - ftp[g][q]) + t1[g][q] * wdiff);
+            const float Rjp = (t0[g][q] - t1[g][q]) * mA;
+            const float Rj = (t1[g][q] - t2[g][q]) * mB;
+            const float Rjm = (t2[g][q] - t3[g][q]) * mC;
+            const float fbt = flux_fast(c, Rjm, Rj, Rjp, t2[g][q], t1[g][q], mB);
+            tz[g][q] = fmaf(dtrf, fmaf(datrf, fbt - ftp[g][q], t1[g][q] * wdiff), t1[g][q]);
             fbtv[g][q] = fbt;
           }
         }
@@ -256,14 +262,17 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
         const int of = d2(d, gi, sjr[q]);
         const double u = U[of + P2 * k0];
         const double rw = R[of + P2 * k0], re = R[of + 1 + P2 * k0];
-        if constexpr (EXACT)
+        if constexpr (EXACT) {
           cx = face_coef(dyte[of] * u, fabs(u * dtime * 2.0 / ((rw + re) * dxte[of])));
-        else
-          cxf = face_coef_f((float)(dyte[of] * u),
-                            fabsf((float)u * (float)(dtime * 2.0) * rcpf((float)((rw + re) * dxte[of]))));
+        } else {
+          const float uf = (float)u;
+          cxf = face_coef_f((float)dyte[of] * uf, fabsf(uf * (float)(dtime * 2.0) *
+                                                        rcpf(((float)rw + (float)re) * (float)dxte[of])));
+        }
       }
       const TT mi = mk<TT>(k, km[q]), mim1 = mk<TT>(k, kmm1);
       const TT mip1 = mk<TT>(k, kmp1), mip2 = mk<TT>(k, kmp2);
+      const TT mX2 = mip2 * mip1, mX1 = mip1 * mi, mX0 = mi * mim1;
       const bool cell = rowok[q] && lane >= 2 && lane < RX - 2 && gi < d.ni;
       double mw64 = 0.0;
       if constexpr (EXACT) {
@@ -287,13 +296,13 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
         const TT tzp2 = __shfl(tz[g][q], lane + 2);
         TT fx = TT(0);
         if (face) {
-          const TT Rjp = (tzp2 - tzp1) * mip2 * mip1;
-          const TT Rj = (tzp1 - tz[g][q]) * mip1 * mi;
-          const TT Rjm = (tz[g][q] - tzm1) * mi * mim1;
+          const TT Rjp = (tzp2 - tzp1) * mX2;
+          const TT Rj = (tzp1 - tz[g][q]) * mX1;
+          const TT Rjm = (tz[g][q] - tzm1) * mX0;
           if constexpr (EXACT)
-            fx = flux_exact(cx, Rjm, Rj, Rjp, tz[g][q], tzp1, mi, mip1);
+            fx = flux_exact(cx, Rjm, Rj, Rjp, tz[g][q], tzp1, mX1);
           else
-            fx = flux_fast(cxf, Rjm, Rj, Rjp, tz[g][q], tzp1, mi, mip1);
+            fx = flux_fast(cxf, Rjm, Rj, Rjp, tz[g][q], tzp1, mX1);
         }
         const TT fxw = __shfl(fx, lane - 1);
         dxf[g][q] = fxw - fx;
@@ -302,7 +311,7 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
           if constexpr (EXACT)
             tx = tz[g][q] + xf64v[q] * (fxw - fx + t1[g][q] * (cx.M - mw64));
           else
-            tx = tz[g][q] + xfac[q] * (fxw - fx + t1[g][q] * (mfe[q] - mfw[q]));
+            tx = fmaf(xfac[q], fmaf(t1[g][q], mfe[q] - mfw[q], fxw - fx), tz[g][q]);
         }
         s_tx[g][ry][lane] = tx;
         txv[g][q] = tx;
@@ -323,28 +332,31 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
         const int of = d2(d, gi, gj);
         const double v = V[of + P2 * k0];
         const double rs = R[of + P2 * k0], rn = R[of + d.NI + P2 * k0];
-        if constexpr (EXACT)
+        if constexpr (EXACT) {
           cy = face_coef(dxtn[of] * v, fabs(v * dtime * 2.0 / ((rs + rn) * dytn[of])));
-        else
-          cyf = face_coef_f((float)(dxtn[of] * v),
-                            fabsf((float)v * (float)(dtime * 2.0) * rcpf((float)((rs + rn) * dytn[of]))));
+        } else {
+          const float vf = (float)v;
+          cyf = face_coef_f((float)dxtn[of] * vf, fabsf(vf * (float)(dtime * 2.0) *
+                                                        rcpf(((float)rs + (float)rn) * (float)dytn[of])));
+        }
       }
       const TT mjm1 = mk<TT>(k, s_km[max(ry - 1, 0)][lane]), mj = mk<TT>(k, s_km[ry][lane]);
       const TT mjp1 = mk<TT>(k, s_km[min(ry + 1, RY - 1)][lane]);
       const TT mjp2 = mk<TT>(k, s_km[min(ry + 2, RY - 1)][lane]);
+      const TT mY2 = mjp2 * mjp1, mY1 = mjp1 * mj, mY0 = mj * mjm1;
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         TT fy = TT(0);
         if (face) {
           const TT tjm1 = s_tx[g][ry - 1][lane], tj = s_tx[g][ry][lane];
           const TT tjp1 = s_tx[g][ry + 1][lane], tjp2 = s_tx[g][ry + 2][lane];
-          const TT Rjp = (tjp2 - tjp1) * mjp2 * mjp1;
-          const TT Rj = (tjp1 - tj) * mjp1 * mj;
-          const TT Rjm = (tj - tjm1) * mj * mjm1;
+          const TT Rjp = (tjp2 - tjp1) * mY2;
+          const TT Rj = (tjp1 - tj) * mY1;
+          const TT Rjm = (tj - tjm1) * mY0;
           if constexpr (EXACT)
-            fy = flux_exact(cy, Rjm, Rj, Rjp, tj, tjp1, mj, mjp1);
+            fy = flux_exact(cy, Rjm, Rj, Rjp, tj, tjp1, mY1);
           else
-            fy = flux_fast(cyf, Rjm, Rj, Rjp, tj, tjp1, mj, mjp1);
+            fy = flux_fast(cyf, Rjm, Rj, Rjp, tj, tjp1, mY1);
         }
         s_fy[g][ry][lane] = fy;
         fyv[g][q] = fy;
@@ -432,7 +444,7 @@ __global__ __launch_bounds__(RX * NW) void k_step_queue(const StepJob<TT> *__res
     __syncthreads();
     if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
     __syncthreads();
-    const int item = s_item;
+    const int item = __builtin_amdgcn_readfirstlane(s_item);
     if (item >= total) return;
     int lo = 0, hi = njobs - 1;
     while (lo < hi) {
@@ -594,10 +606,13 @@ static int ensemble_type(mom_gpu_ensemble *e, const mom_gpu_tracer_step_t *steps
   const bool w16 = sizeof(TT) == 4 && waves == 16 && rpw == 2;
   if (rpw == 2) {
     rc = rc ? rc : queue_launch<TT, 4, 2, 8>(e, by[4], slot++);
-    if (w16) {
-      rc = rc ? rc : queue_launch<TT, 2, 2, 16>(e, by[2], slot++);
-      rc = rc ? rc : queue_launch<TT, 1, 2, 16>(e, by[1], slot++);
-    } else {
+    if constexpr (sizeof(TT) == 4) {
+      if (w16) {
+        rc = rc ? rc : queue_launch<TT, 2, 2, 16>(e, by[2], slot++);
+        rc = rc ? rc : queue_launch<TT, 1, 2, 16>(e, by[1], slot++);
+      }
+    }
+    if (!w16) {
       rc = rc ? rc : queue_launch<TT, 2, 2, 8>(e, by[2], slot++);
       rc = rc ? rc : queue_launch<TT, 1, 2, 8>(e, by[1], slot++);
     }
