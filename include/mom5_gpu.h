@@ -208,6 +208,26 @@ int mom_gpu_vert_diffuse_implicit(mom_gpu_ctx_t *ctx, int ntr,
     const double *rho_dzt_taup1, const double *dzwt, double dtime_t,
     double aidif);
 
+/* ---- advection velocity ------------------------------------------------ */
+
+/* Replaces the B-grid branch of ocean_advection_velocity
+ * (ocean_core/ocean_advection_velocity.F90:568-672; no blobs, OBC or
+ * prescribed inflow): the T-cell advection transports from the U-cell
+ * velocities and continuity,
+ *   uhrho_et = BAY(u(:,:,k,1)*dyu*rho_dzu)/dyte,
+ *   vhrho_nt = BAX(u(:,:,k,2)*dxu*rho_dzu)/dxtn,
+ *   diverge_t = tmask*(BDX_ET(uhrho_et) + BDY_NT(vhrho_nt)),
+ *   wrho_bt(0) = -(pme+river),
+ *   wrho_bt(k) = (rho_dzt_tendency - mass_source + diverge_t + wrho_bt(k-1))*tmask.
+ * u holds Velocity%u(:,:,:,1:2,tau) back to back, rho_dzu
+ * Thickness%rho_dzu(:,:,:,tau).  rho_dzt_tendency, mass_source (zero) and
+ * diverge_t (not written) may be NULL.  The U-cell components (uhrho_eu,
+ * vhrho_nu, wrho_bu) feed the momentum equations and are not computed. */
+int mom_gpu_advection_velocity(mom_gpu_ctx_t *ctx, const double *u,
+    const double *rho_dzu, const double *pme, const double *river,
+    const double *rho_dzt_tendency, const double *mass_source,
+    double *uhrho_et, double *vhrho_nt, double *wrho_bt, double *diverge_t);
+
 /* ---- barotropic -------------------------------------------------------- */
 
 /* Replaces update_ocean_barotropic (ocean_barotropic.F90:2763-2894) for the

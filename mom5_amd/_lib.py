@@ -64,6 +64,7 @@ _PROTOS = {
     "mom_gpu_memcpy_dtoh": (C.c_int, [_vp, _vp, C.c_size_t]),
     "mom_gpu_memcpy_dtod": (C.c_int, [_vp, _vp, C.c_size_t]),
     "mom_gpu_copy": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
+    "mom_gpu_advection_velocity": (C.c_int, [_vp] + [_vp] * 10),
     "mom_gpu_ensemble_create": (C.c_int, [C.c_int, C.POINTER(_vp), C.POINTER(_vp)]),
     "mom_gpu_ensemble_destroy": (C.c_int, [_vp]),
     "mom_gpu_ensemble_step_tracers": (C.c_int, [_vp, C.POINTER(TracerStep)]),

@@ -142,6 +142,18 @@ int mom_gpu_fill(mom_gpu_ctx_t *ctx, double *ptr, double value, size_t n) {
   return launch_fill(ctx, ptr, value, n);
 }
 
+int mom_gpu_advection_velocity(mom_gpu_ctx_t *ctx, const double *u, const double *rho_dzu,
+                               const double *pme, const double *river,
+                               const double *rho_dzt_tendency, const double *mass_source,
+                               double *uhrho_et, double *vhrho_nt, double *wrho_bt,
+                               double *diverge_t) {
+  REQUIRE(nullptr, ctx, "mom_gpu_advection_velocity: NULL context");
+  REQUIRE(ctx, u && rho_dzu && pme && river && uhrho_et && vhrho_nt && wrho_bt,
+          "mom_gpu_advection_velocity: NULL required array");
+  return launch_adv_vel(ctx, u, rho_dzu, pme, river, rho_dzt_tendency, mass_source, uhrho_et,
+                        vhrho_nt, wrho_bt, diverge_t);
+}
+
 int mom_gpu_copy(mom_gpu_ctx_t *ctx, double *dst, const double *src, size_t n) {
   REQUIRE(nullptr, ctx, "mom_gpu_copy: NULL context");
   REQUIRE(ctx, (dst && src) || n == 0, "mom_gpu_copy: NULL pointer");

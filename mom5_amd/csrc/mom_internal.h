@@ -154,6 +154,9 @@ int launch_content(mom_gpu_ctx *ctx, const void *T, int dtype, const double *rho
 int launch_tracer_error(mom_gpu_ctx *ctx, const void *a, int dtype, const double *b,
                         double floor, double *out);
 int launch_fill(mom_gpu_ctx *ctx, double *p, double v, size_t n);
+int launch_adv_vel(mom_gpu_ctx *ctx, const double *u, const double *rho_dzu, const double *pme,
+                   const double *river, const double *rtend, const double *msrc, double *ue,
+                   double *vn, double *wb, double *div);
 int launch_copy(mom_gpu_ctx *ctx, double *dst, const double *src, size_t n);
 int launch_check_masks(mom_gpu_ctx *ctx, const double *tmask,
                        const double *umask, long long *bad);

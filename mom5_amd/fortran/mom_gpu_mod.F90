@@ -73,6 +73,7 @@ module mom_gpu_mod
   public :: mom_gpu_tracer_content, mom_gpu_check
   public :: mom_gpu_step_tracers, mom_gpu_tracer_content_f32, mom_gpu_tracer_error
   public :: mom_gpu_copy, mom_gpu_memcpy_dtod, mom_gpu_get_tuning
+  public :: mom_gpu_advection_velocity
 
   interface
      integer(c_int) function mom_gpu_create(dom, grid, ctx) bind(C, name="mom_gpu_create")
@@ -235,6 +236,15 @@ module mom_gpu_mod
        type(c_ptr), value :: ctx, T, rho_dzt
        real(c_double), intent(out) :: content
      end function mom_gpu_tracer_content
+
+     ! ocean_advection_velocity, ocean_core/ocean_advection_velocity.F90:568-672
+     integer(c_int) function mom_gpu_advection_velocity(ctx, u, rho_dzu, pme, river, &
+          rho_dzt_tendency, mass_source, uhrho_et, vhrho_nt, wrho_bt, diverge_t) &
+          bind(C, name="mom_gpu_advection_velocity")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, u, rho_dzu, pme, river, rho_dzt_tendency, mass_source
+       type(c_ptr), value :: uhrho_et, vhrho_nt, wrho_bt, diverge_t
+     end function mom_gpu_advection_velocity
 
      ! the batched step (all MDFL-Sweby tracers, FP64 or FP32 storage each)
      integer(c_int) function mom_gpu_step_tracers(ctx, s) bind(C, name="mom_gpu_step_tracers")

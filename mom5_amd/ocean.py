@@ -252,6 +252,16 @@ class OceanGPU:
             a.upload(value)
         return a
 
+    # ---- advection velocity ------------------------------------------------
+    def advection_velocity(self, u, rho_dzu, pme, river, uhrho_et, vhrho_nt, wrho_bt,
+                           rho_dzt_tendency=None, mass_source=None, diverge_t=None):
+        """ocean_advection_velocity (B-grid): T-cell transports and wrho_bt
+        from the U-cell velocities u (two components back to back)."""
+        self._chk(self.L.mom_gpu_advection_velocity(
+            self.ctx, _p(u), _p(rho_dzu), _p(pme), _p(river), _p(rho_dzt_tendency),
+            _p(mass_source), _p(uhrho_et), _p(vhrho_nt), _p(wrho_bt), _p(diverge_t)),
+            "advection_velocity")
+
     # ---- tracer advection ------------------------------------------------
     def horz_advect_tracer(self, scheme, T_taum1, th_tendency, *, uhrho_et, vhrho_nt,
                            wrho_bt=None, rho_dzt_tau=None, T_tau=None, dtime=0.0,

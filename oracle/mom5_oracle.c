@@ -918,3 +918,88 @@ double orc_tracer_content(const orc_dom *d, const orc_grid *g,
     }
   return (double)tot;
 }
+
+/* ---- ocean_advection_velocity, B-grid branch ----------------------------
+ * ocean_core/ocean_advection_velocity.F90:568-672 (no blobs, OBC or inflow):
+ *   uhrho_et = BAY(u(:,:,k,1)*dyu*rho_dzu(k))/dyte              (:601-602)
+ *   vhrho_nt = BAX(u(:,:,k,2)*dxu*rho_dzu(k))/dxtn              (:603-604)
+ *   diverge_t = tmask*(BDX_ET(uhrho_et) + BDY_NT(vhrho_nt))     (:607-608)
+ *   wrho_bt(0) = -(pme + river)                                 (:615)
+ *   wrho_bt(k) = (tmp + diverge_t + wrho_bt(k-1))*tmask,
+ *   tmp = rho_dzt_tendency - mass_source                        (:633-636)
+ * with the operators of ocean_core/ocean_operators.F90 (halo = 1):
+ *   BAX :834-846, BAY :889-902, BDX_ET :945-958, BDY_NT :1230-1243.
+ * Each operator is evaluated over the whole data domain into its own array,
+ * as the reference's array-valued functions are. */
+static void orc_bay(const orc_dom *d, const double *a, double *out) {
+  for (int j = d->jsd + 1; j <= d->jed; ++j)
+    for (int i = d->isd; i <= d->ied; ++i) out[I2(i, j)] = 0.5 * (a[I2(i, j)] + a[I2(i, j - 1)]);
+  for (int i = d->isd; i <= d->ied; ++i) out[I2(i, d->jsd)] = 0.0;
+}
+
+static void orc_bax(const orc_dom *d, const double *a, double *out) {
+  for (int j = d->jsd; j <= d->jed; ++j) {
+    for (int i = d->isd + 1; i <= d->ied; ++i)
+      out[I2(i, j)] = 0.5 * (a[I2(i, j)] + a[I2(i - 1, j)]);
+    out[I2(d->isd, j)] = 0.0;
+  }
+}
+
+void orc_advection_velocity(const orc_dom *d, const orc_grid *g, const double *u,
+                            const double *rho_dzu, const double *pme, const double *river,
+                            const double *rho_dzt_tendency, const double *mass_source,
+                            double *uhrho_et, double *vhrho_nt, double *wrho_bt,
+                            double *diverge_t) {
+  const size_t p2 = P2, p3 = p2 * (size_t)d->nk;
+  double *a = (double *)malloc(sizeof(double) * p2);
+  double *b = (double *)malloc(sizeof(double) * p2);
+  double *bdx = (double *)malloc(sizeof(double) * p2);
+  double *bdy = (double *)malloc(sizeof(double) * p2);
+  double *div = (double *)malloc(sizeof(double) * p3);
+  for (int k = 1; k <= d->nk; ++k) {
+    for (int j = d->jsd; j <= d->jed; ++j)
+      for (int i = d->isd; i <= d->ied; ++i)
+        a[I2(i, j)] = u[I3(i, j, k)] * g->dyu[I2(i, j)] * rho_dzu[I3(i, j, k)];
+    orc_bay(d, a, b);
+    for (int j = d->jsd; j <= d->jed; ++j)
+      for (int i = d->isd; i <= d->ied; ++i) uhrho_et[I3(i, j, k)] = b[I2(i, j)] / g->dyte[I2(i, j)];
+    for (int j = d->jsd; j <= d->jed; ++j)
+      for (int i = d->isd; i <= d->ied; ++i)
+        a[I2(i, j)] = u[p3 + I3(i, j, k)] * g->dxu[I2(i, j)] * rho_dzu[I3(i, j, k)];
+    orc_bax(d, a, b);
+    for (int j = d->jsd; j <= d->jed; ++j)
+      for (int i = d->isd; i <= d->ied; ++i) vhrho_nt[I3(i, j, k)] = b[I2(i, j)] / g->dxtn[I2(i, j)];
+    /* BDX_ET(uhrho_et) and BDY_NT(vhrho_nt) */
+    for (int j = d->jsd; j <= d->jed; ++j) {
+      for (int i = d->isd + 1; i <= d->ied; ++i)
+        bdx[I2(i, j)] = (g->dyte[I2(i, j)] * uhrho_et[I3(i, j, k)] -
+                         g->dyte[I2(i - 1, j)] * uhrho_et[I3(i - 1, j, k)]) * g->datr[I2(i, j)];
+      bdx[I2(d->isd, j)] = 0.0;
+    }
+    for (int j = d->jsd + 1; j <= d->jed; ++j)
+      for (int i = d->isd; i <= d->ied; ++i)
+        bdy[I2(i, j)] = (g->dxtn[I2(i, j)] * vhrho_nt[I3(i, j, k)] -
+                         g->dxtn[I2(i, j - 1)] * vhrho_nt[I3(i, j - 1, k)]) * g->datr[I2(i, j)];
+    for (int i = d->isd; i <= d->ied; ++i) bdy[I2(i, d->jsd)] = 0.0;
+    for (int j = d->jsd; j <= d->jed; ++j)
+      for (int i = d->isd; i <= d->ied; ++i)
+        div[I3(i, j, k)] = g->tmask[I3(i, j, k)] * (bdx[I2(i, j)] + bdy[I2(i, j)]);
+  }
+  for (int j = d->jsd; j <= d->jed; ++j)
+    for (int i = d->isd; i <= d->ied; ++i) wrho_bt[I3W(i, j, 0)] = -(pme[I2(i, j)] + river[I2(i, j)]);
+  for (int k = 1; k <= d->nk; ++k)
+    for (int j = d->jsd; j <= d->jed; ++j)
+      for (int i = d->isd; i <= d->ied; ++i) {
+        const double rt = rho_dzt_tendency ? rho_dzt_tendency[I3(i, j, k)] : 0.0;
+        const double ms = mass_source ? mass_source[I3(i, j, k)] : 0.0;
+        const double tmp = rt - ms;
+        wrho_bt[I3W(i, j, k)] = (tmp + div[I3(i, j, k)] + wrho_bt[I3W(i, j, k - 1)]) *
+                                g->tmask[I3(i, j, k)];
+      }
+  if (diverge_t) memcpy(diverge_t, div, sizeof(double) * p3);
+  free(a);
+  free(b);
+  free(bdx);
+  free(bdy);
+  free(div);
+}

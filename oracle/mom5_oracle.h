@@ -141,6 +141,15 @@ double orc_tracer_content(const orc_dom *d, const orc_grid *g,
  * space is (isc-h:iec+h, jsc-h:jec+h, nz).  flags: 1 = x, 2 = y, 3 = both. */
 void orc_halo_update(const orc_dom *d, int h, double *a, int nz, int flags);
 
+/* ocean_core/ocean_advection_velocity.F90:568-672, B-grid branch: T-cell
+ * advection transports from the U-cell velocities u (two 3-D components
+ * back to back) and continuity.  rho_dzt_tendency, mass_source (3-D) and
+ * diverge_t (output) may be NULL.  All arrays on the data domain. */
+void orc_advection_velocity(const orc_dom *d, const orc_grid *g, const double *u,
+    const double *rho_dzu, const double *pme, const double *river,
+    const double *rho_dzt_tendency, const double *mass_source,
+    double *uhrho_et, double *vhrho_nt, double *wrho_bt, double *diverge_t);
+
 #ifdef __cplusplus
 }
 #endif

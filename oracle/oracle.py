@@ -208,3 +208,22 @@ class Oracle:
         rd = self.case.fields["rho_dzt"] if rho_dzt is None else rho_dzt
         self.L.orc_tracer_content.restype = C.c_double
         return self.L.orc_tracer_content(C.byref(self.dom), C.byref(self.grid), _p(T), _p(rd))
+
+    # --- advection velocity ------------------------------------------
+    def advection_velocity(self, u, rho_dzu, pme=None, river=None, rho_dzt_tendency=None,
+                           mass_source=None):
+        """ocean_advection_velocity.F90:568-672 (B-grid): u is (2, nk, NJ, NI).
+        Returns uhrho_et, vhrho_nt, wrho_bt (nk+1 levels), diverge_t."""
+        z2 = np.zeros(self.shape2)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        rho_dzu = np.ascontiguousarray(rho_dzu, dtype=np.float64)
+        pme = z2 if pme is None else np.ascontiguousarray(pme)
+        river = z2 if river is None else np.ascontiguousarray(river)
+        ue, vn, div = (np.zeros(self.shape3) for _ in range(3))
+        wb = np.zeros((self.case.nk + 1,) + self.shape2)
+        self.L.orc_advection_velocity(
+            C.byref(self.dom), C.byref(self.grid), _p(u), _p(rho_dzu), _p(pme), _p(river),
+            _p(rho_dzt_tendency) if rho_dzt_tendency is not None else None,
+            _p(mass_source) if mass_source is not None else None, _p(ue), _p(vn), _p(wb),
+            _p(div))
+        return ue, vn, wb, div
