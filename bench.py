@@ -326,7 +326,7 @@ def run_mixed(args, w):
         grp = g.get_tuning("step_group_f64" if want64 else "step_group_f32")
         rows = g.get_tuning("step_rows_per_wave" if want64 else "step_rows_per_wave_f32")
         waves = 8 if want64 else g.get_tuning("step_waves_f32")
-        if not want64 and (waves == 16 and (rows != 2 or grp > 2)):
+        if not want64 and waves == 16 and rows != 2:
             waves = 8
         key = f"k_step<{'double' if want64 else 'float'}, {grp}, {rows}, {waves}>"
         idx = [n for n in range(ntr) if is64[n] == want64][:grp]

@@ -120,7 +120,7 @@ int mom_gpu_copy(mom_gpu_ctx_t *ctx, double *dst, const double *src, size_t n);
  *   "step_rows_per_wave"               2 or 4: tile height of the FP64 passes
  *   "step_rows_per_wave_f32"           2 or 4: tile height of the FP32 passes
  *   "step_waves_f32"                   8 or 16 wavefronts per FP32 tile (16 is
- *                                      used with 2 rows and 1-2 tracers a pass)
+ *                                      used with 2 rows a wavefront)
  * Unknown knobs and out-of-range values return MOM_GPU_ERR_ARG. */
 int mom_gpu_set_tuning(mom_gpu_ctx_t *ctx, const char *knob, int value);
 int mom_gpu_get_tuning(const mom_gpu_ctx_t *ctx, const char *knob, int *value);
@@ -230,6 +230,16 @@ int mom_gpu_advection_velocity(mom_gpu_ctx_t *ctx, const double *u,
 
 /* ---- barotropic -------------------------------------------------------- */
 
+/* Replaces ocean_barotropic_forcing (ocean_core/ocean_barotropic.F90:
+ * 3058-3141) with the default namelist (no zeroed forcing, no barotropic
+ * friction, no OBC) on the B-grid: forcing_bt(:,:,n) = sum over the wet
+ * U-levels of accel(:,:,k,n) + umask(k=1)*(smf(:,:,n) - bmf(:,:,n)) on the
+ * compute domain, halos as the single-PE update leaves them.  accel holds
+ * Velocity%accel (data domain, nk, 2 components); smf, bmf and forcing_bt
+ * (data domain, 2 components). */
+int mom_gpu_barotropic_forcing(mom_gpu_ctx_t *ctx, const double *accel,
+    const double *smf, const double *bmf, double *forcing_bt);
+
 /* Replaces update_ocean_barotropic (ocean_barotropic.F90:2763-2894) for the
  * B-grid depth-based case: pred_corr_tropic_depth_bgrid (:3225-3579, with
  * barotropic_halo=1 and use_legacy_barotropic_halos=.true., the defaults;
@@ -247,6 +257,77 @@ int mom_gpu_update_ocean_barotropic(mom_gpu_ctx_t *ctx,
     const double *pme, const double *river, const double *source,
     const double *eta_smooth, const double *rho_g,
     double *eta_t_bar_taup1, double *udrho_taup1, double *conv_rho_ud_t);
+
+/* Replaces eta_and_pbot_tendency (ocean_barotropic.F90:2667-2726, no
+ * blobs, no zero_tendency) followed by eta_and_pbot_update (:2290-2385, no
+ * OBC, truncate_eta false, split time stepping).  Depth-based
+ * (pressure_based = 0), on the data domain:
+ *   tendency = tmask(k=1)*(1/rho0)*(conv_rho_ud_t_tau + pme + river + source)
+ *   (Ext_mode%deta_dt); x_taup1 = x_taum1 + dtime*tendency on the compute
+ *   domain (eta_t), halos as the single-PE update leaves them (closed halo
+ *   points keep x_taup1's previous contents).
+ * Pressure-based: tendency = dpatm_dt + grav*tmask(k=1)*(...) (dpbot_dt),
+ * x is anompb, and pbot_t_taup1 = x_taup1 + rho0*grav*ht on the data domain.
+ * dtime is Time_steps%dtime_e; grav defaults to 9.80 in the reference
+ * (ocean_parameters.F90:176).  x_taup1 and tendency must not alias x_taum1;
+ * dpatm_dt, ht and pbot_t_taup1 may be NULL when depth-based. */
+int mom_gpu_eta_and_pbot_update(mom_gpu_ctx_t *ctx, int pressure_based, double rho0,
+    double grav, double dtime, const double *conv_rho_ud_t_tau, const double *pme,
+    const double *river, const double *source, const double *dpatm_dt, const double *ht,
+    const double *x_taum1, double *x_taup1, double *pbot_t_taup1, double *tendency);
+
+/* Grid metrics of REMAP_BT_TO_BU (ocean_operators.F90:462-477): Grd%dte,
+ * dtw, dus, dun, daur (ocean_types.F90:281-293), HOST arrays on the data
+ * domain, copied into the context (the part of ocean_grids_init the
+ * closure needs; mom_gpu_barotropic_closure fails until they are set). */
+int mom_gpu_set_remap_metrics(mom_gpu_ctx_t *ctx, const double *dte, const double *dtw,
+    const double *dus, const double *dun, const double *daur);
+
+/* Replaces the closure of update_ocean_barotropic (ocean_barotropic.F90:
+ * 2906-2921 and 2956-2985; no OBC, tidal or geoid forcing): depth-based,
+ * eta_u = umask(k=1)*REMAP_BT_TO_BU(eta_t_bar); pressure-based, eta_u is
+ * pbot_u = umask(k=1)*REMAP_BT_TO_BU(rho0*grav*ht + anompb_bar); both then
+ * halo-updated.  ps = rho_g*eta_t_bar + patm (Ext_mode%ps), grad_ps =
+ * GRAD_BAROTROPIC_P(ps) and grad_anompb = GRAD_BAROTROPIC_P(anompb_bar)
+ * (ocean_operators.F90:631-676; two components back to back): B-grid when
+ * tmasken is NULL, C-grid with tmasken = Grd%tmasken(:,:,1,1:2) (device).
+ * eta_t_bar and anompb_bar are the itime level (taup1 with two-level
+ * stepping, :2797-2801).  anompb_bar and grad_anompb may both be NULL when
+ * depth-based; ht is read only when pressure-based. */
+int mom_gpu_barotropic_closure(mom_gpu_ctx_t *ctx, int pressure_based, double rho0,
+    double grav, const double *eta_t_bar, const double *anompb_bar, const double *patm,
+    const double *rho_g, const double *ht, const double *tmasken, double *eta_u, double *ps,
+    double *grad_ps, double *grad_anompb);
+
+/* The four split-explicit barotropic schemes update_ocean_barotropic
+ * selects (ocean_barotropic.F90:2829-2843), with barotropic_halo = 1 and no
+ * tides, geoid, friction, smoothing, blobs or OBC, each followed by the
+ * time-mean tail (:2846-2894):
+ *   cgrid = 0, pressure_based = 0: pred_corr_tropic_depth_bgrid :3225-3579
+ *   cgrid = 1, pressure_based = 0: pred_corr_tropic_depth_cgrid :3601-3965
+ *   cgrid = 0, pressure_based = 1: pred_corr_tropic_press_bgrid :3989-4329
+ *   cgrid = 1, pressure_based = 1: pred_corr_tropic_press_cgrid :4355-4693
+ * x is eta_t (depth-based) or anompb (pressure-based).  thick is thicku
+ * (B, depth), Thickness%thicken (C, depth; 2 planes), Thickness%mass_u
+ * (B, pressure) or Thickness%mass_en (C, pressure; 2 planes).  tmasken
+ * (C-grid) = Grd%tmasken(:,:,1,1:2).  smooth is eta_smooth (depth) or
+ * pbot_smooth (pressure).  x_tau is read only when initsum_with_bar = 0;
+ * conv_rho_ud_t may be NULL.  All 2-D fields on the data domain, device
+ * pointers.  The B-grid depth-based case is also served (faster, blocked in
+ * time) by mom_gpu_update_ocean_barotropic; both give the same bits. */
+typedef struct {
+  int cgrid;
+  int pressure_based;
+  double grav;                      /* ocean_parameters.F90:176 (9.80) */
+  const double *x_tau, *x_bar_tau, *udrho_tau, *forcing_bt, *thick, *tmasken;
+  const double *patm, *rho_g;       /* depth-based */
+  const double *dpatm_dt;           /* pressure-based */
+  const double *pme, *river, *source, *smooth;
+  double *x_bar_taup1, *udrho_taup1, *conv_rho_ud_t;
+} mom_gpu_bt_variant_t;
+
+int mom_gpu_update_ocean_barotropic_variant(mom_gpu_ctx_t *ctx, const mom_gpu_bt_params_t *p,
+    const mom_gpu_bt_variant_t *v);
 
 /* ---- batched tracer step ------------------------------------------------ */
 

@@ -36,6 +36,14 @@ class BtParams(C.Structure):
                 ("rho0", C.c_double), ("stepping_a", C.c_int), ("initsum_with_bar", C.c_int)]
 
 
+class BtVariant(C.Structure):
+    _fields_ = [("cgrid", C.c_int), ("pressure_based", C.c_int), ("grav", C.c_double)] + \
+               [(n, _vp) for n in ("x_tau", "x_bar_tau", "udrho_tau", "forcing_bt", "thick",
+                                   "tmasken", "patm", "rho_g", "dpatm_dt", "pme", "river",
+                                   "source", "smooth", "x_bar_taup1", "udrho_taup1",
+                                   "conv_rho_ud_t")]
+
+
 class TracerStep(C.Structure):
     _fields_ = [("ntr", C.c_int), ("T_taum1", C.POINTER(_vp)), ("T_taup1", C.POINTER(_vp)),
                 ("dtype", C.POINTER(C.c_int)), ("th_tendency", C.POINTER(_vp)),
@@ -65,6 +73,13 @@ _PROTOS = {
     "mom_gpu_memcpy_dtod": (C.c_int, [_vp, _vp, C.c_size_t]),
     "mom_gpu_copy": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
     "mom_gpu_advection_velocity": (C.c_int, [_vp] + [_vp] * 10),
+    "mom_gpu_barotropic_forcing": (C.c_int, [_vp] * 5),
+    "mom_gpu_update_ocean_barotropic_variant": (C.c_int, [_vp, C.POINTER(BtParams),
+                                                          C.POINTER(BtVariant)]),
+    "mom_gpu_eta_and_pbot_update": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double,
+                                              C.c_double] + [_dp] * 10),
+    "mom_gpu_set_remap_metrics": (C.c_int, [_vp] + [_dp] * 5),
+    "mom_gpu_barotropic_closure": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double] + [_dp] * 10),
     "mom_gpu_ensemble_create": (C.c_int, [C.c_int, C.POINTER(_vp), C.POINTER(_vp)]),
     "mom_gpu_ensemble_destroy": (C.c_int, [_vp]),
     "mom_gpu_ensemble_step_tracers": (C.c_int, [_vp, C.POINTER(TracerStep)]),

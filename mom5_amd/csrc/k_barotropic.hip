@@ -496,6 +496,158 @@ __global__ void k_bt_conv_ens(const BtTail *__restrict__ T) {
   t.conv[ii + t.d.NI * jj] = -div_ud(t.d, t.ud, t.ud + t.d.P2, t.dxu, t.dyu, t.datr, ii, jj);
 }
 
+// ocean_barotropic_forcing (ocean_barotropic.F90:3058-3141, default namelist,
+// B-grid: tmasken = umask): the column sum of accel over the wet U-levels
+// plus the surface/bottom stress, on the compute domain; a halo column or
+// row takes the value of its cyclic source (what the halo update leaves), a
+// closed halo is zero; then the zero friction terms are added, as there.
+__global__ void k_bt_forcing(KDom d, const double *__restrict__ accel,
+                             const double *__restrict__ smf, const double *__restrict__ bmf,
+                             const int *__restrict__ kmu, double *__restrict__ out) {
+  const int ii = blockIdx.x * blockDim.x + threadIdx.x, jj = blockIdx.y;
+  if (ii >= d.NI) return;
+  int si = ii, sj = jj;
+  bool valid = true;
+  if (jj < 1 || jj > d.nj) {
+    if (d.cyclic_y) sj = 1 + wrapi(jj - 1, d.nj); else valid = false;
+  }
+  if (ii < 1 || ii > d.ni) {
+    if (d.cyclic_x) si = 1 + wrapi(ii - 1, d.ni); else valid = false;
+  }
+  const size_t P2 = (size_t)d.P2, P3 = P2 * (size_t)d.nk;
+  const int o = ii + d.NI * jj, os = si + d.NI * sj;
+  const int km = valid ? kmu[os] : 0;
+  for (int n = 0; n < 2; ++n) {
+    double f = 0.0;
+    if (valid) {
+      double wrk = 0.0;
+      for (int k = 1; k <= d.nk; ++k)
+        wrk = wrk + (k <= km ? 1.0 : 0.0) * accel[P3 * n + P2 * (size_t)(k - 1) + os];
+      f = wrk + (1 <= km ? 1.0 : 0.0) * (smf[P2 * n + os] - bmf[P2 * n + os]);
+    }
+    out[P2 * n + o] = f + 0.0 + 0.0;
+  }
+}
+
+struct EtaIn {
+  const double *conv, *pme, *river, *source, *dpatm_dt, *ht, *x_taum1;
+  const int *kmt;
+  double rho0r, grav, rho0grav, dtime;
+  int pressure_based;
+};
+
+// eta_and_pbot_tendency (ocean_barotropic.F90:2686-2710) at one point.
+__device__ inline double eta_tendency(const EtaIn &e, int o) {
+  const double tm = 1 <= e.kmt[o] ? 1.0 : 0.0;
+  const double sum = e.conv[o] + e.pme[o] + e.river[o] + e.source[o];
+  if (!e.pressure_based) return tm * e.rho0r * sum;
+  return e.dpatm_dt[o] + e.grav * tm * sum;
+}
+
+// eta_and_pbot_tendency then eta_and_pbot_update (:2303-2363): the tendency
+// on the data domain; x_taup1 = x_taum1 + dtime*tendency on the compute
+// domain and, on halo points the update rewrites, at their cyclic source;
+// pressure-based: pbot_t = anompb + rho0grav*ht on the data domain.
+__global__ void k_eta_update(KDom d, EtaIn e, double *__restrict__ x_taup1,
+                             double *__restrict__ pbot, double *__restrict__ tend) {
+  const int ii = blockIdx.x * blockDim.x + threadIdx.x, jj = blockIdx.y;
+  if (ii >= d.NI) return;
+  const int o = ii + d.NI * jj;
+  tend[o] = eta_tendency(e, o);
+  int si, sj;
+  double x;
+  if (halo_source(d, ii, jj, si, sj)) {
+    const int os = si + d.NI * sj;
+    x = e.x_taum1[os] + e.dtime * eta_tendency(e, os);
+    x_taup1[o] = x;
+  } else {
+    x = x_taup1[o];
+  }
+  if (e.pressure_based) pbot[o] = x + e.rho0grav * e.ht[o];
+}
+
+struct ClosureIn {
+  const double *eta_bar, *anompb_bar, *patm, *rho_g, *ht;
+  const double *dte, *dtw, *dus, *dun, *daur, *dxur, *dyur;
+  const double *tmasken, *dxte, *dytn;   // tmasken != NULL: C-grid gradient
+  const int *kmu;
+  double rho0grav;
+  int pressure_based;
+};
+
+// The field REMAP_BT_TO_BU is applied to: eta_t_bar, or rho0grav*ht +
+// anompb_bar for the pressure-based closure (:2908, :2918).
+__device__ inline double remap_src(const ClosureIn &c, int o) {
+  return c.pressure_based ? c.rho0grav * c.ht[o] + c.anompb_bar[o] : c.eta_bar[o];
+}
+
+// umask(k=1)*REMAP_BT_TO_BU at U point (i, j) (ocean_operators.F90:468-475).
+__device__ inline double remap_bu(const KDom &d, const ClosureIn &c, int i, int j) {
+  if (i >= d.NI - 1 || j >= d.NJ - 1) return 0.0;
+  const int o = i + d.NI * j, oe = o + 1, on = o + d.NI, one = on + 1;
+  const double r = (remap_src(c, o) * c.dte[o] * c.dus[o] + remap_src(c, oe) * c.dtw[oe] * c.dus[o]
+                    + remap_src(c, on) * c.dte[on] * c.dun[o]
+                    + remap_src(c, one) * c.dtw[one] * c.dun[o]) * c.daur[o];
+  return (1 <= c.kmu[o] ? 1.0 : 0.0) * r;
+}
+
+// ps = rho_g*eta_t_bar + patm (:2970).
+__device__ inline double surf_p(const ClosureIn &c, int o) {
+  return c.rho_g[o] * c.eta_bar[o] + c.patm[o];
+}
+
+// GRAD_BAROTROPIC_P, halo 1, of a pressure given by a point function:
+// B-grid (ocean_operators.F90:651-660) or C-grid (:664-669).
+template <class P>
+__device__ inline void grad_bt(const KDom &d, const ClosureIn &c, int i, int j, P p, double &gx,
+                              double &gy) {
+  if (i >= d.NI - 1 || j >= d.NJ - 1) {
+    gx = 0.0;
+    gy = 0.0;
+    return;
+  }
+  const int o = i + d.NI * j, oe = o + 1, on = o + d.NI, one = on + 1;
+  if (c.tmasken) {
+    const double pc = p(o);
+    gx = (p(oe) - pc) * (1.0 / (c.dxte[o] + 1.0e-40)) * c.tmasken[o];
+    gy = (p(on) - pc) * (1.0 / (c.dytn[o] + 1.0e-40)) * c.tmasken[(size_t)d.P2 + o];
+    return;
+  }
+  const double pc = p(o), pe = p(oe), pn = p(on), pne = p(one);
+  const double p_fay = 0.5 * (pc + pn);
+  const double p_fax = 0.5 * (pc + pe);
+  const double pip_fay = 0.5 * (pe + pne);
+  const double pjp_fax = 0.5 * (pn + pne);
+  const double um = 1 <= c.kmu[o] ? 1.0 : 0.0;
+  gx = (pip_fay - p_fay) * c.dxur[o] * um;
+  gy = (pjp_fax - p_fax) * c.dyur[o] * um;
+}
+
+// Closure of update_ocean_barotropic (ocean_barotropic.F90:2906-2921,
+// 2956-2985): eta_u (or pbot_u) halo-updated, ps, grad_ps and grad_anompb.
+// Neighbour values of ps are recomputed, not read, so one launch suffices.
+__global__ void k_bt_closure(KDom d, ClosureIn c, double *__restrict__ eta_u,
+                             double *__restrict__ ps, double *__restrict__ grad_ps,
+                             double *__restrict__ grad_ab) {
+  const int ii = blockIdx.x * blockDim.x + threadIdx.x, jj = blockIdx.y;
+  if (ii >= d.NI) return;
+  const int o = ii + d.NI * jj;
+  int si, sj;
+  halo_source(d, ii, jj, si, sj);
+  eta_u[o] = remap_bu(d, c, si, sj);
+  ps[o] = surf_p(c, o);
+  const size_t P2 = (size_t)d.P2;
+  double gx, gy;
+  grad_bt(d, c, ii, jj, [&](int q) { return surf_p(c, q); }, gx, gy);
+  grad_ps[o] = gx;
+  grad_ps[P2 + o] = gy;
+  if (grad_ab) {
+    grad_bt(d, c, ii, jj, [&](int q) { return c.anompb_bar[q]; }, gx, gy);
+    grad_ab[o] = gx;
+    grad_ab[P2 + o] = gy;
+  }
+}
+
 }  // namespace
 
 int launch_barotropic_blocked(mom_gpu_ctx *ctx, const mom_gpu_bt_params_t *p,
@@ -694,5 +846,43 @@ int launch_barotropic_ensemble(mom_gpu_ensemble *e, const mom_gpu_bt_params_t *p
   hipLaunchKernelGGL(k_bt_conv_ens, g2, b2, 0, s, tdev);
   hipLaunchKernelGGL(k_bt_wrap_ens, gw, b2, 0, s, tdev, 1);
   MOM_LAUNCHED(e->lead, "k_bt_tail_ens");
+  return MOM_GPU_OK;
+}
+
+int launch_bt_forcing(mom_gpu_ctx *ctx, const double *accel, const double *smf, const double *bmf,
+                      double *forcing_bt) {
+  const KDom &d = ctx->kd;
+  hipLaunchKernelGGL(k_bt_forcing, dim3((d.NI + 63) / 64, d.NJ), dim3(64), 0, ctx->stream, d,
+                     accel, smf, bmf, ctx->kmu, forcing_bt);
+  MOM_LAUNCHED(ctx, "k_bt_forcing");
+  return MOM_GPU_OK;
+}
+
+int launch_eta_update(mom_gpu_ctx *ctx, int pressure_based, double rho0, double grav,
+                      double dtime, const double *conv, const double *pme, const double *river,
+                      const double *source, const double *dpatm_dt, const double *ht,
+                      const double *x_taum1, double *x_taup1, double *pbot, double *tend) {
+  const KDom &d = ctx->kd;
+  EtaIn e{conv, pme, river, source, dpatm_dt, ht, x_taum1, ctx->kmt,
+          1.0 / rho0, grav, rho0 * grav, dtime, pressure_based};
+  hipLaunchKernelGGL(k_eta_update, dim3((d.NI + 63) / 64, d.NJ), dim3(64), 0, ctx->stream, d, e,
+                     x_taup1, pbot, tend);
+  MOM_LAUNCHED(ctx, "k_eta_update");
+  return MOM_GPU_OK;
+}
+
+int launch_bt_closure(mom_gpu_ctx *ctx, int pressure_based, double rho0, double grav,
+                      const double *eta_bar, const double *anompb_bar, const double *patm,
+                      const double *rho_g, const double *ht, const double *tmasken,
+                      double *eta_u, double *ps, double *grad_ps, double *grad_ab) {
+  const KDom &d = ctx->kd;
+  const double *rm = ctx->remap;
+  const size_t P2 = (size_t)d.P2;
+  ClosureIn c{eta_bar, anompb_bar, patm, rho_g, ht, rm, rm + P2, rm + 2 * P2, rm + 3 * P2,
+              rm + 4 * P2, ctx->dxur, ctx->dyur, tmasken, ctx->dxte, ctx->dytn, ctx->kmu,
+              rho0 * grav, pressure_based};
+  hipLaunchKernelGGL(k_bt_closure, dim3((d.NI + 63) / 64, d.NJ), dim3(64), 0, ctx->stream, d, c,
+                     eta_u, ps, grad_ps, grad_ab);
+  MOM_LAUNCHED(ctx, "k_bt_closure");
   return MOM_GPU_OK;
 }

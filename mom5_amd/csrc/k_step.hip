@@ -20,6 +20,7 @@
 #include "mom_internal.h"
 
 #include <algorithm>
+#include <type_traits>
 #include <cstring>
 
 namespace {
@@ -155,7 +156,10 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
   }
   int km[RPW], offz[RPW], gjr[RPW], sjr[RPW];
   bool ok[RPW], rowok[RPW];
-  double datv[RPW], datrv[RPW], wkm1[RPW];
+  // per-row state: FP64 for FP64 groups (reference rounding), FP32 for FP32
+  // groups (fewer registers; their conservation weight reloads datr in FP64)
+  using RT = typename std::conditional<EXACT, double, float>::type;
+  RT datv[RPW], datrv[RPW], wkm1[RPW];
   TT t0[G][RPW], t1[G][RPW], t2[G][RPW], t3[G][RPW], ftp[G][RPW];
 #pragma unroll
   for (int q = 0; q < RPW; ++q) {
@@ -171,9 +175,9 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
     offz[q] = ok[q] ? d2(d, si, sj) : 0;
     km[q] = ok[q] ? kmt[offz[q]] : 0;
     s_km[ry][lane] = km[q];
-    datv[q] = ok[q] ? dat[offz[q]] : 0.0;
-    datrv[q] = ok[q] ? datr[offz[q]] : 0.0;
-    wkm1[q] = 0.0;
+    datv[q] = ok[q] ? (RT)dat[offz[q]] : RT(0);
+    datrv[q] = ok[q] ? (RT)datr[offz[q]] : RT(0);
+    wkm1[q] = RT(0);
 #pragma unroll
     for (int g = 0; g < G; ++g) {
       ftp[g][q] = TT(0);
@@ -194,7 +198,8 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
   for (int k = 1; k <= nk; ++k) {
     const int k0 = k - 1;
     const int kp1 = min(k + 1, nk), kp2 = min(k + 2, nk), km1 = max(k - 1, 1);
-    double rhov[RPW], wv[RPW], xf64v[RPW];
+    double rhov[RPW], xf64v[RPW];
+    RT wv[RPW];
     float rinv[RPW];   // FP32 groups: about 1/rho_dzt
     TT tz[G][RPW], fbtv[G][RPW], txv[G][RPW], fyv[G][RPW];
     TT dxf[G][RPW];    // FP32 groups: x flux difference of the cell
@@ -203,12 +208,12 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
     // ---- Z ----------------------------------------------------------------
 #pragma unroll
     for (int q = 0; q < RPW; ++q) {
-      rhov[q] = 1.0; wv[q] = 0.0; rinv[q] = 1.0f;
+      rhov[q] = 1.0; wv[q] = RT(0); rinv[q] = 1.0f;
       if (ok[q]) {
         const int o = offz[q];
         const double w = W[o + P2 * k];
         const double r = R[o + P2 * k0];
-        rhov[q] = r; wv[q] = w;
+        rhov[q] = r; wv[q] = (RT)w;
         const TT mkm1 = mk<TT>(km1, km[q]), mk0 = mk<TT>(k, km[q]);
         const TT mkp1 = mk<TT>(kp1, km[q]), mkp2 = mk<TT>(kp2, km[q]);
         const TT mA = mkm1 * mk0, mB = mk0 * mkp1, mC = mkp1 * mkp2;
@@ -231,7 +236,7 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
           const float wf = (float)w;
           const FaceC<float> c = face_coef_f((float)datv[q] * wf, fabsf(wf * (float)dtime * ri));
           const float dtrf = (float)dtime * ri, datrf = (float)datrv[q];
-          const float wdiff = (float)(wkm1[q] - w);
+          const float wdiff = (float)wkm1[q] - wf;
 #pragma unroll
           for (int g = 0; g < G; ++g) {
             const float Rjp = (t0[g][q] - t1[g][q]) * mA;
@@ -387,7 +392,7 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
         // weight rho*dat*factor is dtime to 1e-16 and the fluxes telescope:
         // what is left is the unbiased rounding of the FP32 result.
         const float a32 = (float)(rm1 * rr), c32 = (float)(dtime * rr);
-        const double f64 = dtime * rr * datrv[q] * mi;
+        const double f64 = EXACT ? 0.0 : dtime * rr * datr[offz[q]] * mi;
 #pragma unroll
         for (int g = 0; g < G; ++g) {
           const double thin = tab.th[g] ? tab.th[g][o] : 0.0;
@@ -480,11 +485,12 @@ static void launch_some(mom_gpu_ctx *ctx, int rpw, int waves, const TT *const *T
                         const double *rr, double dtime) {
   StepTab<TT> t{};
   for (int i = 0; i < cnt; ++i) { t.T[i] = Tin[i]; t.out[i] = Tout[i]; t.th[i] = thin[i]; }
-  // 16-wave tiles (32 rows, 28 interior: less halo re-read) fit 128 VGPRs only
-  // for FP32 groups of one or two tracers at two rows per wave
+  // 16-wave tiles (32 rows, 28 interior: less halo re-read) fit 128 VGPRs
+  // only for FP32 groups at two rows per wave
   if constexpr (sizeof(TT) == 4) {
-    if (waves == 16 && rpw == 2 && cnt <= 2) {
-      if (cnt == 2) launch_group<TT, 2, 2, 16>(ctx, t, u, v, w, rho, rm1, rr, dtime);
+    if (waves == 16 && rpw == 2) {
+      if (cnt == 4) launch_group<TT, 4, 2, 16>(ctx, t, u, v, w, rho, rm1, rr, dtime);
+      else if (cnt == 2) launch_group<TT, 2, 2, 16>(ctx, t, u, v, w, rho, rm1, rr, dtime);
       else launch_group<TT, 1, 2, 16>(ctx, t, u, v, w, rho, rm1, rr, dtime);
       return;
     }
@@ -585,7 +591,7 @@ static int ensemble_type(mom_gpu_ensemble *e, const mom_gpu_tracer_step_t *steps
         t.out[q] = (TT *)s.T_taup1[n];
         t.th[q] = s.th_tendency ? s.th_tendency[n] : nullptr;
       }
-      const int w = (sizeof(TT) == 4 && waves == 16 && rpw == 2 && c <= 2) ? 16 : 8;
+      const int w = (sizeof(TT) == 4 && waves == 16 && rpw == 2) ? 16 : 8;
       const int TY = w * rpw - 4;
       const int ntx = (d.ni + TX - 1) / TX;
       const int ntiles = ntx * ((d.nj + TY - 1) / TY);
@@ -605,14 +611,15 @@ static int ensemble_type(mom_gpu_ensemble *e, const mom_gpu_tracer_step_t *steps
   int rc = MOM_GPU_OK;
   const bool w16 = sizeof(TT) == 4 && waves == 16 && rpw == 2;
   if (rpw == 2) {
-    rc = rc ? rc : queue_launch<TT, 4, 2, 8>(e, by[4], slot++);
     if constexpr (sizeof(TT) == 4) {
       if (w16) {
+        rc = rc ? rc : queue_launch<TT, 4, 2, 16>(e, by[4], slot++);
         rc = rc ? rc : queue_launch<TT, 2, 2, 16>(e, by[2], slot++);
         rc = rc ? rc : queue_launch<TT, 1, 2, 16>(e, by[1], slot++);
       }
     }
     if (!w16) {
+      rc = rc ? rc : queue_launch<TT, 4, 2, 8>(e, by[4], slot++);
       rc = rc ? rc : queue_launch<TT, 2, 2, 8>(e, by[2], slot++);
       rc = rc ? rc : queue_launch<TT, 1, 2, 8>(e, by[1], slot++);
     }

@@ -154,6 +154,88 @@ int mom_gpu_advection_velocity(mom_gpu_ctx_t *ctx, const double *u, const double
                         vhrho_nt, wrho_bt, diverge_t);
 }
 
+int mom_gpu_barotropic_forcing(mom_gpu_ctx_t *ctx, const double *accel, const double *smf,
+                               const double *bmf, double *forcing_bt) {
+  REQUIRE(nullptr, ctx, "mom_gpu_barotropic_forcing: NULL context");
+  REQUIRE(ctx, accel && smf && bmf && forcing_bt, "mom_gpu_barotropic_forcing: NULL argument");
+  return launch_bt_forcing(ctx, accel, smf, bmf, forcing_bt);
+}
+
+int mom_gpu_eta_and_pbot_update(mom_gpu_ctx_t *ctx, int pressure_based, double rho0,
+                                double grav, double dtime, const double *conv_rho_ud_t_tau,
+                                const double *pme, const double *river, const double *source,
+                                const double *dpatm_dt, const double *ht, const double *x_taum1,
+                                double *x_taup1, double *pbot_t_taup1, double *tendency) {
+  REQUIRE(nullptr, ctx, "mom_gpu_eta_and_pbot_update: NULL context");
+  REQUIRE(ctx, conv_rho_ud_t_tau && pme && river && source && x_taum1 && x_taup1 && tendency,
+          "mom_gpu_eta_and_pbot_update: NULL argument");
+  REQUIRE(ctx, !pressure_based || (dpatm_dt && ht && pbot_t_taup1),
+          "mom_gpu_eta_and_pbot_update: the pressure-based step needs dpatm_dt, ht and pbot_t");
+  REQUIRE(ctx, (const double *)x_taup1 != x_taum1 && (const double *)tendency != x_taum1,
+          "mom_gpu_eta_and_pbot_update: x_taup1 and tendency must not alias x_taum1");
+  REQUIRE(ctx, rho0 > 0.0, "mom_gpu_eta_and_pbot_update: rho0 must be positive");
+  return launch_eta_update(ctx, pressure_based ? 1 : 0, rho0, grav, dtime, conv_rho_ud_t_tau, pme,
+                           river, source, dpatm_dt, ht, x_taum1, x_taup1, pbot_t_taup1, tendency);
+}
+
+int mom_gpu_set_remap_metrics(mom_gpu_ctx_t *ctx, const double *dte, const double *dtw,
+                              const double *dus, const double *dun, const double *daur) {
+  REQUIRE(nullptr, ctx, "mom_gpu_set_remap_metrics: NULL context");
+  REQUIRE(ctx, dte && dtw && dus && dun && daur, "mom_gpu_set_remap_metrics: NULL argument");
+  const size_t P2 = (size_t)ctx->kd.P2;
+  if (!ctx->remap) MOM_HIP(ctx, hipMalloc(&ctx->remap, 5 * P2 * sizeof(double)));
+  const double *src[5] = {dte, dtw, dus, dun, daur};
+  for (int n = 0; n < 5; ++n)
+    MOM_HIP(ctx, hipMemcpyAsync(ctx->remap + n * P2, src[n], P2 * sizeof(double),
+                                hipMemcpyDefault, ctx->stream));
+  MOM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return MOM_GPU_OK;
+}
+
+int mom_gpu_barotropic_closure(mom_gpu_ctx_t *ctx, int pressure_based, double rho0, double grav,
+                               const double *eta_t_bar, const double *anompb_bar,
+                               const double *patm, const double *rho_g, const double *ht,
+                               const double *tmasken, double *eta_u, double *ps,
+                               double *grad_ps, double *grad_anompb) {
+  REQUIRE(nullptr, ctx, "mom_gpu_barotropic_closure: NULL context");
+  REQUIRE(ctx, ctx->remap,
+          "mom_gpu_barotropic_closure: REMAP_BT_TO_BU metrics not set (mom_gpu_set_remap_metrics)");
+  REQUIRE(ctx, eta_t_bar && patm && rho_g && eta_u && ps && grad_ps,
+          "mom_gpu_barotropic_closure: NULL argument");
+  REQUIRE(ctx, !pressure_based || (ht && anompb_bar),
+          "mom_gpu_barotropic_closure: the pressure-based closure needs ht and anompb_bar");
+  REQUIRE(ctx, !grad_anompb || anompb_bar,
+          "mom_gpu_barotropic_closure: grad_anompb needs anompb_bar");
+  return launch_bt_closure(ctx, pressure_based ? 1 : 0, rho0, grav, eta_t_bar, anompb_bar, patm,
+                           rho_g, ht, tmasken, eta_u, ps, grad_ps, grad_anompb);
+}
+
+int mom_gpu_update_ocean_barotropic_variant(mom_gpu_ctx_t *ctx, const mom_gpu_bt_params_t *p,
+                                            const mom_gpu_bt_variant_t *v) {
+  REQUIRE(nullptr, ctx, "mom_gpu_update_ocean_barotropic_variant: NULL context");
+  REQUIRE(ctx, p && v, "mom_gpu_update_ocean_barotropic_variant: NULL argument");
+  REQUIRE(ctx, p->nts >= 1, "mom_gpu_update_ocean_barotropic_variant: nts=%d < 1", p->nts);
+  REQUIRE(ctx, p->rho0 > 0.0, "mom_gpu_update_ocean_barotropic_variant: rho0 must be positive");
+  REQUIRE(ctx, v->x_bar_tau && v->udrho_tau && v->forcing_bt && v->thick && v->pme &&
+                   v->river && v->source && v->smooth && v->x_bar_taup1 && v->udrho_taup1,
+          "mom_gpu_update_ocean_barotropic_variant: NULL field");
+  REQUIRE(ctx, p->initsum_with_bar || v->x_tau,
+          "mom_gpu_update_ocean_barotropic_variant: x_tau needed when initsum_with_bar = 0");
+  REQUIRE(ctx, !v->cgrid || v->tmasken,
+          "mom_gpu_update_ocean_barotropic_variant: the C-grid needs tmasken");
+  REQUIRE(ctx, v->pressure_based ? v->dpatm_dt != nullptr : (v->patm && v->rho_g),
+          "mom_gpu_update_ocean_barotropic_variant: %s",
+          v->pressure_based ? "the pressure-based scheme needs dpatm_dt"
+                            : "the depth-based scheme needs patm and rho_g");
+  const void *outs[] = {v->x_bar_taup1, v->udrho_taup1, v->conv_rho_ud_t};
+  const void *ins[] = {v->x_tau, v->x_bar_tau, v->udrho_tau};
+  for (const void *o : outs)
+    for (const void *i : ins)
+      REQUIRE(ctx, !o || o != i,
+              "mom_gpu_update_ocean_barotropic_variant: an output aliases x_tau, x_bar_tau or udrho_tau");
+  return launch_bt_variant(ctx, p, v);
+}
+
 int mom_gpu_copy(mom_gpu_ctx_t *ctx, double *dst, const double *src, size_t n) {
   REQUIRE(nullptr, ctx, "mom_gpu_copy: NULL context");
   REQUIRE(ctx, (dst && src) || n == 0, "mom_gpu_copy: NULL pointer");
@@ -164,7 +246,7 @@ static void free_ctx(mom_gpu_ctx *c) {
   if (!c) return;
   double **ds[] = {&c->dxt, &c->dyt, &c->dat, &c->datr, &c->dxte, &c->dyte, &c->dxtn, &c->dytn,
                    &c->dxu, &c->dyu, &c->dxur, &c->dyur, &c->f, &c->dzt, &c->qw, &c->qzw,
-                   &c->bt, &c->tri_e, &c->red};
+                   &c->bt, &c->tri_e, &c->red, &c->remap, &c->btv};
   for (double **p : ds)
     if (*p) hipFree(*p);
   if (c->kmt) hipFree(c->kmt);

@@ -32,6 +32,32 @@ __host__ __device__ inline int wrapi(int g, int n) {
   return g < 0 ? g + n : g;
 }
 
+// Source point of a halo update on a single PE (mpp_update_domains,
+// shared/mpp/include/mpp_do_update.h:186-288): a halo column on a compute
+// row takes its cyclic-x image, a halo row on a compute column its cyclic-y
+// image, a corner only when both axes are cyclic.  Returns false where the
+// update leaves the point alone (the point itself is the source).
+__device__ inline bool halo_source(const KDom &d, int ii, int jj, int &si, int &sj) {
+  si = ii;
+  sj = jj;
+  const bool col = ii < 1 || ii > d.ni, row = jj < 1 || jj > d.nj;
+  if (!col && !row) return true;
+  if (col && row) {
+    if (!(d.cyclic_x && d.cyclic_y)) return false;
+    si = 1 + wrapi(ii - 1, d.ni);
+    sj = 1 + wrapi(jj - 1, d.nj);
+    return true;
+  }
+  if (col) {
+    if (!d.cyclic_x) return false;
+    si = 1 + wrapi(ii - 1, d.ni);
+    return true;
+  }
+  if (!d.cyclic_y) return false;
+  sj = 1 + wrapi(jj - 1, d.nj);
+  return true;
+}
+
 // Fortran MIN/MAX on non-NaN data.  The ternary form pins which operand
 // is returned for +0/-0 so that the result matches the CPU restatement.
 __host__ __device__ inline double fmn(double a, double b) { return (b < a) ? b : a; }
@@ -59,6 +85,10 @@ struct mom_gpu_ctx {
   // steady forcing, cori1, cori2
   double *bt;
   double bt_cori_dtbt;
+  // REMAP_BT_TO_BU metrics dte, dtw, dus, dun, daur (5 planes, optional)
+  double *remap;
+  // state of the barotropic variants (18 planes, allocated on first use)
+  double *btv;
   // invtri work (e coefficients), one 3-D plane set
   double *tri_e;
   // FP32 implicit diffusion: forward increments of up to 4 tracers (lazy)
@@ -154,6 +184,18 @@ int launch_content(mom_gpu_ctx *ctx, const void *T, int dtype, const double *rho
 int launch_tracer_error(mom_gpu_ctx *ctx, const void *a, int dtype, const double *b,
                         double floor, double *out);
 int launch_fill(mom_gpu_ctx *ctx, double *p, double v, size_t n);
+int launch_bt_forcing(mom_gpu_ctx *ctx, const double *accel, const double *smf, const double *bmf,
+                      double *forcing_bt);
+int launch_bt_variant(mom_gpu_ctx *ctx, const mom_gpu_bt_params_t *p,
+                      const mom_gpu_bt_variant_t *v);
+int launch_eta_update(mom_gpu_ctx *ctx, int pressure_based, double rho0, double grav,
+                      double dtime, const double *conv, const double *pme, const double *river,
+                      const double *source, const double *dpatm_dt, const double *ht,
+                      const double *x_taum1, double *x_taup1, double *pbot, double *tend);
+int launch_bt_closure(mom_gpu_ctx *ctx, int pressure_based, double rho0, double grav,
+                      const double *eta_bar, const double *anompb_bar, const double *patm,
+                      const double *rho_g, const double *ht, const double *tmasken,
+                      double *eta_u, double *ps, double *grad_ps, double *grad_ab);
 int launch_adv_vel(mom_gpu_ctx *ctx, const double *u, const double *rho_dzu, const double *pme,
                    const double *river, const double *rtend, const double *msrc, double *ue,
                    double *vn, double *wb, double *div);

@@ -50,6 +50,19 @@ module mom_gpu_mod
      integer(c_int) :: initsum_with_bar
   end type mom_gpu_bt_params_t
 
+  ! the scheme and fields of mom_gpu_update_ocean_barotropic_variant
+  type, bind(C), public :: mom_gpu_bt_variant_t
+     integer(c_int) :: cgrid = 0, pressure_based = 0
+     real(c_double) :: grav = 9.80d0
+     type(c_ptr) :: x_tau = c_null_ptr, x_bar_tau = c_null_ptr, udrho_tau = c_null_ptr
+     type(c_ptr) :: forcing_bt = c_null_ptr, thick = c_null_ptr, tmasken = c_null_ptr
+     type(c_ptr) :: patm = c_null_ptr, rho_g = c_null_ptr, dpatm_dt = c_null_ptr
+     type(c_ptr) :: pme = c_null_ptr, river = c_null_ptr, source = c_null_ptr
+     type(c_ptr) :: smooth = c_null_ptr
+     type(c_ptr) :: x_bar_taup1 = c_null_ptr, udrho_taup1 = c_null_ptr
+     type(c_ptr) :: conv_rho_ud_t = c_null_ptr
+  end type mom_gpu_bt_variant_t
+
   ! the tracer loop of update_ocean_tracer + vert_diffuse_implicit
   ! (ocean_tracers/ocean_tracer.F90:2295-2350,2432): host tables of device
   ! addresses (c_loc of type(c_ptr) arrays), dtype(n) = 8 (FP64) or 4 (FP32)
@@ -73,7 +86,9 @@ module mom_gpu_mod
   public :: mom_gpu_tracer_content, mom_gpu_check
   public :: mom_gpu_step_tracers, mom_gpu_tracer_content_f32, mom_gpu_tracer_error
   public :: mom_gpu_copy, mom_gpu_memcpy_dtod, mom_gpu_get_tuning
-  public :: mom_gpu_advection_velocity
+  public :: mom_gpu_advection_velocity, mom_gpu_barotropic_forcing
+  public :: mom_gpu_eta_and_pbot_update, mom_gpu_set_remap_metrics, mom_gpu_barotropic_closure
+  public :: mom_gpu_update_ocean_barotropic_variant
 
   interface
      integer(c_int) function mom_gpu_create(dom, grid, ctx) bind(C, name="mom_gpu_create")
@@ -229,6 +244,15 @@ module mom_gpu_mod
        type(c_ptr), value :: eta_t_bar_taup1, udrho_taup1, conv_rho_ud_t
      end function mom_gpu_update_ocean_barotropic
 
+     ! pred_corr_tropic_{depth,press}_{bgrid,cgrid}, ocean_barotropic.F90:3225-4693
+     integer(c_int) function mom_gpu_update_ocean_barotropic_variant(ctx, p, v) &
+          bind(C, name="mom_gpu_update_ocean_barotropic_variant")
+       import :: c_int, c_ptr, mom_gpu_bt_params_t, mom_gpu_bt_variant_t
+       type(c_ptr), value :: ctx
+       type(mom_gpu_bt_params_t), intent(in) :: p
+       type(mom_gpu_bt_variant_t), intent(in) :: v
+     end function mom_gpu_update_ocean_barotropic_variant
+
      ! tracer_integrals content, ocean_diag/ocean_tracer_diag.F90:2648-2661
      integer(c_int) function mom_gpu_tracer_content(ctx, T, rho_dzt, content) &
           bind(C, name="mom_gpu_tracer_content")
@@ -245,6 +269,45 @@ module mom_gpu_mod
        type(c_ptr), value :: ctx, u, rho_dzu, pme, river, rho_dzt_tendency, mass_source
        type(c_ptr), value :: uhrho_et, vhrho_nt, wrho_bt, diverge_t
      end function mom_gpu_advection_velocity
+
+     ! ocean_barotropic_forcing, ocean_core/ocean_barotropic.F90:3058-3141
+     integer(c_int) function mom_gpu_barotropic_forcing(ctx, accel, smf, bmf, forcing_bt) &
+          bind(C, name="mom_gpu_barotropic_forcing")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, accel, smf, bmf, forcing_bt
+     end function mom_gpu_barotropic_forcing
+
+     ! eta_and_pbot_tendency + eta_and_pbot_update, ocean_barotropic.F90:2667-2726, 2290-2385
+     integer(c_int) function mom_gpu_eta_and_pbot_update(ctx, pressure_based, rho0, grav, &
+          dtime, conv_rho_ud_t_tau, pme, river, source, dpatm_dt, ht, x_taum1, x_taup1, &
+          pbot_t_taup1, tendency) bind(C, name="mom_gpu_eta_and_pbot_update")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: pressure_based
+       real(c_double), value :: rho0, grav, dtime
+       type(c_ptr), value :: conv_rho_ud_t_tau, pme, river, source, dpatm_dt, ht
+       type(c_ptr), value :: x_taum1, x_taup1, pbot_t_taup1, tendency
+     end function mom_gpu_eta_and_pbot_update
+
+     ! Grd%dte, dtw, dus, dun, daur (host arrays) for REMAP_BT_TO_BU
+     integer(c_int) function mom_gpu_set_remap_metrics(ctx, dte, dtw, dus, dun, daur) &
+          bind(C, name="mom_gpu_set_remap_metrics")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: dte(*), dtw(*), dus(*), dun(*), daur(*)
+     end function mom_gpu_set_remap_metrics
+
+     ! closure of update_ocean_barotropic, ocean_barotropic.F90:2906-2921, 2956-2985
+     integer(c_int) function mom_gpu_barotropic_closure(ctx, pressure_based, rho0, grav, &
+          eta_t_bar, anompb_bar, patm, rho_g, ht, tmasken, eta_u, ps, grad_ps, grad_anompb) &
+          bind(C, name="mom_gpu_barotropic_closure")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: pressure_based
+       real(c_double), value :: rho0, grav
+       type(c_ptr), value :: eta_t_bar, anompb_bar, patm, rho_g, ht, tmasken
+       type(c_ptr), value :: eta_u, ps, grad_ps, grad_anompb
+     end function mom_gpu_barotropic_closure
 
      ! the batched step (all MDFL-Sweby tracers, FP64 or FP32 storage each)
      integer(c_int) function mom_gpu_step_tracers(ctx, s) bind(C, name="mom_gpu_step_tracers")

@@ -26,7 +26,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import BtParams, Domain, Grid, MomGpuError, TracerStep, check, load
+from ._lib import BtParams, BtVariant, Domain, Grid, MomGpuError, TracerStep, check, load
 
 ADVECT_QUICKER = _lib.MOM_ADVECT_QUICKER
 ADVECT_MDFL_SWEBY = _lib.MOM_ADVECT_MDFL_SWEBY
@@ -262,6 +262,41 @@ class OceanGPU:
             _p(mass_source), _p(uhrho_et), _p(vhrho_nt), _p(wrho_bt), _p(diverge_t)),
             "advection_velocity")
 
+    def barotropic_forcing(self, accel, smf, bmf, forcing_bt):
+        """ocean_barotropic_forcing (B-grid, default namelist)."""
+        self._chk(self.L.mom_gpu_barotropic_forcing(self.ctx, _p(accel), _p(smf), _p(bmf),
+                                                    _p(forcing_bt)), "barotropic_forcing")
+
+    def eta_and_pbot_update(self, x_taum1, x_taup1, tendency, *, conv_rho_ud_t_tau, pme, river,
+                            source, dtime, rho0=1035.0, grav=9.80, pressure_based=False,
+                            dpatm_dt=None, ht=None, pbot_t_taup1=None):
+        """eta_and_pbot_tendency + eta_and_pbot_update: eta_t (or anompb)
+        at taup1 and its tendency; pbot_t at taup1 when pressure-based."""
+        self._chk(self.L.mom_gpu_eta_and_pbot_update(
+            self.ctx, int(bool(pressure_based)), float(rho0), float(grav), float(dtime),
+            _p(conv_rho_ud_t_tau), _p(pme), _p(river), _p(source), _p(dpatm_dt), _p(ht),
+            _p(x_taum1), _p(x_taup1), _p(pbot_t_taup1), _p(tendency)), "eta_and_pbot_update")
+
+    def set_remap_metrics(self, dte, dtw, dus, dun, daur):
+        """Grid widths of REMAP_BT_TO_BU (host arrays on the data domain)."""
+        keep = [np.ascontiguousarray(a, dtype=np.float64) for a in (dte, dtw, dus, dun, daur)]
+        for a in keep:
+            if a.shape != self.shape2:
+                raise ValueError(f"remap metric of shape {a.shape}, expected {self.shape2}")
+        self._chk(self.L.mom_gpu_set_remap_metrics(self.ctx, *[a.ctypes.data for a in keep]),
+                  "set_remap_metrics")
+
+    def barotropic_closure(self, eta_t_bar, patm, rho_g, eta_u, ps, grad_ps, *, anompb_bar=None,
+                           grad_anompb=None, pressure_based=False, ht=None, rho0=1035.0,
+                           grav=9.80, tmasken=None):
+        """eta_u (pbot_u) closure, ps, grad_ps and grad_anompb of
+        update_ocean_barotropic (C-grid gradients when tmasken is given)."""
+        self._chk(self.L.mom_gpu_barotropic_closure(
+            self.ctx, int(bool(pressure_based)), float(rho0), float(grav), _p(eta_t_bar),
+            _p(anompb_bar), _p(patm), _p(rho_g), _p(ht), _p(tmasken), _p(eta_u), _p(ps),
+            _p(grad_ps),
+            _p(grad_anompb)), "barotropic_closure")
+
     # ---- tracer advection ------------------------------------------------
     def horz_advect_tracer(self, scheme, T_taum1, th_tendency, *, uhrho_et, vhrho_nt,
                            wrho_bt=None, rho_dzt_tau=None, T_tau=None, dtime=0.0,
@@ -332,6 +367,28 @@ class OceanGPU:
             _p(forcing_bt), _p(thicku), _p(patm), _p(pme), _p(river), _p(source),
             _p(eta_smooth), _p(rho_g), _p(eta_t_bar_taup1), _p(udrho_taup1),
             _p(conv_rho_ud_t)), "update_ocean_barotropic")
+
+    def update_ocean_barotropic_variant(self, *, cgrid, pressure_based, nts, dtbt, x_bar_tau,
+                                        udrho_tau, forcing_bt, thick, pme, river, source, smooth,
+                                        x_bar_taup1, udrho_taup1, conv_rho_ud_t=None, x_tau=None,
+                                        tmasken=None, patm=None, rho_g=None, dpatm_dt=None,
+                                        grav=9.80, pred_corr_gamma=0.2, rho0=1035.0,
+                                        stepping_a=False, initsum_with_bar=True):
+        """pred_corr_tropic_{depth,press}_{bgrid,cgrid} and the time-mean
+        tail of update_ocean_barotropic (x: eta_t or anompb)."""
+        p = BtParams(nts=int(nts), dtbt=float(dtbt), pred_corr_gamma=float(pred_corr_gamma),
+                     rho0=float(rho0), stepping_a=int(bool(stepping_a)),
+                     initsum_with_bar=int(bool(initsum_with_bar)))
+        v = BtVariant(cgrid=int(bool(cgrid)), pressure_based=int(bool(pressure_based)),
+                      grav=float(grav), x_tau=_p(x_tau), x_bar_tau=_p(x_bar_tau),
+                      udrho_tau=_p(udrho_tau), forcing_bt=_p(forcing_bt), thick=_p(thick),
+                      tmasken=_p(tmasken), patm=_p(patm), rho_g=_p(rho_g),
+                      dpatm_dt=_p(dpatm_dt), pme=_p(pme), river=_p(river), source=_p(source),
+                      smooth=_p(smooth), x_bar_taup1=_p(x_bar_taup1),
+                      udrho_taup1=_p(udrho_taup1), conv_rho_ud_t=_p(conv_rho_ud_t))
+        self._chk(self.L.mom_gpu_update_ocean_barotropic_variant(self.ctx, C.byref(p),
+                                                                 C.byref(v)),
+                  "update_ocean_barotropic_variant")
 
     # ---- batched tracer step -------------------------------------------------
     def step_tracers(self, T_taum1s, T_taup1s, *, uhrho_et, vhrho_nt, wrho_bt, rho_dzt_tau,

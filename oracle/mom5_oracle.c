@@ -1003,3 +1003,363 @@ void orc_advection_velocity(const orc_dom *d, const orc_grid *g, const double *u
   free(bdy);
   free(div);
 }
+
+/* ---- ocean_barotropic_forcing ---------------------------------------------
+ * ocean_core/ocean_barotropic.F90:3058-3141 with the default namelist
+ * (zero_nonlinear_forcing_bt = zero_forcing_bt = udrho_lap = udrho_bih =
+ * .false., no OBC).  On the B-grid tmasken(:,:,:,n) = umask
+ * (ocean_core/ocean_grids.F90:1403-1410):
+ *   forcing_bt = 0 on the data domain;
+ *   wrk(n) = sum_k umask(k)*accel(k,n) over the compute domain (k ascending);
+ *   forcing_bt(n) = wrk(n) + umask(1)*(smf(n) - bmf(n));
+ *   mpp_update_domains (single PE: cyclic wrap);
+ *   forcing_bt(n) = forcing_bt(n) + friction_lap(n) + friction_bih(n) (both 0).
+ * accel: (isd:ied,jsd:jed,nk,2); smf, bmf, forcing_bt: (isd:ied,jsd:jed,2). */
+void orc_barotropic_forcing(const orc_dom *d, const orc_grid *g, const double *accel,
+                            const double *smf, const double *bmf, double *forcing_bt) {
+  const size_t p2 = P2, p3 = p2 * (size_t)d->nk;
+  for (size_t o = 0; o < 2 * p2; ++o) forcing_bt[o] = 0.0;
+  for (int n = 0; n < 2; ++n)
+    for (int j = d->jsc; j <= d->jec; ++j)
+      for (int i = d->isc; i <= d->iec; ++i) {
+        double wrk = 0.0;
+        for (int k = 1; k <= d->nk; ++k)
+          wrk = wrk + g->umask[I3(i, j, k)] * accel[p3 * (size_t)n + I3(i, j, k)];
+        forcing_bt[p2 * (size_t)n + I2(i, j)] =
+            wrk + g->umask[I3(i, j, 1)] * (smf[p2 * (size_t)n + I2(i, j)] -
+                                           bmf[p2 * (size_t)n + I2(i, j)]);
+      }
+  orc_halo_update(d, 1, forcing_bt, 2, 3);
+  for (size_t o = 0; o < 2 * p2; ++o) forcing_bt[o] = forcing_bt[o] + 0.0 + 0.0;
+}
+
+/* ocean_barotropic.F90:2667-2700 (eta_and_pbot_tendency, use_blobs false)
+ * followed by :2290-2363 (eta_and_pbot_update, no OBC, truncate_eta false,
+ * splitting).  Depth-based (pressure_based = 0):
+ *   tendency = tmask(1)*rho0r*(conv + pme + river + source) on the data domain;
+ *   x_taup1  = x_taum1 + dtime*tendency on the compute domain, halo-updated.
+ * Pressure-based: tendency = dpatm_dt + grav*tmask(1)*(conv + ...), x is
+ * anompb, and pbot_taup1 = x_taup1 + rho0grav*ht on the data domain. */
+void orc_eta_and_pbot_update(const orc_dom *d, const orc_grid *g, int pressure_based,
+    double rho0, double grav, double dtime, const double *conv_tau, const double *pme,
+    const double *river, const double *source, const double *dpatm_dt, const double *ht,
+    const double *x_taum1, double *x_taup1, double *pbot_taup1, double *tendency) {
+  const double rho0r = 1.0 / rho0;          /* ocean_parameters.F90:260 */
+  const double rho0grav = rho0 * grav;      /* ocean_barotropic.F90:1191 */
+  for (int j = d->jsd; j <= d->jed; ++j)
+    for (int i = d->isd; i <= d->ied; ++i) {
+      const size_t o = I2(i, j);
+      const double tm = g->tmask[I3(i, j, 1)];
+      if (!pressure_based)
+        tendency[o] = tm * rho0r * (conv_tau[o] + pme[o] + river[o] + source[o]);
+      else
+        tendency[o] = dpatm_dt[o] + grav * tm * (conv_tau[o] + pme[o] + river[o] + source[o]);
+    }
+  for (int j = d->jsc; j <= d->jec; ++j)
+    for (int i = d->isc; i <= d->iec; ++i)
+      x_taup1[I2(i, j)] = x_taum1[I2(i, j)] + dtime * tendency[I2(i, j)];
+  orc_halo_update(d, 1, x_taup1, 1, 3);
+  if (pressure_based)
+    for (size_t o = 0; o < P2; ++o) pbot_taup1[o] = x_taup1[o] + rho0grav * ht[o];
+}
+
+/* ocean_operators.F90:462-477, REMAP_BT_TO_BU with halo 1. */
+static void orc_remap_bt_to_bu(const orc_dom *d, const double *a, const double *dte,
+    const double *dtw, const double *dus, const double *dun, const double *daur, double *out) {
+  for (size_t o = 0; o < P2; ++o) out[o] = 0.0;
+  for (int j = d->jsd; j <= d->jed - 1; ++j)
+    for (int i = d->isd; i <= d->ied - 1; ++i)
+      out[I2(i, j)] = (a[I2(i, j)] * dte[I2(i, j)] * dus[I2(i, j)]
+                       + a[I2(i + 1, j)] * dtw[I2(i + 1, j)] * dus[I2(i, j)]
+                       + a[I2(i, j + 1)] * dte[I2(i, j + 1)] * dun[I2(i, j)]
+                       + a[I2(i + 1, j + 1)] * dtw[I2(i + 1, j + 1)] * dun[I2(i, j)])
+                      * daur[I2(i, j)];
+}
+
+/* ocean_operators.F90:631-676, GRAD_BAROTROPIC_P on the B-grid with
+ * halo_in = halo_out = 1; umask_bt = umask(:,:,1), dxur_bt = dxur
+ * (:321-323).  out: (isd:ied, jsd:jed, 2). */
+static void orc_grad_barotropic_p(const orc_dom *d, const orc_grid *g, const double *p,
+                                  double *out) {
+  for (size_t o = 0; o < 2 * P2; ++o) out[o] = 0.0;
+  for (int j = d->jsd; j <= d->jed - 1; ++j)
+    for (int i = d->isd; i <= d->ied - 1; ++i) {
+      const double p_fay = 0.5 * (p[I2(i, j)] + p[I2(i, j + 1)]);
+      const double p_fax = 0.5 * (p[I2(i, j)] + p[I2(i + 1, j)]);
+      const double pip_fay = 0.5 * (p[I2(i + 1, j)] + p[I2(i + 1, j + 1)]);
+      const double pjp_fax = 0.5 * (p[I2(i, j + 1)] + p[I2(i + 1, j + 1)]);
+      out[I2(i, j)] = (pip_fay - p_fay) * g->dxur[I2(i, j)] * g->umask[I3(i, j, 1)];
+      out[P2 + I2(i, j)] = (pjp_fax - p_fax) * g->dyur[I2(i, j)] * g->umask[I3(i, j, 1)];
+    }
+}
+
+/* ocean_barotropic.F90:2906-2921 and :2956-2985 (splitting, no OBC, no
+ * tidal or geoid forcing, zero_eta_u false):
+ *   depth-based:    eta_u = umask(1)*REMAP_BT_TO_BU(eta_t_bar), halo-updated;
+ *   pressure-based: eta_u (pbot_u) = umask(1)*REMAP_BT_TO_BU(rho0grav*ht + anompb_bar);
+ *   ps = rho_g*eta_t_bar + patm; grad_ps = GRAD_BAROTROPIC_P(ps);
+ *   grad_anompb = GRAD_BAROTROPIC_P(anompb_bar).
+ * eta_t_bar and anompb_bar are the itime level (taup1 for two-level
+ * stepping, :2797-2801).  ht is read only when pressure_based;
+ * anompb_bar may be NULL when depth-based (grad_anompb is then not written).
+ * tmasken (Grd%tmasken(:,:,1,1:2)) selects the C-grid gradient
+ * (ocean_operators.F90:664-669); NULL: B-grid. */
+static void orc_grad_p_c(const orc_dom *d, const double *p, const double *dxter,
+                         const double *dytnr, const double *tmasken, double *out);
+
+void orc_barotropic_closure(const orc_dom *d, const orc_grid *g, int pressure_based,
+    double rho0, double grav, const double *eta_t_bar, const double *anompb_bar,
+    const double *patm, const double *rho_g, const double *ht, const double *tmasken,
+    const double *dte, const double *dtw, const double *dus, const double *dun,
+    const double *daur, double *eta_u, double *ps, double *grad_ps, double *grad_anompb) {
+  const double rho0grav = rho0 * grav;
+  double *a = (double *)malloc(P2 * sizeof(double));
+  double *r = (double *)malloc(P2 * sizeof(double));
+  if (!pressure_based) {
+    for (size_t o = 0; o < P2; ++o) a[o] = eta_t_bar[o];
+  } else {
+    for (size_t o = 0; o < P2; ++o) a[o] = rho0grav * ht[o] + anompb_bar[o];
+  }
+  orc_remap_bt_to_bu(d, a, dte, dtw, dus, dun, daur, r);
+  for (int j = d->jsd; j <= d->jed; ++j)
+    for (int i = d->isd; i <= d->ied; ++i)
+      eta_u[I2(i, j)] = g->umask[I3(i, j, 1)] * r[I2(i, j)];
+  orc_halo_update(d, 1, eta_u, 1, 3);
+  for (size_t o = 0; o < P2; ++o) ps[o] = rho_g[o] * eta_t_bar[o] + patm[o];
+  if (!tmasken) {
+    orc_grad_barotropic_p(d, g, ps, grad_ps);
+    if (anompb_bar && grad_anompb) orc_grad_barotropic_p(d, g, anompb_bar, grad_anompb);
+  } else {
+    for (size_t o = 0; o < P2; ++o) {
+      a[o] = 1.0 / (g->dxte[o] + 1.0e-40);   /* Grd%dxter, ocean_grids.F90:956 */
+      r[o] = 1.0 / (g->dytn[o] + 1.0e-40);   /* Grd%dytnr, :959 */
+    }
+    orc_grad_p_c(d, ps, a, r, tmasken, grad_ps);
+    if (anompb_bar && grad_anompb) orc_grad_p_c(d, anompb_bar, a, r, tmasken, grad_anompb);
+  }
+  free(a);
+  free(r);
+}
+
+/* ------------------------------------------------------------------ */
+/* The four split-explicit barotropic schemes of update_ocean_barotropic */
+/* (ocean_barotropic.F90:2829-2843) with barotropic_halo = 1, no tides,  */
+/* geoid, friction, smoothing, blobs or OBC:                            */
+/*   pred_corr_tropic_depth_bgrid :3225-3579 (legacy halos),            */
+/*   pred_corr_tropic_depth_cgrid :3601-3965,                           */
+/*   pred_corr_tropic_press_bgrid :3989-4329 (legacy halos),            */
+/*   pred_corr_tropic_press_cgrid :4355-4693,                           */
+/* and the tail :2846-2894 (time means, conv_rho_ud_t).                 */
+/* ------------------------------------------------------------------ */
+
+/* ocean_operators.F90:526-536, DIV_UD on the C-grid, halo 1.
+ * dxter = 1/(dxte+epsln), dytnr = 1/(dytn+epsln) (ocean_grids.F90:956,959). */
+static void orc_div_ud_c(const orc_dom *d, const orc_grid *g, const double *ud,
+                         const double *dxter, const double *dytnr, double *out) {
+  const double *u1 = ud, *u2 = ud + P2;
+  for (size_t o = 0; o < P2; ++o) out[o] = 0.0;
+  for (int j = d->jsd + 1; j <= d->jed; ++j)
+    for (int i = d->isd + 1; i <= d->ied; ++i) {
+      const double uhim = u1[I2(i - 1, j)] * g->dat[I2(i - 1, j)] * dxter[I2(i - 1, j)];
+      const double vhjm = u2[I2(i, j - 1)] * g->dat[I2(i, j - 1)] * dytnr[I2(i, j - 1)];
+      const double uh = u1[I2(i, j)] * g->dat[I2(i, j)] * dxter[I2(i, j)];
+      const double vh = u2[I2(i, j)] * g->dat[I2(i, j)] * dytnr[I2(i, j)];
+      out[I2(i, j)] = ((uh - uhim) + (vh - vhjm)) * g->datr[I2(i, j)];
+    }
+}
+
+/* ocean_operators.F90:664-669, GRAD_BAROTROPIC_P on the C-grid, halo 1. */
+static void orc_grad_p_c(const orc_dom *d, const double *p, const double *dxter,
+                         const double *dytnr, const double *tmasken, double *out) {
+  for (size_t o = 0; o < 2 * P2; ++o) out[o] = 0.0;
+  for (int j = d->jsd; j <= d->jed - 1; ++j)
+    for (int i = d->isd; i <= d->ied - 1; ++i) {
+      out[I2(i, j)] = (p[I2(i + 1, j)] - p[I2(i, j)]) * dxter[I2(i, j)] * tmasken[I2(i, j)];
+      out[P2 + I2(i, j)] =
+          (p[I2(i, j + 1)] - p[I2(i, j)]) * dytnr[I2(i, j)] * tmasken[P2 + I2(i, j)];
+    }
+}
+
+void orc_update_ocean_barotropic_variant(const orc_dom *d, const orc_grid *g,
+    const orc_bt_params *p, int cgrid, int press, double grav, const double *x_tau,
+    const double *xbar_tau, const double *udrho_tau, const double *forcing_bt,
+    const double *thick, const double *tmasken, const double *patm, const double *rho_g,
+    const double *dpatm_dt, const double *pme, const double *river, const double *source,
+    const double *smooth, double *xbar, double *udrho, double *conv_rho_ud_t) {
+  const int nts = p->nts;
+  const double dtbt = p->dtbt, gamma = p->pred_corr_gamma;
+  const double rho0r = 1.0 / p->rho0;
+  const double dtbt_gamma = dtbt * gamma;                              /* :1195 */
+  const double dtbt_gamma_rho0r = dtbt * gamma * rho0r;                /* :1196 */
+  const double dtbt_rho0r = dtbt * rho0r;                              /* :1197 */
+  const double ab0 = 23.0 / 12.0, ab1 = -16.0 / 12.0, ab2 = 5.0 / 12.0; /* :617-619 */
+  const double *tmask1 = g->tmask;
+  double *x_bt = (double *)calloc(2 * P2, sizeof(double));
+  double *ud_bt = (double *)calloc(4 * P2, sizeof(double));
+  double *cf = (double *)calloc(6 * P2, sizeof(double));    /* coriolis_force_bt(…,2,3) */
+  double *acc = (double *)calloc(2 * P2, sizeof(double));   /* coriolis_accel_bt */
+  double *sf = (double *)malloc(P2 * sizeof(double));
+  double *tmp = (double *)malloc(P2 * sizeof(double));
+  double *ps = (double *)malloc(P2 * sizeof(double));
+  double *gp = (double *)malloc(2 * P2 * sizeof(double));
+  double *cori1 = (double *)malloc(P2 * sizeof(double));
+  double *cori2 = (double *)malloc(P2 * sizeof(double));
+  double *dxter = (double *)malloc(P2 * sizeof(double));
+  double *dytnr = (double *)malloc(P2 * sizeof(double));
+#define XB(o, lev) x_bt[(o) + P2 * (size_t)((lev) - 1)]
+#define UD(o, c, lev) ud_bt[(o) + P2 * ((size_t)((c) - 1) + 2 * (size_t)((lev) - 1))]
+#define CF(o, c, lev) cf[(o) + P2 * ((size_t)((c) - 1) + 2 * (size_t)((lev) - 1))]
+  for (size_t o = 0; o < P2; ++o) {
+    cori1[o] = 0.5 * dtbt * g->f[o];                                  /* :1283-1288 */
+    cori2[o] = 1.0 / (1.0 + cori1[o] * cori1[o]);
+    dxter[o] = 1.0 / (g->dxte[o] + 1.0e-40);
+    dytnr[o] = 1.0 / (g->dytn[o] + 1.0e-40);
+  }
+  int itime = 1;
+  int fstau = (itime + 0) % 2 + 1, fstaup1 = (itime + 1) % 2 + 1;
+  int m2 = (itime + 0) % 3 + 1, m1 = (itime + 1) % 3 + 1, m0 = (itime + 2) % 3 + 1;
+  for (size_t o = 0; o < P2; ++o) {
+    xbar[o] = p->initsum_with_bar ? xbar_tau[o] : x_tau[o];
+    udrho[o] = 0.0;
+    udrho[o + P2] = 0.0;
+    XB(o, fstau) = xbar_tau[o];
+    XB(o, fstaup1) = xbar_tau[o];     /* press: :4103; depth: overwritten */
+    UD(o, 1, fstau) = udrho_tau[o];
+    UD(o, 2, fstau) = udrho_tau[o + P2];
+    if (!press)
+      sf[o] = tmask1[o] * (pme[o] + river[o] + source[o] - smooth[o]);
+    else
+      sf[o] = tmask1[o] * (grav * (pme[o] + river[o] + source[o] - smooth[o]) + dpatm_dt[o]);
+  }
+  /* C-grid: Coriolis force at all three Adams-Bashforth levels (:3711-3730) */
+#define CORI_FORCE(i, j, lev)                                                          \
+  do {                                                                                 \
+    CF(I2(i, j), 1, m0) = 0.25 * (g->f[I2(i, j)] * (UD(I2(i, j), 2, lev) +             \
+                                                    UD(I2((i) + 1, j), 2, lev)) +      \
+                                  g->f[I2(i, (j) - 1)] * (UD(I2(i, (j) - 1), 2, lev) + \
+                                                          UD(I2((i) + 1, (j) - 1), 2, lev))); \
+    CF(I2(i, j), 2, m0) = -(0.25 * (g->f[I2(i, j)] * (UD(I2(i, j), 1, lev) +           \
+                                                      UD(I2(i, (j) + 1), 1, lev)) +    \
+                                    g->f[I2((i) - 1, j)] * (UD(I2((i) - 1, j), 1, lev) + \
+                                                            UD(I2((i) - 1, (j) + 1), 1, lev)))); \
+  } while (0)
+  if (cgrid)
+    for (int j = d->jsc; j <= d->jec; ++j)
+      for (int i = d->isc; i <= d->iec; ++i) {
+        CORI_FORCE(i, j, fstau);
+        for (int c = 1; c <= 2; ++c) {
+          CF(I2(i, j), c, m1) = CF(I2(i, j), c, m0);
+          CF(I2(i, j), c, m2) = CF(I2(i, j), c, m0);
+        }
+      }
+  const int nthick = cgrid ? 2 : 1;
+  for (itime = 1; itime <= nts; ++itime) {
+    fstau = (itime + 0) % 2 + 1;
+    fstaup1 = (itime + 1) % 2 + 1;
+    m2 = (itime + 0) % 3 + 1;
+    m1 = (itime + 1) % 3 + 1;
+    m0 = (itime + 2) % 3 + 1;
+    /* predictor */
+    if ((press && cgrid) || gamma > 0.0) {   /* press_cgrid has no gamma test */
+      if (cgrid) orc_div_ud_c(d, g, &UD(0, 1, fstau), dxter, dytnr, tmp);
+      else orc_div_ud(d, g, &UD(0, 1, fstau), tmp);
+      for (size_t o = 0; o < P2; ++o) {
+        if (!press) {
+          tmp[o] = -tmp[o] * tmask1[o];
+          XB(o, fstaup1) = XB(o, fstau) + dtbt_gamma_rho0r * (tmp[o] + sf[o]);
+        } else {
+          tmp[o] = -grav * tmp[o];
+          XB(o, fstaup1) = XB(o, fstau) + dtbt_gamma * (tmp[o] + sf[o]);
+        }
+      }
+    } else {
+      for (size_t o = 0; o < P2; ++o) XB(o, fstaup1) = XB(o, fstau);
+    }
+    if (!cgrid) orc_halo_update(d, 1, &XB(0, fstaup1), 1, 3);   /* legacy halos */
+    for (size_t o = 0; o < P2; ++o)
+      ps[o] = press ? tmask1[o] * XB(o, fstaup1) : rho_g[o] * XB(o, fstaup1) + patm[o];
+    if (cgrid) {
+      for (size_t o = 0; o < 2 * P2; ++o) acc[o] = 0.0;
+      for (int j = d->jsd + 1; j <= d->jed - 1; ++j)
+        for (int i = d->isd + 1; i <= d->ied - 1; ++i) {
+          CORI_FORCE(i, j, fstau);
+          for (int c = 1; c <= 2; ++c)
+            acc[P2 * (size_t)(c - 1) + I2(i, j)] = ab0 * CF(I2(i, j), c, m0) +
+                                                   ab1 * CF(I2(i, j), c, m1) +
+                                                   ab2 * CF(I2(i, j), c, m2);
+        }
+      orc_grad_p_c(d, ps, dxter, dytnr, tmasken, gp);
+    } else {
+      orc_grad_p(d, g, ps, gp);
+    }
+    for (size_t o = 0; o < P2; ++o) {
+      double pf[2];
+      for (int c = 0; c < 2; ++c) {
+        const double th = thick[(nthick == 2 ? P2 * (size_t)c : 0) + o];
+        pf[c] = press ? -(rho0r * th * gp[P2 * (size_t)c + o]) : -(th * gp[P2 * (size_t)c + o]);
+      }
+      const double u1 = UD(o, 1, fstau), u2 = UD(o, 2, fstau);
+      if (cgrid) {
+        const double a1 = acc[o] + pf[0] + forcing_bt[o];
+        const double a2 = acc[o + P2] + pf[1] + forcing_bt[o + P2];
+        UD(o, 1, fstaup1) = u1 + dtbt * tmasken[o] * (press ? a1 : a1 + 0.0);
+        UD(o, 2, fstaup1) = u2 + dtbt * tmasken[o + P2] * (press ? a2 : a2 + 0.0);
+      } else {
+        double t1, t2;
+        if (!press) {
+          t1 = u1 + dtbt * (0.5 * g->f[o] * u2 + pf[0] + forcing_bt[o] + 0.0);
+          t2 = u2 + dtbt * (-(0.5 * g->f[o] * u1) + pf[1] + forcing_bt[o + P2] + 0.0);
+        } else {
+          t1 = u1 + dtbt * (0.5 * g->f[o] * u2 + pf[0] + forcing_bt[o]);
+          t2 = u2 + dtbt * (-(0.5 * g->f[o] * u1) + pf[1] + forcing_bt[o + P2]);
+        }
+        UD(o, 1, fstaup1) = cori2[o] * (t1 + cori1[o] * t2);
+        UD(o, 2, fstaup1) = cori2[o] * (t2 - cori1[o] * t1);
+      }
+    }
+    if (cgrid) orc_halo_update(d, 1, &UD(0, 1, fstaup1), 2, 3);
+    /* corrector */
+    if (cgrid) orc_div_ud_c(d, g, &UD(0, 1, fstaup1), dxter, dytnr, tmp);
+    else orc_div_ud(d, g, &UD(0, 1, fstaup1), tmp);
+    for (size_t o = 0; o < P2; ++o) {
+      if (!press) {
+        tmp[o] = -tmp[o] * tmask1[o];
+        XB(o, fstaup1) = XB(o, fstau) + dtbt_rho0r * (tmp[o] + sf[o]);
+      } else {
+        tmp[o] = -grav * tmp[o];
+        XB(o, fstaup1) = XB(o, fstau) + dtbt * (tmp[o] + sf[o]);
+      }
+    }
+    /* accumulate */
+    const double wgt = (double)(nts - itime + 1);
+    for (size_t o = 0; o < P2; ++o) {
+      xbar[o] = xbar[o] + XB(o, fstaup1);
+      if (p->stepping_a) {
+        udrho[o] = udrho[o] + UD(o, 1, fstaup1);
+        udrho[o + P2] = udrho[o + P2] + UD(o, 2, fstaup1);
+      } else {
+        udrho[o] = udrho[o] + wgt * UD(o, 1, fstaup1);
+        udrho[o + P2] = udrho[o + P2] + wgt * UD(o, 2, fstaup1);
+      }
+    }
+  }
+#undef CORI_FORCE
+  const double rntsp1 = 1.0 / ((double)nts + 1.0);
+  const double rnts = p->stepping_a ? 1.0 / (double)nts
+                                    : 2.0 / ((double)nts * (double)(nts + 1));
+  for (size_t o = 0; o < P2; ++o) xbar[o] = rntsp1 * xbar[o];
+  orc_halo_update(d, 1, xbar, 1, 3);
+  for (size_t o = 0; o < 2 * P2; ++o) udrho[o] = rnts * udrho[o];
+  orc_halo_update(d, 1, udrho, 2, 3);
+  if (conv_rho_ud_t) {
+    if (cgrid) orc_div_ud_c(d, g, udrho, dxter, dytnr, conv_rho_ud_t);
+    else orc_div_ud(d, g, udrho, conv_rho_ud_t);
+    for (size_t o = 0; o < P2; ++o) conv_rho_ud_t[o] = -conv_rho_ud_t[o];
+    orc_halo_update(d, 1, conv_rho_ud_t, 1, 3);
+  }
+#undef XB
+#undef UD
+#undef CF
+  free(x_bt); free(ud_bt); free(cf); free(acc); free(sf); free(tmp); free(ps); free(gp);
+  free(cori1); free(cori2); free(dxter); free(dytnr);
+}

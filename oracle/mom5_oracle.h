@@ -150,6 +150,49 @@ void orc_advection_velocity(const orc_dom *d, const orc_grid *g, const double *u
     const double *rho_dzt_tendency, const double *mass_source,
     double *uhrho_et, double *vhrho_nt, double *wrho_bt, double *diverge_t);
 
+/* ocean_core/ocean_barotropic.F90:3058-3141 (default namelist, B-grid):
+ * forcing_bt(:,:,n) = sum_k umask*accel(:,:,k,n) + umask(1)*(smf - bmf) on
+ * the compute domain, halo-updated.  accel (data domain, nk, 2); smf, bmf,
+ * forcing_bt (data domain, 2). */
+void orc_barotropic_forcing(const orc_dom *d, const orc_grid *g, const double *accel,
+    const double *smf, const double *bmf, double *forcing_bt);
+
+/* ocean_barotropic.F90:2667-2700 + :2290-2363: surface height (or, when
+ * pressure_based, bottom pressure anomaly) tendency and "big" time step;
+ * all arrays 2-D on the data domain.  dpatm_dt, ht and pbot_taup1 are read
+ * or written only when pressure_based. */
+void orc_eta_and_pbot_update(const orc_dom *d, const orc_grid *g, int pressure_based,
+    double rho0, double grav, double dtime, const double *conv_tau, const double *pme,
+    const double *river, const double *source, const double *dpatm_dt, const double *ht,
+    const double *x_taum1, double *x_taup1, double *pbot_taup1, double *tendency);
+
+/* ocean_barotropic.F90:2906-2921, :2956-2985: eta_u (pbot_u) closure by
+ * REMAP_BT_TO_BU, surface pressure ps and the gradients grad_ps,
+ * grad_anompb ((isd:ied,jsd:jed,2)); B-grid, or C-grid when tmasken is
+ * given. */
+void orc_barotropic_closure(const orc_dom *d, const orc_grid *g, int pressure_based,
+    double rho0, double grav, const double *eta_t_bar, const double *anompb_bar,
+    const double *patm, const double *rho_g, const double *ht, const double *tmasken,
+    const double *dte, const double *dtw, const double *dus, const double *dun,
+    const double *daur, double *eta_u, double *ps, double *grad_ps, double *grad_anompb);
+
+/* The four split-explicit barotropic schemes with barotropic_halo = 1 (no
+ * tides, geoid, friction, smoothing, blobs or OBC) and the tail of
+ * update_ocean_barotropic: cgrid selects pred_corr_tropic_*_cgrid
+ * (ocean_barotropic.F90:3601-3965, 4355-4693) over *_bgrid (:3225-3579,
+ * 3989-4329), press the pressure-based form (x = anompb) over the
+ * depth-based one (x = eta_t).  thick: thicku (B, depth), thicken (C, depth,
+ * 2 planes), mass_u (B, press), mass_en (C, press, 2 planes); tmasken
+ * (C-grid) = Grd%tmasken(:,:,1,1:2); patm, rho_g (depth); dpatm_dt (press);
+ * smooth = eta_smooth (depth) or pbot_smooth (press).  x_tau is read only
+ * when initsum_with_bar = 0. */
+void orc_update_ocean_barotropic_variant(const orc_dom *d, const orc_grid *g,
+    const orc_bt_params *p, int cgrid, int press, double grav, const double *x_tau,
+    const double *xbar_tau, const double *udrho_tau, const double *forcing_bt,
+    const double *thick, const double *tmasken, const double *patm, const double *rho_g,
+    const double *dpatm_dt, const double *pme, const double *river, const double *source,
+    const double *smooth, double *xbar, double *udrho, double *conv_rho_ud_t);
+
 #ifdef __cplusplus
 }
 #endif

@@ -204,6 +204,29 @@ class Oracle:
             _p(eta_bar), _p(ud), _p(conv) if conv is not None else None)
         return eta_bar, ud, conv
 
+    def update_ocean_barotropic_variant(self, nts, dtbt, *, cgrid, press, fields, gamma=0.2,
+                                        stepping_a=False, initsum_with_bar=True, rho0=1035.0,
+                                        grav=9.80, want_conv=True):
+        """One of the four split-explicit schemes + tail.  fields: x_tau,
+        xbar_tau, udrho, forcing_bt, thick, tmasken (C-grid), patm, rho_g
+        (depth), dpatm_dt (press), pme, river, source, smooth.  Returns
+        (xbar, udrho, conv)."""
+        p = BtParams(nts=nts, dtbt=dtbt, pred_corr_gamma=gamma, rho0=rho0,
+                     stepping_a=int(stepping_a), initsum_with_bar=int(initsum_with_bar))
+        f = {k: (np.ascontiguousarray(v, dtype=np.float64) if v is not None else None)
+             for k, v in fields.items()}
+        g = lambda k: _p(f[k]) if f.get(k) is not None else None
+        xbar = np.zeros(self.shape2)
+        ud = np.zeros((2,) + self.shape2)
+        conv = np.zeros(self.shape2) if want_conv else None
+        self.L.orc_update_ocean_barotropic_variant(
+            C.byref(self.dom), C.byref(self.grid), C.byref(p), C.c_int(int(cgrid)),
+            C.c_int(int(press)), C.c_double(grav), g("x_tau"), g("xbar_tau"), g("udrho"),
+            g("forcing_bt"), g("thick"), g("tmasken"), g("patm"), g("rho_g"), g("dpatm_dt"),
+            g("pme"), g("river"), g("source"), g("smooth"), _p(xbar), _p(ud),
+            _p(conv) if conv is not None else None)
+        return xbar, ud, conv
+
     def tracer_content(self, T, rho_dzt=None):
         rd = self.case.fields["rho_dzt"] if rho_dzt is None else rho_dzt
         self.L.orc_tracer_content.restype = C.c_double
@@ -227,3 +250,59 @@ class Oracle:
             _p(mass_source) if mass_source is not None else None, _p(ue), _p(vn), _p(wb),
             _p(div))
         return ue, vn, wb, div
+
+    # --- barotropic forcing ------------------------------------------
+    def barotropic_forcing(self, accel, smf, bmf):
+        """ocean_barotropic_forcing (default namelist, B-grid): accel is
+        (2, nk, NJ, NI), smf and bmf (2, NJ, NI); returns forcing_bt."""
+        accel = np.ascontiguousarray(accel, dtype=np.float64)
+        smf = np.ascontiguousarray(smf, dtype=np.float64)
+        bmf = np.ascontiguousarray(bmf, dtype=np.float64)
+        out = np.zeros((2,) + self.shape2)
+        self.L.orc_barotropic_forcing(C.byref(self.dom), C.byref(self.grid), _p(accel), _p(smf),
+                                      _p(bmf), _p(out))
+        return out
+
+    def eta_and_pbot_update(self, x_taum1, conv_tau, pme, river, source, dtime, *,
+                            pressure_based=False, dpatm_dt=None, ht=None, x_taup1=None,
+                            rho0=1035.0, grav=9.80):
+        """eta_and_pbot_tendency + eta_and_pbot_update: returns (x_taup1,
+        tendency, pbot_taup1 or None).  x_taup1 (optional) is the array's
+        previous contents (closed halos keep them)."""
+        f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        ins = [f64(a) for a in (conv_tau, pme, river, source, x_taum1)]
+        out = f64(np.zeros(self.shape2) if x_taup1 is None else np.array(x_taup1, np.float64))
+        tend = np.zeros(self.shape2)
+        if pressure_based:
+            dpatm_dt, ht = f64(dpatm_dt), f64(ht)
+            pbot = np.zeros(self.shape2)
+        self.L.orc_eta_and_pbot_update(
+            C.byref(self.dom), C.byref(self.grid), C.c_int(int(pressure_based)),
+            C.c_double(rho0), C.c_double(grav), C.c_double(dtime), *[_p(a) for a in ins[:4]],
+            _p(dpatm_dt) if pressure_based else None, _p(ht) if pressure_based else None,
+            _p(ins[4]), _p(out), _p(pbot) if pressure_based else None, _p(tend))
+        return out, tend, (pbot if pressure_based else None)
+
+    def barotropic_closure(self, eta_t_bar, patm, rho_g, ucell, *, anompb_bar=None,
+                           pressure_based=False, ht=None, eta_u=None, rho0=1035.0, grav=9.80,
+                           tmasken=None):
+        """eta_u (pbot_u) closure, ps, grad_ps and grad_anompb.  ucell: dict
+        of dte, dtw, dus, dun, daur.  Returns (eta_u, ps, grad_ps,
+        grad_anompb or None)."""
+        f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        eta_t_bar, patm, rho_g = f64(eta_t_bar), f64(patm), f64(rho_g)
+        uc = [f64(ucell[k]) for k in ("dte", "dtw", "dus", "dun", "daur")]
+        ab = f64(anompb_bar) if anompb_bar is not None else None
+        htp = f64(ht) if ht is not None else None
+        tm = f64(tmasken) if tmasken is not None else None
+        eu = f64(np.zeros(self.shape2) if eta_u is None else np.array(eta_u, np.float64))
+        ps = np.zeros(self.shape2)
+        gps = np.zeros((2,) + self.shape2)
+        gab = np.zeros((2,) + self.shape2) if ab is not None else None
+        self.L.orc_barotropic_closure(
+            C.byref(self.dom), C.byref(self.grid), C.c_int(int(pressure_based)),
+            C.c_double(rho0), C.c_double(grav), _p(eta_t_bar), _p(ab) if ab is not None else None,
+            _p(patm), _p(rho_g), _p(htp) if htp is not None else None,
+            _p(tm) if tm is not None else None, *[_p(a) for a in uc],
+            _p(eu), _p(ps), _p(gps), _p(gab) if gab is not None else None)
+        return eu, ps, gps, gab

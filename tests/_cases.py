@@ -36,3 +36,35 @@ def uniform_case(ni, nj, nk, *, cyclic_x=True, dx=1.0, dy=1.0, dz=1.0, rho=1.0, 
                               dom=dict(isd=0, ied=ni + 1, jsd=0, jed=nj + 1, isc=1, iec=ni,
                                        jsc=1, jec=nj, nk=nk, cyclic_x=int(cyclic_x), cyclic_y=0))
     return c
+
+
+def bt_tmasken(c):
+    """Grd%tmasken(:,:,1,1:2) (ocean_grids.F90:1390-1400): face masks up
+    to iec/jec, halo-updated (cyclic x), zero beyond."""
+    t = c.grid["tmask"][0]
+    NJ, NI = t.shape
+    m = np.zeros((2, NJ, NI))
+    m[0, :-1, :-1] = np.minimum(t[:-1, :-1], t[:-1, 1:])
+    m[1, :-1, :-1] = np.minimum(t[:-1, :-1], t[1:, :-1])
+    if c.dom["cyclic_x"]:
+        m[:, 1:-1, 0] = m[:, 1:-1, -2]
+        m[:, 1:-1, -1] = m[:, 1:-1, 1]
+    return m
+
+
+def bt_variant_fields(c, rng, cgrid, scale=1.0):
+    """Random barotropic-variant inputs on case c (masks respected)."""
+    sh = c.grid["tmask"][0].shape
+    tm = c.grid["tmask"][0]
+    um = c.grid["umask"][0]
+    mask2 = bt_tmasken(c) if cgrid else np.stack([um, um])
+    eta = 0.05 * rng.standard_normal(sh) * tm
+    ud = rng.standard_normal((2,) + sh) * mask2 * scale
+    fb = 1e-3 * rng.standard_normal((2,) + sh) * mask2
+    H = 1000.0 + 3000.0 * rng.random(sh)
+    thick = (H[None] * mask2) if cgrid else H * um
+    return dict(x_tau=eta, xbar_tau=eta.copy(), udrho=ud, forcing_bt=fb, thick=thick,
+                tmasken=bt_tmasken(c) if cgrid else None, patm=np.zeros(sh),
+                rho_g=np.full(sh, 1035.0 * 9.80), dpatm_dt=np.zeros(sh),
+                pme=1e-4 * rng.standard_normal(sh) * tm, river=np.zeros(sh), source=np.zeros(sh),
+                smooth=np.zeros(sh))

@@ -40,3 +40,25 @@ def test_advection_velocity_bitwise(oracle_lib, kind, sources):
                                (ue, vn, wb, div), ref):
         assert np.array_equal(got.to_host(), want), name
     g.close()
+
+
+@pytest.mark.parametrize("cyclic", [True, False])
+def test_barotropic_forcing_bitwise(oracle_lib, cyclic):
+    from mom5_amd import DeviceArray, OceanGPU
+    from mom5_amd.synth import make_case
+    kw = {} if cyclic else dict(cyclic_x=False, lat_s=20.0, lat_n=50.0, lon_span=30.0)
+    c = make_case(66, 40, 8, seed=41, land_frac=0.3, barotropic=False, **kw)
+    print(f"[barotropic forcing 66x40x8 cyclic={cyclic}]", flush=True)
+    nk, NJ, NI = c.nk, c.nj + 2, c.ni + 2
+    rng = np.random.default_rng(5)
+    accel = rng.standard_normal((2, nk, NJ, NI))
+    smf = rng.standard_normal((2, NJ, NI))
+    bmf = rng.standard_normal((2, NJ, NI))
+    ref = oracle_lib.Oracle(c).barotropic_forcing(accel, smf, bmf)
+    g = OceanGPU(c.dom, c.grid)
+    out = DeviceArray((2, NJ, NI))
+    g.barotropic_forcing(DeviceArray.from_host(accel), DeviceArray.from_host(smf),
+                         DeviceArray.from_host(bmf), out)
+    g.sync()
+    assert np.array_equal(out.to_host(), ref)
+    g.close()

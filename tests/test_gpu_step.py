@@ -122,7 +122,7 @@ def test_fp32_results_independent_of_launch_shape(setup):
     dtypes = [np.float64, np.float64] + [np.float32] * (len(Ts) - 2)
     base = None
     for group, rows, waves in [(1, 2, 8), (2, 2, 8), (4, 2, 8), (1, 4, 8), (2, 4, 8), (4, 4, 8),
-                               (1, 2, 16), (2, 2, 16)]:
+                               (1, 2, 16), (2, 2, 16), (4, 2, 16)]:
         gpu.set_tuning("step_group_f32", group)
         gpu.set_tuning("step_rows_per_wave_f32", rows)
         gpu.set_tuning("step_waves_f32", waves)
