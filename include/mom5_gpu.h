@@ -121,6 +121,9 @@ int mom_gpu_copy(mom_gpu_ctx_t *ctx, double *dst, const double *src, size_t n);
  *   "step_rows_per_wave_f32"           2 or 4: tile height of the FP32 passes
  *   "step_waves_f32"                   8 or 16 wavefronts per FP32 tile (16 is
  *                                      used with 2 rows a wavefront)
+ *   "step_waves_f64"                   8 or 12 wavefronts per FP64 tile (12 is
+ *                                      used with 2 rows a wavefront and groups
+ *                                      of up to 2 tracers)
  * Unknown knobs and out-of-range values return MOM_GPU_ERR_ARG. */
 int mom_gpu_set_tuning(mom_gpu_ctx_t *ctx, const char *knob, int value);
 int mom_gpu_get_tuning(const mom_gpu_ctx_t *ctx, const char *knob, int *value);

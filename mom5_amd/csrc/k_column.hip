@@ -194,7 +194,8 @@ __global__ void k_vdiff(VdJob<TT> J) {
 // of T written over the tracer: rounding those (weak-diffusion rows of M are
 // close to the identity) is biased and drifts the tracer content, while the
 // rounding of dT' is not.  Arithmetic stays FP64.
-constexpr int NTF = 4;
+constexpr int NTF = 12;     // FP32 tracers per implicit-diffusion pass
+constexpr int NTF_ENS = 4;  // the same in the ensemble kernel (job tables cost registers)
 struct TriGroupF {
   float *z[NTF];
   const double *stf[NTF];
@@ -215,6 +216,7 @@ struct VdJobF {
   int nbx, nblocks;
 };
 
+template <int MAXN>
 __device__ __forceinline__ void vdiff_inc_col(const VdJobF &J, int gi, int gj) {
   const KDom d = J.d;
   const TriGroupF &g = J.g;
@@ -233,12 +235,13 @@ __device__ __forceinline__ void vdiff_inc_col(const VdJobF &J, int gi, int gj) {
   const int kmo = kmt[o];
   const int kb = max(2, kmo);
   double cprev = 0.0, bet = 0.0, dcb_km1 = 0.0;
-  double dprev[NTF], zkm1[NTF], zk[NTF];
+  double dprev[MAXN];
+  float zkm1[MAXN], zk[MAXN];   // FP32 storage values (exact in FP64)
 #pragma unroll
-  for (int n = 0; n < NTF; ++n) {
+  for (int n = 0; n < MAXN; ++n) {
     dprev[n] = 0.0;
-    zkm1[n] = 0.0;
-    zk[n] = (n < g.n) ? (double)g.z[n][o] : 0.0;
+    zkm1[n] = 0.0f;
+    zk[n] = (n < g.n) ? g.z[n][o] : 0.0f;
   }
   for (int k = 1; k <= nk; ++k) {
     const int kp1 = min(k + 1, nk);
@@ -263,15 +266,16 @@ __device__ __forceinline__ void vdiff_inc_col(const VdJobF &J, int gi, int gj) {
       ework[ok] = e;
     }
 #pragma unroll
-    for (int n = 0; n < NTF; ++n) {
+    for (int n = 0; n < MAXN; ++n) {
       if (n < g.n) {
-        const double zkp1 = (k < nk) ? (double)g.z[n][ok + P2] : 0.0;
+        const float zkp1 = (k < nk) ? g.z[n][ok + P2] : 0.0f;
         double bfl = 0.0;
         if (k == 1)
           bfl = g.stf[n][o] * tdt_aidif * mkk / dhk;
         else if (k == kb)
           bfl = -(g.btf[n][o] * tdt_aidif * mkk / dhk);
-        const double rhs = bfl - (a * (zkm1[n] - zk[n]) + c * (zkp1 - zk[n]));
+        const double zc = zk[n];
+        const double rhs = bfl - (a * ((double)zkm1[n] - zc) + c * ((double)zkp1 - zc));
         const double dk = (k == 1) ? rhs * bet : (rhs - a * dprev[n]) * bet;
         scr[(size_t)n * P3 + ok] = (float)dk;
         dprev[n] = dk;
@@ -286,7 +290,7 @@ __device__ __forceinline__ void vdiff_inc_col(const VdJobF &J, int gi, int gj) {
     const int ok = o + P2 * (nk - 1);
     const double mk = (nk <= kmo) ? 1.0 : 0.0;
 #pragma unroll
-    for (int n = 0; n < NTF; ++n)
+    for (int n = 0; n < MAXN; ++n)
       if (n < g.n) g.z[n][ok] = (float)(((double)g.z[n][ok] + dprev[n]) * mk);
   }
   for (int k = nk - 1; k >= 1; --k) {
@@ -294,7 +298,7 @@ __device__ __forceinline__ void vdiff_inc_col(const VdJobF &J, int gi, int gj) {
     const double e1 = ework[ok + P2];
     const double mk = (k <= kmo) ? 1.0 : 0.0;
 #pragma unroll
-    for (int n = 0; n < NTF; ++n) {
+    for (int n = 0; n < MAXN; ++n) {
       if (n < g.n) {
         const double dT = (double)scr[(size_t)n * P3 + ok] - e1 * dprev[n];
         g.z[n][ok] = (float)(((double)g.z[n][ok] + dT) * mk);
@@ -307,7 +311,7 @@ __device__ __forceinline__ void vdiff_inc_col(const VdJobF &J, int gi, int gj) {
 __global__ void k_vdiff_inc(VdJobF J) {
   const int gi = blockIdx.x * blockDim.x + threadIdx.x;
   if (gi >= J.d.ni) return;
-  vdiff_inc_col(J, gi, blockIdx.y);
+  vdiff_inc_col<NTF>(J, gi, blockIdx.y);
 }
 
 // Ensemble: one group of every domain per launch (groups of one domain share
@@ -327,6 +331,7 @@ __device__ inline const JOB *vd_pick(const JOB *jobs, const VdLaunch &L, int &gi
     const int mid = (lo + hi + 1) >> 1;
     if (L.first[mid] <= b) lo = mid; else hi = mid - 1;
   }
+  lo = __builtin_amdgcn_readfirstlane(lo);   // uniform: the job table is read by scalar loads
   const JOB &J = jobs[lo];
   const int local = b - L.first[lo];
   gi = (local % J.nbx) * 64 + threadIdx.x;
@@ -344,7 +349,7 @@ __global__ __launch_bounds__(64) void k_vdiff_ens(const VdJob<double> *__restric
 __global__ __launch_bounds__(64) void k_vdiff_inc_ens(const VdJobF *__restrict__ jobs, VdLaunch L) {
   int gi, gj;
   const VdJobF *J = vd_pick(jobs, L, gi, gj);
-  if (J) vdiff_inc_col(*J, gi, gj);
+  if (J) vdiff_inc_col<NTF_ENS>(*J, gi, gj);
 }
 
 __global__ void k_set_ints(int *__restrict__ p, int v, int n) {
@@ -643,6 +648,37 @@ int launch_vdiff_implicit(mom_gpu_ctx *ctx, int ntr, double *const *T, const dou
                               index_salt, rho0, dh, dhw, dt, aidif);
 }
 
+// FP32 solver scratch of ctx: one plane per tracer of the largest group.
+// Grows (outside graph capture only) when a call brings a larger group.
+static int ensure_vd_scr(mom_gpu_ctx *ctx, mom_gpu_ctx *err, hipStream_t s, int planes,
+                         const char *what) {
+  if (ctx->vd_scr && ctx->vd_scr_planes >= planes) return MOM_GPU_OK;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  MOM_HIP(err, hipStreamIsCapturing(s, &cs));
+  if (cs != hipStreamCaptureStatusNone)
+    return mom_fail(err, MOM_GPU_ERR_ARG,
+                    "%s: a step with %d FP32 tracers per solver group allocates its scratch; "
+                    "run one such step before capturing a graph", what, planes);
+  if (ctx->vd_scr) {
+    MOM_HIP(err, hipDeviceSynchronize());
+    MOM_HIP(err, hipFree(ctx->vd_scr));
+    ctx->vd_scr = nullptr;
+    ctx->vd_scr_planes = 0;
+  }
+  const size_t p3 = (size_t)ctx->kd.P2 * ctx->kd.nk;
+  MOM_HIP(err, hipMalloc((void **)&ctx->vd_scr, sizeof(float) * (size_t)planes * p3));
+  ctx->vd_scr_planes = planes;
+  return MOM_GPU_OK;
+}
+
+// FP32 tracers of the largest solver group of a call (by mixing class).
+static int fp32_group_planes(int ntr, const int *dtype, int index_salt) {
+  int cnt[2] = {0, 0};
+  for (int n = 0; n < ntr; ++n)
+    if (dtype[n] == 4) ++cnt[n == index_salt ? 1 : 0];
+  return std::min(NTF, std::max(cnt[0], cnt[1]));
+}
+
 int launch_vdiff_mixed(mom_gpu_ctx *ctx, int ntr, void *const *T, const int *dtype,
                        const double *const *stf, const double *const *btf,
                        const double *diff_cbt, int index_salt, double rho0, const double *dh,
@@ -655,15 +691,9 @@ int launch_vdiff_mixed(mom_gpu_ctx *ctx, int ntr, void *const *T, const int *dty
   bool any32 = false;
   for (int n = 0; n < ntr; ++n) any32 = any32 || dtype[n] == 4;
   if (!any32) return MOM_GPU_OK;
-  if (!ctx->vd_scr) {
-    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    MOM_HIP(ctx, hipStreamIsCapturing(ctx->stream, &cs));
-    if (cs != hipStreamCaptureStatusNone)
-      return mom_fail(ctx, MOM_GPU_ERR_ARG,
-                      "step_tracers: the first step with FP32 tracers allocates its scratch; "
-                      "run one step before capturing a graph");
-    MOM_HIP(ctx, hipMalloc((void **)&ctx->vd_scr, sizeof(float) * NTF * p3));
-  }
+  rc = ensure_vd_scr(ctx, ctx, ctx->stream, fp32_group_planes(ntr, dtype, index_salt),
+                     "step_tracers");
+  if (rc) return rc;
   for (int nmix = 1; nmix <= 2; ++nmix) {
     TriGroupF g{};
     auto flush = [&]() {
@@ -830,14 +860,11 @@ int launch_vdiff_ensemble(mom_gpu_ensemble *e, const mom_gpu_tracer_step_t *step
     const int nbx = (d.ni + 63) / 64;
     bool any32 = false;
     for (int n = 0; n < s.ntr; ++n) any32 = any32 || s.dtype[n] == 4;
-    if (any32 && !ctx->vd_scr) {
-      hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-      MOM_HIP(e->lead, hipStreamIsCapturing(e->lead->stream, &cs));
-      if (cs != hipStreamCaptureStatusNone)
-        return mom_fail(e->lead, MOM_GPU_ERR_ARG,
-                        "ensemble: the first step with FP32 tracers allocates solver scratch; "
-                        "run one step before capturing");
-      MOM_HIP(e->lead, hipMalloc((void **)&ctx->vd_scr, sizeof(float) * NTF * p3));
+    if (any32) {
+      const int rc = ensure_vd_scr(ctx, e->lead, e->lead->stream,
+                                   std::min(NTF_ENS, fp32_group_planes(s.ntr, s.dtype, s.index_salt)),
+                                   "ensemble");
+      if (rc) return rc;
     }
     for (int nmix = 1; nmix <= 2; ++nmix) {
       const double *dcb = s.diff_cbt + p3 * (nmix - 1);
@@ -870,7 +897,7 @@ int launch_vdiff_ensemble(mom_gpu_ensemble *e, const mom_gpu_tracer_step_t *step
         } else {
           TriGroupF &g = j32.g;
           g.z[g.n] = (float *)s.T_taup1[n]; g.stf[g.n] = s.stf[n]; g.btf[g.n] = s.btf[n];
-          if (++g.n == NTF) flush32();
+          if (++g.n == NTF_ENS) flush32();
         }
       }
       flush64();

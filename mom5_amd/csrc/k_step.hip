@@ -195,6 +195,46 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
   }
   __syncthreads();
 
+  // Mask depths of the x and y faces: for masks in {0, 1}, mk(k, a)*mk(k, b)
+  // == mk(k, min(a, b)) bit for bit.  FP32 groups (register room to spare)
+  // keep them across the k loop; FP64 groups re-form them per level.
+  auto face_depths = [&](int q, int &x0, int &x1, int &x2, int &y0, int &y1, int &y2) {
+    const int ry = wave + NW * q;
+    const int kmm1 = __shfl(km[q], lane - 1);
+    const int kmp1 = __shfl(km[q], lane + 1);
+    const int kmp2 = __shfl(km[q], lane + 2);
+    x0 = min(km[q], kmm1);
+    x1 = min(kmp1, km[q]);
+    x2 = min(kmp2, kmp1);
+    const int kjm1 = s_km[max(ry - 1, 0)][lane];
+    const int kjp1 = s_km[min(ry + 1, RY - 1)][lane];
+    const int kjp2 = s_km[min(ry + 2, RY - 1)][lane];
+    y0 = min(km[q], kjm1);
+    y1 = min(kjp1, km[q]);
+    y2 = min(kjp2, kjp1);
+  };
+  int kx0[RPW], kx1[RPW], kx2[RPW], ky0[RPW], ky1[RPW], ky2[RPW];
+  // FP32 groups also keep the face metrics (dyte, dxte of the x face, dxtn,
+  // dytn of the y face, in FP32 as they are used) and datr (FP64) in registers
+  float mdyte[RPW], mdxte[RPW], mdxtn[RPW], mdytn[RPW];
+  double mdatr[RPW];
+  if constexpr (!EXACT) {
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+      face_depths(q, kx0[q], kx1[q], kx2[q], ky0[q], ky1[q], ky2[q]);
+      const int ry = wave + NW * q;
+      const bool fx = rowok[q] && lane >= 1 && lane <= RX - 3 && gi >= -1 && gi <= d.ni - 1;
+      const bool fy = ry >= 1 && ry <= RY - 3 && lane >= 2 && lane < RX - 2 && gi < d.ni &&
+                      gjr[q] >= -1 && gjr[q] <= d.nj - 1;
+      const int ox = d2(d, gi, sjr[q]), oy = d2(d, gi, gjr[q]);
+      mdyte[q] = fx ? (float)dyte[ox] : 0.0f;
+      mdxte[q] = fx ? (float)dxte[ox] : 0.0f;
+      mdxtn[q] = fy ? (float)dxtn[oy] : 0.0f;
+      mdytn[q] = fy ? (float)dytn[oy] : 0.0f;
+      mdatr[q] = ok[q] ? datr[offz[q]] : 0.0;
+    }
+  }
+
   for (int k = 1; k <= nk; ++k) {
     const int k0 = k - 1;
     const int kp1 = min(k + 1, nk), kp2 = min(k + 2, nk), km1 = max(k - 1, 1);
@@ -214,9 +254,9 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
         const double w = W[o + P2 * k];
         const double r = R[o + P2 * k0];
         rhov[q] = r; wv[q] = (RT)w;
-        const TT mkm1 = mk<TT>(km1, km[q]), mk0 = mk<TT>(k, km[q]);
-        const TT mkp1 = mk<TT>(kp1, km[q]), mkp2 = mk<TT>(kp2, km[q]);
-        const TT mA = mkm1 * mk0, mB = mk0 * mkp1, mC = mkp1 * mkp2;
+        // mk(km1)*mk(k) = mk(k), mk(k)*mk(kp1) = mk(kp1), mk(kp1)*mk(kp2) =
+        // mk(kp2): km1 <= k <= kp1 <= kp2 and masks in {0, 1}
+        const TT mA = mk<TT>(k, km[q]), mB = mk<TT>(kp1, km[q]), mC = mk<TT>(kp2, km[q]);
         if constexpr (EXACT) {
           const FaceC<double> cz = face_coef(datv[q] * w, fabs(w * dtime / r));
           const double dtr = dtime / r;
@@ -257,9 +297,6 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
 #pragma unroll
     for (int q = 0; q < RPW; ++q) {
       const int ry = wave + NW * q;
-      const int kmm1 = __shfl(km[q], lane - 1);
-      const int kmp1 = __shfl(km[q], lane + 1);
-      const int kmp2 = __shfl(km[q], lane + 2);
       const bool face = rowok[q] && lane >= 1 && lane <= RX - 3 && gi >= -1 && gi <= d.ni - 1;
       FaceC<double> cx{0.0, 0.0, 0.0, 0.0};
       FaceC<float> cxf{0.0f, 0.0f, 0.0f, 0.0f};
@@ -271,13 +308,13 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
           cx = face_coef(dyte[of] * u, fabs(u * dtime * 2.0 / ((rw + re) * dxte[of])));
         } else {
           const float uf = (float)u;
-          cxf = face_coef_f((float)dyte[of] * uf, fabsf(uf * (float)(dtime * 2.0) *
-                                                        rcpf(((float)rw + (float)re) * (float)dxte[of])));
+          cxf = face_coef_f(mdyte[q] * uf, fabsf(uf * (float)(dtime * 2.0) *
+                                                 rcpf(((float)rw + (float)re) * mdxte[q])));
         }
       }
-      const TT mi = mk<TT>(k, km[q]), mim1 = mk<TT>(k, kmm1);
-      const TT mip1 = mk<TT>(k, kmp1), mip2 = mk<TT>(k, kmp2);
-      const TT mX2 = mip2 * mip1, mX1 = mip1 * mi, mX0 = mi * mim1;
+      if constexpr (EXACT) face_depths(q, kx0[q], kx1[q], kx2[q], ky0[q], ky1[q], ky2[q]);
+      const TT mi = mk<TT>(k, km[q]);
+      const TT mX2 = mk<TT>(k, kx2[q]), mX1 = mk<TT>(k, kx1[q]), mX0 = mk<TT>(k, kx0[q]);
       const bool cell = rowok[q] && lane >= 2 && lane < RX - 2 && gi < d.ni;
       double mw64 = 0.0;
       if constexpr (EXACT) {
@@ -341,14 +378,11 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
           cy = face_coef(dxtn[of] * v, fabs(v * dtime * 2.0 / ((rs + rn) * dytn[of])));
         } else {
           const float vf = (float)v;
-          cyf = face_coef_f((float)dxtn[of] * vf, fabsf(vf * (float)(dtime * 2.0) *
-                                                        rcpf(((float)rs + (float)rn) * (float)dytn[of])));
+          cyf = face_coef_f(mdxtn[q] * vf, fabsf(vf * (float)(dtime * 2.0) *
+                                                 rcpf(((float)rs + (float)rn) * mdytn[q])));
         }
       }
-      const TT mjm1 = mk<TT>(k, s_km[max(ry - 1, 0)][lane]), mj = mk<TT>(k, s_km[ry][lane]);
-      const TT mjp1 = mk<TT>(k, s_km[min(ry + 1, RY - 1)][lane]);
-      const TT mjp2 = mk<TT>(k, s_km[min(ry + 2, RY - 1)][lane]);
-      const TT mY2 = mjp2 * mjp1, mY1 = mjp1 * mj, mY0 = mj * mjm1;
+      const TT mY2 = mk<TT>(k, ky2[q]), mY1 = mk<TT>(k, ky1[q]), mY0 = mk<TT>(k, ky0[q]);
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         TT fy = TT(0);
@@ -392,7 +426,8 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
         // weight rho*dat*factor is dtime to 1e-16 and the fluxes telescope:
         // what is left is the unbiased rounding of the FP32 result.
         const float a32 = (float)(rm1 * rr), c32 = (float)(dtime * rr);
-        const double f64 = EXACT ? 0.0 : dtime * rr * datr[offz[q]] * mi;
+        double f64 = 0.0;
+        if constexpr (!EXACT) f64 = dtime * rr * mdatr[q] * mi;
 #pragma unroll
         for (int g = 0; g < G; ++g) {
           const double thin = tab.th[g] ? tab.th[g][o] : 0.0;
@@ -486,12 +521,21 @@ static void launch_some(mom_gpu_ctx *ctx, int rpw, int waves, const TT *const *T
   StepTab<TT> t{};
   for (int i = 0; i < cnt; ++i) { t.T[i] = Tin[i]; t.out[i] = Tout[i]; t.th[i] = thin[i]; }
   // 16-wave tiles (32 rows, 28 interior: less halo re-read) fit 128 VGPRs
-  // only for FP32 groups at two rows per wave
+  // only for FP32 groups at two rows per wave; FP64 groups of up to two
+  // tracers at two rows per wave (at most 170 VGPRs: three waves per SIMD)
+  // fill a compute unit with 12-wave tiles where 8-wave ones leave a third
+  // of its wave slots empty
   if constexpr (sizeof(TT) == 4) {
     if (waves == 16 && rpw == 2) {
       if (cnt == 4) launch_group<TT, 4, 2, 16>(ctx, t, u, v, w, rho, rm1, rr, dtime);
       else if (cnt == 2) launch_group<TT, 2, 2, 16>(ctx, t, u, v, w, rho, rm1, rr, dtime);
       else launch_group<TT, 1, 2, 16>(ctx, t, u, v, w, rho, rm1, rr, dtime);
+      return;
+    }
+  } else {
+    if (waves == 12 && rpw == 2 && cnt <= 2) {
+      if (cnt == 2) launch_group<TT, 2, 2, 12>(ctx, t, u, v, w, rho, rm1, rr, dtime);
+      else launch_group<TT, 1, 2, 12>(ctx, t, u, v, w, rho, rm1, rr, dtime);
       return;
     }
   }
@@ -591,7 +635,8 @@ static int ensemble_type(mom_gpu_ensemble *e, const mom_gpu_tracer_step_t *steps
         t.out[q] = (TT *)s.T_taup1[n];
         t.th[q] = s.th_tendency ? s.th_tendency[n] : nullptr;
       }
-      const int w = (sizeof(TT) == 4 && waves == 16 && rpw == 2) ? 16 : 8;
+      const int w = (sizeof(TT) == 4 && waves == 16 && rpw == 2) ? 16
+                    : (sizeof(TT) == 8 && waves == 12 && rpw == 2 && c <= 2) ? 12 : 8;
       const int TY = w * rpw - 4;
       const int ntx = (d.ni + TX - 1) / TX;
       const int ntiles = ntx * ((d.nj + TY - 1) / TY);
@@ -610,6 +655,7 @@ static int ensemble_type(mom_gpu_ensemble *e, const mom_gpu_tracer_step_t *steps
   }
   int rc = MOM_GPU_OK;
   const bool w16 = sizeof(TT) == 4 && waves == 16 && rpw == 2;
+  const bool w12 = sizeof(TT) == 8 && waves == 12 && rpw == 2;
   if (rpw == 2) {
     if constexpr (sizeof(TT) == 4) {
       if (w16) {
@@ -617,8 +663,14 @@ static int ensemble_type(mom_gpu_ensemble *e, const mom_gpu_tracer_step_t *steps
         rc = rc ? rc : queue_launch<TT, 2, 2, 16>(e, by[2], slot++);
         rc = rc ? rc : queue_launch<TT, 1, 2, 16>(e, by[1], slot++);
       }
+    } else {
+      if (w12) {   // groups of four keep 8-wave tiles
+        rc = rc ? rc : queue_launch<TT, 4, 2, 8>(e, by[4], slot++);
+        rc = rc ? rc : queue_launch<TT, 2, 2, 12>(e, by[2], slot++);
+        rc = rc ? rc : queue_launch<TT, 1, 2, 12>(e, by[1], slot++);
+      }
     }
-    if (!w16) {
+    if (!w16 && !w12) {
       rc = rc ? rc : queue_launch<TT, 4, 2, 8>(e, by[4], slot++);
       rc = rc ? rc : queue_launch<TT, 2, 2, 8>(e, by[2], slot++);
       rc = rc ? rc : queue_launch<TT, 1, 2, 8>(e, by[1], slot++);
@@ -638,8 +690,9 @@ int launch_tracer_step(mom_gpu_ctx *ctx, int ntr, const void *const *T_in, void 
                        const int *dtype, const double *const *th_in, const double *u,
                        const double *v, const double *w, const double *rho,
                        const double *rho_taum1, const double *rho_dztr, double dtime) {
-  launch_all<double>(ctx, ctx->tune.step_group_f64, ctx->tune.step_rows_per_wave, 8, ntr, T_in,
-                     T_out, dtype, th_in, 8, u, v, w, rho, rho_taum1, rho_dztr, dtime);
+  launch_all<double>(ctx, ctx->tune.step_group_f64, ctx->tune.step_rows_per_wave,
+                     ctx->tune.step_waves_f64, ntr, T_in, T_out, dtype, th_in, 8, u, v, w, rho,
+                     rho_taum1, rho_dztr, dtime);
   launch_all<float>(ctx, ctx->tune.step_group_f32, ctx->tune.step_rows_per_wave_f32,
                     ctx->tune.step_waves_f32, ntr, T_in, T_out, dtype, th_in, 4, u, v, w, rho,
                     rho_taum1, rho_dztr, dtime);
@@ -654,8 +707,8 @@ int launch_step_ensemble(mom_gpu_ensemble *e, const mom_gpu_tracer_step_t *steps
   int rc = launch_set_ints(L, e->counters, 0, mom_gpu_ensemble::NCOUNT);
   if (rc) return rc;
   int slot = 0;
-  rc = ensemble_type<double>(e, steps, 8, L->tune.step_group_f64, L->tune.step_rows_per_wave, 8,
-                             slot);
+  rc = ensemble_type<double>(e, steps, 8, L->tune.step_group_f64, L->tune.step_rows_per_wave,
+                             L->tune.step_waves_f64, slot);
   if (rc) return rc;
   return ensemble_type<float>(e, steps, 4, L->tune.step_group_f32, L->tune.step_rows_per_wave_f32,
                               L->tune.step_waves_f32, slot);

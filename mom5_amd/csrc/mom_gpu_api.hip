@@ -104,6 +104,10 @@ static int *tuning_slot(mom_gpu_ctx *ctx, const char *knob, int *lo, int *hi) {
     *lo = 8; *hi = 16;
     return &ctx->tune.step_waves_f32;
   }
+  if (!std::strcmp(knob, "step_waves_f64")) {
+    *lo = 8; *hi = 12;
+    return &ctx->tune.step_waves_f64;
+  }
   return nullptr;
 }
 
@@ -120,6 +124,8 @@ int mom_gpu_set_tuning(mom_gpu_ctx_t *ctx, const char *knob, int value) {
           "mom_gpu_set_tuning: %s must be 2 or 4", knob);
   REQUIRE(ctx, slot != &ctx->tune.step_waves_f32 || value == 8 || value == 16,
           "mom_gpu_set_tuning: step_waves_f32 must be 8 or 16");
+  REQUIRE(ctx, slot != &ctx->tune.step_waves_f64 || value == 8 || value == 12,
+          "mom_gpu_set_tuning: step_waves_f64 must be 8 or 12");
   REQUIRE(ctx, slot != &ctx->tune.step_group_f32 || value != 3,
           "mom_gpu_set_tuning: step_group_f32 must be 1, 2 or 4");
   *slot = value;
@@ -295,6 +301,7 @@ int mom_gpu_create(const mom_gpu_domain_t *dom, const mom_gpu_grid_t *grid, mom_
   c->tune.step_rows_per_wave = 2;
   c->tune.step_rows_per_wave_f32 = 2;
   c->tune.step_waves_f32 = 16;
+  c->tune.step_waves_f64 = 8;
   const size_t P2 = (size_t)k.P2, nk = (size_t)k.nk;
 
 #define CK(call)                                                                      \

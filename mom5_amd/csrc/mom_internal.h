@@ -91,8 +91,10 @@ struct mom_gpu_ctx {
   double *btv;
   // invtri work (e coefficients), one 3-D plane set
   double *tri_e;
-  // FP32 implicit diffusion: forward increments of up to 4 tracers (lazy)
+  // FP32 implicit diffusion: forward increments, one plane per tracer of
+  // the largest solver group seen (lazy)
   float *vd_scr;
+  int vd_scr_planes;
   // reduction partials
   double *red;
   int red_n;
@@ -105,6 +107,7 @@ struct mom_gpu_ctx {
     int step_rows_per_wave;        // 2 or 4 (FP64 groups)
     int step_rows_per_wave_f32;    // 2 or 4 (FP32 groups)
     int step_waves_f32;            // 8 or 16 wavefronts per tile (FP32 groups)
+    int step_waves_f64;            // 8 or 12 wavefronts per tile (FP64 groups)
   } tune;
   char err[512];
 };

@@ -57,8 +57,9 @@ def autotune(step, reps=3, barotropic=True):
         fp32 = [c_ + (8,) for c_ in itertools.product((1, 2, 4), (2, 4))] + \
             [(1, 2, 16), (2, 2, 16), (4, 2, 16)]
         for tag, size, knobs, cands in (
-                ("fp64", 8, ("step_group_f64", "step_rows_per_wave"),
-                 list(itertools.product((1, 2), (2, 4)))),
+                ("fp64", 8, ("step_group_f64", "step_rows_per_wave", "step_waves_f64"),
+                 [c_ + (8,) for c_ in itertools.product((1, 2), (2, 4))] +
+                 [(1, 2, 12), (2, 2, 12)]),
                 ("fp32", 4, ("step_group_f32", "step_rows_per_wave_f32", "step_waves_f32"),
                  fp32)):
             idx = [n for n in range(step.ntr) if step.tracer(n).element_size() == size]

@@ -60,12 +60,14 @@ def _run_step(case, gpu, dev, host_tracers, dtypes):
     return [a.to_host() for a in tout]
 
 
-@pytest.mark.parametrize("group,rows", [(1, 4), (2, 2), (1, 2)])
-def test_fp64_step_bitwise_equal_to_plain_path_and_oracle(setup, group, rows):
+@pytest.mark.parametrize("group,rows,waves", [(1, 4, 8), (2, 2, 8), (1, 2, 8), (2, 2, 12),
+                                              (1, 2, 12)])
+def test_fp64_step_bitwise_equal_to_plain_path_and_oracle(setup, group, rows, waves):
     from mom5_amd import DeviceArray
     case, orc, gpu, dev = setup
     gpu.set_tuning("step_group_f64", group)
     gpu.set_tuning("step_rows_per_wave", rows)
+    gpu.set_tuning("step_waves_f64", waves)
     Ts = [case.tracers[0][1], case.tracers[1][1]]
     got = _run_step(case, gpu, dev, Ts, [np.float64, np.float64])
     ref = _oracle_step(case, orc, Ts)
@@ -83,6 +85,7 @@ def test_fp64_step_bitwise_equal_to_plain_path_and_oracle(setup, group, rows):
                               [DeviceArray.from_host(s) for s in f["btf"][:2]], dev["diff_cbt"],
                               dev["rho_dzt"], dev["dzwt"], case.dt, index_salt=1)
     gpu.sync()
+    gpu.set_tuning("step_waves_f64", 8)
     for n in range(2):
         a = got[n][:, 1:-1, 1:-1]
         assert np.array_equal(a, plain[n].to_host()[:, 1:-1, 1:-1]), f"tracer {n} vs plain"
