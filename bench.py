@@ -265,7 +265,7 @@ def measured_traffic(workload, key):
 def run_mixed(args, w):
     import dataclasses
     import gc
-    from mom5_amd import DeviceArray
+    from mom5_amd import DeviceArray, OceanGPU
     from mom5_amd.step import BatchedStep, PlainStep
     from mom5_amd.synth import make_case_device
     steps = w["steps"] if args.steps is None else args.steps
@@ -325,9 +325,9 @@ def run_mixed(args, w):
     for want64, elem in ((True, 8), (False, 4)):
         grp = g.get_tuning("step_group_f64" if want64 else "step_group_f32")
         rows = g.get_tuning("step_rows_per_wave" if want64 else "step_rows_per_wave_f32")
-        waves = 8 if want64 else g.get_tuning("step_waves_f32")
-        if not want64 and waves == 16 and rows != 2:
-            waves = 8
+        waves = g.get_tuning("step_waves_f64" if want64 else "step_waves_f32")
+        if rows != 2 or (want64 and (waves != 12 or grp > 2)) or (not want64 and waves != 16):
+            waves = 8                        # the shapes the library launches (tile variants)
         key = f"k_step<{'double' if want64 else 'float'}, {grp}, {rows}, {waves}>"
         idx = [n for n in range(ntr) if is64[n] == want64][:grp]
         if len(idx) < grp:
@@ -357,9 +357,15 @@ def run_mixed(args, w):
     # keep the batched results (FP64 on the device, FP32 on the host), free the rest
     fin64 = {n: bs.tracer(n) for n in range(ntr) if is64[n]}
     fin32 = {n: bs.tracer(n).to_host() for n in range(ntr) if not is64[n]}
+    tuning_used = {k: g.get_tuning(k) for k in TUNING_KEYS}
     del bs, cand
     case.tracers = []
     gc.collect()
+    # a fresh context for the plain path: the mixed one holds its solver
+    # scratch (one FP32 plane per passive tracer), which the plain run's
+    # twelve FP64 tracers need the room of
+    g.close()
+    g = OceanGPU(case.dom, case.grid)
 
     # the plain per-tracer FP64 path from the same initial state
     pcase = dataclasses.replace(case, tracers=[
@@ -414,7 +420,7 @@ def run_mixed(args, w):
            "tracer_dtypes": {"f64": n64, "f32": n32}, "schemes": "mdfl_sweby (all)",
            "barotropic_nts": case.nts,
            "path": ("batched mixed" if n32 else "batched FP64") + " (mom_gpu_step_tracers)",
-           "tuning": {k: g.get_tuning(k) for k in TUNING_KEYS}}
+           "tuning": tuning_used}
     return dict(value=value, ms=ms, steps=steps, warmup=warm,
                 dtype="f64+f32" if n32 else "f64", cfg=cfg, roof=roof, extra=extra, ntr=ntr)
 
@@ -429,7 +435,7 @@ def _tune_summary(rep):
 
 TUNING_KEYS = ("barotropic_subcycles_per_launch", "mdfl_rows_per_wave", "step_group_f64",
                "step_group_f32", "step_rows_per_wave", "step_rows_per_wave_f32",
-               "step_waves_f32")
+               "step_waves_f32", "step_waves_f64")
 
 
 def run_plain(args, w):

@@ -297,11 +297,11 @@ int mom_gpu_create(const mom_gpu_domain_t *dom, const mom_gpu_grid_t *grid, mom_
   c->tune.bt_subcycles_per_launch = 4;
   c->tune.mdfl_rows_per_wave = 4;
   c->tune.step_group_f64 = 2;
-  c->tune.step_group_f32 = 2;
+  c->tune.step_group_f32 = 4;
   c->tune.step_rows_per_wave = 2;
   c->tune.step_rows_per_wave_f32 = 2;
   c->tune.step_waves_f32 = 16;
-  c->tune.step_waves_f64 = 8;
+  c->tune.step_waves_f64 = 12;
   const size_t P2 = (size_t)k.P2, nk = (size_t)k.nk;
 
 #define CK(call)                                                                      \
