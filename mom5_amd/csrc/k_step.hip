@@ -347,7 +347,14 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
             fx = flux_fast(cxf, Rjm, Rj, Rjp, tz[g][q], tzp1, mX1);
         }
         const TT fxw = __shfl(fx, lane - 1);
-        dxf[g][q] = fxw - fx;
+        if constexpr (EXACT) {
+          dxf[g][q] = fxw - fx;
+        } else {
+          // FP32: the z and x parts of the flux sum, formed now so that only
+          // one value per tracer stays live through the y sweep
+          dxf[g][q] = (fbtv[g][q] - ftp[g][q]) + (fxw - fx);
+          ftp[g][q] = fbtv[g][q];
+        }
         TT tx = TT(0);
         if (cell) {
           if constexpr (EXACT)
@@ -398,7 +405,7 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
             fy = flux_fast(cyf, Rjm, Rj, Rjp, tj, tjp1, mY1);
         }
         s_fy[g][ry][lane] = fy;
-        fyv[g][q] = fy;
+        if constexpr (EXACT) fyv[g][q] = fy;   // FP32 reads it back from LDS
       }
     }
     __syncthreads();
@@ -440,8 +447,7 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
             tab.out[g][o] = (rm1 * Tc + dtime * th) * rr;
           } else {
             const float Tc = t1[g][q];
-            const float dflux = (fbtv[g][q] - ftp[g][q]) + dxf[g][q] +
-                                (s_fy[g][ry - 1][lane] - fyv[g][q]);
+            const float dflux = dxf[g][q] + (s_fy[g][ry - 1][lane] - s_fy[g][ry][lane]);
             tab.out[g][o] = Tc * a32 + (float)(f64 * (double)dflux) + (float)thin * c32;
           }
         }
@@ -454,7 +460,7 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
       wkm1[q] = wv[q];
 #pragma unroll
       for (int g = 0; g < G; ++g) {
-        ftp[g][q] = fbtv[g][q];
+        if constexpr (EXACT) ftp[g][q] = fbtv[g][q];
         t0[g][q] = t1[g][q]; t1[g][q] = t2[g][q]; t2[g][q] = t3[g][q];
         if (ok[q] && k < nk) t3[g][q] = tab.T[g][offz[q] + P2 * (min(k + 3, nk) - 1)];
       }
