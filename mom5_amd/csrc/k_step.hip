@@ -34,6 +34,33 @@ constexpr double EPSLN = 1.0e-40;   // shared/constants/constants.F90:123
 template <typename TT>
 __device__ inline TT mk(int k, int kmt) { return (k <= kmt) ? TT(1) : TT(0); }
 
+// Neighbour lanes of a 64-wide region row through DPP wavefront shifts
+// (v_mov_b32_dpp wave_shr:1 / wave_shl:1, full rate, no LDS traffic).  The
+// first (last) lane receives 0: callers use lane-1 only from lane 1 up and
+// lane+1, lane+2 only up to lane 62, 61.
+__device__ inline int dpp_prev(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, false); }
+__device__ inline int dpp_next(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xf, 0xf, false); }
+__device__ inline float lane_prev(float x) {
+  return __builtin_bit_cast(float, dpp_prev(__builtin_bit_cast(int, x)));
+}
+__device__ inline float lane_next(float x) {
+  return __builtin_bit_cast(float, dpp_next(__builtin_bit_cast(int, x)));
+}
+__device__ inline double lane_prev(double x) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+  const unsigned long long lo = (unsigned)dpp_prev((int)(unsigned)b);
+  const unsigned long long hi = (unsigned)dpp_prev((int)(unsigned)(b >> 32));
+  return __builtin_bit_cast(double, lo | (hi << 32));
+}
+__device__ inline double lane_next(double x) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+  const unsigned long long lo = (unsigned)dpp_next((int)(unsigned)b);
+  const unsigned long long hi = (unsigned)dpp_next((int)(unsigned)(b >> 32));
+  return __builtin_bit_cast(double, lo | (hi << 32));
+}
+__device__ inline int lane_prev(int x) { return dpp_prev(x); }
+__device__ inline int lane_next(int x) { return dpp_next(x); }
+
 // tracer-independent limiter coefficients of one face
 template <typename TT>
 struct FaceC {
@@ -81,17 +108,30 @@ __device__ inline FaceC<float> face_coef_f(float massflux, float cfl) {
   return c;
 }
 
-// division-free limited flux (FP32)
+// division-free limited flux (FP32).  The hardware min/max (v_min3_f32,
+// v_max3_f32 and v_min/max_f32 on finite data) replace compare-and-select:
+// FP32 tracers need no particular signed zero, only the same code everywhere.
 __device__ inline float psi_r(const FaceC<float> &c, float a, float r) {
   const float lin = fmaf(c.d0, a, c.d1 * r);
   const float kr = c.kk * r;
-  return (a >= 0.0f) ? fmxf(0.0f, fmnf(fmnf(a, lin), kr)) : fmnf(0.0f, fmxf(fmxf(a, lin), kr));
+  const float up = __builtin_fmaxf(0.0f, __builtin_fminf(__builtin_fminf(a, lin), kr));
+  const float dn = __builtin_fminf(0.0f, __builtin_fmaxf(__builtin_fmaxf(a, lin), kr));
+  return (a >= 0.0f) ? up : dn;
 }
-__device__ inline float flux_fast(const FaceC<float> &c, float Rjm, float Rj, float Rjp, float Tpos,
-                                  float Tneg, float mab) {
+// The upwind weights of a face with the 1/2 and the face mask folded in:
+// mp = (M + |M|)/2 * mab, mn = (M - |M|)/2 * mab (power-of-two and 0/1
+// scalings are exact, so the flux is the same value as the unfolded form).
+struct FaceW {
+  float mp, mn;
+};
+__device__ inline FaceW face_w(const FaceC<float> &c, float mab) {
+  return FaceW{0.5f * (c.M + fabsf(c.M)) * mab, 0.5f * (c.M - fabsf(c.M)) * mab};
+}
+__device__ inline float flux_fast(const FaceC<float> &c, const FaceW &w, float Rjm, float Rj,
+                                  float Rjp, float Tpos, float Tneg) {
   const float pP = psi_r(c, Rj, Rjm);
   const float pM = psi_r(c, Rj, Rjp);
-  return 0.5f * fmaf(c.M + fabsf(c.M), Tpos + pP, (c.M - fabsf(c.M)) * (Tneg - pM)) * mab;
+  return fmaf(w.mp, Tpos + pP, w.mn * (Tneg - pM));
 }
 
 template <typename TT>
@@ -200,9 +240,9 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
   // keep them across the k loop; FP64 groups re-form them per level.
   auto face_depths = [&](int q, int &x0, int &x1, int &x2, int &y0, int &y1, int &y2) {
     const int ry = wave + NW * q;
-    const int kmm1 = __shfl(km[q], lane - 1);
-    const int kmp1 = __shfl(km[q], lane + 1);
-    const int kmp2 = __shfl(km[q], lane + 2);
+    const int kmm1 = lane_prev(km[q]);
+    const int kmp1 = lane_next(km[q]);
+    const int kmp2 = lane_next(kmp1);
     x0 = min(km[q], kmm1);
     x1 = min(kmp1, km[q]);
     x2 = min(kmp2, kmp1);
@@ -275,14 +315,17 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
           rinv[q] = ri;
           const float wf = (float)w;
           const FaceC<float> c = face_coef_f((float)datv[q] * wf, fabsf(wf * (float)dtime * ri));
+          const FaceW wz = face_w(c, mB);
           const float dtrf = (float)dtime * ri, datrf = (float)datrv[q];
           const float wdiff = (float)wkm1[q] - wf;
 #pragma unroll
+          // where the face is wet (mB = 1) so is the cell above (mA = 1);
+          // where it is dry its weights are 0: only mC needs applying
           for (int g = 0; g < G; ++g) {
-            const float Rjp = (t0[g][q] - t1[g][q]) * mA;
-            const float Rj = (t1[g][q] - t2[g][q]) * mB;
+            const float Rjp = t0[g][q] - t1[g][q];
+            const float Rj = t1[g][q] - t2[g][q];
             const float Rjm = (t2[g][q] - t3[g][q]) * mC;
-            const float fbt = flux_fast(c, Rjm, Rj, Rjp, t2[g][q], t1[g][q], mB);
+            const float fbt = flux_fast(c, wz, Rjm, Rj, Rjp, t2[g][q], t1[g][q]);
             tz[g][q] = fmaf(dtrf, fmaf(datrf, fbt - ftp[g][q], t1[g][q] * wdiff), t1[g][q]);
             fbtv[g][q] = fbt;
           }
@@ -318,7 +361,7 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
       const bool cell = rowok[q] && lane >= 2 && lane < RX - 2 && gi < d.ni;
       double mw64 = 0.0;
       if constexpr (EXACT) {
-        mw64 = __shfl(cx.M, lane - 1);
+        mw64 = lane_prev(cx.M);
         // dtime*tmask*datr/rho (x and y sweeps)
         const double xf64 = dtime * (double)mi * datrv[q] / rhov[q];
         xf64v[q] = xf64;
@@ -328,25 +371,27 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
       } else {
         xf64v[q] = 0.0;
         mfe[q] = cxf.M;
-        mfw[q] = __shfl(cxf.M, lane - 1);
+        mfw[q] = lane_prev(cxf.M);
         xfac[q] = (float)dtime * mi * (float)datrv[q] * rinv[q];
       }
+      FaceW wx{0.0f, 0.0f};
+      if constexpr (!EXACT) wx = face_w(cxf, mX1);
 #pragma unroll
       for (int g = 0; g < G; ++g) {
-        const TT tzm1 = __shfl(tz[g][q], lane - 1);
-        const TT tzp1 = __shfl(tz[g][q], lane + 1);
-        const TT tzp2 = __shfl(tz[g][q], lane + 2);
+        const TT tzm1 = lane_prev(tz[g][q]);
+        const TT tzp1 = lane_next(tz[g][q]);
+        const TT tzp2 = lane_next(tzp1);
         TT fx = TT(0);
         if (face) {
           const TT Rjp = (tzp2 - tzp1) * mX2;
-          const TT Rj = (tzp1 - tz[g][q]) * mX1;
+          const TT Rj = EXACT ? (tzp1 - tz[g][q]) * mX1 : tzp1 - tz[g][q];  // FP32: face weights carry mX1
           const TT Rjm = (tz[g][q] - tzm1) * mX0;
           if constexpr (EXACT)
             fx = flux_exact(cx, Rjm, Rj, Rjp, tz[g][q], tzp1, mX1);
           else
-            fx = flux_fast(cxf, Rjm, Rj, Rjp, tz[g][q], tzp1, mX1);
+            fx = flux_fast(cxf, wx, Rjm, Rj, Rjp, tz[g][q], tzp1);
         }
-        const TT fxw = __shfl(fx, lane - 1);
+        const TT fxw = lane_prev(fx);
         if constexpr (EXACT) {
           dxf[g][q] = fxw - fx;
         } else {
@@ -390,6 +435,8 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
         }
       }
       const TT mY2 = mk<TT>(k, ky2[q]), mY1 = mk<TT>(k, ky1[q]), mY0 = mk<TT>(k, ky0[q]);
+      FaceW wy{0.0f, 0.0f};
+      if constexpr (!EXACT) wy = face_w(cyf, mY1);
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         TT fy = TT(0);
@@ -397,12 +444,12 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
           const TT tjm1 = s_tx[g][ry - 1][lane], tj = s_tx[g][ry][lane];
           const TT tjp1 = s_tx[g][ry + 1][lane], tjp2 = s_tx[g][ry + 2][lane];
           const TT Rjp = (tjp2 - tjp1) * mY2;
-          const TT Rj = (tjp1 - tj) * mY1;
+          const TT Rj = EXACT ? (tjp1 - tj) * mY1 : tjp1 - tj;   // FP32: face weights carry mY1
           const TT Rjm = (tj - tjm1) * mY0;
           if constexpr (EXACT)
             fy = flux_exact(cy, Rjm, Rj, Rjp, tj, tjp1, mY1);
           else
-            fy = flux_fast(cyf, Rjm, Rj, Rjp, tj, tjp1, mY1);
+            fy = flux_fast(cyf, wy, Rjm, Rj, Rjp, tj, tjp1);
         }
         s_fy[g][ry][lane] = fy;
         if constexpr (EXACT) fyv[g][q] = fy;   // FP32 reads it back from LDS
