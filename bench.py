@@ -435,7 +435,7 @@ def _tune_summary(rep):
 
 TUNING_KEYS = ("barotropic_subcycles_per_launch", "mdfl_rows_per_wave", "step_group_f64",
                "step_group_f32", "step_rows_per_wave", "step_rows_per_wave_f32",
-               "step_waves_f32", "step_waves_f64")
+               "step_waves_f32", "step_waves_f64", "vdiff_prefetch")
 
 
 def run_plain(args, w):

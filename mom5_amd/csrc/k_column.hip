@@ -216,7 +216,10 @@ struct VdJobF {
   int nbx, nblocks;
 };
 
-template <int MAXN>
+// PF = true: the loads of level k+1 (k-1 on the way back) are issued before
+// level k is computed, so that each column has two levels of loads in
+// flight (same arithmetic, same results).
+template <int MAXN, bool PF>
 __device__ __forceinline__ void vdiff_inc_col(const VdJobF &J, int gi, int gj) {
   const KDom d = J.d;
   const TriGroupF &g = J.g;
@@ -236,21 +239,42 @@ __device__ __forceinline__ void vdiff_inc_col(const VdJobF &J, int gi, int gj) {
   const int kb = max(2, kmo);
   double cprev = 0.0, bet = 0.0, dcb_km1 = 0.0;
   double dprev[MAXN];
-  float zkm1[MAXN], zk[MAXN];   // FP32 storage values (exact in FP64)
+  float zkm1[MAXN], zk[MAXN], zn[MAXN];   // FP32 storage values (exact in FP64)
 #pragma unroll
   for (int n = 0; n < MAXN; ++n) {
     dprev[n] = 0.0;
     zkm1[n] = 0.0f;
     zk[n] = (n < g.n) ? g.z[n][o] : 0.0f;
+    zn[n] = (PF && n < g.n && nk > 1) ? g.z[n][o + P2] : 0.0f;
   }
+  // level-k inputs (PF: loaded one level ahead)
+  double dcb_n = dcbase[o], dh_n = dh[o], dhwu_n = dhw[o], dhwl_n = dhw[o + P2];
   for (int k = 1; k <= nk; ++k) {
     const int kp1 = min(k + 1, nk);
     const int ok = o + P2 * (k - 1);
-    const double dcb_k = rho0 * dcbase[ok];
+    double dcbk, dhk, dhwu, dhwl;
+    float zkp1v[MAXN];
+    if constexpr (PF) {
+      dcbk = dcb_n; dhk = dh_n; dhwu = dhwu_n; dhwl = dhwl_n;
+#pragma unroll
+      for (int n = 0; n < MAXN; ++n) zkp1v[n] = zn[n];
+      if (k < nk) {
+        dcb_n = dcbase[ok + P2]; dh_n = dh[ok + P2];
+        dhwu_n = dhwl; dhwl_n = dhw[o + P2 * (k + 1)];
+#pragma unroll
+        for (int n = 0; n < MAXN; ++n)
+          zn[n] = (n < g.n && k + 1 < nk) ? g.z[n][ok + 2 * P2] : 0.0f;
+      }
+    } else {
+      dcbk = dcbase[ok]; dhk = dh[ok];
+      dhwu = dhw[o + P2 * (k - 1)]; dhwl = dhw[o + P2 * k];
+#pragma unroll
+      for (int n = 0; n < MAXN; ++n) zkp1v[n] = (n < g.n && k < nk) ? g.z[n][ok + P2] : 0.0f;
+    }
+    const double dcb_k = rho0 * dcbk;
     if (k == 1) dcb_km1 = dcb_k;
-    const double dhk = dh[ok];
-    const double factu = tdt_aidif / (dhw[o + P2 * (k - 1)] * dhk);
-    const double factl = tdt_aidif / (dhw[o + P2 * k] * dhk);
+    const double factu = tdt_aidif / (dhwu * dhk);
+    const double factl = tdt_aidif / (dhwl * dhk);
     const double mkk = (k <= kmo) ? 1.0 : 0.0;
     const double mkp1 = (kp1 <= kmo) ? 1.0 : 0.0;
     double a = -dcb_km1 * factu * mkk;
@@ -268,7 +292,7 @@ __device__ __forceinline__ void vdiff_inc_col(const VdJobF &J, int gi, int gj) {
 #pragma unroll
     for (int n = 0; n < MAXN; ++n) {
       if (n < g.n) {
-        const float zkp1 = (k < nk) ? g.z[n][ok + P2] : 0.0f;
+        const float zkp1 = (k < nk) ? zkp1v[n] : 0.0f;
         double bfl = 0.0;
         if (k == 1)
           bfl = g.stf[n][o] * tdt_aidif * mkk / dhk;
@@ -293,25 +317,62 @@ __device__ __forceinline__ void vdiff_inc_col(const VdJobF &J, int gi, int gj) {
     for (int n = 0; n < MAXN; ++n)
       if (n < g.n) g.z[n][ok] = (float)(((double)g.z[n][ok] + dprev[n]) * mk);
   }
+  // back substitution, k = nk-1 .. 1 (PF: level k-1 loaded while k is done)
+  float sn[MAXN], tnx[MAXN];
+  double en = 0.0;
+  if constexpr (PF) {
+    if (nk >= 2) {
+      const int ok = o + P2 * (nk - 2);
+      en = ework[ok + P2];
+#pragma unroll
+      for (int n = 0; n < MAXN; ++n) {
+        sn[n] = (n < g.n) ? scr[(size_t)n * P3 + ok] : 0.0f;
+        tnx[n] = (n < g.n) ? g.z[n][ok] : 0.0f;
+      }
+    }
+  }
   for (int k = nk - 1; k >= 1; --k) {
     const int ok = o + P2 * (k - 1);
-    const double e1 = ework[ok + P2];
+    double e1;
+    float sv[MAXN], tv[MAXN];
+    if constexpr (PF) {
+      e1 = en;
+#pragma unroll
+      for (int n = 0; n < MAXN; ++n) { sv[n] = sn[n]; tv[n] = tnx[n]; }
+      if (k > 1) {
+        const int om = ok - P2;
+        en = ework[om + P2];
+#pragma unroll
+        for (int n = 0; n < MAXN; ++n) {
+          sn[n] = (n < g.n) ? scr[(size_t)n * P3 + om] : 0.0f;
+          tnx[n] = (n < g.n) ? g.z[n][om] : 0.0f;
+        }
+      }
+    } else {
+      e1 = ework[ok + P2];
+#pragma unroll
+      for (int n = 0; n < MAXN; ++n) {
+        sv[n] = (n < g.n) ? scr[(size_t)n * P3 + ok] : 0.0f;
+        tv[n] = (n < g.n) ? g.z[n][ok] : 0.0f;
+      }
+    }
     const double mk = (k <= kmo) ? 1.0 : 0.0;
 #pragma unroll
     for (int n = 0; n < MAXN; ++n) {
       if (n < g.n) {
-        const double dT = (double)scr[(size_t)n * P3 + ok] - e1 * dprev[n];
-        g.z[n][ok] = (float)(((double)g.z[n][ok] + dT) * mk);
+        const double dT = (double)sv[n] - e1 * dprev[n];
+        g.z[n][ok] = (float)(((double)tv[n] + dT) * mk);
         dprev[n] = dT;
       }
     }
   }
 }
 
-__global__ void k_vdiff_inc(VdJobF J) {
+template <bool PF>
+__global__ __launch_bounds__(64) void k_vdiff_inc(VdJobF J) {
   const int gi = blockIdx.x * blockDim.x + threadIdx.x;
   if (gi >= J.d.ni) return;
-  vdiff_inc_col<NTF>(J, gi, blockIdx.y);
+  vdiff_inc_col<NTF, PF>(J, gi, blockIdx.y);
 }
 
 // Ensemble: one group of every domain per launch (groups of one domain share
@@ -349,7 +410,7 @@ __global__ __launch_bounds__(64) void k_vdiff_ens(const VdJob<double> *__restric
 __global__ __launch_bounds__(64) void k_vdiff_inc_ens(const VdJobF *__restrict__ jobs, VdLaunch L) {
   int gi, gj;
   const VdJobF *J = vd_pick(jobs, L, gi, gj);
-  if (J) vdiff_inc_col<NTF_ENS>(*J, gi, gj);
+  if (J) vdiff_inc_col<NTF_ENS, false>(*J, gi, gj);
 }
 
 __global__ void k_set_ints(int *__restrict__ p, int v, int n) {
@@ -700,7 +761,12 @@ int launch_vdiff_mixed(mom_gpu_ctx *ctx, int ntr, void *const *T, const int *dty
       if (g.n == 0) return;
       VdJobF J{d, g, diff_cbt + p3 * (nmix - 1), rho0, ctx->kmt, dh, dhw, dt, aidif,
                ctx->tri_e, ctx->vd_scr, 0, 0};
-      hipLaunchKernelGGL(k_vdiff_inc, dim3((d.ni + 63) / 64, d.nj), dim3(64), 0, ctx->stream, J);
+      if (ctx->tune.vdiff_prefetch)
+        hipLaunchKernelGGL(k_vdiff_inc<true>, dim3((d.ni + 63) / 64, d.nj), dim3(64), 0,
+                           ctx->stream, J);
+      else
+        hipLaunchKernelGGL(k_vdiff_inc<false>, dim3((d.ni + 63) / 64, d.nj), dim3(64), 0,
+                           ctx->stream, J);
       g.n = 0;
     };
     for (int n = 0; n < ntr; ++n) {

@@ -38,8 +38,8 @@ __device__ inline TT mk(int k, int kmt) { return (k <= kmt) ? TT(1) : TT(0); }
 // (v_mov_b32_dpp wave_shr:1 / wave_shl:1, full rate, no LDS traffic).  The
 // first (last) lane receives 0: callers use lane-1 only from lane 1 up and
 // lane+1, lane+2 only up to lane 62, 61.
-__device__ inline int dpp_prev(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, false); }
-__device__ inline int dpp_next(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xf, 0xf, false); }
+__device__ inline int dpp_prev(int x) { return __builtin_amdgcn_mov_dpp(x, 0x138, 0xf, 0xf, true); }
+__device__ inline int dpp_next(int x) { return __builtin_amdgcn_mov_dpp(x, 0x130, 0xf, 0xf, true); }
 __device__ inline float lane_prev(float x) {
   return __builtin_bit_cast(float, dpp_prev(__builtin_bit_cast(int, x)));
 }
@@ -108,15 +108,18 @@ __device__ inline FaceC<float> face_coef_f(float massflux, float cfl) {
   return c;
 }
 
-// division-free limited flux (FP32).  The hardware min/max (v_min3_f32,
-// v_max3_f32 and v_min/max_f32 on finite data) replace compare-and-select:
-// FP32 tracers need no particular signed zero, only the same code everywhere.
-__device__ inline float psi_r(const FaceC<float> &c, float a, float r) {
-  const float lin = fmaf(c.d0, a, c.d1 * r);
+// division-free limited flux (FP32).  The hardware min/max/median
+// (v_min_f32, v_max_f32, v_med3_f32 on finite data) replace compare-and-
+// select: FP32 tracers need no particular signed zero, only the same code
+// everywhere.
+// d0a = d0*a is shared by the two limited fluxes of a face
+__device__ inline float psi_r(const FaceC<float> &c, float d0a, float a, float r) {
+  const float lin = fmaf(c.d1, r, d0a);
   const float kr = c.kk * r;
-  const float up = __builtin_fmaxf(0.0f, __builtin_fminf(__builtin_fminf(a, lin), kr));
-  const float dn = __builtin_fminf(0.0f, __builtin_fmaxf(__builtin_fmaxf(a, lin), kr));
-  return (a >= 0.0f) ? up : dn;
+  // a >= 0: max(0, min(a, lin, kr)) = med3(0, a, min(lin, kr));
+  // a <  0: min(0, max(a, lin, kr)) = med3(0, a, max(lin, kr))
+  const float m = (a >= 0.0f) ? __builtin_fminf(lin, kr) : __builtin_fmaxf(lin, kr);
+  return __builtin_amdgcn_fmed3f(0.0f, a, m);
 }
 // The upwind weights of a face with the 1/2 and the face mask folded in:
 // mp = (M + |M|)/2 * mab, mn = (M - |M|)/2 * mab (power-of-two and 0/1
@@ -129,8 +132,9 @@ __device__ inline FaceW face_w(const FaceC<float> &c, float mab) {
 }
 __device__ inline float flux_fast(const FaceC<float> &c, const FaceW &w, float Rjm, float Rj,
                                   float Rjp, float Tpos, float Tneg) {
-  const float pP = psi_r(c, Rj, Rjm);
-  const float pM = psi_r(c, Rj, Rjp);
+  const float d0a = c.d0 * Rj;
+  const float pP = psi_r(c, d0a, Rj, Rjm);
+  const float pM = psi_r(c, d0a, Rj, Rjp);
   return fmaf(w.mp, Tpos + pP, w.mn * (Tneg - pM));
 }
 
@@ -495,7 +499,9 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
           } else {
             const float Tc = t1[g][q];
             const float dflux = dxf[g][q] + (s_fy[g][ry - 1][lane] - s_fy[g][ry][lane]);
-            tab.out[g][o] = Tc * a32 + (float)(f64 * (double)dflux) + (float)thin * c32;
+            float tn = Tc * a32 + (float)(f64 * (double)dflux);
+            if (tab.th[g]) tn = tn + (float)thin * c32;
+            tab.out[g][o] = tn;
           }
         }
       }

@@ -108,6 +108,10 @@ static int *tuning_slot(mom_gpu_ctx *ctx, const char *knob, int *lo, int *hi) {
     *lo = 8; *hi = 12;
     return &ctx->tune.step_waves_f64;
   }
+  if (!std::strcmp(knob, "vdiff_prefetch")) {
+    *lo = 0; *hi = 1;
+    return &ctx->tune.vdiff_prefetch;
+  }
   return nullptr;
 }
 
@@ -302,6 +306,7 @@ int mom_gpu_create(const mom_gpu_domain_t *dom, const mom_gpu_grid_t *grid, mom_
   c->tune.step_rows_per_wave_f32 = 2;
   c->tune.step_waves_f32 = 16;
   c->tune.step_waves_f64 = 12;
+  c->tune.vdiff_prefetch = 0;
   const size_t P2 = (size_t)k.P2, nk = (size_t)k.nk;
 
 #define CK(call)                                                                      \

@@ -108,6 +108,7 @@ struct mom_gpu_ctx {
     int step_rows_per_wave_f32;    // 2 or 4 (FP32 groups)
     int step_waves_f32;            // 8 or 16 wavefronts per tile (FP32 groups)
     int step_waves_f64;            // 8 or 12 wavefronts per tile (FP64 groups)
+    int vdiff_prefetch;            // FP32 implicit diffusion: loads one level ahead (0, 1)
   } tune;
   char err[512];
 };

@@ -68,6 +68,13 @@ def autotune(step, reps=3, barotropic=True):
             src = [step.T[n][cur] for n in idx]
             dst = [step.T[n][nxt] for n in idx]
             report[tag] = _search(g, knobs, cands, lambda: g.step_tracers(src, dst, **kw), reps)
+            if tag == "fp32" and step.diffuse:
+                # the FP32 implicit diffusion (timed with the tuned advection)
+                kwd = dict(kw, stfs=[step.stf[n] for n in idx], btfs=[step.btf[n] for n in idx],
+                           dzwt=d["dzwt"], diff_cbt=d["diff_cbt"], index_salt=-1)
+                report["fp32_vdiff"] = _search(
+                    g, ("vdiff_prefetch",), [(0,), (1,)],
+                    lambda: g.step_tracers(src, dst, **kwd), reps)
     else:
         th = step.th
         T0 = step.tracer(0)
