@@ -291,6 +291,10 @@ def run_mixed(args, w):
         log(f"autotuned in {time.time() - t0:.1f}s: " + "; ".join(
             f"{k} {v['best']} {v['best_ms']:.3f} ms (default {v['default_ms']:.3f})"
             for k, v in tuned.items()))
+        for k, v in tuned.items():
+            log(f"  {k} candidates: " + ", ".join(
+                "/".join(str(x) for kk, x in t.items() if kk != "ms") + f" {t['ms']:.2f}"
+                for t in v["tried"]))
     c0 = [g.tracer_content(bs.tracer(n), rho) for n in range(ntr)]
     bs.run(warm)
     g.sync()

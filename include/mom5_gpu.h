@@ -127,7 +127,7 @@ int mom_gpu_copy(mom_gpu_ctx_t *ctx, double *dst, const double *src, size_t n);
  *   "vdiff_prefetch"                   0 or 1: the FP32 implicit diffusion loads
  *                                      one level ahead
  * Defaults (the fastest measured at 3600x2700x75 with T, S and ten FP32
- * tracers): 4, 4, 2, 4, 2, 2, 16, 12, 0 in the order above.  Unknown knobs and
+ * tracers): 4, 4, 2, 4, 2, 2, 16, 12, 1 in the order above.  Unknown knobs and
  * out-of-range values return MOM_GPU_ERR_ARG. */
 int mom_gpu_set_tuning(mom_gpu_ctx_t *ctx, const char *knob, int value);
 int mom_gpu_get_tuning(const mom_gpu_ctx_t *ctx, const char *knob, int *value);

@@ -306,7 +306,7 @@ int mom_gpu_create(const mom_gpu_domain_t *dom, const mom_gpu_grid_t *grid, mom_
   c->tune.step_rows_per_wave_f32 = 2;
   c->tune.step_waves_f32 = 16;
   c->tune.step_waves_f64 = 12;
-  c->tune.vdiff_prefetch = 0;
+  c->tune.vdiff_prefetch = 1;
   const size_t P2 = (size_t)k.P2, nk = (size_t)k.nk;
 
 #define CK(call)                                                                      \

@@ -118,20 +118,23 @@ def test_fp32_passive_error_vs_fp64(setup, group, rows):
 
 
 def test_fp32_results_independent_of_launch_shape(setup):
-    """Grouping and tile height change which threads do the work, not the
-    arithmetic of any tracer: FP32 results are bit-for-bit the same."""
+    """Grouping, tile height and the column solver's load schedule change
+    which threads do the work and when, not the arithmetic of any tracer:
+    FP32 results are bit-for-bit the same."""
     case, orc, gpu, dev = setup
     Ts = [t for _, t in case.tracers]
     dtypes = [np.float64, np.float64] + [np.float32] * (len(Ts) - 2)
     base = None
-    for group, rows, waves in [(1, 2, 8), (2, 2, 8), (4, 2, 8), (1, 4, 8), (2, 4, 8), (4, 4, 8),
-                               (1, 2, 16), (2, 2, 16), (4, 2, 16)]:
+    for group, rows, waves, pf in [(1, 2, 8, 0), (2, 2, 8, 1), (4, 2, 8, 0), (1, 4, 8, 1),
+                                   (2, 4, 8, 0), (4, 4, 8, 1), (1, 2, 16, 0), (2, 2, 16, 1),
+                                   (4, 2, 16, 0), (4, 2, 16, 1)]:
         gpu.set_tuning("step_group_f32", group)
         gpu.set_tuning("step_rows_per_wave_f32", rows)
         gpu.set_tuning("step_waves_f32", waves)
+        gpu.set_tuning("vdiff_prefetch", pf)
         got = [g[:, 1:-1, 1:-1] for g in _run_step(case, gpu, dev, Ts, dtypes)]
         if base is None:
             base = got
         for n in range(len(Ts)):
-            assert np.array_equal(got[n], base[n]), (group, rows, waves, n)
+            assert np.array_equal(got[n], base[n]), (group, rows, waves, pf, n)
     gpu.set_tuning("step_waves_f32", 8)
