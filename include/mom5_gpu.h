@@ -116,11 +116,15 @@ int mom_gpu_copy(mom_gpu_ctx_t *ctx, double *dst, const double *src, size_t n);
  *                                      n>0: n subcycles per launch (1..9)
  *   "mdfl_rows_per_wave"               2 or 4: tile height of the MDFL kernel
  *   "step_group_f64"                   1 or 2 FP64 tracers per batched pass
- *   "step_group_f32"                   1, 2 or 4 FP32 tracers per batched pass
+ *   "step_group_f32"                   1, 2, 4, 5, 8 or 10 FP32 tracers per
+ *                                      batched pass (at most what the tile
+ *                                      shape below is built for: 4, or 10 with
+ *                                      1 row x 16 wavefronts, 5 with 2 x 12)
  *   "step_rows_per_wave"               2 or 4: tile height of the FP64 passes
- *   "step_rows_per_wave_f32"           2 or 4: tile height of the FP32 passes
- *   "step_waves_f32"                   8 or 16 wavefronts per FP32 tile (16 is
- *                                      used with 2 rows a wavefront)
+ *   "step_rows_per_wave_f32"           1, 2 or 4: tile height of the FP32 passes
+ *   "step_waves_f32"                   8, 12 or 16 wavefronts per FP32 tile (16
+ *                                      with 1 or 2 rows a wavefront, 12 with 2;
+ *                                      other pairs use 8)
  *   "step_waves_f64"                   8 or 12 wavefronts per FP64 tile (12 is
  *                                      used with 2 rows a wavefront and groups
  *                                      of up to 2 tracers)

@@ -140,9 +140,9 @@ __device__ inline float flux_fast(const FaceC<float> &c, const FaceW &w, float R
 
 template <typename TT>
 struct StepTab {
-  const TT *T[8];      // tracer at taum1 (= tau)
-  TT *out[8];          // tracer at taup1
-  const double *th[8]; // other th_tendency contributions (NULL: none)
+  const TT *T[12];      // tracer at taum1 (= tau)
+  TT *out[12];          // tracer at taup1
+  const double *th[12]; // other th_tendency contributions (NULL: none)
 };
 
 // Everything one pass of one domain needs: the batched kernel takes one
@@ -554,88 +554,144 @@ __global__ __launch_bounds__(RX * NW) void k_step_queue(const StepJob<TT> *__res
   }
 }
 
-template <typename TT, int G, int RPW, int NW = 8>
-void launch_group(mom_gpu_ctx *ctx, const StepTab<TT> &tab, const double *u, const double *v,
-                  const double *w, const double *rho, const double *rm1, const double *rr,
-                  double dtime) {
-  constexpr int TY = NW * RPW - 4;
-  const KDom &d = ctx->kd;
-  const int ntx = (d.ni + TX - 1) / TX;
-  const int nty = (d.nj + TY - 1) / TY;
-  const int ntiles = ntx * nty;
-  const int per = (ntiles + 7) / 8;
-  const StepJob<TT> J{d, tab, u, v, w, rho, rm1, rr, ctx->kmt, ctx->dat, ctx->datr,
-                      ctx->dxte, ctx->dyte, ctx->dxtn, ctx->dytn, dtime, ntx, ntiles};
-  hipLaunchKernelGGL((k_step<TT, G, RPW, NW>), dim3(8 * per), dim3(RX * NW), 0, ctx->stream, J,
-                     per);
-}
+// ---- launch shapes ------------------------------------------------------------
+// A shape is (rows per wavefront, wavefronts per tile); each has the group
+// sizes it is instantiated for.  Knob values that name no shape fall back to
+// 8 wavefronts.  Tiles are RX x (RPW*NW) with a 2-cell halo.
+//   FP64: (2, 8), (4, 8): 1, 2, 4 tracers; (2, 12): 1, 2 (168 VGPRs, 3 waves/SIMD)
+//   FP32: (2, 8), (4, 8), (2, 16): 1, 2, 4; (1, 16): 1, 2, 4, 8, 10 (one row a
+//         wavefront leaves room for ten tracers in 128 VGPRs); (2, 12): 1, 2, 5
+struct Shape {
+  int rpw, nw;
+};
 
-
-// Launch one group of up to G tracers of type TT taken from list[0..cnt).
 template <typename TT>
-static void launch_some(mom_gpu_ctx *ctx, int rpw, int waves, const TT *const *Tin,
-                        TT *const *Tout, const double *const *thin, int cnt, const double *u,
-                        const double *v, const double *w, const double *rho, const double *rm1,
-                        const double *rr, double dtime) {
-  StepTab<TT> t{};
-  for (int i = 0; i < cnt; ++i) { t.T[i] = Tin[i]; t.out[i] = Tout[i]; t.th[i] = thin[i]; }
-  // 16-wave tiles (32 rows, 28 interior: less halo re-read) fit 128 VGPRs
-  // only for FP32 groups at two rows per wave; FP64 groups of up to two
-  // tracers at two rows per wave (at most 170 VGPRs: three waves per SIMD)
-  // fill a compute unit with 12-wave tiles where 8-wave ones leave a third
-  // of its wave slots empty
+static Shape shape_of(int rpw, int waves) {
+  if (sizeof(TT) == 4) {
+    if (rpw == 1 && waves == 16) return {1, 16};
+    if (rpw == 2 && waves == 16) return {2, 16};
+    if (rpw == 2 && waves == 12) return {2, 12};
+  } else {
+    if (rpw == 2 && waves == 12) return {2, 12};
+  }
+  return {rpw == 4 ? 4 : 2, 8};
+}
+
+// Group sizes of a shape, largest first (0-terminated).
+template <typename TT>
+static const int *group_sizes(Shape sh) {
+  static const int g124[] = {4, 2, 1, 0}, g12[] = {2, 1, 0}, g125[] = {5, 2, 1, 0};
+  static const int g12_4_8_10[] = {10, 8, 4, 2, 1, 0};
+  if (sizeof(TT) == 4) {
+    if (sh.rpw == 1) return g12_4_8_10;
+    if (sh.nw == 12) return g125;
+    return g124;
+  }
+  return sh.nw == 12 ? g12 : g124;
+}
+
+// Largest group size of the shape no larger than min(left, gmax).
+template <typename TT>
+static int next_group(Shape sh, int left, int gmax) {
+  for (const int *g = group_sizes<TT>(sh); *g; ++g)
+    if (*g <= left && *g <= gmax) return *g;
+  return 1;
+}
+
+// Calls F.template run<G, RPW, NW>() for the runtime (g, shape).
+template <typename TT, typename F>
+static int with_shape(int g, Shape sh, F &&f) {
   if constexpr (sizeof(TT) == 4) {
-    if (waves == 16 && rpw == 2) {
-      if (cnt == 4) launch_group<TT, 4, 2, 16>(ctx, t, u, v, w, rho, rm1, rr, dtime);
-      else if (cnt == 2) launch_group<TT, 2, 2, 16>(ctx, t, u, v, w, rho, rm1, rr, dtime);
-      else launch_group<TT, 1, 2, 16>(ctx, t, u, v, w, rho, rm1, rr, dtime);
-      return;
+    if (sh.rpw == 1) {
+      switch (g) {
+        case 10: return f.template run<10, 1, 16>();
+        case 8: return f.template run<8, 1, 16>();
+        case 4: return f.template run<4, 1, 16>();
+        case 2: return f.template run<2, 1, 16>();
+        default: return f.template run<1, 1, 16>();
+      }
+    }
+    if (sh.nw == 12) {
+      switch (g) {
+        case 5: return f.template run<5, 2, 12>();
+        case 2: return f.template run<2, 2, 12>();
+        default: return f.template run<1, 2, 12>();
+      }
+    }
+    if (sh.nw == 16) {
+      switch (g) {
+        case 4: return f.template run<4, 2, 16>();
+        case 2: return f.template run<2, 2, 16>();
+        default: return f.template run<1, 2, 16>();
+      }
     }
   } else {
-    if (waves == 12 && rpw == 2 && cnt <= 2) {
-      if (cnt == 2) launch_group<TT, 2, 2, 12>(ctx, t, u, v, w, rho, rm1, rr, dtime);
-      else launch_group<TT, 1, 2, 12>(ctx, t, u, v, w, rho, rm1, rr, dtime);
-      return;
+    if (sh.nw == 12) {
+      switch (g) {
+        case 2: return f.template run<2, 2, 12>();
+        default: return f.template run<1, 2, 12>();
+      }
     }
   }
-  if (rpw == 2) {
-    if (cnt == 4) launch_group<TT, 4, 2>(ctx, t, u, v, w, rho, rm1, rr, dtime);
-    else if (cnt == 2) launch_group<TT, 2, 2>(ctx, t, u, v, w, rho, rm1, rr, dtime);
-    else launch_group<TT, 1, 2>(ctx, t, u, v, w, rho, rm1, rr, dtime);
-  } else {
-    if (cnt == 4) launch_group<TT, 4, 4>(ctx, t, u, v, w, rho, rm1, rr, dtime);
-    else if (cnt == 2) launch_group<TT, 2, 4>(ctx, t, u, v, w, rho, rm1, rr, dtime);
-    else launch_group<TT, 1, 4>(ctx, t, u, v, w, rho, rm1, rr, dtime);
+  if (sh.rpw == 4) {
+    switch (g) {
+      case 4: return f.template run<4, 4, 8>();
+      case 2: return f.template run<2, 4, 8>();
+      default: return f.template run<1, 4, 8>();
+    }
+  }
+  switch (g) {
+    case 4: return f.template run<4, 2, 8>();
+    case 2: return f.template run<2, 2, 8>();
+    default: return f.template run<1, 2, 8>();
   }
 }
+
+template <typename TT>
+struct GroupLaunch {
+  mom_gpu_ctx *ctx;
+  const StepTab<TT> &tab;
+  const double *u, *v, *w, *rho, *rm1, *rr;
+  double dtime;
+  template <int G, int RPW, int NW>
+  int run() const {
+    constexpr int TY = NW * RPW - 4;
+    const KDom &d = ctx->kd;
+    const int ntx = (d.ni + TX - 1) / TX;
+    const int nty = (d.nj + TY - 1) / TY;
+    const int ntiles = ntx * nty;
+    const int per = (ntiles + 7) / 8;
+    const StepJob<TT> J{d, tab, u, v, w, rho, rm1, rr, ctx->kmt, ctx->dat, ctx->datr,
+                        ctx->dxte, ctx->dyte, ctx->dxtn, ctx->dytn, dtime, ntx, ntiles};
+    hipLaunchKernelGGL((k_step<TT, G, RPW, NW>), dim3(8 * per), dim3(RX * NW), 0, ctx->stream,
+                       J, per);
+    return MOM_GPU_OK;
+  }
+};
 
 template <typename TT>
 static void launch_all(mom_gpu_ctx *ctx, int gmax, int rpw, int waves, int ntr,
-                       const void *const *T_in,
-                       void *const *T_out, const int *dtype, const double *const *th_in,
-                       int want, const double *u, const double *v, const double *w,
-                       const double *rho, const double *rm1, const double *rr, double dtime) {
-  const TT *tin[4];
-  TT *tout[4];
-  const double *th[4];
-  int g = 0;
-  auto flush = [&]() {
-    int i = 0;
-    while (i < g) {                 // groups of 4, 2, 1 no larger than gmax
-      int c = (g - i >= 4 && gmax >= 4) ? 4 : (g - i >= 2 && gmax >= 2) ? 2 : 1;
-      launch_some<TT>(ctx, rpw, waves, tin + i, tout + i, th + i, c, u, v, w, rho, rm1, rr, dtime);
-      i += c;
+                       const void *const *T_in, void *const *T_out, const int *dtype,
+                       const double *const *th_in, int want, const double *u, const double *v,
+                       const double *w, const double *rho, const double *rm1, const double *rr,
+                       double dtime) {
+  const Shape sh = shape_of<TT>(rpw, waves);
+  std::vector<int> idx;
+  for (int n = 0; n < ntr; ++n)
+    if (dtype[n] == want) idx.push_back(n);
+  size_t i = 0;
+  while (i < idx.size()) {
+    const int c = next_group<TT>(sh, (int)(idx.size() - i), gmax);
+    StepTab<TT> t{};
+    for (int q = 0; q < c; ++q) {
+      const int n = idx[i + q];
+      t.T[q] = (const TT *)T_in[n];
+      t.out[q] = (TT *)T_out[n];
+      t.th[q] = th_in ? th_in[n] : nullptr;
     }
-    g = 0;
-  };
-  for (int n = 0; n < ntr; ++n) {
-    if (dtype[n] != want) continue;
-    tin[g] = (const TT *)T_in[n];
-    tout[g] = (TT *)T_out[n];
-    th[g] = th_in ? th_in[n] : nullptr;
-    if (++g == 4) flush();
+    with_shape<TT>(c, sh, GroupLaunch<TT>{ctx, t, u, v, w, rho, rm1, rr, dtime});
+    i += c;
   }
-  flush();
 }
 
 
@@ -671,10 +727,22 @@ static int queue_launch(mom_gpu_ensemble *e, std::vector<StepJob<TT>> &jobs, int
 }
 
 template <typename TT>
+struct QueueLaunch {
+  mom_gpu_ensemble *e;
+  std::vector<StepJob<TT>> &jobs;
+  int slot;
+  template <int G, int RPW, int NW>
+  int run() const { return queue_launch<TT, G, RPW, NW>(e, jobs, slot); }
+};
+
+template <typename TT>
 static int ensemble_type(mom_gpu_ensemble *e, const mom_gpu_tracer_step_t *steps, int want,
                          int gmax, int rpw, int waves, int &slot) {
-  // jobs bucketed by group size: groups of 4, 2, 1 no larger than gmax, per domain
-  std::vector<StepJob<TT>> by[5];
+  // jobs bucketed by group size (the shape's sizes no larger than gmax), per domain
+  const Shape sh = shape_of<TT>(rpw, waves);
+  constexpr int GMAX = 12;
+  std::vector<StepJob<TT>> by[GMAX + 1];
+  const int TY = sh.nw * sh.rpw - 4;
   for (size_t dd = 0; dd < e->ctx.size(); ++dd) {
     mom_gpu_ctx *ctx = e->ctx[dd];
     const mom_gpu_tracer_step_t &s = steps[dd];
@@ -685,8 +753,7 @@ static int ensemble_type(mom_gpu_ensemble *e, const mom_gpu_tracer_step_t *steps
       if (s.dtype[n] == want) idx.push_back(n);
     size_t i = 0;
     while (i < idx.size()) {
-      const size_t left = idx.size() - i;
-      const int c = (left >= 4 && gmax >= 4) ? 4 : (left >= 2 && gmax >= 2) ? 2 : 1;
+      const int c = next_group<TT>(sh, (int)(idx.size() - i), gmax);
       StepTab<TT> t{};
       for (int q = 0; q < c; ++q) {
         const int n = idx[i + q];
@@ -694,9 +761,6 @@ static int ensemble_type(mom_gpu_ensemble *e, const mom_gpu_tracer_step_t *steps
         t.out[q] = (TT *)s.T_taup1[n];
         t.th[q] = s.th_tendency ? s.th_tendency[n] : nullptr;
       }
-      const int w = (sizeof(TT) == 4 && waves == 16 && rpw == 2) ? 16
-                    : (sizeof(TT) == 8 && waves == 12 && rpw == 2 && c <= 2) ? 12 : 8;
-      const int TY = w * rpw - 4;
       const int ntx = (d.ni + TX - 1) / TX;
       const int ntiles = ntx * ((d.nj + TY - 1) / TY);
       by[c].emplace_back();
@@ -713,32 +777,8 @@ static int ensemble_type(mom_gpu_ensemble *e, const mom_gpu_tracer_step_t *steps
     }
   }
   int rc = MOM_GPU_OK;
-  const bool w16 = sizeof(TT) == 4 && waves == 16 && rpw == 2;
-  const bool w12 = sizeof(TT) == 8 && waves == 12 && rpw == 2;
-  if (rpw == 2) {
-    if constexpr (sizeof(TT) == 4) {
-      if (w16) {
-        rc = rc ? rc : queue_launch<TT, 4, 2, 16>(e, by[4], slot++);
-        rc = rc ? rc : queue_launch<TT, 2, 2, 16>(e, by[2], slot++);
-        rc = rc ? rc : queue_launch<TT, 1, 2, 16>(e, by[1], slot++);
-      }
-    } else {
-      if (w12) {   // groups of four keep 8-wave tiles
-        rc = rc ? rc : queue_launch<TT, 4, 2, 8>(e, by[4], slot++);
-        rc = rc ? rc : queue_launch<TT, 2, 2, 12>(e, by[2], slot++);
-        rc = rc ? rc : queue_launch<TT, 1, 2, 12>(e, by[1], slot++);
-      }
-    }
-    if (!w16 && !w12) {
-      rc = rc ? rc : queue_launch<TT, 4, 2, 8>(e, by[4], slot++);
-      rc = rc ? rc : queue_launch<TT, 2, 2, 8>(e, by[2], slot++);
-      rc = rc ? rc : queue_launch<TT, 1, 2, 8>(e, by[1], slot++);
-    }
-  } else {
-    rc = rc ? rc : queue_launch<TT, 4, 4, 8>(e, by[4], slot++);
-    rc = rc ? rc : queue_launch<TT, 2, 4, 8>(e, by[2], slot++);
-    rc = rc ? rc : queue_launch<TT, 1, 4, 8>(e, by[1], slot++);
-  }
+  for (const int *g = group_sizes<TT>(sh); *g && rc == MOM_GPU_OK; ++g)
+    if (!by[*g].empty()) rc = with_shape<TT>(*g, sh, QueueLaunch<TT>{e, by[*g], slot++});
   return rc;
 }
 }  // namespace

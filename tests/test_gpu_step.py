@@ -127,7 +127,8 @@ def test_fp32_results_independent_of_launch_shape(setup):
     base = None
     for group, rows, waves, pf in [(1, 2, 8, 0), (2, 2, 8, 1), (4, 2, 8, 0), (1, 4, 8, 1),
                                    (2, 4, 8, 0), (4, 4, 8, 1), (1, 2, 16, 0), (2, 2, 16, 1),
-                                   (4, 2, 16, 0), (4, 2, 16, 1)]:
+                                   (4, 2, 16, 0), (4, 2, 16, 1), (10, 1, 16, 1), (8, 1, 16, 0),
+                                   (4, 1, 16, 1), (5, 2, 12, 0)]:
         gpu.set_tuning("step_group_f32", group)
         gpu.set_tuning("step_rows_per_wave_f32", rows)
         gpu.set_tuning("step_waves_f32", waves)
@@ -138,3 +139,38 @@ def test_fp32_results_independent_of_launch_shape(setup):
         for n in range(len(Ts)):
             assert np.array_equal(got[n], base[n]), (group, rows, waves, pf, n)
     gpu.set_tuning("step_waves_f32", 8)
+
+
+@pytest.mark.parametrize("cyclic", [True, False])
+def test_fp32_large_groups_bitwise_and_accurate(oracle_lib, cyclic):
+    """Ten FP32 tracers in one pass (1 row x 16 wavefronts), two passes of
+    five (2 x 12) and the 4 + 4 + 2 passes of 2 x 16 tiles give the same
+    bits; the result is within the FP32 bound of the FP64 restatement."""
+    from mom5_amd import OceanGPU
+    from mom5_amd.synth import make_case
+    kw = {} if cyclic else dict(cyclic_x=False, lat_s=20.0, lat_n=50.0, lon_span=30.0)
+    case = make_case(100, 52, 10, land_frac=0.3, nts=10, seed=21, n_passive=10, stf_scale=1e-3,
+                     **kw)
+    print(f"[ten FP32 tracers 100x52x10 cyclic={cyclic}]", flush=True)
+    from mom5_amd import DeviceArray
+    gpu = OceanGPU(case.dom, case.grid)
+    dev = {k: DeviceArray.from_host(v) for k, v in case.fields.items()
+           if isinstance(v, np.ndarray)}
+    Ts = [t for _, t in case.tracers]
+    dtypes = [np.float64, np.float64] + [np.float32] * (len(Ts) - 2)
+    base = None
+    for group, rows, waves in [(4, 2, 16), (10, 1, 16), (8, 1, 16), (5, 2, 12)]:
+        gpu.set_tuning("step_group_f32", group)
+        gpu.set_tuning("step_rows_per_wave_f32", rows)
+        gpu.set_tuning("step_waves_f32", waves)
+        got = [g[:, 1:-1, 1:-1] for g in _run_step(case, gpu, dev, Ts, dtypes)]
+        if base is None:
+            base = got
+        for n in range(len(Ts)):
+            assert np.array_equal(got[n], base[n]), (group, rows, waves, n)
+    ref = _oracle_step(case, oracle_lib.Oracle(case), Ts)
+    for n in range(2, len(Ts)):
+        r = ref[n][:, 1:-1, 1:-1]
+        err = np.abs(base[n].astype(np.float64) - r).max() / np.abs(r).max()
+        assert err <= 1e-5, (n, err)
+    gpu.close()
