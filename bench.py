@@ -330,8 +330,10 @@ def run_mixed(args, w):
         grp = g.get_tuning("step_group_f64" if want64 else "step_group_f32")
         rows = g.get_tuning("step_rows_per_wave" if want64 else "step_rows_per_wave_f32")
         waves = g.get_tuning("step_waves_f64" if want64 else "step_waves_f32")
-        if rows != 2 or (want64 and (waves != 12 or grp > 2)) or (not want64 and waves != 16):
-            waves = 8                        # the shapes the library launches (tile variants)
+        # the tile shapes the library builds (others fall back to 8 wavefronts)
+        shapes = {(2, 12)} if want64 else {(1, 16), (2, 16), (2, 12)}
+        if (rows, waves) not in shapes:
+            waves = 8
         key = f"k_step<{'double' if want64 else 'float'}, {grp}, {rows}, {waves}>"
         idx = [n for n in range(ntr) if is64[n] == want64][:grp]
         if len(idx) < grp:

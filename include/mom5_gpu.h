@@ -116,7 +116,7 @@ int mom_gpu_copy(mom_gpu_ctx_t *ctx, double *dst, const double *src, size_t n);
  *                                      n>0: n subcycles per launch (1..9)
  *   "mdfl_rows_per_wave"               2 or 4: tile height of the MDFL kernel
  *   "step_group_f64"                   1 or 2 FP64 tracers per batched pass
- *   "step_group_f32"                   1, 2, 4, 5, 8 or 10 FP32 tracers per
+ *   "step_group_f32"                   1, 2, 4, 5, 6, 8 or 10 FP32 tracers per
  *                                      batched pass (at most what the tile
  *                                      shape below is built for: 4, or 10 with
  *                                      1 row x 16 wavefronts, 5 with 2 x 12)
@@ -131,7 +131,7 @@ int mom_gpu_copy(mom_gpu_ctx_t *ctx, double *dst, const double *src, size_t n);
  *   "vdiff_prefetch"                   0 or 1: the FP32 implicit diffusion loads
  *                                      one level ahead
  * Defaults (the fastest measured at 3600x2700x75 with T, S and ten FP32
- * tracers): 4, 4, 2, 4, 2, 2, 16, 12, 1 in the order above.  Unknown knobs and
+ * tracers): 4, 4, 2, 10, 2, 1, 16, 12, 1 in the order above.  Unknown knobs and
  * out-of-range values return MOM_GPU_ERR_ARG. */
 int mom_gpu_set_tuning(mom_gpu_ctx_t *ctx, const char *knob, int value);
 int mom_gpu_get_tuning(const mom_gpu_ctx_t *ctx, const char *knob, int *value);

@@ -559,7 +559,7 @@ __global__ __launch_bounds__(RX * NW) void k_step_queue(const StepJob<TT> *__res
 // sizes it is instantiated for.  Knob values that name no shape fall back to
 // 8 wavefronts.  Tiles are RX x (RPW*NW) with a 2-cell halo.
 //   FP64: (2, 8), (4, 8): 1, 2, 4 tracers; (2, 12): 1, 2 (168 VGPRs, 3 waves/SIMD)
-//   FP32: (2, 8), (4, 8), (2, 16): 1, 2, 4; (1, 16): 1, 2, 4, 8, 10 (one row a
+//   FP32: (2, 8), (4, 8), (2, 16): 1, 2, 4; (1, 16): 1, 2, 4, 6, 8, 10 (one row a
 //         wavefront leaves room for ten tracers in 128 VGPRs); (2, 12): 1, 2, 5
 struct Shape {
   int rpw, nw;
@@ -581,7 +581,7 @@ static Shape shape_of(int rpw, int waves) {
 template <typename TT>
 static const int *group_sizes(Shape sh) {
   static const int g124[] = {4, 2, 1, 0}, g12[] = {2, 1, 0}, g125[] = {5, 2, 1, 0};
-  static const int g12_4_8_10[] = {10, 8, 4, 2, 1, 0};
+  static const int g12_4_8_10[] = {10, 8, 6, 4, 2, 1, 0};
   if (sizeof(TT) == 4) {
     if (sh.rpw == 1) return g12_4_8_10;
     if (sh.nw == 12) return g125;
@@ -606,6 +606,7 @@ static int with_shape(int g, Shape sh, F &&f) {
       switch (g) {
         case 10: return f.template run<10, 1, 16>();
         case 8: return f.template run<8, 1, 16>();
+        case 6: return f.template run<6, 1, 16>();
         case 4: return f.template run<4, 1, 16>();
         case 2: return f.template run<2, 1, 16>();
         default: return f.template run<1, 1, 16>();

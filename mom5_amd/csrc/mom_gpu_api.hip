@@ -132,8 +132,8 @@ int mom_gpu_set_tuning(mom_gpu_ctx_t *ctx, const char *knob, int value) {
   REQUIRE(ctx, slot != &ctx->tune.step_waves_f64 || value == 8 || value == 12,
           "mom_gpu_set_tuning: step_waves_f64 must be 8 or 12");
   REQUIRE(ctx, slot != &ctx->tune.step_group_f32 || value == 1 || value == 2 || value == 4 ||
-                   value == 5 || value == 8 || value == 10,
-          "mom_gpu_set_tuning: step_group_f32 must be 1, 2, 4, 5, 8 or 10");
+                   value == 5 || value == 6 || value == 8 || value == 10,
+          "mom_gpu_set_tuning: step_group_f32 must be 1, 2, 4, 5, 6, 8 or 10");
   *slot = value;
   return MOM_GPU_OK;
 }
@@ -303,9 +303,9 @@ int mom_gpu_create(const mom_gpu_domain_t *dom, const mom_gpu_grid_t *grid, mom_
   c->tune.bt_subcycles_per_launch = 4;
   c->tune.mdfl_rows_per_wave = 4;
   c->tune.step_group_f64 = 2;
-  c->tune.step_group_f32 = 4;
+  c->tune.step_group_f32 = 10;
   c->tune.step_rows_per_wave = 2;
-  c->tune.step_rows_per_wave_f32 = 2;
+  c->tune.step_rows_per_wave_f32 = 1;
   c->tune.step_waves_f32 = 16;
   c->tune.step_waves_f64 = 12;
   c->tune.vdiff_prefetch = 1;

@@ -55,8 +55,8 @@ def autotune(step, reps=3, barotropic=True):
                   rho_dzt_tau=d["rho_dzt"], rho_dztr=d["rho_dztr"], dtime=c.dt)
         cur, nxt = step.cur, 1 - step.cur
         fp32 = [c_ + (8,) for c_ in itertools.product((1, 2, 4), (2, 4))] + \
-            [(1, 2, 16), (2, 2, 16), (4, 2, 16), (4, 1, 16), (8, 1, 16), (10, 1, 16),
-             (5, 2, 12)]
+            [(1, 2, 16), (2, 2, 16), (4, 2, 16), (4, 1, 16), (6, 1, 16), (8, 1, 16),
+             (10, 1, 16), (5, 2, 12)]
         for tag, size, knobs, cands in (
                 ("fp64", 8, ("step_group_f64", "step_rows_per_wave", "step_waves_f64"),
                  [c_ + (8,) for c_ in itertools.product((1, 2), (2, 4))] +

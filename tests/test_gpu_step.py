@@ -159,7 +159,7 @@ def test_fp32_large_groups_bitwise_and_accurate(oracle_lib, cyclic):
     Ts = [t for _, t in case.tracers]
     dtypes = [np.float64, np.float64] + [np.float32] * (len(Ts) - 2)
     base = None
-    for group, rows, waves in [(4, 2, 16), (10, 1, 16), (8, 1, 16), (5, 2, 12)]:
+    for group, rows, waves in [(4, 2, 16), (10, 1, 16), (8, 1, 16), (6, 1, 16), (5, 2, 12)]:
         gpu.set_tuning("step_group_f32", group)
         gpu.set_tuning("step_rows_per_wave_f32", rows)
         gpu.set_tuning("step_waves_f32", waves)
