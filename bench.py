@@ -331,7 +331,7 @@ def run_mixed(args, w):
         rows = g.get_tuning("step_rows_per_wave" if want64 else "step_rows_per_wave_f32")
         waves = g.get_tuning("step_waves_f64" if want64 else "step_waves_f32")
         # the tile shapes the library builds (others fall back to 8 wavefronts)
-        shapes = {(2, 12)} if want64 else {(1, 16), (2, 16), (2, 12)}
+        shapes = {(2, 12), (1, 16)} if want64 else {(1, 16), (2, 16), (2, 12)}
         if (rows, waves) not in shapes:
             waves = 8
         key = f"k_step<{'double' if want64 else 'float'}, {grp}, {rows}, {waves}>"

@@ -115,19 +115,20 @@ int mom_gpu_copy(mom_gpu_ctx_t *ctx, double *dst, const double *src, size_t n);
  *   "barotropic_subcycles_per_launch"  0: three launches per subcycle;
  *                                      n>0: n subcycles per launch (1..9)
  *   "mdfl_rows_per_wave"               2 or 4: tile height of the MDFL kernel
- *   "step_group_f64"                   1 or 2 FP64 tracers per batched pass
+ *   "step_group_f64"                   1, 2 or 4 FP64 tracers per batched pass
+ *                                      (4 with 1 row x 16 or 2 x 8 wavefronts)
  *   "step_group_f32"                   1, 2, 4, 5, 6, 8 or 10 FP32 tracers per
  *                                      batched pass (at most what the tile
  *                                      shape below is built for: 4, or 10 with
  *                                      1 row x 16 wavefronts, 5 with 2 x 12)
- *   "step_rows_per_wave"               2 or 4: tile height of the FP64 passes
+ *   "step_rows_per_wave"               1, 2 or 4: tile height of the FP64 passes
  *   "step_rows_per_wave_f32"           1, 2 or 4: tile height of the FP32 passes
  *   "step_waves_f32"                   8, 12 or 16 wavefronts per FP32 tile (16
  *                                      with 1 or 2 rows a wavefront, 12 with 2;
  *                                      other pairs use 8)
- *   "step_waves_f64"                   8 or 12 wavefronts per FP64 tile (12 is
- *                                      used with 2 rows a wavefront and groups
- *                                      of up to 2 tracers)
+ *   "step_waves_f64"                   8, 12 or 16 wavefronts per FP64 tile (12
+ *                                      with 2 rows a wavefront and groups of up
+ *                                      to 2 tracers, 16 with 1 row)
  *   "vdiff_prefetch"                   0 or 1: the FP32 implicit diffusion loads
  *                                      one level ahead
  * Defaults (the fastest measured at 3600x2700x75 with T, S and ten FP32

@@ -558,7 +558,8 @@ __global__ __launch_bounds__(RX * NW) void k_step_queue(const StepJob<TT> *__res
 // A shape is (rows per wavefront, wavefronts per tile); each has the group
 // sizes it is instantiated for.  Knob values that name no shape fall back to
 // 8 wavefronts.  Tiles are RX x (RPW*NW) with a 2-cell halo.
-//   FP64: (2, 8), (4, 8): 1, 2, 4 tracers; (2, 12): 1, 2 (168 VGPRs, 3 waves/SIMD)
+//   FP64: (2, 8), (4, 8): 1, 2, 4 tracers; (2, 12): 1, 2 (168 VGPRs, 3 waves/SIMD);
+//         (1, 16): 1, 2, 4 (at most 128 VGPRs, 4 waves/SIMD)
 //   FP32: (2, 8), (4, 8), (2, 16): 1, 2, 4; (1, 16): 1, 2, 4, 6, 8, 10 (one row a
 //         wavefront leaves room for ten tracers in 128 VGPRs); (2, 12): 1, 2, 5
 struct Shape {
@@ -573,6 +574,7 @@ static Shape shape_of(int rpw, int waves) {
     if (rpw == 2 && waves == 12) return {2, 12};
   } else {
     if (rpw == 2 && waves == 12) return {2, 12};
+    if (rpw == 1 && waves == 16) return {1, 16};
   }
   return {rpw == 4 ? 4 : 2, 8};
 }
@@ -587,7 +589,7 @@ static const int *group_sizes(Shape sh) {
     if (sh.nw == 12) return g125;
     return g124;
   }
-  return sh.nw == 12 ? g12 : g124;
+  return sh.nw == 12 ? g12 : g124;   // (1, 16), (2, 8), (4, 8): 4, 2, 1
 }
 
 // Largest group size of the shape no larger than min(left, gmax).
@@ -631,6 +633,13 @@ static int with_shape(int g, Shape sh, F &&f) {
       switch (g) {
         case 2: return f.template run<2, 2, 12>();
         default: return f.template run<1, 2, 12>();
+      }
+    }
+    if (sh.rpw == 1) {
+      switch (g) {
+        case 4: return f.template run<4, 1, 16>();
+        case 2: return f.template run<2, 1, 16>();
+        default: return f.template run<1, 1, 16>();
       }
     }
   }

@@ -85,7 +85,7 @@ static int *tuning_slot(mom_gpu_ctx *ctx, const char *knob, int *lo, int *hi) {
     return &ctx->tune.mdfl_rows_per_wave;
   }
   if (!std::strcmp(knob, "step_group_f64")) {
-    *lo = 1; *hi = 2;
+    *lo = 1; *hi = 4;
     return &ctx->tune.step_group_f64;
   }
   if (!std::strcmp(knob, "step_group_f32")) {
@@ -93,7 +93,7 @@ static int *tuning_slot(mom_gpu_ctx *ctx, const char *knob, int *lo, int *hi) {
     return &ctx->tune.step_group_f32;
   }
   if (!std::strcmp(knob, "step_rows_per_wave")) {
-    *lo = 2; *hi = 4;
+    *lo = 1; *hi = 4;
     return &ctx->tune.step_rows_per_wave;
   }
   if (!std::strcmp(knob, "step_rows_per_wave_f32")) {
@@ -105,7 +105,7 @@ static int *tuning_slot(mom_gpu_ctx *ctx, const char *knob, int *lo, int *hi) {
     return &ctx->tune.step_waves_f32;
   }
   if (!std::strcmp(knob, "step_waves_f64")) {
-    *lo = 8; *hi = 12;
+    *lo = 8; *hi = 16;
     return &ctx->tune.step_waves_f64;
   }
   if (!std::strcmp(knob, "vdiff_prefetch")) {
@@ -122,15 +122,17 @@ int mom_gpu_set_tuning(mom_gpu_ctx_t *ctx, const char *knob, int value) {
   REQUIRE(ctx, slot, "mom_gpu_set_tuning: unknown knob '%s'", knob ? knob : "(null)");
   REQUIRE(ctx, value >= lo && value <= hi, "mom_gpu_set_tuning: %s=%d outside [%d,%d]", knob,
           value, lo, hi);
-  REQUIRE(ctx, (slot != &ctx->tune.mdfl_rows_per_wave && slot != &ctx->tune.step_rows_per_wave) ||
-                   value == 2 || value == 4,
+  REQUIRE(ctx, slot != &ctx->tune.mdfl_rows_per_wave || value == 2 || value == 4,
           "mom_gpu_set_tuning: %s must be 2 or 4", knob);
-  REQUIRE(ctx, slot != &ctx->tune.step_rows_per_wave_f32 || value == 1 || value == 2 || value == 4,
-          "mom_gpu_set_tuning: step_rows_per_wave_f32 must be 1, 2 or 4");
+  REQUIRE(ctx, (slot != &ctx->tune.step_rows_per_wave_f32 && slot != &ctx->tune.step_rows_per_wave) ||
+                   value == 1 || value == 2 || value == 4,
+          "mom_gpu_set_tuning: %s must be 1, 2 or 4", knob);
+  REQUIRE(ctx, slot != &ctx->tune.step_group_f64 || value != 3,
+          "mom_gpu_set_tuning: step_group_f64 must be 1, 2 or 4");
   REQUIRE(ctx, slot != &ctx->tune.step_waves_f32 || value == 8 || value == 12 || value == 16,
           "mom_gpu_set_tuning: step_waves_f32 must be 8, 12 or 16");
-  REQUIRE(ctx, slot != &ctx->tune.step_waves_f64 || value == 8 || value == 12,
-          "mom_gpu_set_tuning: step_waves_f64 must be 8 or 12");
+  REQUIRE(ctx, slot != &ctx->tune.step_waves_f64 || value == 8 || value == 12 || value == 16,
+          "mom_gpu_set_tuning: step_waves_f64 must be 8, 12 or 16");
   REQUIRE(ctx, slot != &ctx->tune.step_group_f32 || value == 1 || value == 2 || value == 4 ||
                    value == 5 || value == 6 || value == 8 || value == 10,
           "mom_gpu_set_tuning: step_group_f32 must be 1, 2, 4, 5, 6, 8 or 10");

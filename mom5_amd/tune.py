@@ -59,8 +59,8 @@ def autotune(step, reps=3, barotropic=True):
              (10, 1, 16), (5, 2, 12)]
         for tag, size, knobs, cands in (
                 ("fp64", 8, ("step_group_f64", "step_rows_per_wave", "step_waves_f64"),
-                 [c_ + (8,) for c_ in itertools.product((1, 2), (2, 4))] +
-                 [(1, 2, 12), (2, 2, 12)]),
+                 [c_ + (8,) for c_ in itertools.product((1, 2, 4), (2, 4))] +
+                 [(1, 2, 12), (2, 2, 12), (1, 1, 16), (2, 1, 16), (4, 1, 16)]),
                 ("fp32", 4, ("step_group_f32", "step_rows_per_wave_f32", "step_waves_f32"),
                  fp32)):
             idx = [n for n in range(step.ntr) if step.tracer(n).element_size() == size]

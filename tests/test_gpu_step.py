@@ -61,7 +61,7 @@ def _run_step(case, gpu, dev, host_tracers, dtypes):
 
 
 @pytest.mark.parametrize("group,rows,waves", [(1, 4, 8), (2, 2, 8), (1, 2, 8), (2, 2, 12),
-                                              (1, 2, 12)])
+                                              (1, 2, 12), (2, 1, 16), (1, 1, 16), (4, 1, 16)])
 def test_fp64_step_bitwise_equal_to_plain_path_and_oracle(setup, group, rows, waves):
     from mom5_amd import DeviceArray
     case, orc, gpu, dev = setup
