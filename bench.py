@@ -131,6 +131,14 @@ def step_kernel_bytes(ni, nj, nk, elem, group):
     return ni * nj * nk * (40 + 2 * elem * group) + ni * nj * 52
 
 
+def vdiff_kernel_bytes(ni, nj, nk, elem, ntr):
+    """Algorithmic bytes of one implicit-diffusion launch over `ntr`
+    tracers (the invtri routine-boundary minimum): per cell each tracer read
+    and written once and dcb, rho_dzt, dzwt read once (24 B); per column
+    stf and btf of each tracer and kmt."""
+    return ni * nj * nk * (2 * elem * ntr + 24) + ni * nj * (16 * ntr + 4)
+
+
 def mixed_step_bytes(ni, nj, nk, n64, n32, barotropic=True):
     """Algorithmic bytes of one whole step, each distinct input read once and
     each output written once:
@@ -349,6 +357,24 @@ def run_mixed(args, w):
                          launches_per_step=nl))
         log(f"{cand[-1]['kernel']}: {k_ms:.3f} ms/launch x {nl} per step")
         del src, dst
+    if n32 and bs.diffuse:
+        # the FP32 implicit diffusion: the tracer step with minus without it
+        idx = [n for n in range(ntr) if not is64[n]]
+        src = [bs.tracer(n) for n in idx]
+        dst = [bs.T[n][1 - bs.cur] for n in idx]
+        kwd = dict(kw, stfs=[bs.stf[n] for n in idx], btfs=[bs.btf[n] for n in idx],
+                   dzwt=d["dzwt"], diff_cbt=d["diff_cbt"], index_salt=-1)
+        t_with = time_launches(g, lambda: g.step_tracers(src, dst, **kwd))
+        t_adv = time_launches(g, lambda: g.step_tracers(src, dst, **kw))
+        nvd = -(-n32 // 12)                 # solver groups of up to 12 tracers
+        key = f"k_vdiff_inc<{'true' if g.get_tuning('vdiff_prefetch') else 'false'}>"
+        cand.append(dict(kernel=f"{key} (implicit vertical diffusion of the FP32 tracers, "
+                                f"increment form; timed as the step with minus without it)",
+                         key=key, kernel_ms=(t_with - t_adv) / nvd,
+                         bytes_per_launch=vdiff_kernel_bytes(ni, nj, nk, 4, min(n32, 12)),
+                         launches_per_step=nvd))
+        log(f"{cand[-1]['kernel']}: {cand[-1]['kernel_ms']:.3f} ms/launch x {nvd} per step")
+        del src, dst
     dom = max(cand, key=lambda c: c["kernel_ms"] * c["launches_per_step"])
     achieved = dom["bytes_per_launch"] / (dom["kernel_ms"] / 1e3) / 1e9
     traffic, tsrc = measured_traffic(w["name"] if not args.shape else None, dom.pop("key"))
@@ -521,6 +547,20 @@ def profile_kernels(args, w):
         g.sync()
         log(f"k_step<{'double' if size == 8 else 'float'},G={grp}> x3: algorithmic "
             f"{step_kernel_bytes(ni, nj, nk, size, grp)} bytes per dispatch")
+    # the FP32 implicit diffusion of every FP32 tracer (with their advection pass)
+    sel = [t for t in trs if t.element_size() == 4]
+    if sel:
+        out = [DeviceArray(t.shape, t.dtype) for t in sel]
+        ns = len(sel)
+        up = lambda a: a if isinstance(a, DeviceArray) else DeviceArray.from_host(a)
+        kwd = dict(kw, stfs=[up(a) for a in f["stf"][2:2 + ns]],
+                   btfs=[up(a) for a in f["btf"][2:2 + ns]], dzwt=up(f["dzwt"]),
+                   diff_cbt=up(f["diff_cbt"]), index_salt=-1)
+        for _ in range(3):
+            g.step_tracers(sel, out, **kwd)
+        g.sync()
+        log(f"k_vdiff_inc x3 ({ns} FP32 tracers): algorithmic "
+            f"{vdiff_kernel_bytes(ni, nj, nk, 4, ns)} bytes per dispatch")
 
 
 def main():
