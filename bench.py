@@ -467,7 +467,7 @@ def _tune_summary(rep):
 
 TUNING_KEYS = ("barotropic_subcycles_per_launch", "mdfl_rows_per_wave", "step_group_f64",
                "step_group_f32", "step_rows_per_wave", "step_rows_per_wave_f32",
-               "step_waves_f32", "step_waves_f64", "vdiff_prefetch")
+               "step_waves_f32", "step_waves_f64", "vdiff_prefetch", "vdiff_block")
 
 
 def run_plain(args, w):

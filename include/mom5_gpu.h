@@ -131,8 +131,10 @@ int mom_gpu_copy(mom_gpu_ctx_t *ctx, double *dst, const double *src, size_t n);
  *                                      to 2 tracers, 16 with 1 row)
  *   "vdiff_prefetch"                   0 or 1: the FP32 implicit diffusion loads
  *                                      one level ahead
+ *   "vdiff_block"                      64 or 256 threads a block of the FP32
+ *                                      implicit diffusion
  * Defaults (the fastest measured at 3600x2700x75 with T, S and ten FP32
- * tracers): 4, 4, 2, 10, 2, 1, 16, 12, 1 in the order above.  Unknown knobs and
+ * tracers): 4, 4, 2, 10, 2, 1, 16, 12, 1, 64 in the order above.  Unknown knobs and
  * out-of-range values return MOM_GPU_ERR_ARG. */
 int mom_gpu_set_tuning(mom_gpu_ctx_t *ctx, const char *knob, int value);
 int mom_gpu_get_tuning(const mom_gpu_ctx_t *ctx, const char *knob, int *value);

@@ -368,8 +368,11 @@ __device__ __forceinline__ void vdiff_inc_col(const VdJobF &J, int gi, int gj) {
   }
 }
 
+// Launched with 64 or 256 threads a block (knob vdiff_block): four
+// wavefronts of one block sweep 256 neighbouring columns together, so each
+// level's loads and stores reach HBM as 1-2 KB runs instead of 256-512 B.
 template <bool PF>
-__global__ __launch_bounds__(64) void k_vdiff_inc(VdJobF J) {
+__global__ __launch_bounds__(256) void k_vdiff_inc(VdJobF J) {
   const int gi = blockIdx.x * blockDim.x + threadIdx.x;
   if (gi >= J.d.ni) return;
   vdiff_inc_col<NTF, PF>(J, gi, blockIdx.y);
@@ -761,11 +764,12 @@ int launch_vdiff_mixed(mom_gpu_ctx *ctx, int ntr, void *const *T, const int *dty
       if (g.n == 0) return;
       VdJobF J{d, g, diff_cbt + p3 * (nmix - 1), rho0, ctx->kmt, dh, dhw, dt, aidif,
                ctx->tri_e, ctx->vd_scr, 0, 0};
+      const int bx = ctx->tune.vdiff_block;
       if (ctx->tune.vdiff_prefetch)
-        hipLaunchKernelGGL(k_vdiff_inc<true>, dim3((d.ni + 63) / 64, d.nj), dim3(64), 0,
+        hipLaunchKernelGGL(k_vdiff_inc<true>, dim3((d.ni + bx - 1) / bx, d.nj), dim3(bx), 0,
                            ctx->stream, J);
       else
-        hipLaunchKernelGGL(k_vdiff_inc<false>, dim3((d.ni + 63) / 64, d.nj), dim3(64), 0,
+        hipLaunchKernelGGL(k_vdiff_inc<false>, dim3((d.ni + bx - 1) / bx, d.nj), dim3(bx), 0,
                            ctx->stream, J);
       g.n = 0;
     };

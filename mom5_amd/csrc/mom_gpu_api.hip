@@ -112,6 +112,10 @@ static int *tuning_slot(mom_gpu_ctx *ctx, const char *knob, int *lo, int *hi) {
     *lo = 0; *hi = 1;
     return &ctx->tune.vdiff_prefetch;
   }
+  if (!std::strcmp(knob, "vdiff_block")) {
+    *lo = 64; *hi = 256;
+    return &ctx->tune.vdiff_block;
+  }
   return nullptr;
 }
 
@@ -127,6 +131,8 @@ int mom_gpu_set_tuning(mom_gpu_ctx_t *ctx, const char *knob, int value) {
   REQUIRE(ctx, (slot != &ctx->tune.step_rows_per_wave_f32 && slot != &ctx->tune.step_rows_per_wave) ||
                    value == 1 || value == 2 || value == 4,
           "mom_gpu_set_tuning: %s must be 1, 2 or 4", knob);
+  REQUIRE(ctx, slot != &ctx->tune.vdiff_block || value == 64 || value == 256,
+          "mom_gpu_set_tuning: vdiff_block must be 64 or 256");
   REQUIRE(ctx, slot != &ctx->tune.step_group_f64 || value != 3,
           "mom_gpu_set_tuning: step_group_f64 must be 1, 2 or 4");
   REQUIRE(ctx, slot != &ctx->tune.step_waves_f32 || value == 8 || value == 12 || value == 16,
@@ -311,6 +317,7 @@ int mom_gpu_create(const mom_gpu_domain_t *dom, const mom_gpu_grid_t *grid, mom_
   c->tune.step_waves_f32 = 16;
   c->tune.step_waves_f64 = 12;
   c->tune.vdiff_prefetch = 1;
+  c->tune.vdiff_block = 64;
   const size_t P2 = (size_t)k.P2, nk = (size_t)k.nk;
 
 #define CK(call)                                                                      \

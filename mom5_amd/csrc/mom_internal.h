@@ -109,6 +109,7 @@ struct mom_gpu_ctx {
     int step_waves_f32;            // 8 or 16 wavefronts per tile (FP32 groups)
     int step_waves_f64;            // 8 or 12 wavefronts per tile (FP64 groups)
     int vdiff_prefetch;            // FP32 implicit diffusion: loads one level ahead (0, 1)
+    int vdiff_block;               // FP32 implicit diffusion: threads a block (64, 256)
   } tune;
   char err[512];
 };

@@ -74,7 +74,7 @@ def autotune(step, reps=3, barotropic=True):
                 kwd = dict(kw, stfs=[step.stf[n] for n in idx], btfs=[step.btf[n] for n in idx],
                            dzwt=d["dzwt"], diff_cbt=d["diff_cbt"], index_salt=-1)
                 report["fp32_vdiff"] = _search(
-                    g, ("vdiff_prefetch",), [(0,), (1,)],
+                    g, ("vdiff_prefetch", "vdiff_block"), [(0, 64), (1, 64), (0, 256), (1, 256)],
                     lambda: g.step_tracers(src, dst, **kwd), reps)
     else:
         th = step.th
