@@ -358,21 +358,34 @@ def run_mixed(args, w):
         log(f"{cand[-1]['kernel']}: {k_ms:.3f} ms/launch x {nl} per step")
         del src, dst
     if n32 and bs.diffuse:
-        # the FP32 implicit diffusion: the tracer step with minus without it
-        idx = [n for n in range(ntr) if not is64[n]]
+        # the implicit diffusion: the tracer step with minus without it.  With
+        # vdiff_merge (T, S and up to 12 FP32 tracers) it is one launch of
+        # k_vdiff_mix for every tracer; otherwise the FP32 tracers' k_vdiff_inc.
+        merged = g.get_tuning("vdiff_merge") == 1 and n32 <= 12 and n64 <= 2
+        pf = "true" if g.get_tuning("vdiff_prefetch") else "false"
+        if merged:
+            idx = list(range(ntr))
+            nb = 4 if n32 <= 4 else 8 if n32 <= 8 else 10 if n32 <= 10 else 12
+            key = f"k_vdiff_mix<{pf}, {nb}>"
+            what = "implicit vertical diffusion of every tracer in one launch"
+            vbytes = ni * nj * nk * (8 * n32 + 16 * n64 + 32) + ni * nj * (16 * ntr + 4)
+            nvd = 1
+        else:
+            idx = [n for n in range(ntr) if not is64[n]]
+            key = f"k_vdiff_inc<{pf}>"
+            what = "implicit vertical diffusion of the FP32 tracers, increment form"
+            vbytes = vdiff_kernel_bytes(ni, nj, nk, 4, min(n32, 12))
+            nvd = -(-n32 // 12)             # solver groups of up to 12 tracers
         src = [bs.tracer(n) for n in idx]
         dst = [bs.T[n][1 - bs.cur] for n in idx]
         kwd = dict(kw, stfs=[bs.stf[n] for n in idx], btfs=[bs.btf[n] for n in idx],
-                   dzwt=d["dzwt"], diff_cbt=d["diff_cbt"], index_salt=-1)
+                   dzwt=d["dzwt"], diff_cbt=d["diff_cbt"],
+                   index_salt=bs.index_salt if merged else -1)
         t_with = time_launches(g, lambda: g.step_tracers(src, dst, **kwd))
         t_adv = time_launches(g, lambda: g.step_tracers(src, dst, **kw))
-        nvd = -(-n32 // 12)                 # solver groups of up to 12 tracers
-        key = f"k_vdiff_inc<{'true' if g.get_tuning('vdiff_prefetch') else 'false'}>"
-        cand.append(dict(kernel=f"{key} (implicit vertical diffusion of the FP32 tracers, "
-                                f"increment form; timed as the step with minus without it)",
-                         key=key, kernel_ms=(t_with - t_adv) / nvd,
-                         bytes_per_launch=vdiff_kernel_bytes(ni, nj, nk, 4, min(n32, 12)),
-                         launches_per_step=nvd))
+        cand.append(dict(kernel=f"{key} ({what}; timed as the tracer step with minus without "
+                                f"it)", key=key, kernel_ms=(t_with - t_adv) / nvd,
+                         bytes_per_launch=vbytes, launches_per_step=nvd))
         log(f"{cand[-1]['kernel']}: {cand[-1]['kernel_ms']:.3f} ms/launch x {nvd} per step")
         del src, dst
     dom = max(cand, key=lambda c: c["kernel_ms"] * c["launches_per_step"])
@@ -467,7 +480,8 @@ def _tune_summary(rep):
 
 TUNING_KEYS = ("barotropic_subcycles_per_launch", "mdfl_rows_per_wave", "step_group_f64",
                "step_group_f32", "step_rows_per_wave", "step_rows_per_wave_f32",
-               "step_waves_f32", "step_waves_f64", "vdiff_prefetch", "vdiff_block")
+               "step_waves_f32", "step_waves_f64", "vdiff_prefetch", "vdiff_block",
+               "vdiff_merge")
 
 
 def run_plain(args, w):
@@ -547,20 +561,16 @@ def profile_kernels(args, w):
         g.sync()
         log(f"k_step<{'double' if size == 8 else 'float'},G={grp}> x3: algorithmic "
             f"{step_kernel_bytes(ni, nj, nk, size, grp)} bytes per dispatch")
-    # the FP32 implicit diffusion of every FP32 tracer (with their advection pass)
-    sel = [t for t in trs if t.element_size() == 4]
-    if sel:
-        out = [DeviceArray(t.shape, t.dtype) for t in sel]
-        ns = len(sel)
-        up = lambda a: a if isinstance(a, DeviceArray) else DeviceArray.from_host(a)
-        kwd = dict(kw, stfs=[up(a) for a in f["stf"][2:2 + ns]],
-                   btfs=[up(a) for a in f["btf"][2:2 + ns]], dzwt=up(f["dzwt"]),
-                   diff_cbt=up(f["diff_cbt"]), index_salt=-1)
-        for _ in range(3):
-            g.step_tracers(sel, out, **kwd)
-        g.sync()
-        log(f"k_vdiff_inc x3 ({ns} FP32 tracers): algorithmic "
-            f"{vdiff_kernel_bytes(ni, nj, nk, 4, ns)} bytes per dispatch")
+    # the implicit diffusion of every tracer of the step (with the advection passes)
+    out = [DeviceArray(t.shape, t.dtype) for t in trs]
+    ns = len(trs)
+    up = lambda a: a if isinstance(a, DeviceArray) else DeviceArray.from_host(a)
+    kwd = dict(kw, stfs=[up(a) for a in f["stf"][:ns]], btfs=[up(a) for a in f["btf"][:ns]],
+               dzwt=up(f["dzwt"]), diff_cbt=up(f["diff_cbt"]), index_salt=1)
+    for _ in range(3):
+        g.step_tracers(trs, out, **kwd)
+    g.sync()
+    log(f"implicit diffusion of {ns} tracers x3 (k_vdiff_mix when merged)")
 
 
 def main():

@@ -133,8 +133,13 @@ int mom_gpu_copy(mom_gpu_ctx_t *ctx, double *dst, const double *src, size_t n);
  *                                      one level ahead
  *   "vdiff_block"                      64 or 256 threads a block of the FP32
  *                                      implicit diffusion
+ *   "vdiff_merge"                      0 or 1: a step with FP32 tracers solves
+ *                                      the implicit diffusion of every tracer
+ *                                      in one launch (up to 12 FP32 tracers of
+ *                                      mixing class 1 and two FP64 tracers of
+ *                                      each class; other mixes: per group)
  * Defaults (the fastest measured at 3600x2700x75 with T, S and ten FP32
- * tracers): 4, 4, 2, 10, 2, 1, 16, 12, 1, 64 in the order above.  Unknown knobs and
+ * tracers): 4, 4, 2, 10, 2, 1, 16, 12, 1, 256, 1 in the order above.  Unknown knobs and
  * out-of-range values return MOM_GPU_ERR_ARG. */
 int mom_gpu_set_tuning(mom_gpu_ctx_t *ctx, const char *knob, int value);
 int mom_gpu_get_tuning(const mom_gpu_ctx_t *ctx, const char *knob, int *value);

@@ -378,6 +378,200 @@ __global__ __launch_bounds__(256) void k_vdiff_inc(VdJobF J) {
   vdiff_inc_col<NTF, PF>(J, gi, blockIdx.y);
 }
 
+// Every tracer of a mixed step in one launch: FP32 tracers (mixing class 1,
+// increment form as k_vdiff_inc) and up to two FP64 tracers of each mixing
+// class (the full form of vdiff_col, bit for bit).  dh, dhw and the class-1
+// factorisation are read and formed once for all of them; class 2 (salt)
+// has its own dcb, factorisation and e plane.
+constexpr int NT64 = 2;
+struct VdMixJob {
+  KDom d;
+  TriGroupF g32;                     // class 1, FP32 storage
+  double *z1[NT64], *z2[NT64];       // FP64 tracers of class 1 and class 2
+  const double *stf1[NT64], *btf1[NT64], *stf2[NT64], *btf2[NT64];
+  int n1, n2;
+  const double *dcb1, *dcb2;         // diff_cbt planes of the two classes
+  double rho0;
+  const int *kmt;
+  const double *dh, *dhw;
+  double tdt, aidif;
+  double *ew1, *ew2;
+  float *scr;
+};
+
+// One class's factorisation step at level k (vdiff_col's arithmetic).
+struct TriCoef {
+  double a, c, bet, cprev, dcb_km1;
+};
+__device__ inline void tri_level(TriCoef &t, double dcb_k, double factu, double factl, double mkk,
+                                 double mkp1, int k, int nk, double *ew, int ok) {
+  if (k == 1) t.dcb_km1 = dcb_k;
+  double a = -t.dcb_km1 * factu * mkk;
+  double c = -dcb_k * factl * mkp1;
+  if (k == 1) a = 0.0;
+  if (k == nk) c = 0.0;
+  const double b = 1.0 - a - c;
+  if (k == 1) {
+    t.bet = mkk / (b + EPS_TRI);
+  } else {
+    const double e = t.cprev * t.bet;
+    t.bet = mkk / (b - a * e + EPS_TRI);
+    ew[ok] = e;
+  }
+  t.a = a;
+  t.c = c;
+}
+
+template <bool PF, int N32>
+__global__ __launch_bounds__(256) void k_vdiff_mix(VdMixJob J) {
+  const int gi = blockIdx.x * blockDim.x + threadIdx.x, gj = blockIdx.y;
+  const KDom d = J.d;
+  if (gi >= d.ni) return;
+  const TriGroupF &g = J.g32;
+  const int nk = d.nk, P2 = d.P2;
+  const size_t P3 = (size_t)P2 * nk;
+  const int o = d2(d, gi, gj);
+  const double tdt_aidif = J.tdt * J.aidif;
+  const int kmo = J.kmt[o];
+  const int kb = max(2, kmo);
+  const bool two = J.n2 > 0;
+  TriCoef t1{0.0, 0.0, 0.0, 0.0, 0.0}, t2{0.0, 0.0, 0.0, 0.0, 0.0};
+  double dprev[N32], zp1[NT64], zp2[NT64];
+  float zkm1[N32], zk[N32], zn[N32];
+#pragma unroll
+  for (int n = 0; n < N32; ++n) {
+    dprev[n] = 0.0;
+    zkm1[n] = 0.0f;
+    zk[n] = (n < g.n) ? g.z[n][o] : 0.0f;
+    zn[n] = (PF && n < g.n && nk > 1) ? g.z[n][o + P2] : 0.0f;
+  }
+#pragma unroll
+  for (int n = 0; n < NT64; ++n) { zp1[n] = 0.0; zp2[n] = 0.0; }
+  double dcb1_n = J.dcb1[o], dcb2_n = two ? J.dcb2[o] : 0.0, dh_n = J.dh[o];
+  double dhwu_n = J.dhw[o], dhwl_n = J.dhw[o + P2];
+  for (int k = 1; k <= nk; ++k) {
+    const int kp1 = min(k + 1, nk);
+    const int ok = o + P2 * (k - 1);
+    double dcb1k, dcb2k, dhk, dhwu, dhwl;
+    float zkp1v[N32];
+    if constexpr (PF) {
+      dcb1k = dcb1_n; dcb2k = dcb2_n; dhk = dh_n; dhwu = dhwu_n; dhwl = dhwl_n;
+#pragma unroll
+      for (int n = 0; n < N32; ++n) zkp1v[n] = zn[n];
+      if (k < nk) {
+        dcb1_n = J.dcb1[ok + P2];
+        if (two) dcb2_n = J.dcb2[ok + P2];
+        dh_n = J.dh[ok + P2];
+        dhwu_n = dhwl;
+        dhwl_n = J.dhw[o + P2 * (k + 1)];
+#pragma unroll
+        for (int n = 0; n < N32; ++n)
+          zn[n] = (n < g.n && k + 1 < nk) ? g.z[n][ok + 2 * P2] : 0.0f;
+      }
+    } else {
+      dcb1k = J.dcb1[ok];
+      dcb2k = two ? J.dcb2[ok] : 0.0;
+      dhk = J.dh[ok];
+      dhwu = J.dhw[o + P2 * (k - 1)];
+      dhwl = J.dhw[o + P2 * k];
+#pragma unroll
+      for (int n = 0; n < N32; ++n) zkp1v[n] = (n < g.n && k < nk) ? g.z[n][ok + P2] : 0.0f;
+    }
+    const double factu = tdt_aidif / (dhwu * dhk);
+    const double factl = tdt_aidif / (dhwl * dhk);
+    const double mkk = (k <= kmo) ? 1.0 : 0.0;
+    const double mkp1 = (kp1 <= kmo) ? 1.0 : 0.0;
+    // dcb = rho0*diff_cbt + K33 with K33 = 0 (ocean_vert_mix.F90:3467-3484)
+    const double dcb1_k = J.rho0 * dcb1k + 0.0;
+    tri_level(t1, dcb1_k, factu, factl, mkk, mkp1, k, nk, J.ew1, ok);
+    double dcb2_k = 0.0;
+    if (two) {
+      dcb2_k = J.rho0 * dcb2k + 0.0;
+      tri_level(t2, dcb2_k, factu, factl, mkk, mkp1, k, nk, J.ew2, ok);
+    }
+    // FP64 tracers: forward sweep of invtri, stored in place
+    auto fwd64 = [&](double *z, const double *stf, const double *btf, const TriCoef &t,
+                     double &zprev) {
+      const double zo = z[ok];
+      double f;
+      if (k == 1)
+        f = zo + stf[o] * tdt_aidif * mkk / dhk;
+      else if (k == kb)
+        f = zo - btf[o] * tdt_aidif * mkk / dhk;
+      else
+        f = zo * mkk;
+      const double zkv = (k == 1) ? f * t.bet : (f - t.a * zprev) * t.bet;
+      z[ok] = zkv;
+      zprev = zkv;
+    };
+#pragma unroll
+    for (int n = 0; n < NT64; ++n) {
+      if (n < J.n1) fwd64(J.z1[n], J.stf1[n], J.btf1[n], t1, zp1[n]);
+      if (n < J.n2) fwd64(J.z2[n], J.stf2[n], J.btf2[n], t2, zp2[n]);
+    }
+    // FP32 tracers: increment form (k_vdiff_inc)
+#pragma unroll
+    for (int n = 0; n < N32; ++n) {
+      if (n < g.n) {
+        const float zkp1 = (k < nk) ? zkp1v[n] : 0.0f;
+        double bfl = 0.0;
+        if (k == 1)
+          bfl = g.stf[n][o] * tdt_aidif * mkk / dhk;
+        else if (k == kb)
+          bfl = -(g.btf[n][o] * tdt_aidif * mkk / dhk);
+        const double zc = zk[n];
+        const double rhs = bfl - (t1.a * ((double)zkm1[n] - zc) + t1.c * ((double)zkp1 - zc));
+        const double dk = (k == 1) ? rhs * t1.bet : (rhs - t1.a * dprev[n]) * t1.bet;
+        J.scr[(size_t)n * P3 + ok] = (float)dk;
+        dprev[n] = dk;
+        zkm1[n] = zk[n];
+        zk[n] = zkp1;
+      }
+    }
+    t1.cprev = t1.c;
+    t1.dcb_km1 = dcb1_k;
+    if (two) {
+      t2.cprev = t2.c;
+      t2.dcb_km1 = dcb2_k;
+    }
+  }
+  {
+    const int ok = o + P2 * (nk - 1);
+    const double mk = (nk <= kmo) ? 1.0 : 0.0;
+#pragma unroll
+    for (int n = 0; n < N32; ++n)
+      if (n < g.n) g.z[n][ok] = (float)(((double)g.z[n][ok] + dprev[n]) * mk);
+  }
+  // back substitution
+  for (int k = nk - 1; k >= 1; --k) {
+    const int ok = o + P2 * (k - 1);
+    const double e1 = J.ew1[ok + P2];
+    const double e2 = two ? J.ew2[ok + P2] : 0.0;
+#pragma unroll
+    for (int n = 0; n < NT64; ++n) {
+      if (n < J.n1) {
+        const double zkv = J.z1[n][ok] - e1 * zp1[n];
+        J.z1[n][ok] = zkv;
+        zp1[n] = zkv;
+      }
+      if (n < J.n2) {
+        const double zkv = J.z2[n][ok] - e2 * zp2[n];
+        J.z2[n][ok] = zkv;
+        zp2[n] = zkv;
+      }
+    }
+    const double mk = (k <= kmo) ? 1.0 : 0.0;
+#pragma unroll
+    for (int n = 0; n < N32; ++n) {
+      if (n < g.n) {
+        const double dT = (double)J.scr[(size_t)n * P3 + ok] - e1 * dprev[n];
+        g.z[n][ok] = (float)(((double)g.z[n][ok] + dT) * mk);
+        dprev[n] = dT;
+      }
+    }
+  }
+}
+
 // Ensemble: one group of every domain per launch (groups of one domain share
 // that domain's solver scratch, so they go in successive launches).
 constexpr int VD_MAXJ = 64;
@@ -743,10 +937,80 @@ static int fp32_group_planes(int ntr, const int *dtype, int index_salt) {
   return std::min(NTF, std::max(cnt[0], cnt[1]));
 }
 
+// One launch for the whole mix when it fits k_vdiff_mix: FP32 tracers only of
+// class 1 (at most NTF), at most NT64 FP64 tracers of each class.
+static int vdiff_mix_one(mom_gpu_ctx *ctx, int ntr, void *const *T, const int *dtype,
+                         const double *const *stf, const double *const *btf,
+                         const double *diff_cbt, int index_salt, double rho0, const double *dh,
+                         const double *dhw, double dt, double aidif, bool &done) {
+  done = false;
+  VdMixJob J;
+  std::memset((void *)&J, 0, sizeof J);
+  for (int n = 0; n < ntr; ++n) {
+    const int cls = (n == index_salt) ? 2 : 1;
+    if (dtype[n] == 4) {
+      if (cls != 1 || J.g32.n == NTF) return MOM_GPU_OK;
+      J.g32.z[J.g32.n] = (float *)T[n]; J.g32.stf[J.g32.n] = stf[n];
+      J.g32.btf[J.g32.n] = btf[n]; ++J.g32.n;
+    } else if (cls == 1) {
+      if (J.n1 == NT64) return MOM_GPU_OK;
+      J.z1[J.n1] = (double *)T[n]; J.stf1[J.n1] = stf[n]; J.btf1[J.n1] = btf[n]; ++J.n1;
+    } else {
+      if (J.n2 == NT64) return MOM_GPU_OK;
+      J.z2[J.n2] = (double *)T[n]; J.stf2[J.n2] = stf[n]; J.btf2[J.n2] = btf[n]; ++J.n2;
+    }
+  }
+  if (J.g32.n == 0) return MOM_GPU_OK;
+  const KDom &d = ctx->kd;
+  const size_t p3 = (size_t)d.P2 * d.nk;
+  int rc = ensure_vd_scr(ctx, ctx, ctx->stream, J.g32.n, "step_tracers");
+  if (rc) return rc;
+  if (J.n2 > 0 && !ctx->tri_e2) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    MOM_HIP(ctx, hipStreamIsCapturing(ctx->stream, &cs));
+    if (cs != hipStreamCaptureStatusNone)
+      return mom_fail(ctx, MOM_GPU_ERR_ARG,
+                      "step_tracers: the first mixed step allocates a second factorisation "
+                      "plane; run one step before capturing a graph");
+    MOM_HIP(ctx, hipMalloc((void **)&ctx->tri_e2, p3 * sizeof(double)));
+  }
+  J.d = d;
+  J.dcb1 = diff_cbt;
+  J.dcb2 = diff_cbt + p3;
+  J.rho0 = rho0; J.kmt = ctx->kmt; J.dh = dh; J.dhw = dhw; J.tdt = dt; J.aidif = aidif;
+  J.ew1 = ctx->tri_e; J.ew2 = ctx->tri_e2; J.scr = ctx->vd_scr;
+  const int bx = ctx->tune.vdiff_block;
+  const dim3 grid((d.ni + bx - 1) / bx, d.nj), block(bx);
+  // register arrays sized to the group (4, 8, 10 or 12 FP32 tracers)
+  const int nb = J.g32.n <= 4 ? 4 : J.g32.n <= 8 ? 8 : J.g32.n <= 10 ? 10 : 12;
+#define MOM_VDMIX(PFV, NV) hipLaunchKernelGGL((k_vdiff_mix<PFV, NV>), grid, block, 0, ctx->stream, J)
+  if (ctx->tune.vdiff_prefetch) {
+    if (nb == 4) MOM_VDMIX(true, 4);
+    else if (nb == 8) MOM_VDMIX(true, 8);
+    else if (nb == 10) MOM_VDMIX(true, 10);
+    else MOM_VDMIX(true, 12);
+  } else {
+    if (nb == 4) MOM_VDMIX(false, 4);
+    else if (nb == 8) MOM_VDMIX(false, 8);
+    else if (nb == 10) MOM_VDMIX(false, 10);
+    else MOM_VDMIX(false, 12);
+  }
+#undef MOM_VDMIX
+  MOM_LAUNCHED(ctx, "k_vdiff_mix");
+  done = true;
+  return MOM_GPU_OK;
+}
+
 int launch_vdiff_mixed(mom_gpu_ctx *ctx, int ntr, void *const *T, const int *dtype,
                        const double *const *stf, const double *const *btf,
                        const double *diff_cbt, int index_salt, double rho0, const double *dh,
                        const double *dhw, double dt, double aidif) {
+  if (ctx->tune.vdiff_merge) {
+    bool done = false;
+    const int rc0 = vdiff_mix_one(ctx, ntr, T, dtype, stf, btf, diff_cbt, index_salt, rho0, dh,
+                                  dhw, dt, aidif, done);
+    if (rc0 || done) return rc0;
+  }
   int rc = vdiff_groups<double>(ctx, ntr, T, dtype, 8, stf, btf, diff_cbt, nullptr, index_salt,
                                 rho0, dh, dhw, dt, aidif);
   if (rc) return rc;

@@ -116,6 +116,10 @@ static int *tuning_slot(mom_gpu_ctx *ctx, const char *knob, int *lo, int *hi) {
     *lo = 64; *hi = 256;
     return &ctx->tune.vdiff_block;
   }
+  if (!std::strcmp(knob, "vdiff_merge")) {
+    *lo = 0; *hi = 1;
+    return &ctx->tune.vdiff_merge;
+  }
   return nullptr;
 }
 
@@ -266,7 +270,7 @@ static void free_ctx(mom_gpu_ctx *c) {
   if (!c) return;
   double **ds[] = {&c->dxt, &c->dyt, &c->dat, &c->datr, &c->dxte, &c->dyte, &c->dxtn, &c->dytn,
                    &c->dxu, &c->dyu, &c->dxur, &c->dyur, &c->f, &c->dzt, &c->qw, &c->qzw,
-                   &c->bt, &c->tri_e, &c->red, &c->remap, &c->btv};
+                   &c->bt, &c->tri_e, &c->tri_e2, &c->red, &c->remap, &c->btv};
   for (double **p : ds)
     if (*p) hipFree(*p);
   if (c->kmt) hipFree(c->kmt);
@@ -317,7 +321,8 @@ int mom_gpu_create(const mom_gpu_domain_t *dom, const mom_gpu_grid_t *grid, mom_
   c->tune.step_waves_f32 = 16;
   c->tune.step_waves_f64 = 12;
   c->tune.vdiff_prefetch = 1;
-  c->tune.vdiff_block = 64;
+  c->tune.vdiff_block = 256;
+  c->tune.vdiff_merge = 1;
   const size_t P2 = (size_t)k.P2, nk = (size_t)k.nk;
 
 #define CK(call)                                                                      \

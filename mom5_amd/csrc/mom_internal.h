@@ -91,6 +91,7 @@ struct mom_gpu_ctx {
   double *btv;
   // invtri work (e coefficients), one 3-D plane set
   double *tri_e;
+  double *tri_e2;   // second factorisation plane (salt's mixing class), lazy
   // FP32 implicit diffusion: forward increments, one plane per tracer of
   // the largest solver group seen (lazy)
   float *vd_scr;
@@ -110,6 +111,7 @@ struct mom_gpu_ctx {
     int step_waves_f64;            // 8 or 12 wavefronts per tile (FP64 groups)
     int vdiff_prefetch;            // FP32 implicit diffusion: loads one level ahead (0, 1)
     int vdiff_block;               // FP32 implicit diffusion: threads a block (64, 256)
+    int vdiff_merge;               // mixed steps: all tracers in one diffusion launch (0, 1)
   } tune;
   char err[512];
 };

@@ -69,13 +69,15 @@ def autotune(step, reps=3, barotropic=True):
             src = [step.T[n][cur] for n in idx]
             dst = [step.T[n][nxt] for n in idx]
             report[tag] = _search(g, knobs, cands, lambda: g.step_tracers(src, dst, **kw), reps)
-            if tag == "fp32" and step.diffuse:
-                # the FP32 implicit diffusion (timed with the tuned advection)
-                kwd = dict(kw, stfs=[step.stf[n] for n in idx], btfs=[step.btf[n] for n in idx],
-                           dzwt=d["dzwt"], diff_cbt=d["diff_cbt"], index_salt=-1)
-                report["fp32_vdiff"] = _search(
-                    g, ("vdiff_prefetch", "vdiff_block"), [(0, 64), (1, 64), (0, 256), (1, 256)],
-                    lambda: g.step_tracers(src, dst, **kwd), reps)
+        if step.diffuse and any(step.tracer(n).element_size() == 4 for n in range(step.ntr)):
+            # the implicit diffusion of a step with FP32 tracers: the whole
+            # tracer step (tuned advection passes) per candidate
+            a = step.step_args()
+            src_all, dst_all = a.pop("T_taum1s"), a.pop("T_taup1s")
+            report["vdiff"] = _search(
+                g, ("vdiff_prefetch", "vdiff_block", "vdiff_merge"),
+                [(pf, bx, mg) for mg in (0, 1) for pf in (0, 1) for bx in (64, 256)],
+                lambda: g.step_tracers(src_all, dst_all, **a), reps)
     else:
         th = step.th
         T0 = step.tracer(0)

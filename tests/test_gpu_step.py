@@ -134,6 +134,7 @@ def test_fp32_results_independent_of_launch_shape(setup):
         gpu.set_tuning("step_waves_f32", waves)
         gpu.set_tuning("vdiff_prefetch", pf)
         gpu.set_tuning("vdiff_block", 256 if group % 2 else 64)
+        gpu.set_tuning("vdiff_merge", 1 if rows == 2 else 0)
         got = [g[:, 1:-1, 1:-1] for g in _run_step(case, gpu, dev, Ts, dtypes)]
         if base is None:
             base = got
