@@ -322,7 +322,7 @@ int mom_gpu_create(const mom_gpu_domain_t *dom, const mom_gpu_grid_t *grid, mom_
   c->tune.step_waves_f64 = 12;
   c->tune.vdiff_prefetch = 1;
   c->tune.vdiff_block = 256;
-  c->tune.vdiff_merge = 1;
+  c->tune.vdiff_merge = 0;
   const size_t P2 = (size_t)k.P2, nk = (size_t)k.nk;
 
 #define CK(call)                                                                      \
