@@ -361,9 +361,19 @@ def run_mixed(args, w):
         # the implicit diffusion: the tracer step with minus without it.  With
         # vdiff_merge (T, S and up to 12 FP32 tracers) it is one launch of
         # k_vdiff_mix for every tracer; otherwise the FP32 tracers' k_vdiff_inc.
-        merged = g.get_tuning("vdiff_merge") == 1 and n32 <= 12 and n64 <= 2
+        twin = g.get_tuning("vdiff_solver") == 1 and 2 <= nk <= 80
+        merged = not twin and g.get_tuning("vdiff_merge") == 1 and n32 <= 12 and n64 <= 2
         pf = "true" if g.get_tuning("vdiff_prefetch") else "false"
-        if merged:
+        if twin:
+            idx = [n for n in range(ntr) if not is64[n]]
+            h = -(-nk // 2)
+            h = next(v for v in (8, 16, 24, 32, 40) if v >= h)
+            key = f"k_vdiff_twin<{h}>"
+            what = ("implicit vertical diffusion of the FP32 tracers, two wavefronts per "
+                    "column, levels in registers")
+            vbytes = vdiff_kernel_bytes(ni, nj, nk, 4, min(n32, 12))
+            nvd = -(-n32 // 12)             # solver groups of up to 12 tracers
+        elif merged:
             idx = list(range(ntr))
             nb = 4 if n32 <= 4 else 8 if n32 <= 8 else 10 if n32 <= 10 else 12
             key = f"k_vdiff_mix<{pf}, {nb}>"
@@ -481,7 +491,7 @@ def _tune_summary(rep):
 TUNING_KEYS = ("barotropic_subcycles_per_launch", "mdfl_rows_per_wave", "step_group_f64",
                "step_group_f32", "step_rows_per_wave", "step_rows_per_wave_f32",
                "step_waves_f32", "step_waves_f64", "vdiff_prefetch", "vdiff_block",
-               "vdiff_merge")
+               "vdiff_merge", "vdiff_solver")
 
 
 def run_plain(args, w):

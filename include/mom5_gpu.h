@@ -111,7 +111,8 @@ int mom_gpu_fill(mom_gpu_ctx_t *ctx, double *ptr, double value, size_t n);
 int mom_gpu_copy(mom_gpu_ctx_t *ctx, double *dst, const double *src, size_t n);
 
 /* Launch-shape knobs, set per context (the autotuner, mom5_amd/tune.py,
- * drives these; results do not depend on them, bit for bit).  Knobs:
+ * drives these; results do not depend on them, bit for bit, except FP32
+ * results on vdiff_solver).  Knobs:
  *   "barotropic_subcycles_per_launch"  0: three launches per subcycle;
  *                                      n>0: n subcycles per launch (1..9)
  *   "mdfl_rows_per_wave"               2 or 4: tile height of the MDFL kernel
@@ -138,8 +139,22 @@ int mom_gpu_copy(mom_gpu_ctx_t *ctx, double *dst, const double *src, size_t n);
  *                                      in one launch (up to 12 FP32 tracers of
  *                                      mixing class 1 and two FP64 tracers of
  *                                      each class; other mixes: per group)
+ *   "vdiff_solver"                     0 or 1: the FP32 tracers' implicit
+ *                                      diffusion.  0: one lane per column, the
+ *                                      forward sweep through a scratch plane
+ *                                      per tracer; 1: two wavefronts per 64
+ *                                      columns, every level in registers (the
+ *                                      two halves of a column eliminated from
+ *                                      both ends), no scratch (2 <= nk <= 80;
+ *                                      other nk use 0).  The two round
+ *                                      differently: FP32 results depend on
+ *                                      this knob, within the same error bound;
+ *                                      FP64 results never do.  With 1,
+ *                                      vdiff_prefetch, vdiff_block and
+ *                                      vdiff_merge have no effect.  Single
+ *                                      contexts only (ensembles use 0).
  * Defaults (the fastest measured at 3600x2700x75 with T, S and ten FP32
- * tracers): 4, 4, 2, 10, 2, 1, 16, 12, 1, 256, 0 in the order above.  Unknown knobs and
+ * tracers): 4, 4, 2, 10, 2, 1, 16, 12, 1, 256, 0, 1 in the order above.  Unknown knobs and
  * out-of-range values return MOM_GPU_ERR_ARG. */
 int mom_gpu_set_tuning(mom_gpu_ctx_t *ctx, const char *knob, int value);
 int mom_gpu_get_tuning(const mom_gpu_ctx_t *ctx, const char *knob, int *value);

@@ -378,6 +378,207 @@ __global__ __launch_bounds__(256) void k_vdiff_inc(VdJobF J) {
   vdiff_inc_col<NTF, PF>(J, gi, blockIdx.y);
 }
 
+// FP32 tracers, every level of a column in registers: two wavefronts per 64
+// columns.  The block's first wavefront holds the top nt = ceil(nk/2) levels
+// of each column and eliminates downward (Thomas); the second holds the other
+// nb = nk - nt levels and eliminates upward, which is the same recurrence with
+// the sub- and super-diagonal swapped.  The two partial eliminations meet at
+// the interface between levels nt and nt + 1, where
+//     x_top = d_top - E_top x_bot,   x_bot = d_bot - E_bot x_top
+// is solved with the other wavefront's (d, E) of the column (through LDS);
+// each wavefront then back-substitutes over its own levels.
+//
+// A wavefront has H register slots; its nact levels fill slots ph..H-1
+// (ph = H - nact), so both meet in slot H-1: slot ph + r is level r + 1
+// (top) or nk - r (bottom), the same plane for all lanes (a scalar base and
+// one lane offset), and slot H holds the other half's last level.  Slots
+// below ph are inert rows (P = E = 0, no loads or stores) that precede the
+// surface or bottom row, whose coupling to them is zero.
+//
+// The system is invtri's (ocean_util.F90:175-236) in increment form, as in
+// vdiff_inc_col: x = T(taup1) - T.  Coefficients and the factorisation are
+// formed in FP64 (the same expressions), kept as FP32 multipliers
+// P = p bet, E = q bet (p couples a row to the previous slot, q to the next);
+// each tracer's sweeps run in FP32.  Every tracer is read once and written
+// once; there is no scratch plane.
+// Wavefront-uniform base + 32-bit lane byte offset.  The base is pinned in
+// scalar registers, so that each access is one global load/store of the
+// scalar-base + vector-offset form instead of a per-slot 64-bit vector
+// address that the compiler would otherwise form (and keep) for each slot.
+#define MOM_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ T ld_at(const T *base, unsigned byte_off) {
+  asm volatile("" : "+s"(base));
+  return *(const MOM_GLOBAL T *)((const MOM_GLOBAL char *)base + byte_off);
+}
+template <typename T>
+__device__ __forceinline__ void st_at(T *base, unsigned byte_off, T v) {
+  asm volatile("" : "+s"(base));
+  *(MOM_GLOBAL T *)((MOM_GLOBAL char *)base + byte_off) = v;
+}
+// the LDS exchange of the two wavefronts: a barrier that orders LDS only, so
+// that loads in flight (the next tracer) are not waited for
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template <int H>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
+void k_vdiff_twin(VdJobF J) {
+  __shared__ float xch[2][2][64];            // [tracer parity][half][column]
+  __shared__ double ech[2][64];
+  const KDom d = J.d;
+  const int half = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const bool top = half == 0;
+  const int lane = threadIdx.x & 63;
+  const int col = blockIdx.x * 64 + lane;
+  const bool valid = col < d.ni;
+  const int nk = d.nk;
+  const int nt = (nk + 1) >> 1;
+  const int nact = top ? nt : nk - nt;
+  const int ph = H - nact;
+  const unsigned o = (unsigned)d2(d, valid ? col : d.ni - 1, blockIdx.y);
+  const unsigned ob4 = o * 4u, ob8 = o * 8u;  // lane byte offsets in a plane
+  const long sP = (long)d.P2;
+  const long step = top ? sP : -sP;           // plane stride from slot to slot
+  const long p0 = top ? 0 : (long)(nk - 1) * sP;   // plane of slot ph
+  const int kmo = ld_at(J.kmt, ob4);
+  const int kb = max(2, kmo);
+  const double tda = J.tdt * J.aidif;
+  // slot ph + r: dh plane = its level's; the next-side interface (between
+  // its level and the next slot's): diff_cbt plane = upper level - 1,
+  // dzwt plane = the interface index.  Inert slots read slot ph's planes.
+  // The loads of a chunk of CH slots are issued before the previous chunk
+  // is computed (the empty asm keeps later chunks from being hoisted).
+  constexpr int CH = 8, NCH = (H + CH - 1) / CH;
+  const double *dhp = J.dh + p0;
+  const double *dcp = J.dcbase + p0 - (top ? 0 : sP);
+  const double *dwp = J.dhw + p0 + (top ? sP : 0);
+  double Ldh[NCH * CH], Ldc[NCH * CH], Ldw[NCH * CH];
+  auto cload = [&](int c) {
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const int j = c * CH + u;
+      const long r = max(j - ph, 0);
+      Ldh[j] = ld_at(dhp + r * step, ob8);
+      Ldc[j] = ld_at(dcp + r * step, ob8);
+      Ldw[j] = ld_at(dwp + r * step, ob8);
+    }
+  };
+  float P[H], E[H];
+  double Dp = 0.0, Wp = 1.0, mdp = 0.0, Eprev = 0.0;
+  // bet and dh of the surface row (top) and of row kb (bottom flux)
+  double bet1 = 0.0, dh1 = 1.0, betkb = 0.0, dhkb = 1.0;
+  int jkb = -1;
+  cload(0);
+#pragma unroll
+  for (int c = 0; c < NCH; ++c) {
+    if (c + 1 < NCH) cload(c + 1);
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const int j = c * CH + u;
+      if (j >= H) break;
+      const bool real = j >= ph;              // uniform
+      const int r = j - ph;
+      const int lev = top ? r + 1 : nk - r;
+      const double dhk = Ldh[j], Dn = J.rho0 * Ldc[j], Wn = Ldw[j];
+      const double mkk = (real && lev <= kmo) ? 1.0 : 0.0;
+      // a coefficient is masked by the deeper of its two levels
+      const double mdn = real ? (top ? ((lev + 1 <= kmo) ? 1.0 : 0.0) : mkk) : 0.0;
+      const double p = (r <= 0) ? 0.0 : -Dp * (tda / (Wp * dhk)) * mdp;
+      const double q = -Dn * (tda / (Wn * dhk)) * mdn;
+      const double b = 1.0 - p - q;
+      const double bet = mkk / (b - p * Eprev + EPS_TRI);
+      Eprev = q * bet;
+      // the FP32 multipliers are formed here (not where they are used), so
+      // that only they, not their FP64 sources, stay in registers
+      float pj = (float)(p * bet), ej = (float)Eprev;
+      asm volatile("" : "+v"(pj), "+v"(ej));
+      P[j] = pj;
+      E[j] = ej;
+      if (real && lev == 1) { bet1 = bet; dh1 = dhk; }
+      const bool atkb = real && lev == kb;
+      betkb = atkb ? bet : betkb;
+      dhkb = atkb ? dhk : dhkb;
+      jkb = atkb ? j : jkb;
+      Dp = Dn; Wp = Wn; mdp = mdn;
+      // one slot at a time: the divisions of later slots are not hoisted
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  // bfl of invtri's rows 1 and kb, times bet (mask: bet is 0 on dry rows)
+  const float sstf = (float)(bet1 * tda / dh1);
+  const float sbtf = (float)(-(betkb * tda / dhkb));
+  ech[half][lane] = Eprev;
+  lds_barrier();
+  const double Eo = ech[1 - half][lane];
+  const float Es = (float)Eprev;
+  const float den = (float)(1.0 / (1.0 - Eprev * Eo));
+  // wet slots: top j < ph + kmo, bottom j >= ph + nk - kmo
+  const int wlo = top ? 0 : ph + nk - kmo, whi = top ? ph + kmo : H;
+  const int js = top ? ph : -1;               // the surface row
+
+  float T[H + 1], Tn[H + 1];
+  double stf = 0.0, btf = 0.0, stfn = 0.0, btfn = 0.0;
+  // branch-free: inert slots read slot ph's level again (their value is
+  // not used; P = E = 0 there)
+  auto load = [&](int n, int phl, float *dst, double &s, double &bt) {
+    const float *zp = J.g.z[n] + p0;
+#pragma unroll
+    for (int j = 0; j <= H; ++j) {
+      dst[j] = ld_at(zp, ob4);
+      zp += (j >= phl) ? step : 0;
+    }
+    s = ld_at(J.g.stf[n], ob8);
+    bt = ld_at(J.g.btf[n], ob8);
+  };
+  if (J.g.n > 0) load(0, ph, T, stf, btf);
+#pragma nounroll
+  for (int n = 0; n < J.g.n; ++n) {
+    // slot bounds and thresholds opaque to the compiler, so that the
+    // per-slot compares and addresses are formed in the loop instead of
+    // being kept in registers across it
+    int phl = ph, jsl = js, jkb_l = jkb, wlo_l = wlo, whi_l = whi;
+    asm volatile("" : "+s"(phl), "+s"(jsl), "+v"(jkb_l), "+v"(wlo_l), "+v"(whi_l));
+    if (n + 1 < J.g.n) load(n + 1, phl, Tn, stfn, btfn);
+    const float inj0 = (float)stf * sstf, injb = (float)btf * sbtf;
+    float dv[H];
+    float dprev = 0.0f;
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+      const float t1 = (j > 0 ? T[j - 1] - T[j] : 0.0f) + dprev;
+      const float t2 = T[j + 1] - T[j];
+      float dj = fmaf(-P[j], t1, -(E[j] * t2));
+      dj += (j == jsl) ? inj0 : 0.0f;
+      dj += (j == jkb_l) ? injb : 0.0f;
+      dv[j] = dj;
+      dprev = dj;
+    }
+    xch[n & 1][half][lane] = dprev;
+    lds_barrier();
+    const float dother = xch[n & 1][1 - half][lane];
+    float x = (dprev - Es * dother) * den;
+    float *zp = J.g.z[n] + p0 + (long)(nact - 1) * step;
+#pragma unroll
+    for (int j = H - 1; j >= 0; --j) {
+      if (j < H - 1) x = fmaf(-E[j], x, dv[j]);   // slot H-1 keeps the meeting value
+      if (j >= phl) {
+        const bool wet = j >= wlo_l && j < whi_l;
+        if (valid) st_at(zp, ob4, wet ? T[j] + x : 0.0f);
+        zp -= step;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j <= H; ++j) T[j] = Tn[j];
+    stf = stfn;
+    btf = btfn;
+  }
+}
+
 // Every tracer of a mixed step in one launch: FP32 tracers (mixing class 1,
 // increment form as k_vdiff_inc) and up to two FP64 tracers of each mixing
 // class (the full form of vdiff_col, bit for bit).  dh, dhw and the class-1
@@ -1001,11 +1202,32 @@ static int vdiff_mix_one(mom_gpu_ctx *ctx, int ntr, void *const *T, const int *d
   return MOM_GPU_OK;
 }
 
+// k_vdiff_twin register slots per wavefront for nk levels, H >= ceil(nk/2)
+// (0: nk outside 2..80, where k_vdiff_inc serves).
+static int twin_slots(int nk) {
+  if (nk < 2 || nk > 80) return 0;
+  const int h = (nk + 1) / 2;
+  return h <= 8 ? 8 : h <= 16 ? 16 : h <= 24 ? 24 : h <= 32 ? 32 : 40;
+}
+
+static void launch_twin(mom_gpu_ctx *ctx, const VdJobF &J) {
+  const KDom &d = J.d;
+  const dim3 grid((d.ni + 63) / 64, d.nj), block(128);
+  switch (twin_slots(d.nk)) {
+    case 8: hipLaunchKernelGGL(k_vdiff_twin<8>, grid, block, 0, ctx->stream, J); break;
+    case 16: hipLaunchKernelGGL(k_vdiff_twin<16>, grid, block, 0, ctx->stream, J); break;
+    case 24: hipLaunchKernelGGL(k_vdiff_twin<24>, grid, block, 0, ctx->stream, J); break;
+    case 32: hipLaunchKernelGGL(k_vdiff_twin<32>, grid, block, 0, ctx->stream, J); break;
+    default: hipLaunchKernelGGL(k_vdiff_twin<40>, grid, block, 0, ctx->stream, J); break;
+  }
+}
+
 int launch_vdiff_mixed(mom_gpu_ctx *ctx, int ntr, void *const *T, const int *dtype,
                        const double *const *stf, const double *const *btf,
                        const double *diff_cbt, int index_salt, double rho0, const double *dh,
                        const double *dhw, double dt, double aidif) {
-  if (ctx->tune.vdiff_merge) {
+  const bool twin = ctx->tune.vdiff_solver == 1 && twin_slots(ctx->kd.nk) > 0;
+  if (ctx->tune.vdiff_merge && !twin) {
     bool done = false;
     const int rc0 = vdiff_mix_one(ctx, ntr, T, dtype, stf, btf, diff_cbt, index_salt, rho0, dh,
                                   dhw, dt, aidif, done);
@@ -1019,9 +1241,11 @@ int launch_vdiff_mixed(mom_gpu_ctx *ctx, int ntr, void *const *T, const int *dty
   bool any32 = false;
   for (int n = 0; n < ntr; ++n) any32 = any32 || dtype[n] == 4;
   if (!any32) return MOM_GPU_OK;
-  rc = ensure_vd_scr(ctx, ctx, ctx->stream, fp32_group_planes(ntr, dtype, index_salt),
-                     "step_tracers");
-  if (rc) return rc;
+  if (!twin) {
+    rc = ensure_vd_scr(ctx, ctx, ctx->stream, fp32_group_planes(ntr, dtype, index_salt),
+                       "step_tracers");
+    if (rc) return rc;
+  }
   for (int nmix = 1; nmix <= 2; ++nmix) {
     TriGroupF g{};
     auto flush = [&]() {
@@ -1029,7 +1253,9 @@ int launch_vdiff_mixed(mom_gpu_ctx *ctx, int ntr, void *const *T, const int *dty
       VdJobF J{d, g, diff_cbt + p3 * (nmix - 1), rho0, ctx->kmt, dh, dhw, dt, aidif,
                ctx->tri_e, ctx->vd_scr, 0, 0};
       const int bx = ctx->tune.vdiff_block;
-      if (ctx->tune.vdiff_prefetch)
+      if (twin)
+        launch_twin(ctx, J);
+      else if (ctx->tune.vdiff_prefetch)
         hipLaunchKernelGGL(k_vdiff_inc<true>, dim3((d.ni + bx - 1) / bx, d.nj), dim3(bx), 0,
                            ctx->stream, J);
       else
@@ -1044,7 +1270,7 @@ int launch_vdiff_mixed(mom_gpu_ctx *ctx, int ntr, void *const *T, const int *dty
     }
     flush();
   }
-  MOM_LAUNCHED(ctx, "k_vdiff_inc");
+  MOM_LAUNCHED(ctx, twin ? "k_vdiff_twin" : "k_vdiff_inc");
   return MOM_GPU_OK;
 }
 

@@ -120,6 +120,10 @@ static int *tuning_slot(mom_gpu_ctx *ctx, const char *knob, int *lo, int *hi) {
     *lo = 0; *hi = 1;
     return &ctx->tune.vdiff_merge;
   }
+  if (!std::strcmp(knob, "vdiff_solver")) {
+    *lo = 0; *hi = 1;
+    return &ctx->tune.vdiff_solver;
+  }
   return nullptr;
 }
 
@@ -323,6 +327,7 @@ int mom_gpu_create(const mom_gpu_domain_t *dom, const mom_gpu_grid_t *grid, mom_
   c->tune.vdiff_prefetch = 1;
   c->tune.vdiff_block = 256;
   c->tune.vdiff_merge = 0;
+  c->tune.vdiff_solver = 1;
   const size_t P2 = (size_t)k.P2, nk = (size_t)k.nk;
 
 #define CK(call)                                                                      \

@@ -112,6 +112,7 @@ struct mom_gpu_ctx {
     int vdiff_prefetch;            // FP32 implicit diffusion: loads one level ahead (0, 1)
     int vdiff_block;               // FP32 implicit diffusion: threads a block (64, 256)
     int vdiff_merge;               // mixed steps: all tracers in one diffusion launch (0, 1)
+    int vdiff_solver;              // FP32 diffusion: 0 lane per column + scratch, 1 registers
   } tune;
   char err[512];
 };

@@ -74,9 +74,13 @@ def autotune(step, reps=3, barotropic=True):
             # tracer step (tuned advection passes) per candidate
             a = step.step_args()
             src_all, dst_all = a.pop("T_taum1s"), a.pop("T_taup1s")
+            # the register-resident solver (vdiff_solver 1) ignores the
+            # other three knobs
             report["vdiff"] = _search(
-                g, ("vdiff_prefetch", "vdiff_block", "vdiff_merge"),
-                [(pf, bx, mg) for mg in (0, 1) for pf in (0, 1) for bx in (64, 256)],
+                g, ("vdiff_prefetch", "vdiff_block", "vdiff_merge", "vdiff_solver"),
+                [(pf, bx, mg, 0) for mg in (0, 1) for pf in (0, 1) for bx in (64, 256)] +
+                [(g.get_tuning("vdiff_prefetch"), g.get_tuning("vdiff_block"),
+                  g.get_tuning("vdiff_merge"), 1)],
                 lambda: g.step_tracers(src_all, dst_all, **a), reps)
     else:
         th = step.th

@@ -124,6 +124,7 @@ def test_fp32_results_independent_of_launch_shape(setup):
     case, orc, gpu, dev = setup
     Ts = [t for _, t in case.tracers]
     dtypes = [np.float64, np.float64] + [np.float32] * (len(Ts) - 2)
+    gpu.set_tuning("vdiff_solver", 0)
     base = None
     for group, rows, waves, pf in [(1, 2, 8, 0), (2, 2, 8, 1), (4, 2, 8, 0), (1, 4, 8, 1),
                                    (2, 4, 8, 0), (4, 4, 8, 1), (1, 2, 16, 0), (2, 2, 16, 1),
@@ -140,7 +141,61 @@ def test_fp32_results_independent_of_launch_shape(setup):
             base = got
         for n in range(len(Ts)):
             assert np.array_equal(got[n], base[n]), (group, rows, waves, pf, n)
+    # the register-resident solver: its own bits, the same for every shape
+    gpu.set_tuning("vdiff_solver", 1)
+    base1 = None
+    for group, rows, waves in [(1, 2, 8), (4, 2, 16), (2, 1, 16), (4, 1, 16), (5, 2, 12)]:
+        gpu.set_tuning("step_group_f32", group)
+        gpu.set_tuning("step_rows_per_wave_f32", rows)
+        gpu.set_tuning("step_waves_f32", waves)
+        got = [g[:, 1:-1, 1:-1] for g in _run_step(case, gpu, dev, Ts, dtypes)]
+        if base1 is None:
+            base1 = got
+        for n in range(len(Ts)):
+            assert np.array_equal(got[n], base1[n]), (group, rows, waves, n)
+    for n in range(2):                 # FP64 tracers do not depend on it
+        assert np.array_equal(base1[n], base[n])
+    gpu.set_tuning("vdiff_solver", 0)
     gpu.set_tuning("step_waves_f32", 8)
+
+
+@pytest.mark.parametrize("nk,mix", [(2, 1.0), (3, 1.0), (9, 1.0), (16, 1.0), (17, 1.0), (33, 1.0),
+                                    (64, 1.0), (75, 1.0), (75, 100.0), (80, 1.0), (81, 1.0)])
+def test_fp32_twin_solver(oracle_lib, nk, mix):
+    """vdiff_solver 1 (two wavefronts per column, levels in registers) at
+    every register-slot size, odd and even nk (an inert slot on one half),
+    shallow and full columns, weak and strong mixing (diff_cbt x 100: up to
+    1 m2/s); nk = 81 is beyond the register solver and takes solver 0.  FP32
+    tracers within the FP32 bound of the FP64 restatement, FP64 tracers bit
+    for bit, and within the bound of solver 0's results."""
+    from mom5_amd import DeviceArray, OceanGPU
+    from mom5_amd.synth import make_case
+    case = make_case(70, 18, nk, land_frac=0.3, nts=8, seed=100 + nk, n_passive=5,
+                     stf_scale=1e-3)
+    case.fields["diff_cbt"] = case.fields["diff_cbt"] * mix
+    print(f"[twin solver 70x18x{nk} mix x{mix}]", flush=True)
+    gpu = OceanGPU(case.dom, case.grid)
+    dev = {k: DeviceArray.from_host(v) for k, v in case.fields.items()
+           if isinstance(v, np.ndarray)}
+    Ts = [t for _, t in case.tracers]
+    dtypes = [np.float64, np.float64] + [np.float32] * (len(Ts) - 2)
+    got = {}
+    for solver in (0, 1):
+        gpu.set_tuning("vdiff_solver", solver)
+        got[solver] = [g[:, 1:-1, 1:-1] for g in _run_step(case, gpu, dev, Ts, dtypes)]
+    gpu.close()
+    ref = [r[:, 1:-1, 1:-1] for r in _oracle_step(case, oracle_lib.Oracle(case), Ts)]
+    for n in range(2):
+        assert np.array_equal(got[1][n], ref[n]), n
+    for n in range(2, len(Ts)):
+        scale = np.abs(ref[n]).max()
+        e1 = np.abs(got[1][n].astype(np.float64) - ref[n]).max() / scale
+        e0 = np.abs(got[0][n].astype(np.float64) - ref[n]).max() / scale
+        d10 = np.abs(got[1][n].astype(np.float64) - got[0][n]).max() / scale
+        print(f"tracer {n}: solver 1 {e1:.2e}, solver 0 {e0:.2e}, |1 - 0| {d10:.2e}")
+        assert e1 <= 1e-5 and d10 <= 2e-5, (n, e1, d10)
+        if nk > 80:
+            assert np.array_equal(got[1][n], got[0][n])
 
 
 @pytest.mark.parametrize("cyclic", [True, False])
