@@ -366,9 +366,14 @@ __device__ __forceinline__ void bt_tile(const BtJob &J, int tile) {
   }
 }
 
-__global__ __launch_bounds__(BNT) void k_bt_block(BtJob J) {
-  if ((int)blockIdx.x >= J.ntiles) return;
-  bt_tile(J, blockIdx.x);
+// Tiles in XCD-aware order: blocks are dealt to the 8 XCDs round-robin, so
+// each XCD takes per_xcd consecutive tiles and the halo rows that x-neighbours
+// both read are fetched into one L2.
+__global__ __launch_bounds__(BNT) void k_bt_block(BtJob J, int per_xcd) {
+  const int b = blockIdx.x;
+  const int tile = (b & 7) * per_xcd + (b >> 3);
+  if (tile >= J.ntiles) return;
+  bt_tile(J, tile);
 }
 
 // Ensemble: launch ell of every domain that still has subcycles; the static
@@ -680,7 +685,8 @@ int launch_barotropic_blocked(mom_gpu_ctx *ctx, const mom_gpu_bt_params_t *p,
     const int ntx = (xcount + TX - 1) / TX, nty = (d.NJ + TY - 1) / TY;
     const BtJob J{d, in, eta_src, ud_src, eta_buf[which], ud_buf[which], eta_bar, ud, s0, nsub,
                   p->nts, p->stepping_a, p->dtbt, c_pred, c_corr, xbase, xcount, ntx, ntx * nty};
-    hipLaunchKernelGGL(k_bt_block, dim3(ntx * nty), dim3(BNT), 0, s, J);
+    const int per_xcd = (ntx * nty + 7) / 8;
+    hipLaunchKernelGGL(k_bt_block, dim3(8 * per_xcd), dim3(BNT), 0, s, J, per_xcd);
     MOM_LAUNCHED(ctx, "k_bt_block");
     eta_src = eta_buf[which];
     ud_src = ud_buf[which];

@@ -180,11 +180,16 @@ __device__ __forceinline__ void vdiff_col(const VdJob<TT> &J, int gi, int gj) {
   }
 }
 
+// 64-column tiles in XCD-aware order (see k_vdiff_twin): J.nbx tiles a row,
+// J.nblocks in all, per_xcd consecutive tiles per XCD.
 template <typename TT>
-__global__ void k_vdiff(VdJob<TT> J) {
-  const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_vdiff(VdJob<TT> J, int per_xcd) {
+  const int b = blockIdx.x;
+  const int tile = (b & 7) * per_xcd + (b >> 3);
+  if (tile >= J.nblocks) return;
+  const int gi = (tile % J.nbx) * 64 + threadIdx.x;
   if (gi >= J.d.ni) return;
-  vdiff_col(J, gi, blockIdx.y);
+  vdiff_col(J, gi, tile / J.nbx);
 }
 
 // FP32 storage: the same tridiagonal system solved for the increment,
@@ -424,22 +429,23 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// One 64-column tile (J.nbx tiles a row) of job J; every thread of the
+// block calls it (it holds two block barriers per tracer).
 template <int H>
-__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
-void k_vdiff_twin(VdJobF J) {
+__device__ __forceinline__ void twin_tile(const VdJobF &J, int tile) {
   __shared__ float xch[2][2][64];            // [tracer parity][half][column]
   __shared__ double ech[2][64];
   const KDom d = J.d;
   const int half = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const bool top = half == 0;
   const int lane = threadIdx.x & 63;
-  const int col = blockIdx.x * 64 + lane;
+  const int col = (tile % J.nbx) * 64 + lane;
   const bool valid = col < d.ni;
   const int nk = d.nk;
   const int nt = (nk + 1) >> 1;
   const int nact = top ? nt : nk - nt;
   const int ph = H - nact;
-  const unsigned o = (unsigned)d2(d, valid ? col : d.ni - 1, blockIdx.y);
+  const unsigned o = (unsigned)d2(d, valid ? col : d.ni - 1, tile / J.nbx);
   const unsigned ob4 = o * 4u, ob8 = o * 8u;  // lane byte offsets in a plane
   const long sP = (long)d.P2;
   const long step = top ? sP : -sP;           // plane stride from slot to slot
@@ -485,7 +491,11 @@ void k_vdiff_twin(VdJobF J) {
       const bool real = j >= ph;              // uniform
       const int r = j - ph;
       const int lev = top ? r + 1 : nk - r;
-      const double dhk = Ldh[j], Dn = J.rho0 * Ldc[j], Wn = Ldw[j];
+      double dhk = Ldh[j], Wn = Ldw[j];
+      // tie this slot's divisions to the previous slot's result: computed
+      // one slot at a time, they are not all held in registers at once
+      asm volatile("" : "+v"(dhk), "+v"(Wn) : "v"(Eprev));
+      const double Dn = J.rho0 * Ldc[j];
       const double mkk = (real && lev <= kmo) ? 1.0 : 0.0;
       // a coefficient is masked by the deeper of its two levels
       const double mdn = real ? (top ? ((lev + 1 <= kmo) ? 1.0 : 0.0) : mkk) : 0.0;
@@ -577,6 +587,18 @@ void k_vdiff_twin(VdJobF J) {
     stf = stfn;
     btf = btfn;
   }
+}
+
+// Blocks are dealt to the 8 XCDs round-robin; each XCD takes a contiguous
+// run of 64-column tiles, so the cache lines that two neighbouring tiles
+// share (rows are not line-aligned) are fetched into one L2, once.
+template <int H>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
+void k_vdiff_twin(VdJobF J, int per_xcd) {
+  const int b = blockIdx.x;
+  const int tile = (b & 7) * per_xcd + (b >> 3);
+  if (tile >= J.nblocks) return;             // the whole block
+  twin_tile<H>(J, tile);
 }
 
 // Every tracer of a mixed step in one launch: FP32 tracers (mixing class 1,
@@ -809,6 +831,24 @@ __global__ __launch_bounds__(64) void k_vdiff_inc_ens(const VdJobF *__restrict__
   int gi, gj;
   const VdJobF *J = vd_pick(jobs, L, gi, gj);
   if (J) vdiff_inc_col<NTF_ENS, false>(*J, gi, gj);
+}
+
+// Ensemble, register-resident FP32 solver: every group of every domain in one
+// launch (no scratch is shared); tiles in XCD-aware order over the whole
+// table, H the largest slot count the domains need.
+template <int H>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
+void k_vdiff_twin_ens(const VdJobF *__restrict__ jobs, VdLaunch L, int per_xcd) {
+  const int b = blockIdx.x;
+  const int t = (b & 7) * per_xcd + (b >> 3);
+  if (t >= L.first[L.njobs]) return;         // the whole block
+  int lo = 0, hi = L.njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (L.first[mid] <= t) lo = mid; else hi = mid - 1;
+  }
+  lo = __builtin_amdgcn_readfirstlane(lo);
+  twin_tile<H>(jobs[lo], t - L.first[lo]);
 }
 
 __global__ void k_set_ints(int *__restrict__ p, int v, int n) {
@@ -1067,9 +1107,10 @@ static int vdiff_groups(mom_gpu_ctx *ctx, int ntr, void *const *T, const int *dt
     auto flush = [&](const double *k33) -> int {
       if (g.n == 0) return MOM_GPU_OK;
       g.k33 = k33;
+      const int nbx = (d.ni + 63) / 64, per_xcd = (nbx * d.nj + 7) / 8;
       VdJob<TT> J{d, g, diff_cbt + p3 * (nmix - 1), rho0, ctx->kmt, dh, dhw, dt, aidif,
-                  ctx->tri_e, 0, 0};
-      hipLaunchKernelGGL(k_vdiff<TT>, dim3((d.ni + 63) / 64, d.nj), dim3(64), 0, ctx->stream, J);
+                  ctx->tri_e, nbx, nbx * d.nj};
+      hipLaunchKernelGGL(k_vdiff<TT>, dim3(8 * per_xcd), dim3(64), 0, ctx->stream, J, per_xcd);
       MOM_LAUNCHED(ctx, "k_vdiff");
       g.n = 0;
       return MOM_GPU_OK;
@@ -1210,15 +1251,18 @@ static int twin_slots(int nk) {
   return h <= 8 ? 8 : h <= 16 ? 16 : h <= 24 ? 24 : h <= 32 ? 32 : 40;
 }
 
-static void launch_twin(mom_gpu_ctx *ctx, const VdJobF &J) {
+static void launch_twin(mom_gpu_ctx *ctx, VdJobF J) {
   const KDom &d = J.d;
-  const dim3 grid((d.ni + 63) / 64, d.nj), block(128);
+  J.nbx = (d.ni + 63) / 64;
+  J.nblocks = J.nbx * d.nj;
+  const int per_xcd = (J.nblocks + 7) / 8;
+  const dim3 grid(8 * per_xcd), block(128);
   switch (twin_slots(d.nk)) {
-    case 8: hipLaunchKernelGGL(k_vdiff_twin<8>, grid, block, 0, ctx->stream, J); break;
-    case 16: hipLaunchKernelGGL(k_vdiff_twin<16>, grid, block, 0, ctx->stream, J); break;
-    case 24: hipLaunchKernelGGL(k_vdiff_twin<24>, grid, block, 0, ctx->stream, J); break;
-    case 32: hipLaunchKernelGGL(k_vdiff_twin<32>, grid, block, 0, ctx->stream, J); break;
-    default: hipLaunchKernelGGL(k_vdiff_twin<40>, grid, block, 0, ctx->stream, J); break;
+    case 8: hipLaunchKernelGGL(k_vdiff_twin<8>, grid, block, 0, ctx->stream, J, per_xcd); break;
+    case 16: hipLaunchKernelGGL(k_vdiff_twin<16>, grid, block, 0, ctx->stream, J, per_xcd); break;
+    case 24: hipLaunchKernelGGL(k_vdiff_twin<24>, grid, block, 0, ctx->stream, J, per_xcd); break;
+    case 32: hipLaunchKernelGGL(k_vdiff_twin<32>, grid, block, 0, ctx->stream, J, per_xcd); break;
+    default: hipLaunchKernelGGL(k_vdiff_twin<40>, grid, block, 0, ctx->stream, J, per_xcd); break;
   }
 }
 
@@ -1354,7 +1398,7 @@ int launch_check_masks(mom_gpu_ctx *ctx, const double *tmask, const double *umas
   int *host = (int *)malloc(n * sizeof(int));
   hipError_t e1 = hipMemcpyAsync(host, dev, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
   hipError_t e2 = hipStreamSynchronize(ctx->stream);
-  hipFree(dev);
+  (void)hipFree(dev);
   if (le != hipSuccess || e1 != hipSuccess || e2 != hipSuccess) {
     free(host);
     return mom_fail(ctx, MOM_GPU_ERR_HIP, "mask check failed: %s",
@@ -1408,6 +1452,15 @@ static int vd_rounds(mom_gpu_ensemble *e, std::vector<std::vector<JOB>> &groups,
 int launch_vdiff_ensemble(mom_gpu_ensemble *e, const mom_gpu_tracer_step_t *steps) {
   const size_t nd = e->ctx.size();
   if (nd > (size_t)VD_MAXJ) return mom_fail(e->lead, MOM_GPU_ERR_ARG, "ensemble: too many domains");
+  // the lead context's vdiff_solver chooses for every domain; the register
+  // solver needs every domain's nk within its range
+  bool twin = e->lead->tune.vdiff_solver == 1;
+  int hmax = 0;
+  for (size_t dd = 0; dd < nd; ++dd) {
+    const int h = twin_slots(e->ctx[dd]->kd.nk);
+    twin = twin && h > 0;
+    hmax = std::max(hmax, h);
+  }
   std::vector<std::vector<VdJob<double>>> g64(nd);
   std::vector<std::vector<VdJobF>> g32(nd);
   for (size_t dd = 0; dd < nd; ++dd) {
@@ -1420,7 +1473,7 @@ int launch_vdiff_ensemble(mom_gpu_ensemble *e, const mom_gpu_tracer_step_t *step
     const int nbx = (d.ni + 63) / 64;
     bool any32 = false;
     for (int n = 0; n < s.ntr; ++n) any32 = any32 || s.dtype[n] == 4;
-    if (any32) {
+    if (any32 && !twin) {
       const int rc = ensure_vd_scr(ctx, e->lead, e->lead->stream,
                                    std::min(NTF_ENS, fp32_group_planes(s.ntr, s.dtype, s.index_salt)),
                                    "ensemble");
@@ -1457,7 +1510,7 @@ int launch_vdiff_ensemble(mom_gpu_ensemble *e, const mom_gpu_tracer_step_t *step
         } else {
           TriGroupF &g = j32.g;
           g.z[g.n] = (float *)s.T_taup1[n]; g.stf[g.n] = s.stf[n]; g.btf[g.n] = s.btf[n];
-          if (++g.n == NTF_ENS) flush32();
+          if (++g.n == (twin ? NTF : NTF_ENS)) flush32();
         }
       }
       flush64();
@@ -1466,5 +1519,33 @@ int launch_vdiff_ensemble(mom_gpu_ensemble *e, const mom_gpu_tracer_step_t *step
   }
   int rc = vd_rounds(e, g64, k_vdiff_ens, "k_vdiff_ens");
   if (rc) return rc;
-  return vd_rounds(e, g32, k_vdiff_inc_ens, "k_vdiff_inc_ens");
+  if (!twin) return vd_rounds(e, g32, k_vdiff_inc_ens, "k_vdiff_inc_ens");
+  // register-resident solver: all jobs at once, VD_MAXJ a launch
+  std::vector<VdJobF> all;
+  for (auto &g : g32) all.insert(all.end(), g.begin(), g.end());
+  for (size_t at = 0; at < all.size(); at += VD_MAXJ) {
+    const size_t nj = std::min((size_t)VD_MAXJ, all.size() - at);
+    VdLaunch L{};
+    int total = 0;
+    for (size_t j = 0; j < nj; ++j) {
+      L.first[j] = total;
+      total += all[at + j].nblocks;
+    }
+    L.first[nj] = total;
+    L.njobs = (int)nj;
+    const void *dev = ens_table(e, all.data() + at, sizeof(VdJobF) * nj);
+    if (!dev) return MOM_GPU_ERR_HIP;
+    const int per_xcd = (total + 7) / 8;
+    const dim3 grid(8 * per_xcd), block(128);
+    hipStream_t st = e->lead->stream;
+    switch (hmax) {
+      case 8: hipLaunchKernelGGL(k_vdiff_twin_ens<8>, grid, block, 0, st, (const VdJobF *)dev, L, per_xcd); break;
+      case 16: hipLaunchKernelGGL(k_vdiff_twin_ens<16>, grid, block, 0, st, (const VdJobF *)dev, L, per_xcd); break;
+      case 24: hipLaunchKernelGGL(k_vdiff_twin_ens<24>, grid, block, 0, st, (const VdJobF *)dev, L, per_xcd); break;
+      case 32: hipLaunchKernelGGL(k_vdiff_twin_ens<32>, grid, block, 0, st, (const VdJobF *)dev, L, per_xcd); break;
+      default: hipLaunchKernelGGL(k_vdiff_twin_ens<40>, grid, block, 0, st, (const VdJobF *)dev, L, per_xcd); break;
+    }
+    MOM_LAUNCHED(e->lead, "k_vdiff_twin_ens");
+  }
+  return MOM_GPU_OK;
 }
