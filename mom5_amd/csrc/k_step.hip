@@ -488,8 +488,8 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
         if constexpr (!EXACT) f64 = dtime * rr * mdatr[q] * mi;
 #pragma unroll
         for (int g = 0; g < G; ++g) {
-          const double thin = tab.th[g] ? tab.th[g][o] : 0.0;
           if constexpr (EXACT) {
+            const double thin = tab.th[g] ? tab.th[g][o] : 0.0;
             const double Tc = t1[g][q];
             double tr = txv[g][q] + xf64v[q] * (s_fy[g][ry - 1][lane] - fyv[g][q]);
             tr = tr + dtime * Tc / r * brk;
@@ -499,9 +499,10 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
           } else {
             const float Tc = t1[g][q];
             const float dflux = dxf[g][q] + (s_fy[g][ry - 1][lane] - s_fy[g][ry][lane]);
-            float tn = Tc * a32 + (float)(f64 * (double)dflux);
-            if (tab.th[g]) tn = tn + (float)thin * c32;
-            tab.out[g][o] = tn;
+            // other th_tendency contributions: k_add_th_f32 adds
+            // (float)th * c32 to this value afterwards, bit for bit the
+            // same as here, so that FP32 passes hold no th pointers
+            tab.out[g][o] = Tc * a32 + (float)(f64 * (double)dflux);
           }
         }
       }
@@ -657,6 +658,27 @@ static int with_shape(int g, Shape sh, F &&f) {
   }
 }
 
+// FP32 tracers with other th_tendency contributions: the last operation of
+// k_step's FP32 update, out = out + (float)th * (float)(dtime / rho(taup1)),
+// over every level of the compute domain, as a second pass.
+__global__ void k_add_th_f32(KDom d, float *__restrict__ out, const double *__restrict__ th,
+                             const double *__restrict__ R, const double *__restrict__ Rr,
+                             double dtime) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.ni) return;
+  const int o = d2(d, i, blockIdx.y) + d.P2 * blockIdx.z;
+  const double rr = Rr ? Rr[o] : 1.0 / (R[o] + EPSLN);
+  const float c32 = (float)(dtime * rr);
+  out[o] = out[o] + (float)th[o] * c32;
+}
+
+// (launch errors surface at the caller's next launch check)
+static void add_th_f32(hipStream_t s, const KDom &d, float *out, const double *th,
+                       const double *R, const double *Rr, double dtime) {
+  hipLaunchKernelGGL(k_add_th_f32, dim3((d.ni + 255) / 256, d.nj, d.nk), dim3(256), 0, s, d, out,
+                     th, R, Rr, dtime);
+}
+
 template <typename TT>
 struct GroupLaunch {
   mom_gpu_ctx *ctx;
@@ -697,9 +719,14 @@ static void launch_all(mom_gpu_ctx *ctx, int gmax, int rpw, int waves, int ntr,
       const int n = idx[i + q];
       t.T[q] = (const TT *)T_in[n];
       t.out[q] = (TT *)T_out[n];
-      t.th[q] = th_in ? th_in[n] : nullptr;
+      t.th[q] = (th_in && sizeof(TT) == 8) ? th_in[n] : nullptr;
     }
     with_shape<TT>(c, sh, GroupLaunch<TT>{ctx, t, u, v, w, rho, rm1, rr, dtime});
+    if (sizeof(TT) == 4 && th_in)
+      for (int q = 0; q < c; ++q)
+        if (th_in[idx[i + q]])
+          add_th_f32(ctx->stream, ctx->kd, (float *)T_out[idx[i + q]], th_in[idx[i + q]], rho,
+                     rr, dtime);
     i += c;
   }
 }
@@ -769,7 +796,7 @@ static int ensemble_type(mom_gpu_ensemble *e, const mom_gpu_tracer_step_t *steps
         const int n = idx[i + q];
         t.T[q] = (const TT *)s.T_taum1[n];
         t.out[q] = (TT *)s.T_taup1[n];
-        t.th[q] = s.th_tendency ? s.th_tendency[n] : nullptr;
+        t.th[q] = (s.th_tendency && sizeof(TT) == 8) ? s.th_tendency[n] : nullptr;
       }
       const int ntx = (d.ni + TX - 1) / TX;
       const int ntiles = ntx * ((d.nj + TY - 1) / TY);
@@ -789,6 +816,19 @@ static int ensemble_type(mom_gpu_ensemble *e, const mom_gpu_tracer_step_t *steps
   int rc = MOM_GPU_OK;
   for (const int *g = group_sizes<TT>(sh); *g && rc == MOM_GPU_OK; ++g)
     if (!by[*g].empty()) rc = with_shape<TT>(*g, sh, QueueLaunch<TT>{e, by[*g], slot++});
+  if (rc || sizeof(TT) == 8) return rc;
+  bool added = false;
+  for (size_t dd = 0; dd < e->ctx.size(); ++dd) {
+    const mom_gpu_tracer_step_t &s = steps[dd];
+    if (!s.th_tendency) continue;
+    for (int n = 0; n < s.ntr; ++n)
+      if (s.dtype[n] == want && s.th_tendency[n]) {
+        add_th_f32(e->lead->stream, e->ctx[dd]->kd, (float *)s.T_taup1[n], s.th_tendency[n],
+                   s.rho_dzt_tau, s.rho_dztr, s.dtime);
+        added = true;
+      }
+  }
+  if (added) MOM_LAUNCHED(e->lead, "k_add_th_f32");
   return rc;
 }
 }  // namespace

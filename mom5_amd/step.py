@@ -182,6 +182,8 @@ class BatchedStep(_Captured):
             self.eta_bar = [_copy(f["eta_t_bar"]), _zeros_like(self.dev["eta_t_bar"])]
             self.udrho = [_copy(f["udrho"]), _zeros_like(self.dev["udrho"])]
             self.conv = _zeros_like(self.dev["eta_t_bar"])
+        # other th_tendency contributions per tracer (device arrays), or None
+        self.th = None
         self.cur = 0
 
     def tracer(self, n):
@@ -197,7 +199,7 @@ class BatchedStep(_Captured):
                     rho_dzt_tau=d["rho_dzt"], rho_dztr=d["rho_dztr"], dtime=c.dt,
                     stfs=self.stf, btfs=self.btf, dzwt=d["dzwt"] if self.diffuse else None,
                     diff_cbt=d["diff_cbt"] if self.diffuse else None,
-                    index_salt=self.index_salt)
+                    index_salt=self.index_salt, th_tendencies=self.th)
 
     def bt_args(self):
         """Barotropic parameters and fields of this step at the current level."""

@@ -48,6 +48,13 @@ def test_ensemble_equals_members_alone_and_graph_replays():
     cases = make_regional_ensemble(SHAPES, n_passive=3)
     alone = _members(cases, fresh=True)
     ens_m = _members(cases, fresh=True)
+    # other th_tendency terms on two of the domains (FP64 and FP32 tracers)
+    from mom5_amd import DeviceArray
+    rng = np.random.default_rng(3)
+    for k in (1, 3):
+        ths = [DeviceArray.from_host(1e-3 * rng.standard_normal(alone[k].tracer(n).shape))
+               for n in range(alone[k].ntr)]
+        alone[k].th = ens_m[k].th = ths
     ens = EnsembleStep(ens_m)
     for m in alone:
         m.run(3)

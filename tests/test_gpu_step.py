@@ -117,6 +117,45 @@ def test_fp32_passive_error_vs_fp64(setup, group, rows):
         assert rms <= 1e-6
 
 
+def test_step_with_other_tendencies(setup):
+    """th_tendency inputs (the other tendency terms the step adds to the
+    advection): FP64 tracers bit for bit the CPU restatement of th +
+    advection + explicit update; FP32 tracers bit for bit the FP32 update's
+    definition, T(taup1) = [update without th] + th * (dtime / rho(taup1)),
+    each operand rounded to FP32 (the library applies that last addition in
+    a second pass), at every cell of the compute domain."""
+    from mom5_amd import DeviceArray
+    case, orc, gpu, dev = setup
+    f = case.fields
+    Ts = [t for _, t in case.tracers]
+    dtypes = [np.float64, np.float64] + [np.float32] * (len(Ts) - 2)
+    rng = np.random.default_rng(17)
+    ths = [1e-3 * rng.standard_normal(t.shape) for t in Ts]
+    kw = dict(uhrho_et=dev["uhrho_et"], vhrho_nt=dev["vhrho_nt"], wrho_bt=dev["wrho_bt"],
+              rho_dzt_tau=dev["rho_dzt"], dtime=case.dt, rho_dztr=dev["rho_dztr"])
+    outs = {}
+    for with_th in (False, True):
+        tin = [DeviceArray.from_host(t.astype(dt)) for t, dt in zip(Ts, dtypes)]
+        tout = [DeviceArray(t.shape, dt) for t, dt in zip(Ts, dtypes)]
+        for a in tout:
+            a.zero()
+        extra = dict(th_tendencies=[DeviceArray.from_host(t) for t in ths]) if with_th else {}
+        gpu.step_tracers(tin, tout, **kw, **extra)
+        gpu.sync()
+        outs[with_th] = [a.to_host()[:, 1:-1, 1:-1] for a in tout]
+    for n in range(2):
+        th = ths[n].copy()
+        orc.horz_advect_mdfl(Ts[n], th, case.dt, lim=1.0)
+        want = orc.update_explicit(Ts[n], th, case.dt)[:, 1:-1, 1:-1]
+        assert np.array_equal(outs[True][n].view(np.uint64), want.view(np.uint64)), n
+    c32 = (case.dt * f["rho_dztr"][:, 1:-1, 1:-1]).astype(np.float32)
+    for n in range(2, len(Ts)):
+        th32 = ths[n][:, 1:-1, 1:-1].astype(np.float32)
+        want = (outs[False][n] + th32 * c32).astype(np.float32)
+        assert np.array_equal(outs[True][n].view(np.uint32), want.view(np.uint32)), n
+        assert not np.array_equal(outs[True][n], outs[False][n])
+
+
 def test_fp32_results_independent_of_launch_shape(setup):
     """Grouping, tile height and the column solver's load schedule change
     which threads do the work and when, not the arithmetic of any tracer:
