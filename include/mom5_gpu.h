@@ -153,8 +153,19 @@ int mom_gpu_copy(mom_gpu_ctx_t *ctx, double *dst, const double *src, size_t n);
  *                                      vdiff_prefetch, vdiff_block and
  *                                      vdiff_merge have no effect.  Single
  *                                      contexts only (ensembles use 0).
+ *   "step_streams"                     1 or 2: with 2, mom_gpu_step_tracers
+ *                                      runs its FP32 tracers (advection,
+ *                                      update, diffusion) on a second stream
+ *                                      of the context, overlapping the FP64
+ *                                      tracers and the barotropic calls that
+ *                                      follow; needs the register-resident
+ *                                      FP32 solver when diffusing.  Every
+ *                                      other entry point except the
+ *                                      barotropic ones, mom_gpu_sync and
+ *                                      mom_gpu_mark first wait for that
+ *                                      stream, so results are the same.
  * Defaults (the fastest measured at 3600x2700x75 with T, S and ten FP32
- * tracers): 4, 4, 2, 10, 2, 1, 16, 12, 1, 256, 0, 1 in the order above.  Unknown knobs and
+ * tracers): 4, 4, 2, 10, 2, 1, 16, 12, 1, 256, 0, 1, 1 in the order above.  Unknown knobs and
  * out-of-range values return MOM_GPU_ERR_ARG. */
 int mom_gpu_set_tuning(mom_gpu_ctx_t *ctx, const char *knob, int value);
 int mom_gpu_get_tuning(const mom_gpu_ctx_t *ctx, const char *knob, int *value);

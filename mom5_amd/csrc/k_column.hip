@@ -1266,6 +1266,10 @@ static void launch_twin(mom_gpu_ctx *ctx, VdJobF J) {
   }
 }
 
+bool vdiff_fp32_scratch(const mom_gpu_ctx *c) {
+  return !(c->tune.vdiff_solver == 1 && twin_slots(c->kd.nk) > 0);
+}
+
 int launch_vdiff_mixed(mom_gpu_ctx *ctx, int ntr, void *const *T, const int *dtype,
                        const double *const *stf, const double *const *btf,
                        const double *diff_cbt, int index_salt, double rho0, const double *dh,

@@ -275,6 +275,7 @@ extern "C" {
 
 int mom_synth_tracer(mom_gpu_ctx_t *ctx, void *out, int dtype, int kind, unsigned long long seed,
                      const double *zt) {
+  if (ctx) side_join(ctx);
   SREQ(nullptr, ctx, "mom_synth_tracer: NULL context");
   SREQ(ctx, out && zt, "mom_synth_tracer: NULL argument");
   SREQ(ctx, dtype == 8 || dtype == 4, "mom_synth_tracer: dtype must be 8 or 4");
@@ -296,6 +297,7 @@ int mom_synth_tracer(mom_gpu_ctx_t *ctx, void *out, int dtype, int kind, unsigne
 
 int mom_synth_loguniform(mom_gpu_ctx_t *ctx, double *out, int nplanes, unsigned long long seed,
                          double lo, double hi) {
+  if (ctx) side_join(ctx);
   SREQ(nullptr, ctx, "mom_synth_loguniform: NULL context");
   SREQ(ctx, out && nplanes >= 1 && lo > 0.0 && hi >= lo, "mom_synth_loguniform: bad argument");
   const KDom &d = ctx->kd;
@@ -307,6 +309,7 @@ int mom_synth_loguniform(mom_gpu_ctx_t *ctx, double *out, int nplanes, unsigned 
 }
 
 int mom_synth_profile(mom_gpu_ctx_t *ctx, double *out, const double *prof, int nlev) {
+  if (ctx) side_join(ctx);
   SREQ(nullptr, ctx, "mom_synth_profile: NULL context");
   SREQ(ctx, out && prof && nlev >= 1, "mom_synth_profile: bad argument");
   const KDom &d = ctx->kd;
@@ -322,6 +325,7 @@ int mom_synth_profile(mom_gpu_ctx_t *ctx, double *out, const double *prof, int n
 int mom_synth_transports_unit(mom_gpu_ctx_t *ctx, const double *psi, const double *ov,
                               const double *hden, const int *kface, const double *prof,
                               double rho0, double *du, double *dv, double *mov) {
+  if (ctx) side_join(ctx);
   SREQ(nullptr, ctx, "mom_synth_transports_unit: NULL context");
   SREQ(ctx, psi && ov && hden && kface && prof && du && dv && mov,
        "mom_synth_transports_unit: NULL argument");
@@ -337,6 +341,7 @@ int mom_synth_transports_unit(mom_gpu_ctx_t *ctx, const double *psi, const doubl
 }
 
 int mom_synth_maxabs(mom_gpu_ctx_t *ctx, const double *a, size_t n, double *out) {
+  if (ctx) side_join(ctx);
   SREQ(nullptr, ctx, "mom_synth_maxabs: NULL context");
   SREQ(ctx, a && out && n > 0, "mom_synth_maxabs: bad argument");
   SREQ(ctx, ctx->red_n <= 1024, "mom_synth_maxabs: reduction buffer larger than expected");
@@ -355,6 +360,7 @@ int mom_synth_maxabs(mom_gpu_ctx_t *ctx, const double *a, size_t n, double *out)
 
 int mom_synth_transports_combine(mom_gpu_ctx_t *ctx, double *du, double *dv, const double *mov,
                                  double fac) {
+  if (ctx) side_join(ctx);
   SREQ(nullptr, ctx, "mom_synth_transports_combine: NULL context");
   SREQ(ctx, du && dv && mov, "mom_synth_transports_combine: NULL argument");
   const KDom &d = ctx->kd;
@@ -372,6 +378,7 @@ int mom_synth_transports_combine(mom_gpu_ctx_t *ctx, double *du, double *dv, con
 
 int mom_synth_cfl(mom_gpu_ctx_t *ctx, const double *du, const double *dv, const double *rdz,
                   double scale, double dtt, double *colmax, double *out) {
+  if (ctx) side_join(ctx);
   SREQ(nullptr, ctx, "mom_synth_cfl: NULL context");
   SREQ(ctx, du && dv && rdz && colmax && out, "mom_synth_cfl: NULL argument");
   const KDom &d = ctx->kd;
@@ -387,6 +394,7 @@ int mom_synth_cfl(mom_gpu_ctx_t *ctx, const double *du, const double *dv, const 
 
 int mom_synth_transports_finish(mom_gpu_ctx_t *ctx, double *du, double *dv, double scale,
                                 double *wrho) {
+  if (ctx) side_join(ctx);
   SREQ(nullptr, ctx, "mom_synth_transports_finish: NULL context");
   SREQ(ctx, du && dv && wrho, "mom_synth_transports_finish: NULL argument");
   const KDom &d = ctx->kd;

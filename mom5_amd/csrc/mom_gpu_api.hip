@@ -22,6 +22,14 @@ int mom_fail(mom_gpu_ctx *ctx, int code, const char *fmt, ...) {
     if (!(cond)) return mom_fail((ctx), MOM_GPU_ERR_ARG, __VA_ARGS__); \
   } while (0)
 
+void side_join(mom_gpu_ctx *c) {
+  if (c->side_pending) {
+    (void)hipStreamWaitEvent(c->stream, c->side_done, 0);
+    c->side_pending = false;
+  }
+  c->fork_valid = false;
+}
+
 extern "C" {
 
 const char *mom_gpu_last_error(const mom_gpu_ctx_t *ctx) { return ctx ? ctx->err : g_err; }
@@ -124,6 +132,10 @@ static int *tuning_slot(mom_gpu_ctx *ctx, const char *knob, int *lo, int *hi) {
     *lo = 0; *hi = 1;
     return &ctx->tune.vdiff_solver;
   }
+  if (!std::strcmp(knob, "step_streams")) {
+    *lo = 1; *hi = 2;
+    return &ctx->tune.step_streams;
+  }
   return nullptr;
 }
 
@@ -166,6 +178,7 @@ int mom_gpu_get_tuning(const mom_gpu_ctx_t *ctx, const char *knob, int *value) {
 
 int mom_gpu_fill(mom_gpu_ctx_t *ctx, double *ptr, double value, size_t n) {
   REQUIRE(nullptr, ctx, "mom_gpu_fill: NULL context");
+  side_join(ctx);
   REQUIRE(ctx, ptr || n == 0, "mom_gpu_fill: NULL pointer");
   return launch_fill(ctx, ptr, value, n);
 }
@@ -176,6 +189,7 @@ int mom_gpu_advection_velocity(mom_gpu_ctx_t *ctx, const double *u, const double
                                double *uhrho_et, double *vhrho_nt, double *wrho_bt,
                                double *diverge_t) {
   REQUIRE(nullptr, ctx, "mom_gpu_advection_velocity: NULL context");
+  side_join(ctx);
   REQUIRE(ctx, u && rho_dzu && pme && river && uhrho_et && vhrho_nt && wrho_bt,
           "mom_gpu_advection_velocity: NULL required array");
   return launch_adv_vel(ctx, u, rho_dzu, pme, river, rho_dzt_tendency, mass_source, uhrho_et,
@@ -266,6 +280,7 @@ int mom_gpu_update_ocean_barotropic_variant(mom_gpu_ctx_t *ctx, const mom_gpu_bt
 
 int mom_gpu_copy(mom_gpu_ctx_t *ctx, double *dst, const double *src, size_t n) {
   REQUIRE(nullptr, ctx, "mom_gpu_copy: NULL context");
+  side_join(ctx);
   REQUIRE(ctx, (dst && src) || n == 0, "mom_gpu_copy: NULL pointer");
   return launch_copy(ctx, dst, src, n);
 }
@@ -282,6 +297,9 @@ static void free_ctx(mom_gpu_ctx *c) {
   if (c->vd_scr) hipFree(c->vd_scr);
   for (int i = 0; i < 8; ++i)
     if (c->ev[i]) hipEventDestroy(c->ev[i]);
+  if (c->side_done) hipEventDestroy(c->side_done);
+  if (c->fork_ev) hipEventDestroy(c->fork_ev);
+  if (c->side) hipStreamDestroy(c->side);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
@@ -328,6 +346,7 @@ int mom_gpu_create(const mom_gpu_domain_t *dom, const mom_gpu_grid_t *grid, mom_
   c->tune.vdiff_block = 256;
   c->tune.vdiff_merge = 0;
   c->tune.vdiff_solver = 1;
+  c->tune.step_streams = 1;
   const size_t P2 = (size_t)k.P2, nk = (size_t)k.nk;
 
 #define CK(call)                                                                      \
@@ -343,6 +362,9 @@ int mom_gpu_create(const mom_gpu_domain_t *dom, const mom_gpu_grid_t *grid, mom_
   CK(hipGetDevice(&c->device));
   CK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   for (int i = 0; i < 8; ++i) CK(hipEventCreate(&c->ev[i]));
+  CK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+  CK(hipEventCreateWithFlags(&c->side_done, hipEventDisableTiming));
+  CK(hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
   struct {
     double **dst;
     const double *src;
@@ -391,13 +413,15 @@ int mom_gpu_create(const mom_gpu_domain_t *dom, const mom_gpu_grid_t *grid, mom_
 
 int mom_gpu_destroy(mom_gpu_ctx_t *ctx) {
   if (!ctx) return MOM_GPU_OK;
-  hipStreamSynchronize(ctx->stream);
+  (void)hipStreamSynchronize(ctx->side);
+  (void)hipStreamSynchronize(ctx->stream);
   free_ctx(ctx);
   return MOM_GPU_OK;
 }
 
 int mom_gpu_sync(mom_gpu_ctx_t *ctx) {
   REQUIRE(nullptr, ctx, "mom_gpu_sync: NULL context");
+  side_join(ctx);
   MOM_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return MOM_GPU_OK;
 }
@@ -405,6 +429,7 @@ int mom_gpu_sync(mom_gpu_ctx_t *ctx) {
 int mom_gpu_mark(mom_gpu_ctx_t *ctx, int slot) {
   REQUIRE(nullptr, ctx, "mom_gpu_mark: NULL context");
   REQUIRE(ctx, slot >= 0 && slot < 8, "mom_gpu_mark: slot %d out of range", slot);
+  side_join(ctx);
   MOM_HIP(ctx, hipEventRecord(ctx->ev[slot], ctx->stream));
   return MOM_GPU_OK;
 }
@@ -483,6 +508,13 @@ static int check_step(mom_gpu_ctx *ctx, const mom_gpu_tracer_step_t *s, const ch
 
 int mom_gpu_ensemble_step_tracers(mom_gpu_ensemble_t *e, const mom_gpu_tracer_step_t *steps) {
   REQUIRE(nullptr, e && steps, "mom_gpu_ensemble_step_tracers: NULL argument");
+  for (mom_gpu_ctx *c : e->ctx) {    // members' pending second-stream work
+    if (c->side_pending) {
+      MOM_HIP(e->lead, hipStreamWaitEvent(e->lead->stream, c->side_done, 0));
+      c->side_pending = false;
+    }
+    c->fork_valid = false;
+  }
   for (size_t d = 0; d < e->ctx.size(); ++d) {
     const int rc = check_step(e->ctx[d], &steps[d], "mom_gpu_ensemble_step_tracers");
     if (rc) return mom_fail(e->lead, rc, "domain %zu: %s", d, e->ctx[d]->err);
@@ -522,6 +554,7 @@ int mom_gpu_graph_begin(mom_gpu_ctx_t *ctx) {
   hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
   MOM_HIP(ctx, hipStreamIsCapturing(ctx->stream, &st));
   REQUIRE(ctx, st == hipStreamCaptureStatusNone, "mom_gpu_graph_begin: already capturing");
+  side_join(ctx);
   MOM_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
   return MOM_GPU_OK;
 }
@@ -531,14 +564,21 @@ int mom_gpu_graph_end(mom_gpu_ctx_t *ctx, mom_gpu_graph_t **out) {
   REQUIRE(ctx, out, "mom_gpu_graph_end: NULL output");
   *out = nullptr;
   hipGraph_t g = nullptr;
+  side_join(ctx);                 // the second stream rejoins the capture
   hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+  ctx->fork_valid = false;        // events recorded in the capture are graph nodes
   if (e != hipSuccess || !g) {
     if (g) (void)hipGraphDestroy(g);
-    // a stream whose capture was invalidated may stay unusable: replace it
+    // streams whose capture was invalidated may stay unusable: replace them
     hipStream_t fresh = nullptr;
     if (hipStreamCreateWithFlags(&fresh, hipStreamNonBlocking) == hipSuccess) {
       (void)hipStreamDestroy(ctx->stream);
       ctx->stream = fresh;
+    }
+    fresh = nullptr;
+    if (hipStreamCreateWithFlags(&fresh, hipStreamNonBlocking) == hipSuccess) {
+      (void)hipStreamDestroy(ctx->side);
+      ctx->side = fresh;
     }
     (void)hipGetLastError();
     return mom_fail(ctx, MOM_GPU_ERR_HIP,
@@ -561,6 +601,7 @@ int mom_gpu_graph_end(mom_gpu_ctx_t *ctx, mom_gpu_graph_t **out) {
 int mom_gpu_graph_launch(mom_gpu_ctx_t *ctx, const mom_gpu_graph_t *graph) {
   REQUIRE(nullptr, ctx, "mom_gpu_graph_launch: NULL context");
   REQUIRE(ctx, graph, "mom_gpu_graph_launch: NULL graph");
+  side_join(ctx);
   MOM_HIP(ctx, hipGraphLaunch(graph->exec, ctx->stream));
   return MOM_GPU_OK;
 }
@@ -584,6 +625,7 @@ int mom_gpu_horz_advect_tracer(mom_gpu_ctx_t *ctx, int scheme, const double *T_t
                                int limit_with_upwind, double dtime, double *th_tendency,
                                double *wrk1, double *flux_x, double *flux_y, double *flux_z) {
   REQUIRE(nullptr, ctx, "mom_gpu_horz_advect_tracer: NULL context");
+  side_join(ctx);
   REQUIRE(ctx, T_taum1 && uhrho_et && vhrho_nt && th_tendency,
           "mom_gpu_horz_advect_tracer: NULL required array");
   const size_t b3 = p3bytes(ctx);
@@ -618,6 +660,7 @@ int mom_gpu_vert_advect_tracer(mom_gpu_ctx_t *ctx, int scheme, const double *T_t
                                const double *tmask_limit, double *th_tendency, double *wrk1,
                                double *flux_z) {
   REQUIRE(nullptr, ctx, "mom_gpu_vert_advect_tracer: NULL context");
+  side_join(ctx);
   REQUIRE(ctx, th_tendency, "mom_gpu_vert_advect_tracer: th_tendency required");
   switch (scheme) {
     case MOM_ADVECT_MDFL_SWEBY:
@@ -643,6 +686,7 @@ int mom_gpu_advect_sweby_all(mom_gpu_ctx_t *ctx, int ntr, const double *const *T
                              const double *wrho_bt, const double *rho_dzt_tau, double dtime,
                              double *const *th_tendency, double *const *wrk1) {
   REQUIRE(nullptr, ctx, "mom_gpu_advect_sweby_all: NULL context");
+  side_join(ctx);
   REQUIRE(ctx, ntr >= 0 && T_taum1 && th_tendency && uhrho_et && vhrho_nt && wrho_bt && rho_dzt_tau,
           "mom_gpu_advect_sweby_all: NULL required array");
   REQUIRE(ctx, dtime > 0.0, "mom_gpu_advect_sweby_all: dtime must be positive");
@@ -664,6 +708,7 @@ int mom_gpu_compute_tmask_limit(mom_gpu_ctx_t *ctx, int ntr, const double *const
                                 const double *min_limit, const double *max_limit, int index_temp,
                                 int index_salt, int ts_same, double *const *tmask_limit) {
   REQUIRE(nullptr, ctx, "mom_gpu_compute_tmask_limit: NULL context");
+  side_join(ctx);
   REQUIRE(ctx, ntr >= 0 && T_tau && min_limit && max_limit && tmask_limit,
           "mom_gpu_compute_tmask_limit: NULL argument");
   REQUIRE(ctx, index_temp < ntr && index_salt < ntr, "mom_gpu_compute_tmask_limit: bad index");
@@ -678,6 +723,7 @@ int mom_gpu_update_tracer_explicit(mom_gpu_ctx_t *ctx, const double *T_taum1,
                                    const double *th_tendency, const double *rho_dzt_taum1,
                                    const double *rho_dztr, double dtime, double *T_taup1) {
   REQUIRE(nullptr, ctx, "mom_gpu_update_tracer_explicit: NULL context");
+  side_join(ctx);
   REQUIRE(ctx, T_taum1 && th_tendency && rho_dzt_taum1 && rho_dztr && T_taup1,
           "mom_gpu_update_tracer_explicit: NULL argument");
   return launch_explicit(ctx, T_taum1, th_tendency, rho_dzt_taum1, rho_dztr, dtime, T_taup1);
@@ -687,6 +733,7 @@ int mom_gpu_invtri(mom_gpu_ctx_t *ctx, double *z, const double *topbc, const dou
                    const double *dcb, double tdt, const int *kmz, const double *mask,
                    const double *dh, const double *dhw, double aidif) {
   REQUIRE(nullptr, ctx, "mom_gpu_invtri: NULL context");
+  side_join(ctx);
   REQUIRE(ctx, z && topbc && botbc && dcb && kmz && mask && dh && dhw, "mom_gpu_invtri: NULL argument");
   return launch_invtri(ctx, z, topbc, botbc, dcb, tdt, kmz, mask, dh, dhw, aidif);
 }
@@ -697,6 +744,7 @@ int mom_gpu_vert_diffuse_implicit(mom_gpu_ctx_t *ctx, int ntr, double *const *T_
                                   int index_salt, double rho0, const double *rho_dzt_taup1,
                                   const double *dzwt, double dtime_t, double aidif) {
   REQUIRE(nullptr, ctx, "mom_gpu_vert_diffuse_implicit: NULL context");
+  side_join(ctx);
   REQUIRE(ctx, ntr >= 0 && T_taup1 && stf && btf && diff_cbt && rho_dzt_taup1 && dzwt,
           "mom_gpu_vert_diffuse_implicit: NULL argument");
   for (int n = 0; n < ntr; ++n)
@@ -750,27 +798,70 @@ static int check_step(mom_gpu_ctx *ctx, const mom_gpu_tracer_step_t *s, const ch
   return MOM_GPU_OK;
 }
 
-int mom_gpu_step_tracers(mom_gpu_ctx_t *ctx, const mom_gpu_tracer_step_t *s) {
-  REQUIRE(nullptr, ctx, "mom_gpu_step_tracers: NULL context");
-  int rc = check_step(ctx, s, "mom_gpu_step_tracers");
-  if (rc) return rc;
-  if (s->ntr == 0) return MOM_GPU_OK;
+// The tracers of dtype[n] (the others are passed as 0 and skipped) on the
+// context's current stream: advection + explicit update, then the implicit
+// vertical diffusion.
+static int step_part(mom_gpu_ctx *ctx, const mom_gpu_tracer_step_t *s, const int *dtype) {
   const double *rm1 = s->rho_dzt_taum1 ? s->rho_dzt_taum1 : s->rho_dzt_tau;
-  rc = launch_tracer_step(ctx, s->ntr, s->T_taum1, s->T_taup1, s->dtype, s->th_tendency,
-                          s->uhrho_et, s->vhrho_nt, s->wrho_bt, s->rho_dzt_tau, rm1,
-                          s->rho_dztr, s->dtime);
+  int rc = launch_tracer_step(ctx, s->ntr, s->T_taum1, s->T_taup1, dtype, s->th_tendency,
+                              s->uhrho_et, s->vhrho_nt, s->wrho_bt, s->rho_dzt_tau, rm1,
+                              s->rho_dztr, s->dtime);
   if (rc) return rc;
   if (s->stf && s->btf && s->aidif != 0.0) {
     const double *dh = s->rho_dzt_taup1 ? s->rho_dzt_taup1 : s->rho_dzt_tau;
-    rc = launch_vdiff_mixed(ctx, s->ntr, s->T_taup1, s->dtype, s->stf, s->btf, s->diff_cbt,
+    rc = launch_vdiff_mixed(ctx, s->ntr, s->T_taup1, dtype, s->stf, s->btf, s->diff_cbt,
                             s->index_salt, s->rho0, dh, s->dzwt, s->dtime, s->aidif);
   }
   return rc;
 }
 
+int mom_gpu_step_tracers(mom_gpu_ctx_t *ctx, const mom_gpu_tracer_step_t *s) {
+  REQUIRE(nullptr, ctx, "mom_gpu_step_tracers: NULL context");
+  int rc = check_step(ctx, s, "mom_gpu_step_tracers");
+  if (rc) return rc;
+  if (s->ntr == 0) return MOM_GPU_OK;
+  // step_streams = 2: the FP32 tracers go on the second stream, where their
+  // (VALU-bound) advection pass overlaps the main stream's memory-bound
+  // solves and barotropic subcycles.  They read only this call's inputs and
+  // write only FP32 arrays, which no main-stream work of a step touches, and
+  // use no context scratch (the register-resident solver).
+  std::vector<int> d64(s->ntr, 0), d32(s->ntr, 0);
+  int n32 = 0;
+  for (int n = 0; n < s->ntr; ++n) {
+    if (s->dtype[n] == 4) { d32[n] = 4; ++n32; } else { d64[n] = s->dtype[n]; }
+  }
+  const bool diffuse = s->stf && s->btf && s->aidif != 0.0;
+  const bool two = ctx->tune.step_streams == 2 && n32 > 0 &&
+                   (!diffuse || !vdiff_fp32_scratch(ctx));
+  if (!two) {
+    side_join(ctx);
+    return step_part(ctx, s, s->dtype);
+  }
+  // fork: the second stream waits for the main stream's position at the start
+  // of the previous step call (only barotropic work may have been queued
+  // since; it touches none of a step's arrays), else for its position now
+  if (!ctx->fork_valid) MOM_HIP(ctx, hipEventRecord(ctx->fork_ev, ctx->stream));
+  MOM_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
+  MOM_HIP(ctx, hipEventRecord(ctx->fork_ev, ctx->stream));
+  ctx->fork_valid = true;
+  if (n32 < s->ntr) {
+    rc = step_part(ctx, s, d64.data());
+    if (rc) return rc;
+  }
+  hipStream_t main = ctx->stream;
+  ctx->stream = ctx->side;
+  rc = step_part(ctx, s, d32.data());
+  ctx->stream = main;
+  if (rc) return rc;
+  MOM_HIP(ctx, hipEventRecord(ctx->side_done, ctx->side));
+  ctx->side_pending = true;
+  return MOM_GPU_OK;
+}
+
 int mom_gpu_tracer_content(mom_gpu_ctx_t *ctx, const double *T, const double *rho_dzt,
                            double *content) {
   REQUIRE(nullptr, ctx, "mom_gpu_tracer_content: NULL context");
+  side_join(ctx);
   REQUIRE(ctx, T && rho_dzt && content, "mom_gpu_tracer_content: NULL argument");
   return launch_content(ctx, T, 8, rho_dzt, content);
 }
@@ -778,6 +869,7 @@ int mom_gpu_tracer_content(mom_gpu_ctx_t *ctx, const double *T, const double *rh
 int mom_gpu_tracer_content_f32(mom_gpu_ctx_t *ctx, const float *T, const double *rho_dzt,
                                double *content) {
   REQUIRE(nullptr, ctx, "mom_gpu_tracer_content_f32: NULL context");
+  side_join(ctx);
   REQUIRE(ctx, T && rho_dzt && content, "mom_gpu_tracer_content_f32: NULL argument");
   return launch_content(ctx, T, 4, rho_dzt, content);
 }
@@ -785,6 +877,7 @@ int mom_gpu_tracer_content_f32(mom_gpu_ctx_t *ctx, const float *T, const double 
 int mom_gpu_tracer_error(mom_gpu_ctx_t *ctx, const void *a, int dtype, const double *b,
                          double rel_floor, double *out) {
   REQUIRE(nullptr, ctx, "mom_gpu_tracer_error: NULL context");
+  side_join(ctx);
   REQUIRE(ctx, a && b && out, "mom_gpu_tracer_error: NULL argument");
   REQUIRE(ctx, dtype == 8 || dtype == 4, "mom_gpu_tracer_error: dtype must be 8 or 4");
   return launch_tracer_error(ctx, a, dtype, b, rel_floor, out);

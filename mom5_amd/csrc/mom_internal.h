@@ -71,6 +71,15 @@ struct mom_gpu_ctx {
   int device;
   hipStream_t stream;
   hipEvent_t ev[8];
+  // Second stream (default priority): the FP32 passive-tracer work of a
+  // batched step when step_streams = 2.  side_done marks the last work
+  // queued there; fork_ev the main stream's position at the start of the
+  // previous step call, which a step's side work waits for (fork_valid:
+  // nothing else touched the step's inputs since).  side_join makes the
+  // main stream wait for pending side work.
+  hipStream_t side;
+  hipEvent_t side_done, fork_ev;
+  bool side_pending, fork_valid;
   // device copies of the grid (owned)
   double *dxt, *dyt, *dat, *datr, *dxte, *dyte, *dxtn, *dytn;
   double *dxu, *dyu, *dxur, *dyur, *f;
@@ -113,9 +122,16 @@ struct mom_gpu_ctx {
     int vdiff_block;               // FP32 implicit diffusion: threads a block (64, 256)
     int vdiff_merge;               // mixed steps: all tracers in one diffusion launch (0, 1)
     int vdiff_solver;              // FP32 diffusion: 0 lane per column + scratch, 1 registers
+    int step_streams;              // 2: a step's FP32 tracers on the second stream
   } tune;
   char err[512];
 };
+
+// The main stream waits for the context's pending second-stream work.
+void side_join(mom_gpu_ctx *c);
+// Whether the FP32 implicit diffusion of this context uses its solver scratch
+// (vdiff_solver 0, or nk outside the register solver's range).
+bool vdiff_fp32_scratch(const mom_gpu_ctx *c);
 
 // Error handling: every entry point returns through these.
 int mom_fail(mom_gpu_ctx *ctx, int code, const char *fmt, ...);

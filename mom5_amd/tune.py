@@ -90,6 +90,17 @@ def autotune(step, reps=3, barotropic=True):
             lambda: g.horz_advect_tracer("mdfl_sweby", T0, th, uhrho_et=d["uhrho_et"],
                                          vhrho_nt=d["vhrho_nt"], wrho_bt=d["wrho_bt"],
                                          rho_dzt_tau=d["rho_dzt"], dtime=c.dt), reps)
+    if isinstance(step, BatchedStep) and step.barotropic and any(
+            step.tracer(n).element_size() == 4 for n in range(step.ntr)):
+        # FP32 tracers on a second stream: timed as whole steps (tracer call
+        # and barotropic call, each writing the next time level only)
+        a = step.step_args()
+        src_all, dst_all = a.pop("T_taum1s"), a.pop("T_taup1s")
+        bt = step.bt_args()
+        report["streams"] = _search(
+            g, ("step_streams",), [(1,), (2,)],
+            lambda: (g.step_tracers(src_all, dst_all, **a), g.update_ocean_barotropic(**bt)),
+            reps)
     if barotropic and getattr(step, "barotropic", c.dtbt > 0.0) and c.dtbt > 0.0:
         cur, nxt = step.cur, 1 - step.cur
         report["barotropic"] = _search(

@@ -19,12 +19,17 @@ def _same(a, b):
     return np.array_equal(a.to_host(), b.to_host())
 
 
-def test_batched_mixed_step_graph_replays_equal_eager():
+@pytest.mark.parametrize("streams", [1, 2])
+def test_batched_mixed_step_graph_replays_equal_eager(streams):
+    """streams = 2: the FP32 tracers of each captured step run on the
+    context's second stream (graph branches joined at the end)."""
     from mom5_amd.step import BatchedStep
     from mom5_amd.synth import make_case_device
     case = make_case_device(150, 90, 14, seed=9, nts=10, n_passive=3, stf_scale=1e-3)
-    print("[graph: batched mixed 150x90x14, 2 FP64 + 3 FP32, barotropic nts=10]", flush=True)
+    print(f"[graph: batched mixed 150x90x14, 2 FP64 + 3 FP32, barotropic nts=10, "
+          f"streams={streams}]", flush=True)
     g = case.info["gpu"]
+    g.set_tuning("step_streams", streams)
     dt = [np.float64, np.float64] + [np.float32] * 3
 
     def fresh():

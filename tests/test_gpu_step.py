@@ -156,6 +156,52 @@ def test_step_with_other_tendencies(setup):
         assert not np.array_equal(outs[True][n], outs[False][n])
 
 
+def test_two_streams_equal_one_stream():
+    """step_streams = 2 (FP32 tracers on the context's second stream,
+    overlapping the FP64 tracers and the barotropic calls) gives the bits of
+    step_streams = 1, with diffusion and other th_tendency terms, over steps
+    that alternate the two modes on one context (the joins at the
+    entry points order them) and with reads in between."""
+    from mom5_amd import DeviceArray
+    from mom5_amd.step import BatchedStep
+    from mom5_amd.synth import make_case
+    case = make_case(90, 50, 11, land_frac=0.3, nts=8, seed=31, n_passive=4, stf_scale=1e-3)
+    print("[two streams vs one: 90x50x11, 2 FP64 + 4 FP32, barotropic nts=8]", flush=True)
+    dt = [np.float64, np.float64] + [np.float32] * 4
+    import dataclasses
+    from mom5_amd import OceanGPU
+    g = OceanGPU(case.dom, case.grid)
+
+    def member():
+        c = dataclasses.replace(case, tracers=[(nm, t.astype(dt[i]))
+                                               for i, (nm, t) in enumerate(case.tracers)])
+        m = BatchedStep(c, gpu=g)
+        rng = np.random.default_rng(8)
+        m.th = [DeviceArray.from_host(1e-3 * rng.standard_normal(case.tracers[0][1].shape))
+                for _ in range(m.ntr)]
+        return m
+
+    one, two = member(), member()
+    for k in range(4):
+        g.set_tuning("step_streams", 2)
+        two.run(1)
+        if k == 1:
+            # a read of the FP32 state between steps (joins the second stream)
+            g.tracer_content(two.tracer(3), two.dev["rho_dzt"])
+        g.set_tuning("step_streams", 1)
+        one.run(1)
+    g.set_tuning("step_streams", 2)
+    two.run(2)
+    g.set_tuning("step_streams", 1)
+    one.run(2)
+    g.sync()
+    for n in range(one.ntr):
+        assert np.array_equal(one.tracer(n).to_host(), two.tracer(n).to_host()), n
+    assert np.array_equal(one.eta_bar[one.cur].to_host(), two.eta_bar[two.cur].to_host())
+    assert np.array_equal(one.udrho[one.cur].to_host(), two.udrho[two.cur].to_host())
+    g.close()
+
+
 def test_fp32_results_independent_of_launch_shape(setup):
     """Grouping, tile height and the column solver's load schedule change
     which threads do the work and when, not the arithmetic of any tracer:
