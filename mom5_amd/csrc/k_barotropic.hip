@@ -196,11 +196,28 @@ constexpr int BRY = 32;          // region rows
 constexpr int BPT = 2;           // region rows per wavefront
 constexpr int BNT = BRX * BRY / BPT;
 
+// sf: tmask * (pme + river + source - eta_smooth) and mask: bit 0 tmask(k=1),
+// bit 1 umask(k=1), formed once per call by k_bt_sfmask (constant over the
+// subcycles) instead of being re-read from their six sources by every launch
 struct BtIn {
-  const double *rho_g, *patm, *thicku, *forcing, *f, *pme, *river, *source, *eta_smooth;
+  const double *rho_g, *patm, *thicku, *forcing, *f;
   const double *dxu, *dyu, *datr, *dxur, *dyur;
-  const int *kmt, *kmu;
+  const double *sf;
+  const unsigned char *mask;
 };
+
+__global__ void k_bt_sfmask(KDom d, const double *__restrict__ pme,
+                            const double *__restrict__ river, const double *__restrict__ source,
+                            const double *__restrict__ eta_smooth, const int *__restrict__ kmt,
+                            const int *__restrict__ kmu, double *__restrict__ sf,
+                            unsigned char *__restrict__ mask) {
+  const int ii = blockIdx.x * blockDim.x + threadIdx.x, jj = blockIdx.y;
+  if (ii >= d.NI) return;
+  const int o = ii + d.NI * jj;
+  const double tm = (1 <= kmt[o]) ? 1.0 : 0.0;
+  sf[o] = tm * (pme[o] + river[o] + source[o] - eta_smooth[o]);
+  mask[o] = (unsigned char)((1 <= kmt[o] ? 1 : 0) | (1 <= kmu[o] ? 2 : 0));
+}
 
 // One launch of the blocked subcycle for one domain: nsub subcycles from s0.
 struct BtJob {
@@ -259,15 +276,16 @@ __device__ __forceinline__ void bt_tile(const BtJob &J, int tile) {
     const int o = present[q] ? col + d.NI * j : 0;
     off[q] = o;
     if (present[q]) {
-      tm[q] = (1 <= in.kmt[o]) ? 1.0 : 0.0;
-      sf[q] = tm[q] * (in.pme[o] + in.river[o] + in.source[o] - in.eta_smooth[o]);
+      const unsigned char mb = in.mask[o];
+      tm[q] = (mb & 1) ? 1.0 : 0.0;
+      sf[q] = in.sf[o];
       datr[q] = in.datr[o]; dyu[q] = in.dyu[o]; dxu[q] = in.dxu[o];
       rho_g[q] = in.rho_g[o]; patm[q] = in.patm[o]; thk[q] = in.thicku[o];
       F1[q] = in.forcing[o]; F2[q] = in.forcing[o + P2]; fc[q] = in.f[o];
       cori1[q] = 0.5 * dtbt * fc[q];
       cori2[q] = 1.0 / (1.0 + cori1[q] * cori1[q]);
       dxur[q] = in.dxur[o]; dyur[q] = in.dyur[o];
-      um[q] = (1 <= in.kmu[o]) ? 1.0 : 0.0;
+      um[q] = (mb & 2) ? 1.0 : 0.0;
       eta[q] = eta_src[o]; u1[q] = ud_src[o]; u2[q] = ud_src[o + P2];
       if (interior[q]) { ebv[q] = eb[o]; ub1[q] = ub[o]; ub2[q] = ub[o + P2]; }
       else { ebv[q] = ub1[q] = ub2[q] = 0.0; }
@@ -672,8 +690,14 @@ int launch_barotropic_blocked(mom_gpu_ctx *ctx, const mom_gpu_bt_params_t *p,
   hipLaunchKernelGGL(k_bt_acc_init, grid, block, 0, s, d, p->initsum_with_bar, eta_t,
                      eta_bar_tau, eta_bar, ud);
   MOM_LAUNCHED(ctx, "k_bt_acc_init");
-  BtIn in{rho_g, patm, thicku, forcing, ctx->f, pme, river, source, eta_smooth,
-          ctx->dxu, ctx->dyu, ctx->datr, ctx->dxur, ctx->dyur, ctx->kmt, ctx->kmu};
+  // sf and the masks in planes 6 and 7 of the subcycle buffer (0-5 hold the
+  // ping-pong state)
+  double *sfp = ctx->bt + 6 * P2;
+  unsigned char *maskp = (unsigned char *)(ctx->bt + 7 * P2);
+  hipLaunchKernelGGL(k_bt_sfmask, grid, block, 0, s, d, pme, river, source, eta_smooth,
+                     ctx->kmt, ctx->kmu, sfp, maskp);
+  BtIn in{rho_g, patm, thicku, forcing, ctx->f, ctx->dxu, ctx->dyu, ctx->datr, ctx->dxur,
+          ctx->dyur, sfp, maskp};
   const int xbase = d.cyclic_x ? 1 : 0, xcount = d.cyclic_x ? d.ni : d.NI;
   double *eta_buf[2] = {ctx->bt, ctx->bt + P2};
   double *ud_buf[2] = {ctx->bt + 2 * P2, ctx->bt + 4 * P2};
@@ -785,9 +809,13 @@ int launch_barotropic_ensemble(mom_gpu_ensemble *e, const mom_gpu_bt_params_t *p
     const double rho0r = 1.0 / p[k].rho0;
     BtDom &D = doms[k];
     D.d = d;
-    D.in = BtIn{x.rho_g, x.patm, x.thicku, x.forcing_bt, ctx->f, x.pme, x.river, x.source,
-                x.eta_smooth, ctx->dxu, ctx->dyu, ctx->datr, ctx->dxur, ctx->dyur, ctx->kmt,
-                ctx->kmu};
+    double *sfp = ctx->bt + 6 * P2;
+    unsigned char *maskp = (unsigned char *)(ctx->bt + 7 * P2);
+    hipLaunchKernelGGL(k_bt_sfmask, dim3((d.NI + 255) / 256, d.NJ), dim3(256), 0,
+                       e->lead->stream, d, x.pme, x.river, x.source, x.eta_smooth, ctx->kmt,
+                       ctx->kmu, sfp, maskp);
+    D.in = BtIn{x.rho_g, x.patm, x.thicku, x.forcing_bt, ctx->f, ctx->dxu, ctx->dyu, ctx->datr,
+                ctx->dxur, ctx->dyur, sfp, maskp};
     D.eta_bar_tau = x.eta_t_bar_tau;
     D.ud_tau = x.udrho_tau;
     D.eta_buf[0] = ctx->bt;
