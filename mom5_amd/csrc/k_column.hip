@@ -180,6 +180,106 @@ __device__ __forceinline__ void vdiff_col(const VdJob<TT> &J, int gi, int gj) {
   }
 }
 
+// vdiff_col for a group of one tracer (TT = double), the deepest TAIL levels
+// kept in registers between the sweeps: the forward sweep reaches them last
+// and the back substitution needs them first, so their z and e never go to
+// memory (the same arithmetic, in the same order; nk > TAIL).
+template <int TAIL>
+__device__ __forceinline__ void vdiff_col1(const VdJob<double> &J, int gi, int gj) {
+  const KDom d = J.d;
+  const TriGroup<double> &g = J.g;
+  const double *__restrict__ dcbase = J.dcbase;
+  const double rho0 = J.rho0;
+  const double *__restrict__ dh = J.dh;
+  const double *__restrict__ dhw = J.dhw;
+  const double tdt_aidif = J.tdt * J.aidif;
+  double *__restrict__ ework = J.ework;
+  double *__restrict__ z = g.z[0];
+  const int nk = d.nk, P2 = d.P2;
+  const int o = d2(d, gi, gj);
+  const int kmo = J.kmt[o];
+  const int kb = max(2, kmo);
+  const double stf = g.stf[0][o], btf = g.btf[0][o];
+  double cprev = 0.0, bet = 0.0, zprev = 0.0, dcb_km1 = 0.0;
+  // one level of the forward sweep (vdiff_col's expressions); returns e(k)
+  auto level = [&](int k, double &zk) -> double {
+    const int kp1 = min(k + 1, nk);
+    int ok = o + P2 * (k - 1);
+    // the level's addresses wait for the previous level's result, so that
+    // the unrolled tail issues one level's loads at a time
+    asm volatile("" : "+v"(ok) : "v"(zprev));
+    const double dcb_k = rho0 * dcbase[ok] + (g.k33 ? g.k33[ok] : 0.0);
+    if (k == 1) dcb_km1 = dcb_k;
+    const double dhk = dh[ok];
+    const double factu = tdt_aidif / (dhw[ok] * dhk);
+    const double factl = tdt_aidif / (dhw[ok + P2] * dhk);
+    const double mkk = (k <= kmo) ? 1.0 : 0.0;
+    const double mkp1 = (kp1 <= kmo) ? 1.0 : 0.0;
+    double a = -dcb_km1 * factu * mkk;
+    double c = -dcb_k * factl * mkp1;
+    if (k == 1) a = 0.0;
+    if (k == nk) c = 0.0;
+    const double b = 1.0 - a - c;
+    double e = 0.0;
+    if (k == 1) {
+      bet = mkk / (b + EPS_TRI);
+    } else {
+      e = cprev * bet;
+      bet = mkk / (b - a * e + EPS_TRI);
+    }
+    const double zo = z[ok];
+    double f;
+    if (k == 1)
+      f = zo + stf * tdt_aidif * mkk / dhk;
+    else if (k == kb)
+      f = zo - btf * tdt_aidif * mkk / dhk;
+    else
+      f = zo * mkk;
+    zk = (k == 1) ? f * bet : (f - a * zprev) * bet;
+    zprev = zk;
+    cprev = c;
+    dcb_km1 = dcb_k;
+    return e;
+  };
+  const int kt = nk - TAIL;
+  for (int k = 1; k <= kt; ++k) {
+    double zk;
+    const double e = level(k, zk);
+    const int ok = o + P2 * (k - 1);
+    if (k > 1) ework[ok] = e;
+    z[ok] = zk;
+  }
+  double et[TAIL], zt[TAIL];
+#pragma unroll
+  for (int t = 0; t < TAIL; ++t) et[t] = level(kt + 1 + t, zt[t]);
+  z[o + P2 * (nk - 1)] = zt[TAIL - 1];
+  zprev = zt[TAIL - 1];
+#pragma unroll
+  for (int t = TAIL - 2; t >= 0; --t) {
+    const double zk = zt[t] - et[t + 1] * zprev;
+    z[o + P2 * (kt + t)] = zk;
+    zprev = zk;
+  }
+  for (int k = kt; k >= 1; --k) {
+    const int ok = o + P2 * (k - 1);
+    const double e1 = (k == kt) ? et[0] : ework[ok + P2];
+    const double zk = z[ok] - e1 * zprev;
+    z[ok] = zk;
+    zprev = zk;
+  }
+}
+
+constexpr int VD_TAIL = 20;
+
+__global__ __launch_bounds__(64) void k_vdiff1(VdJob<double> J, int per_xcd) {
+  const int b = blockIdx.x;
+  const int tile = (b & 7) * per_xcd + (b >> 3);
+  if (tile >= J.nblocks) return;
+  const int gi = (tile % J.nbx) * 64 + threadIdx.x;
+  if (gi >= J.d.ni) return;
+  vdiff_col1<VD_TAIL>(J, gi, tile / J.nbx);
+}
+
 // 64-column tiles in XCD-aware order (see k_vdiff_twin): J.nbx tiles a row,
 // J.nblocks in all, per_xcd consecutive tiles per XCD.
 template <typename TT>
@@ -1110,7 +1210,16 @@ static int vdiff_groups(mom_gpu_ctx *ctx, int ntr, void *const *T, const int *dt
       const int nbx = (d.ni + 63) / 64, per_xcd = (nbx * d.nj + 7) / 8;
       VdJob<TT> J{d, g, diff_cbt + p3 * (nmix - 1), rho0, ctx->kmt, dh, dhw, dt, aidif,
                   ctx->tri_e, nbx, nbx * d.nj};
-      hipLaunchKernelGGL(k_vdiff<TT>, dim3(8 * per_xcd), dim3(64), 0, ctx->stream, J, per_xcd);
+      if constexpr (sizeof(TT) == 8) {
+        if (g.n == 1 && d.nk > VD_TAIL)
+          hipLaunchKernelGGL(k_vdiff1, dim3(8 * per_xcd), dim3(64), 0, ctx->stream,
+                             (const VdJob<double> &)J, per_xcd);
+        else
+          hipLaunchKernelGGL(k_vdiff<TT>, dim3(8 * per_xcd), dim3(64), 0, ctx->stream, J,
+                             per_xcd);
+      } else {
+        hipLaunchKernelGGL(k_vdiff<TT>, dim3(8 * per_xcd), dim3(64), 0, ctx->stream, J, per_xcd);
+      }
       MOM_LAUNCHED(ctx, "k_vdiff");
       g.n = 0;
       return MOM_GPU_OK;
