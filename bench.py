@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Throughput of one baroclinic step of the MOM5 hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 3|1|0]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 
 A step is the reference's order (ocean_model.F90:1441-2108, the part this
 build implements): the tracer update (advection into th_tendency, explicit
@@ -9,7 +9,7 @@ update, vert_diffuse_implicit/invtri) for every tracer, then the barotropic
 subcycle with its time-averaging tail.  Inputs are seeded synthetic fields
 resident in HBM before timing starts.
 
---config 3 (default): 3600 x 2700 x 75, T and S in FP64 plus 10 passive
+--workload mixed (default): 3600 x 2700 x 75, T and S in FP64 plus 10 passive
 tracers in FP32, 120 barotropic subcycles, advanced by the batched step
 (mom_gpu_step_tracers + update_ocean_barotropic).  The same case is then run
 through the plain per-tracer FP64 path for the same number of steps: that run
@@ -17,8 +17,10 @@ gives the FP64 rate the batched step is compared with, the bitwise check of
 T and S, and the FP64 reference for each passive tracer's error.  Masked
 tracer content (compensated FP64 sums) is reported before and after.
 
---config 1: 1440 x 1080 x 50 plain FP64 path (T, S MDFL-Sweby, one Quicker
-tracer, 80 subcycles).  --config 0: 90 x 60 x 25, MDFL only.
+--workload batched_fp64: 1440 x 1080 x 50, 32 FP64 tracers through the batched
+step.  --workload ensemble: 24 regional domains as one ensemble.  --workload
+plain_fp64: 1440 x 1080 x 50 plain FP64 path (T, S MDFL-Sweby, one Quicker
+tracer, 80 subcycles).  --workload small: 90 x 60 x 25, MDFL only.
 
 Prints one JSON line: tracer-cell-updates/s of the step (value), the roofline
 of the dominant kernel (HIP-event timed on the context's stream) and a CPU
@@ -40,18 +42,18 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
-    3: dict(name="cfg3_mixed_3600x2700x75", ni=3600, nj=2700, nk=75, land_frac=0.30,
+    "mixed": dict(name="mixed_3600x2700x75", ni=3600, nj=2700, nk=75, land_frac=0.30,
             nts=120, n_passive=10, mixed=True, passive_dtype=np.float32, steps=18, warmup=2,
             schemes=["mdfl_sweby"] * 12, sample=dict(ni=240, nj=180)),
-    2: dict(name="cfg2_batched_fp64_1440x1080x50", ni=1440, nj=1080, nk=50, land_frac=0.30,
+    "batched_fp64": dict(name="batched_fp64_1440x1080x50", ni=1440, nj=1080, nk=50, land_frac=0.30,
             nts=80, n_passive=30, mixed=True, passive_dtype=np.float64, steps=20, warmup=2,
             schemes=["mdfl_sweby"] * 32, sample=dict(ni=240, nj=180)),
-    4: dict(name="cfg4_ensemble_24_regional", ndom=24, n_passive=6, ensemble=True, steps=20,
+    "ensemble": dict(name="ensemble_24_regional", ndom=24, n_passive=6, ensemble=True, steps=20,
             warmup=2, sample=dict(ni=200, nj=150, nk=40, nts=80, regional=True)),
-    1: dict(name="cfg1_plain_fp64_1440x1080x50", ni=1440, nj=1080, nk=50, land_frac=0.30,
+    "plain_fp64": dict(name="plain_fp64_1440x1080x50", ni=1440, nj=1080, nk=50, land_frac=0.30,
             nts=80, n_passive=1, schemes=["mdfl_sweby", "mdfl_sweby", "quicker"],
             sample=dict(ni=480, nj=360)),
-    0: dict(name="cfg0_plain_fp64_90x60x25", ni=90, nj=60, nk=25, land_frac=0.25, nts=0,
+    "small": dict(name="plain_fp64_90x60x25", ni=90, nj=60, nk=25, land_frac=0.25, nts=0,
             n_passive=0, schemes=["mdfl_sweby", "mdfl_sweby"], sample=dict(ni=90, nj=60)),
 }
 
@@ -259,7 +261,7 @@ def measured_traffic(workload, key):
     rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs committed under profiles/
     (scaled by the copy-kernel calibration recorded there); None if that
     kernel and launch shape were not measured."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_cfg3_3600x2700x75.json")
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_mixed_3600x2700x75.json")
     if workload is None or not os.path.exists(path):
         return None, None
     with open(path) as fh:
@@ -587,10 +589,10 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
-                    help="timed steps (default 18 for config 3, 20 otherwise)")
+                    help="timed steps (default 18 for the mixed workload, 20 otherwise)")
     ap.add_argument("--warmup", type=int, default=None,
-                    help="untimed steps first (default 2 for config 3, 3 otherwise)")
-    ap.add_argument("--config", type=int, default=3, choices=sorted(WORKLOADS))
+                    help="untimed steps first (default 2 for the mixed workload, 3 otherwise)")
+    ap.add_argument("--workload", default="mixed", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="only launch the kernels measured with rocprofv3 --pmc (mixed workload)")
@@ -604,7 +606,7 @@ def main():
               "the multi-GPU decomposition is a design, not an implementation.", file=sys.stderr)
         sys.exit(2)
 
-    w = WORKLOADS[args.config]
+    w = WORKLOADS[args.workload]
     if args.profile_kernels:
         profile_kernels(args, w)
         return

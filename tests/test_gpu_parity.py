@@ -10,7 +10,7 @@ pytestmark = pytest.mark.gpu
 
 CASES = {
     # name: (ni, nj, nk, kwargs)
-    "cfg0_90x60x25": (90, 60, 25, dict(land_frac=0.25, barotropic=False)),
+    "small_90x60x25": (90, 60, 25, dict(land_frac=0.25, barotropic=False)),
     "odd_cyclic_37x23x7": (37, 23, 7, dict(land_frac=0.3, nts=12, seed=11)),
     "regional_41x29x9": (41, 29, 9, dict(land_frac=0.2, cyclic_x=False, lat_s=20.0,
                                          lat_n=50.0, lon_span=30.0, nts=16, seed=12)),

@@ -30,57 +30,57 @@ def _synth(**kw):
 
 
 @pytest.fixture(scope="module")
-def cfg0():
+def small():
     return _synth(ni=90, nj=60, nk=25, land_frac=0.25, barotropic=False)
 
 
-def test_zero_velocity_zero_tendency(orc, cfg0):
-    o = orc.Oracle(cfg0)
-    f = dict(cfg0.fields)
+def test_zero_velocity_zero_tendency(orc, small):
+    o = orc.Oracle(small)
+    f = dict(small.fields)
     for k in ("uhrho_et", "vhrho_nt", "wrho_bt"):
         f[k] = np.zeros_like(f[k])
     for lim in (0.0, 1.0):
-        adv = o.advect_mdfl_sweby(cfg0.tracers[0][1], cfg0.dt, lim=lim, fields=f)
+        adv = o.advect_mdfl_sweby(small.tracers[0][1], small.dt, lim=lim, fields=f)
         assert np.all(adv == 0.0)
 
 
-def test_constancy_with_w_from_continuity(orc, cfg0):
-    o = orc.Oracle(cfg0)
-    const = 7.25 * cfg0.grid["tmask"]
-    adv_T = o.advect_mdfl_sweby(cfg0.tracers[0][1], cfg0.dt)
+def test_constancy_with_w_from_continuity(orc, small):
+    o = orc.Oracle(small)
+    const = 7.25 * small.grid["tmask"]
+    adv_T = o.advect_mdfl_sweby(small.tracers[0][1], small.dt)
     scale = np.abs(adv_T).max()
     for lim in (0.0, 1.0):
-        adv = o.advect_mdfl_sweby(const, cfg0.dt, lim=lim)
+        adv = o.advect_mdfl_sweby(const, small.dt, lim=lim)
         assert np.abs(adv).max() <= 1e-12 * scale * 7.25
 
 
-def test_constancy_detects_a_wrong_correction_sign(orc, cfg0):
+def test_constancy_detects_a_wrong_correction_sign(orc, small):
     """The same check fails when w is not the continuity field."""
-    o = orc.Oracle(cfg0)
-    f = dict(cfg0.fields)
-    f["wrho_bt"] = -cfg0.fields["wrho_bt"]
-    const = 7.25 * cfg0.grid["tmask"]
-    adv = o.advect_mdfl_sweby(const, cfg0.dt, fields=f)
-    scale = np.abs(o.advect_mdfl_sweby(cfg0.tracers[0][1], cfg0.dt)).max()
+    o = orc.Oracle(small)
+    f = dict(small.fields)
+    f["wrho_bt"] = -small.fields["wrho_bt"]
+    const = 7.25 * small.grid["tmask"]
+    adv = o.advect_mdfl_sweby(const, small.dt, fields=f)
+    scale = np.abs(o.advect_mdfl_sweby(small.tracers[0][1], small.dt)).max()
     assert np.abs(adv).max() > 1e-6 * scale
 
 
 @pytest.mark.parametrize("lim", [0.0, 1.0])
-def test_conservation(orc, cfg0, lim):
-    o = orc.Oracle(cfg0)
-    dat = cfg0.grid["dat"][1:-1, 1:-1]
-    for _, T in cfg0.tracers:
-        adv = o.advect_mdfl_sweby(T, cfg0.dt, lim=lim)
+def test_conservation(orc, small, lim):
+    o = orc.Oracle(small)
+    dat = small.grid["dat"][1:-1, 1:-1]
+    for _, T in small.tracers:
+        adv = o.advect_mdfl_sweby(T, small.dt, lim=lim)
         tot = (adv * dat[None]).sum()
         assert abs(tot) <= 1e-12 * np.abs(adv * dat[None]).sum()
 
 
-def test_conservation_through_explicit_update(orc, cfg0):
-    o = orc.Oracle(cfg0)
-    T = cfg0.tracers[0][1]
+def test_conservation_through_explicit_update(orc, small):
+    o = orc.Oracle(small)
+    T = small.tracers[0][1]
     th = np.zeros_like(T)
-    o.horz_advect_mdfl(T, th, cfg0.dt)
-    Tn = o.update_explicit(T, th, cfg0.dt)
+    o.horz_advect_mdfl(T, th, small.dt)
+    Tn = o.update_explicit(T, th, small.dt)
     c0, c1 = o.tracer_content(T), o.tracer_content(Tn)
     assert abs(c1 - c0) <= 1e-13 * abs(c0)
 
@@ -192,16 +192,16 @@ def test_sweby_is_tvd_dst_is_not(orc):
     assert tn_d.min() < Ti.min() - 1e-3 or tn_d.max() > Ti.max() + 1e-3
 
 
-def test_sweby_all_matches_per_tracer_to_rounding(orc, cfg0):
-    o = orc.Oracle(cfg0)
-    Ts = [t for _, t in cfg0.tracers]
+def test_sweby_all_matches_per_tracer_to_rounding(orc, small):
+    o = orc.Oracle(small)
+    Ts = [t for _, t in small.tracers]
     th_all = [np.zeros_like(t) for t in Ts]
-    o.sweby_all(Ts, th_all, cfg0.dt)
+    o.sweby_all(Ts, th_all, small.dt)
     for n, T in enumerate(Ts):
         th = np.zeros_like(T)
-        o.horz_advect_mdfl(T, th, cfg0.dt, lim=1.0)
+        o.horz_advect_mdfl(T, th, small.dt, lim=1.0)
         # the tendency is rho*(T_new - T)/dt: its rounding scale is rho*|T|/dt
-        scale = np.abs(cfg0.fields["rho_dzt"] * T).max() / cfg0.dt
+        scale = np.abs(small.fields["rho_dzt"] * T).max() / small.dt
         diff = np.abs(th_all[n] - th).max()
         assert diff <= 1e-13 * scale
         assert diff > 0.0, "the two routines round differently (MOM doc claims bitwise)"
