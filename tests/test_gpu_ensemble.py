@@ -12,7 +12,10 @@ pytestmark = pytest.mark.gpu
 SHAPES = [dict(ni=60, nj=40, nk=8, nts=10, lat_s=10.0, lat_n=14.0, lon_span=6.0, seed=11),
           dict(ni=131, nj=70, nk=12, nts=16, lat_s=-40.0, lat_n=-33.0, lon_span=13.1, seed=12),
           dict(ni=37, nj=23, nk=7, nts=6, lat_s=30.0, lat_n=32.3, lon_span=3.7, seed=13),
-          dict(ni=90, nj=95, nk=10, nts=12, lat_s=-5.0, lat_n=4.5, lon_span=9.0, seed=14)]
+          dict(ni=90, nj=95, nk=10, nts=12, lat_s=-5.0, lat_n=4.5, lon_span=9.0, seed=14),
+          # nk = 21: the ensemble's register-resident FP32 solver runs every
+          # domain with 16 register slots a wavefront, the others alone use 8
+          dict(ni=48, nj=30, nk=21, nts=8, lat_s=20.0, lat_n=23.0, lon_span=4.8, seed=15)]
 
 
 def _members(cases, fresh):
@@ -44,7 +47,7 @@ def test_ensemble_equals_members_alone_and_graph_replays():
     # the members alone and the ensemble share each domain's context (its
     # solver and subcycle scratch), so the two never run at the same time
     from mom5_amd.ensemble import EnsembleStep, make_regional_ensemble
-    print("[ensemble: 4 regional domains, 2 FP64 + 3 FP32 tracers each]", flush=True)
+    print("[ensemble: 5 regional domains, 2 FP64 + 3 FP32 tracers each]", flush=True)
     cases = make_regional_ensemble(SHAPES, n_passive=3)
     alone = _members(cases, fresh=True)
     ens_m = _members(cases, fresh=True)
