@@ -17,9 +17,9 @@ import numpy as np
 import pytest
 
 
-def system(nk, rng, strong):
+def system(nk, rng, strong, kmt=None):
     """One column: a, c (<= 0, a(1) = c(nk) = 0, zero below kmt), T, bfl."""
-    kmt = int(rng.integers(0, nk + 1))
+    kmt = int(rng.integers(0, nk + 1)) if kmt is None else kmt
     d = 1e-4 + (10.0 if strong else 0.01) * rng.random(nk + 1) ** 4
     a = np.zeros(nk + 1)
     c = np.zeros(nk + 1)
@@ -137,10 +137,8 @@ def test_meeting_with_swapped_multipliers_fails():
     nk, H = 20, 16
     bad = 0
     for _ in range(20):
-        kmt, a, c, T, bfl = system(nk, rng, strong=True)
-        if kmt < nk:
-            continue
+        kmt, a, c, T, bfl = system(nk, rng, strong=True, kmt=nk)
         want = dense(nk, kmt, a, c, T, bfl)
         got = twin(nk, H, kmt, a, c, T, bfl, swap_meeting=True)
         bad += not np.allclose(got, want, rtol=1e-9, atol=1e-9)
-    assert bad > 0
+    assert bad == 20
