@@ -384,36 +384,15 @@ __device__ __forceinline__ void bt_tile(const BtJob &J, int tile) {
   }
 }
 
-// Position t of the launch order -> row-major tile index.  Tiles are taken in
-// strips BT_STRIP tiles wide, top to bottom within a strip, so that a tile's
-// y-neighbour follows it BT_STRIP positions later (not a whole tile row) and
-// the halo rows both read are still in L2; the last strip is narrower when
-// BT_STRIP does not divide ntx.  A bijection of [0, ntx*nty).
-constexpr int BT_STRIP = 8;
-__device__ __forceinline__ int bt_strip_tile(int t, int ntx, int nty) {
-  const int full = ntx - ntx % BT_STRIP;   // columns in whole strips
-  int row, col;
-  if (t < full * nty) {
-    const int s = t / (BT_STRIP * nty), r = t - s * (BT_STRIP * nty);
-    row = r / BT_STRIP;
-    col = s * BT_STRIP + r % BT_STRIP;
-  } else {
-    const int w = ntx - full, r = t - full * nty;
-    row = r / w;
-    col = full + r % w;
-  }
-  return row * ntx + col;
-}
-
 // Tiles in XCD-aware order: blocks are dealt to the 8 XCDs round-robin, so
-// each XCD takes per_xcd consecutive positions of the strip order and the
-// halo rows and columns that neighbouring tiles both read are fetched into
-// one L2.
+// each XCD takes per_xcd consecutive positions of the order of 8-wide strips
+// (strip_tile) and the halo rows and columns that neighbouring tiles both
+// read are fetched into one L2.
 __global__ __launch_bounds__(BNT) void k_bt_block(BtJob J, int per_xcd) {
   const int b = blockIdx.x;
   const int t = (b & 7) * per_xcd + (b >> 3);
   if (t >= J.ntiles) return;
-  bt_tile(J, bt_strip_tile(t, J.ntx, J.ntiles / J.ntx));
+  bt_tile(J, strip_tile<8>(t, J.ntx, J.ntiles / J.ntx));
 }
 
 // Ensemble: launch ell of every domain that still has subcycles; the static

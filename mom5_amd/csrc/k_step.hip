@@ -526,9 +526,11 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
 template <typename TT, int G, int RPW, int NW>
 __global__ __launch_bounds__(RX * NW) void k_step(StepJob<TT> J, int per_xcd) {
   const int b = blockIdx.x;
-  const int tile = (b & 7) * per_xcd + (b >> 3);   // XCD-aware: neighbours share an L2
-  if (tile >= J.ntiles) return;
-  step_tile<TT, G, RPW, NW>(J, tile);
+  // XCD-aware: each XCD takes per_xcd consecutive positions of the order of
+  // 8-wide strips, so x- and y-neighbours share their halo in one L2
+  const int t = (b & 7) * per_xcd + (b >> 3);
+  if (t >= J.ntiles) return;
+  step_tile<TT, G, RPW, NW>(J, strip_tile<8>(t, J.ntx, J.ntiles / J.ntx));
 }
 
 // Ensemble pass: a persistent grid takes (job, tile) items from a device-side

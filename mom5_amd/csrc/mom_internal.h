@@ -26,6 +26,27 @@ __host__ __device__ inline int d2(const KDom &d, int gi, int gj) {
   return (gi + 1) + d.NI * (gj + 1);
 }
 
+// Position t of a launch order -> row-major tile index of an ntx x nty grid.
+// Tiles are taken in strips S tiles wide, top to bottom within a strip, so
+// that a tile's y-neighbour follows it S positions later (not a whole tile
+// row) and the halo rows both read are still in L2; the last strip is
+// narrower when S does not divide ntx.  A bijection of [0, ntx*nty).
+template <int S>
+__device__ __forceinline__ int strip_tile(int t, int ntx, int nty) {
+  const int full = ntx - ntx % S;   // columns in whole strips
+  int row, col;
+  if (t < full * nty) {
+    const int s = t / (S * nty), r = t - s * (S * nty);
+    row = r / S;
+    col = s * S + r % S;
+  } else {
+    const int w = ntx - full, r = t - full * nty;
+    row = r / w;
+    col = full + r % w;
+  }
+  return row * ntx + col;
+}
+
 // Wrap of a compute index on a cyclic axis.
 __host__ __device__ inline int wrapi(int g, int n) {
   g %= n;
