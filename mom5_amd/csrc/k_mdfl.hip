@@ -78,10 +78,12 @@ __global__ __launch_bounds__(RX * NW) void k_mdfl(
   __shared__ int s_km[RY][RX];
 
   // Blocks are dealt to the 8 XCDs round-robin; give each XCD a contiguous
-  // run of tiles so that halo re-reads hit its own L2.
+  // run of positions of the order of 8-wide strips (strip_tile), so that the
+  // halo rows and columns that neighbouring tiles both read hit its own L2.
   const int b = blockIdx.x;
-  const int tile = (b & 7) * per_xcd + (b >> 3);
-  if (tile >= ntiles) return;
+  const int t = (b & 7) * per_xcd + (b >> 3);
+  if (t >= ntiles) return;
+  const int tile = strip_tile<8>(t, ntx, ntiles / ntx);
   const int tix = tile % ntx, tiy = tile / ntx;
   const int gi0 = tix * TX, gj0 = tiy * TY;
 
