@@ -785,21 +785,10 @@ static void orc_div_ud(const orc_dom *d, const orc_grid *g, const double *ud,
   }
 }
 
-/* ocean_operators.F90:631-676 GRAD_BAROTROPIC_P (B-grid), data domain. */
-static void orc_grad_p(const orc_dom *d, const orc_grid *g, const double *p,
-                       double *out /* 2 planes */) {
-  for (size_t o = 0; o < 2 * P2; ++o) out[o] = 0.0;
-  for (int j = d->jsd; j <= d->jed - 1; ++j)
-    for (int i = d->isd; i <= d->ied - 1; ++i) {
-      const double p_fay = 0.5 * (p[I2(i, j)] + p[I2(i, j + 1)]);
-      const double p_fax = 0.5 * (p[I2(i, j)] + p[I2(i + 1, j)]);
-      const double pip_fay = 0.5 * (p[I2(i + 1, j)] + p[I2(i + 1, j + 1)]);
-      const double pjp_fax = 0.5 * (p[I2(i, j + 1)] + p[I2(i + 1, j + 1)]);
-      const double um = g->umask[I2(i, j)];
-      out[I2(i, j)] = (pip_fay - p_fay) * g->dxur[I2(i, j)] * um;
-      out[I2(i, j) + P2] = (pjp_fax - p_fax) * g->dyur[I2(i, j)] * um;
-    }
-}
+/* GRAD_BAROTROPIC_P on the B-grid (ocean_operators.F90:631-676), defined
+ * with the barotropic closure below; the subcycle loops use the same one. */
+static void orc_grad_barotropic_p(const orc_dom *d, const orc_grid *g, const double *p,
+                                  double *out);
 
 /* ------------------------------------------------------------------ */
 /* ocean_barotropic.F90:3225-3579 pred_corr_tropic_depth_bgrid with the  */
@@ -856,7 +845,7 @@ void orc_update_ocean_barotropic(const orc_dom *d, const orc_grid *g,
     /* legacy halo update of eta_t_bt(fstaup1) (:3383) */
     orc_halo_update(d, 1, &ETA(0, fstaup1), 1, 3);
     for (size_t o = 0; o < P2; ++o) ps[o] = rho_g[o] * ETA(o, fstaup1) + patm[o];
-    orc_grad_p(d, g, ps, gp);
+    orc_grad_barotropic_p(d, g, ps, gp);
     for (size_t o = 0; o < P2; ++o) {       /* :3464-3486 */
       const double pf1 = -(thicku[o] * gp[o]);
       const double pf2 = -(thicku[o] * gp[o + P2]);
@@ -1290,7 +1279,7 @@ void orc_update_ocean_barotropic_variant(const orc_dom *d, const orc_grid *g,
         }
       orc_grad_p_c(d, ps, dxter, dytnr, tmasken, gp);
     } else {
-      orc_grad_p(d, g, ps, gp);
+      orc_grad_barotropic_p(d, g, ps, gp);
     }
     for (size_t o = 0; o < P2; ++o) {
       double pf[2];

@@ -9,7 +9,10 @@ ocean_operators.F90:507-676):
   walls) rotates U by exactly 2*atan(f*dtbt/2) per subcycle with |U| kept:
   the closed form of the Crank-Nicolson Coriolis step, averaged with the
   stepping-B weights nts-s+1 (or uniform weights for stepping A);
-* constant fields average to themselves (sum of weights = 1/rnts).
+* constant fields average to themselves (sum of weights = 1/rnts);
+* one subcycle with gamma = 0 and f = 0 gives U = -dtbt*H*grad(p_s) exactly
+  for a quadratic surface height (pins the B-grid gradient's sign, index
+  offsets, metric and mask).
 """
 import math
 
@@ -83,3 +86,34 @@ def test_constant_fields_average_to_themselves(oracle_lib):
     assert np.allclose(ud[0][mid], 2.0, rtol=1e-14)
     assert np.allclose(ud[1][mid], 0.0, atol=1e-14)
     assert np.allclose(eb[mid], 0.5, rtol=1e-15)
+
+
+@pytest.mark.parametrize("stepping_a", [False, True])
+def test_bgrid_pressure_gradient_known_answer(oracle_lib, stepping_a):
+    """One subcycle with gamma = 0 and f = 0 on a closed box: the predictor
+    leaves eta unchanged, so U = -dtbt*thicku*GRAD_BAROTROPIC_P(rho_g*eta)
+    (ocean_barotropic.F90:3409,3464-3486; ocean_operators.F90:650-660).
+    For eta quadratic in x and y the B-grid difference of the two-point
+    averages is the exact derivative at the U point (i+1/2, j+1/2), so
+    U = -dtbt*H*rho_g*(2a*x_u, 2b*y_u) at wet U points and 0 elsewhere.
+    A flipped sign, a backward difference (2i-1 in place of 2i+1), dyur in
+    place of dxur (dx != dy) or a missing umask fails."""
+    dx, dy, dtbt, H, rho_g = 2.0e4, 1.5e4, 20.0, 3000.0, 1.0e4
+    c = uniform_case(18, 14, 2, cyclic_x=False, dx=dx, dy=dy, f=0.0)
+    o = oracle_lib.Oracle(c)
+    NJ, NI = c.nj + 2, c.ni + 2
+    jj, ii = np.meshgrid(np.arange(NJ, dtype=np.float64), np.arange(NI, dtype=np.float64),
+                         indexing="ij")
+    a, b = 3.0e-11, -2.0e-11
+    eta = a * (ii * dx) ** 2 + b * (jj * dy) ** 2
+    fl = _bt_fields(c, hu=H)
+    fl.update(eta_t=eta, eta_t_bar=eta, rho_g=np.full((NJ, NI), rho_g))
+    _, ud, _ = o.update_ocean_barotropic(1, dtbt, gamma=0.0, stepping_a=stepping_a, fields=fl)
+    um = c.grid["umask"][0]
+    want_x = -dtbt * H * rho_g * 2.0 * a * (ii + 0.5) * dx * um
+    want_y = -dtbt * H * rho_g * 2.0 * b * (jj + 0.5) * dy * um
+    scale = np.abs(want_x).max()
+    assert um.sum() > 100
+    assert np.abs(ud[0] - want_x).max() <= 1e-12 * scale
+    assert np.abs(ud[1] - want_y).max() <= 1e-12 * np.abs(want_y).max()
+    assert np.all(ud[:, um == 0] == 0)

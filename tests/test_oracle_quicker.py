@@ -7,7 +7,10 @@
   at cell centres is exact (three-point interpolation), in x, y and z;
 * the scheme is in flux form (masked integral of dat*tendency vanishes),
   and zero velocity gives zero tendency;
-* limit_with_upwind switches a flagged face to the first-order upwind flux.
+* limit_with_upwind switches a flagged face to the first-order upwind flux;
+* a constant tracer on a coastal grid with w from continuity has zero total
+  (horizontal + vertical) tendency, which holds only through the coastal and
+  bottom substitutions of the upwind stencil.
 """
 import numpy as np
 import pytest
@@ -118,3 +121,31 @@ def test_limit_with_upwind_uses_first_order_flux(oracle_lib):
     _, (fx, _) = o.horz_quicker(T, T, tmask_limit=lim, limit_with_upwind=True, want_flux=True)
     # u > 0: upwind flux through the east face of i is u*T(i)
     assert np.array_equal(fx[:, 1:-1, 0:-2], 2.0 * T[:, 1:-1, 0:-2])
+
+
+def test_coastal_constancy_with_w_from_continuity(oracle_lib):
+    """T = c*tmask on a grid with coasts and a sloping bottom, w from
+    continuity: horizontal plus vertical Quicker tendency vanishes to
+    round-off.  The centred weights sum to 1 and the curvature weights to 0,
+    so a face flux is vel*c only if every upwind stencil point holds c; next
+    to land that holds only through the coastal substitutions
+    (rnormsk/soutmsk/eastmsk/westmsk, ocean_tracer_advect.F90:2577-2606) and
+    the bottom substitution kp2 (:3010-3011), which replace the land value 0
+    by the wet neighbour.  Dropping any of them leaves an O(c*vel) residual
+    at coastal or bottom cells.  The vertical upmsk (:3010) is 0 at every
+    wet cell of a z-level grid (the cell above a wet cell is wet), so the
+    result does not depend on it."""
+    from mom5_amd.synth import make_case
+    case = make_case(48, 36, 12, nts=10, seed=23, land_frac=0.3, barotropic=False)
+    o = oracle_lib.Oracle(case)
+    c = 4.5
+    T = c * case.grid["tmask"]
+    res = o.horz_quicker(T, T) + o.vert_quicker(T, T)
+    # scale: the size of either part alone for this field
+    scale = np.abs(o.horz_quicker(T, T)).max()
+    assert scale > 0
+    assert np.abs(res).max() <= 1e-12 * scale, np.abs(res).max() / scale
+    # coasts and bottoms are there to test: wet cells with a land neighbour
+    tm = case.grid["tmask"][:, 1:-1, 1:-1]
+    land_nb = (case.grid["tmask"][:, 1:-1, :-2] == 0) | (case.grid["tmask"][:, :-2, 1:-1] == 0)
+    assert np.sum((tm > 0) & land_nb) > 100
