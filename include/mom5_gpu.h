@@ -291,7 +291,9 @@ int mom_gpu_barotropic_forcing(mom_gpu_ctx_t *ctx, const double *accel,
  * no tides, geoid, friction, smoothing or OBC) and the time-averaging tail
  * (:2804-2811, :2846-2888).  udrho_* and forcing_bt hold the two velocity
  * components back to back.  conv_rho_ud_t (= -DIV_UD of the averaged
- * transport) may be NULL.  Supports cyclic_y = 0 only.
+ * transport) may be NULL.  Supports cyclic_y = 0 only.  The outputs must
+ * not alias eta_t_tau, eta_t_bar_tau or udrho_tau (MOM passes other time
+ * levels).
  * Precondition (what mpp_update_domains leaves in MOM on a cyclic-x
  * domain): halo column isd equals column iec and ied equals isc, on every
  * row, for the state and for the 2-D inputs. */
@@ -420,8 +422,14 @@ int mom_gpu_step_tracers(mom_gpu_ctx_t *ctx, const mom_gpu_tracer_step_t *s);
  * tile) items from a device-side queue, largest domain first; the implicit
  * diffusion and each block of barotropic subcycles cover every domain in one
  * grid.  Launches go to the stream of ctxs[0] (capture that context to graph
- * a whole ensemble step); the member contexts must not run work of their own
- * meanwhile.  Results are bit-for-bit those of the per-domain calls. */
+ * a whole ensemble step).  Outside a capture an ensemble call is ordered
+ * after work already queued on every member context and before work queued
+ * on them afterwards; while ctxs[0] is being captured the other members must
+ * be idle (mom_gpu_sync them first), since their streams take no part in the
+ * graph.  Calls that overlap with an ensemble call from another host thread
+ * are not ordered.  Launch tables used while capturing stay allocated until
+ * mom_gpu_ensemble_destroy (at most 64 tables are held).  Results are
+ * bit-for-bit those of the per-domain calls. */
 typedef struct mom_gpu_ensemble mom_gpu_ensemble_t;
 
 /* The 2-D barotropic fields of one domain (as mom_gpu_update_ocean_barotropic). */
@@ -436,7 +444,8 @@ int mom_gpu_ensemble_destroy(mom_gpu_ensemble_t *ens);
 /* steps[d]: domain d's batched tracer step (see mom_gpu_step_tracers). */
 int mom_gpu_ensemble_step_tracers(mom_gpu_ensemble_t *ens, const mom_gpu_tracer_step_t *steps);
 /* p[d], f[d]: domain d's barotropic parameters and fields (ndom <= 64,
- * cyclic_y = 0 for every domain). */
+ * cyclic_y = 0 for every domain; outputs must not alias eta_t_tau,
+ * eta_t_bar_tau or udrho_tau). */
 int mom_gpu_ensemble_update_barotropic(mom_gpu_ensemble_t *ens, const mom_gpu_bt_params_t *p,
                                        const mom_gpu_bt_fields_t *f);
 

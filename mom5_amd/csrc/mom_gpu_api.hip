@@ -299,6 +299,7 @@ static void free_ctx(mom_gpu_ctx *c) {
     if (c->ev[i]) hipEventDestroy(c->ev[i]);
   if (c->side_done) hipEventDestroy(c->side_done);
   if (c->fork_ev) hipEventDestroy(c->fork_ev);
+  if (c->join_ev) hipEventDestroy(c->join_ev);
   if (c->side) hipStreamDestroy(c->side);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -365,6 +366,7 @@ int mom_gpu_create(const mom_gpu_domain_t *dom, const mom_gpu_grid_t *grid, mom_
   CK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
   CK(hipEventCreateWithFlags(&c->side_done, hipEventDisableTiming));
   CK(hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
+  CK(hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming));
   struct {
     double **dst;
     const double *src;
@@ -446,19 +448,31 @@ int mom_gpu_elapsed(mom_gpu_ctx_t *ctx, int s0, int s1, float *ms) {
 }  // extern "C"
 
 const void *ens_table(mom_gpu_ensemble *e, const void *host, size_t bytes) {
-  for (auto &t : e->tables)
-    if (t.host.size() == bytes && std::memcmp(t.host.data(), host, bytes) == 0) return t.dev;
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-  if (hipStreamIsCapturing(e->lead->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) {
+  if (hipStreamIsCapturing(e->lead->stream, &cs) != hipSuccess) {
+    mom_fail(e->lead, MOM_GPU_ERR_HIP, "ensemble: hipStreamIsCapturing failed");
+    return nullptr;
+  }
+  const bool capturing = cs != hipStreamCaptureStatusNone;
+  const int hit = e->tables.lookup(host, bytes, capturing);
+  if (hit >= 0) return e->tables.entries[hit].dev;
+  if (capturing) {
     mom_fail(e->lead, MOM_GPU_ERR_ARG,
              "ensemble: a new launch table while capturing; run one step of each time-level "
              "parity before capturing");
     return nullptr;
   }
-  if (e->tables.size() >= 64) {             // keep the cache bounded
+  const int v = e->tables.victim();       // keep the cache bounded
+  if (v == -2) {
+    mom_fail(e->lead, MOM_GPU_ERR_ARG,
+             "ensemble: %zu launch tables are held by captured graphs; destroy the ensemble "
+             "(and its graphs) to release them", e->tables.entries.size());
+    return nullptr;
+  }
+  if (v >= 0) {
     if (hipStreamSynchronize(e->lead->stream) != hipSuccess) return nullptr;
-    (void)hipFree(e->tables.front().dev);
-    e->tables.erase(e->tables.begin());
+    (void)hipFree(e->tables.entries[v].dev);
+    e->tables.erase(v);
   }
   void *dev = nullptr;
   hipError_t err = hipMalloc(&dev, bytes);
@@ -468,9 +482,40 @@ const void *ens_table(mom_gpu_ensemble *e, const void *host, size_t bytes) {
     mom_fail(e->lead, MOM_GPU_ERR_HIP, "ensemble table: %s", hipGetErrorString(err));
     return nullptr;
   }
-  const unsigned char *h = (const unsigned char *)host;
-  e->tables.push_back({std::vector<unsigned char>(h, h + bytes), dev});
+  e->tables.add(host, bytes, dev);
   return dev;
+}
+
+// Ensemble calls run on the lead stream.  Outside a capture, the lead stream
+// first waits for work already queued on every member's stream, and each
+// member stream afterwards waits for the ensemble's work, so calls on a
+// member context before and after an ensemble call stay ordered with it.
+// During a capture the members must be idle (synchronised) instead: their
+// streams take no part in the graph.
+static int ens_capturing(mom_gpu_ensemble *e, bool *capturing) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  MOM_HIP(e->lead, hipStreamIsCapturing(e->lead->stream, &cs));
+  *capturing = cs != hipStreamCaptureStatusNone;
+  return MOM_GPU_OK;
+}
+
+static int ens_join_members(mom_gpu_ensemble *e, bool capturing) {
+  for (mom_gpu_ctx *c : e->ctx) {
+    side_join(c);                           // members' pending second-stream work
+    if (capturing || c->stream == e->lead->stream) continue;
+    MOM_HIP(e->lead, hipEventRecord(c->join_ev, c->stream));
+    MOM_HIP(e->lead, hipStreamWaitEvent(e->lead->stream, c->join_ev, 0));
+  }
+  return MOM_GPU_OK;
+}
+
+static int ens_release_members(mom_gpu_ensemble *e, bool capturing) {
+  if (capturing) return MOM_GPU_OK;
+  MOM_HIP(e->lead, hipEventRecord(e->lead->join_ev, e->lead->stream));
+  for (mom_gpu_ctx *c : e->ctx)
+    if (c->stream != e->lead->stream)
+      MOM_HIP(e->lead, hipStreamWaitEvent(c->stream, e->lead->join_ev, 0));
+  return MOM_GPU_OK;
 }
 
 extern "C" {
@@ -498,7 +543,7 @@ int mom_gpu_ensemble_create(int ndom, mom_gpu_ctx_t *const *ctxs, mom_gpu_ensemb
 int mom_gpu_ensemble_destroy(mom_gpu_ensemble_t *e) {
   if (!e) return MOM_GPU_OK;
   (void)hipStreamSynchronize(e->lead->stream);
-  for (auto &t : e->tables) (void)hipFree(t.dev);
+  for (auto &t : e->tables.entries) (void)hipFree(t.dev);
   if (e->counters) (void)hipFree(e->counters);
   delete e;
   return MOM_GPU_OK;
@@ -508,20 +553,17 @@ static int check_step(mom_gpu_ctx *ctx, const mom_gpu_tracer_step_t *s, const ch
 
 int mom_gpu_ensemble_step_tracers(mom_gpu_ensemble_t *e, const mom_gpu_tracer_step_t *steps) {
   REQUIRE(nullptr, e && steps, "mom_gpu_ensemble_step_tracers: NULL argument");
-  for (mom_gpu_ctx *c : e->ctx) {    // members' pending second-stream work
-    if (c->side_pending) {
-      MOM_HIP(e->lead, hipStreamWaitEvent(e->lead->stream, c->side_done, 0));
-      c->side_pending = false;
-    }
-    c->fork_valid = false;
-  }
   for (size_t d = 0; d < e->ctx.size(); ++d) {
     const int rc = check_step(e->ctx[d], &steps[d], "mom_gpu_ensemble_step_tracers");
     if (rc) return mom_fail(e->lead, rc, "domain %zu: %s", d, e->ctx[d]->err);
   }
-  int rc = launch_step_ensemble(e, steps);
-  if (rc) return rc;
-  return launch_vdiff_ensemble(e, steps);
+  bool capturing = false;
+  int rc = ens_capturing(e, &capturing);
+  if (!rc) rc = ens_join_members(e, capturing);
+  if (!rc) rc = launch_step_ensemble(e, steps);
+  if (!rc) rc = launch_vdiff_ensemble(e, steps);
+  if (!rc) rc = ens_release_members(e, capturing);
+  return rc;
 }
 
 int mom_gpu_ensemble_update_barotropic(mom_gpu_ensemble_t *e, const mom_gpu_bt_params_t *p,
@@ -538,8 +580,20 @@ int mom_gpu_ensemble_update_barotropic(mom_gpu_ensemble_t *e, const mom_gpu_bt_p
                          x.eta_t_bar_taup1 && x.udrho_taup1 &&
                          (p[d].initsum_with_bar || x.eta_t_tau),
             "mom_gpu_ensemble_update_barotropic: domain %zu: NULL required field", d);
+    const void *outs[] = {x.eta_t_bar_taup1, x.udrho_taup1, x.conv_rho_ud_t};
+    const void *ins[] = {x.eta_t_tau, x.eta_t_bar_tau, x.udrho_tau};
+    for (const void *o : outs)
+      for (const void *i : ins)
+        REQUIRE(e->lead, !o || o != i,
+                "mom_gpu_ensemble_update_barotropic: domain %zu: an output aliases eta_t_tau, "
+                "eta_t_bar_tau or udrho_tau", d);
   }
-  return launch_barotropic_ensemble(e, p, f);
+  bool capturing = false;
+  int rc = ens_capturing(e, &capturing);
+  if (!rc) rc = ens_join_members(e, capturing);
+  if (!rc) rc = launch_barotropic_ensemble(e, p, f);
+  if (!rc) rc = ens_release_members(e, capturing);
+  return rc;
 }
 
 // ---- hipGraph capture of whole steps -----------------------------------------
@@ -772,6 +826,15 @@ int mom_gpu_update_ocean_barotropic(mom_gpu_ctx_t *ctx, const mom_gpu_bt_params_
   REQUIRE(ctx, p->nts >= 1, "mom_gpu_update_ocean_barotropic: nts must be >= 1");
   REQUIRE(ctx, p->dtbt > 0.0 && p->rho0 > 0.0, "mom_gpu_update_ocean_barotropic: bad dtbt/rho0");
   REQUIRE(ctx, !ctx->kd.cyclic_y, "mom_gpu_update_ocean_barotropic: cyclic_y is not supported");
+  {
+    const void *outs[] = {eta_t_bar_taup1, udrho_taup1, conv_rho_ud_t};
+    const void *ins[] = {eta_t_tau, eta_t_bar_tau, udrho_tau};
+    for (const void *o : outs)
+      for (const void *i : ins)
+        REQUIRE(ctx, !o || o != i,
+                "mom_gpu_update_ocean_barotropic: an output aliases eta_t_tau, eta_t_bar_tau "
+                "or udrho_tau");
+  }
   return launch_barotropic(ctx, p, eta_t_tau ? eta_t_tau : eta_t_bar_tau, eta_t_bar_tau, udrho_tau,
                            forcing_bt, thicku, patm, pme, river, source, eta_smooth, rho_g,
                            eta_t_bar_taup1, udrho_taup1, conv_rho_ud_t);

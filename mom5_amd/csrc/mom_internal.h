@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "mom5_gpu.h"
+#include "table_cache.h"
 
 // Compact domain description passed to kernels by value.  Compute-domain
 // indices are 0-based (gi in [0, ni), gj in [0, nj)); the data domain has a
@@ -100,6 +101,8 @@ struct mom_gpu_ctx {
   // main stream wait for pending side work.
   hipStream_t side;
   hipEvent_t side_done, fork_ev;
+  // ordering of ensemble calls against the member's own stream
+  hipEvent_t join_ev;
   bool side_pending, fork_valid;
   // device copies of the grid (owned)
   double *dxt, *dyt, *dat, *datr, *dxte, *dyte, *dxtn, *dytn;
@@ -254,16 +257,13 @@ struct mom_gpu_ensemble {
   mom_gpu_ctx *lead = nullptr;
   int *counters = nullptr;          // device work-queue counters
   static constexpr int NCOUNT = 16;
-  struct Table {
-    std::vector<unsigned char> host;
-    void *dev;
-  };
-  std::vector<Table> tables;        // launch tables, shared by content
+  TableCache<void *> tables;        // launch tables, shared by content
 };
 
 // Device copy of a launch table; identical tables (the two parities of
 // two-level time stepping) share one copy.  Uploading a new table is refused
-// while the lead stream is being captured.  nullptr on failure.
+// while the lead stream is being captured; tables used during a capture are
+// kept until the ensemble is destroyed.  nullptr on failure.
 const void *ens_table(mom_gpu_ensemble *e, const void *host, size_t bytes);
 int launch_step_ensemble(mom_gpu_ensemble *e, const mom_gpu_tracer_step_t *steps);
 int launch_vdiff_ensemble(mom_gpu_ensemble *e, const mom_gpu_tracer_step_t *steps);

@@ -68,3 +68,33 @@ def bt_variant_fields(c, rng, cgrid, scale=1.0):
                 rho_g=np.full(sh, 1035.0 * 9.80), dpatm_dt=np.zeros(sh),
                 pme=1e-4 * rng.standard_normal(sh) * tm, river=np.zeros(sh), source=np.zeros(sh),
                 smooth=np.zeros(sh))
+
+
+def oracle_tracer_step(orc, Ts, stfs, btfs, dt, *, index_salt=1, ths=None, fields=None):
+    """The oracle's sequence for one batched tracer step of FP64 tracers Ts:
+    per tracer th = (other tendencies or 0) + MDFL-Sweby advection, the
+    explicit update, then the implicit vertical diffusion of all of them
+    (ocean_tracer.F90:2295-2350, 2432).  Returns the T(taup1) arrays."""
+    outs = []
+    for n, T in enumerate(Ts):
+        th = np.zeros_like(T) if ths is None or ths[n] is None else np.array(ths[n], copy=True)
+        orc.horz_advect_mdfl(T, th, dt, lim=1.0, fields=fields)
+        outs.append(orc.update_explicit(T, th, dt, fields=fields))
+    orc.vert_diffuse_implicit(outs, stfs, btfs, dt, index_salt=index_salt, fields=fields)
+    return outs
+
+
+def host_case(case):
+    """A host copy of a case whose 3-D fields and tracers live on the device
+    (mom5_amd.synth.make_case_device), with the 3-D masks the oracle reads."""
+    import dataclasses
+    grid = dict(case.grid)
+    nk = case.nk
+    kk = np.arange(1, nk + 1)[:, None, None]
+    grid["tmask"] = (kk <= grid["kmt"][None]).astype(np.float64)
+    grid["umask"] = (kk <= grid["kmu"][None]).astype(np.float64)
+    conv = lambda v: v.to_host() if hasattr(v, "to_host") else v
+    fields = {k: (conv(v) if not isinstance(v, list) else [conv(x) for x in v])
+              for k, v in case.fields.items()}
+    tracers = [(nm, conv(t).astype(np.float64)) for nm, t in case.tracers]
+    return dataclasses.replace(case, grid=grid, fields=fields, tracers=tracers)

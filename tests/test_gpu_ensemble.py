@@ -80,3 +80,55 @@ def test_ensemble_equals_members_alone_and_graph_replays():
         _assert_same(alone, ens_m, f"replay {r}")
     graph.close()
     ens.close()
+
+
+def test_ensemble_members_match_oracle():
+    """Every member of an ensemble stepped three times against the oracle's
+    own step sequence on that member's domain (MDFL-Sweby advection,
+    explicit update, implicit vertical diffusion, B-grid barotropic
+    subcycles with their time averages): T and S and the barotropic state
+    bit for bit, the FP32 passive tracers within 1e-5 of max |T| (the FP64
+    oracle carries them in FP64).  One member also adds other th_tendency
+    terms."""
+    from mom5_amd import DeviceArray
+    from mom5_amd.ensemble import EnsembleStep, make_regional_ensemble
+    from oracle.oracle import Oracle
+    from _cases import host_case, oracle_tracer_step
+    nsteps = 3
+    print("[ensemble vs oracle: 5 regional domains, 2 FP64 + 3 FP32 tracers, 3 steps]",
+          flush=True)
+    cases = make_regional_ensemble(SHAPES, n_passive=3)
+    members = _members(cases, fresh=True)
+    rng = np.random.default_rng(5)
+    th_host = {2: [1e-3 * rng.standard_normal(members[2].tracer(n).shape)
+                   for n in range(members[2].ntr)]}
+    members[2].th = [DeviceArray.from_host(t) for t in th_host[2]]
+    # the oracle's initial state: each member's initial arrays widened to FP64
+    init = [[m.tracer(n).to_host().astype(np.float64) for n in range(m.ntr)] for m in members]
+    ens = EnsembleStep(members)
+    ens.run(nsteps)
+    ens.gpu.sync()
+    for k, (c, m) in enumerate(zip(cases, members)):
+        hc = host_case(c)
+        o = Oracle(hc)
+        f = hc.fields
+        Ts = init[k]
+        eb, ud = f["eta_t_bar"].copy(), f["udrho"].copy()
+        for _ in range(nsteps):
+            Ts = oracle_tracer_step(o, Ts, f["stf"][:m.ntr], f["btf"][:m.ntr], hc.dt,
+                                    ths=th_host.get(k))
+            fl = dict(f)
+            fl.update(eta_t=eb, eta_t_bar=eb, udrho=ud)
+            eb, ud, _ = o.update_ocean_barotropic(hc.nts, hc.dtbt, fields=fl)
+        wet = hc.grid["tmask"][:, 1:-1, 1:-1] > 0
+        for n in range(m.ntr):
+            got = m.tracer(n).to_host()[:, 1:-1, 1:-1]
+            ref = Ts[n][:, 1:-1, 1:-1]
+            if n < 2:
+                assert np.array_equal(got, ref), (k, n)
+            else:
+                err = np.abs(got.astype(np.float64)[wet] - ref[wet]).max()
+                assert err <= 1e-5 * np.abs(ref[wet]).max(), (k, n, err)
+        assert np.array_equal(m.eta_bar[m.cur].to_host(), eb), k
+        assert np.array_equal(m.udrho[m.cur].to_host(), ud), k
+    ens.close()

@@ -316,3 +316,28 @@ def test_fp32_large_groups_bitwise_and_accurate(oracle_lib, cyclic):
         err = np.abs(base[n].astype(np.float64) - r).max() / np.abs(r).max()
         assert err <= 1e-5, (n, err)
     gpu.close()
+
+
+@pytest.mark.parametrize("rows,waves", [(1, 16), (2, 8), (4, 8)])
+def test_fp64_full_groups_of_four(oracle_lib, rows, waves):
+    """Eight FP64 tracers at step_group_f64 = 4: two full passes of the
+    four-tracer kernel, against the oracle's per-tracer sequence."""
+    from mom5_amd import DeviceArray, OceanGPU
+    from mom5_amd.synth import make_case
+    from _cases import oracle_tracer_step
+    case = make_case(97, 45, 9, land_frac=0.3, nts=8, seed=44, n_passive=6, stf_scale=1e-3)
+    print(f"[8 FP64 tracers 97x45x9, group 4, {rows} rows x {waves} waves]", flush=True)
+    gpu = OceanGPU(case.dom, case.grid)
+    gpu.set_tuning("step_group_f64", 4)
+    gpu.set_tuning("step_rows_per_wave", rows)
+    gpu.set_tuning("step_waves_f64", waves)
+    dev = {k: DeviceArray.from_host(v) for k, v in case.fields.items()
+           if isinstance(v, np.ndarray)}
+    Ts = [t for _, t in case.tracers]
+    assert len(Ts) == 8
+    got = _run_step(case, gpu, dev, Ts, [np.float64] * 8)
+    gpu.close()
+    f = case.fields
+    ref = oracle_tracer_step(oracle_lib.Oracle(case), Ts, f["stf"][:8], f["btf"][:8], case.dt)
+    for n in range(8):
+        assert np.array_equal(got[n][:, 1:-1, 1:-1], ref[n][:, 1:-1, 1:-1]), n
