@@ -255,6 +255,15 @@ int mom_gpu_vert_diffuse_implicit(mom_gpu_ctx_t *ctx, int ntr,
 
 /* ---- advection velocity ------------------------------------------------ */
 
+/* Replaces mpp_update_domains(field, Dom%domain2d) on a single PE
+ * (shared/mpp/include/mpp_do_update.h:186-288, ocean_core/ocean_domains.F90:
+ * 123-230) for a device-resident field of nz levels on the data domain: a
+ * halo column of a compute row takes its cyclic-x image, a halo row of a
+ * compute column its cyclic-y image, a corner only when both axes are
+ * cyclic; halo points on a closed axis are left as they are.  MOM calls it
+ * on T(taup1) after the tracer update (ocean_model.F90:2049). */
+int mom_gpu_update_domains(mom_gpu_ctx_t *ctx, double *field, int nz);
+
 /* Replaces the B-grid branch of ocean_advection_velocity
  * (ocean_core/ocean_advection_velocity.F90:568-672; no blobs, OBC or
  * prescribed inflow): the T-cell advection transports from the U-cell
@@ -378,11 +387,11 @@ int mom_gpu_update_ocean_barotropic_variant(mom_gpu_ctx_t *ctx, const mom_gpu_bt
 
 /* ---- batched tracer step ------------------------------------------------ */
 
-/* All MDFL-Sweby tracers of one baroclinic step: replaces the tracer loop of
+/* All tracers of one baroclinic step: replaces the tracer loop of
  * update_ocean_tracer (ocean_tracers/ocean_tracer.F90:2295-2350: advection
  * into th_tendency and the explicit update to taup1) followed by
  * vert_diffuse_implicit (:2432, ocean_param/vertical/ocean_vert_mix.F90:
- * 3426-3513), for tracers whose horizontal scheme is MDFL_SWEBY.
+ * 3426-3513), for tracers whose advection scheme is MDFL_SWEBY or QUICKER.
  *
  * Tracers are advanced in groups that share one pass over the shared
  * fields.  dtype[n] = 8: FP64 storage, reference operation order --
@@ -396,7 +405,23 @@ int mom_gpu_update_ocean_barotropic_variant(mom_gpu_ctx_t *ctx, const mom_gpu_bt
  * rho_dzt_taum1 (default rho_dzt_tau, two-level stepping), rho_dztr
  * (default 1/(rho_dzt_tau+epsln), ocean_thickness.F90:1567),
  * rho_dzt_taup1 (default rho_dzt_tau), stf/btf (NULL: no implicit vertical
- * diffusion). */
+ * diffusion).
+ *
+ * scheme (host array, NULL: every tracer MOM_ADVECT_MDFL_SWEBY) selects
+ * MOM_ADVECT_QUICKER per tracer (FP64 storage only).  A Quicker tracer gets
+ * horz_advect_tracer and vert_advect_tracer (ocean_tracer_advect.F90:
+ * 2538-2653, 2981-3031) into th_tendency, with its tmask_limit
+ * (compute_tmask_limit, ocean_tracer.F90:2788-2892) formed inside the
+ * kernel from T_taum1 (= T(tau)) and the range min_limit[n]..max_limit[n];
+ * with tmask_limit_ts_same and index_temp, index_salt >= 0 the temperature
+ * and salinity flags are OR'ed (the partner's T_taum1 is read);
+ * limit_with_upwind applies the upwind fallback to the horizontal fluxes
+ * (the vertical one always uses it).  Values beyond the compute domain are
+ * read as the halo update leaves them (cyclic wrap; nothing beyond a closed
+ * boundary), so T_taup1 equals bit for bit mom_gpu_compute_tmask_limit,
+ * mom_gpu_horz/vert_advect_tracer into a zeroed th_tendency (plus
+ * th_tendency[n]) and mom_gpu_update_tracer_explicit on halo-updated
+ * inputs.  Ensembles take MDFL-Sweby tracers only. */
 typedef struct {
   int ntr;
   const void *const *T_taum1;       /* host table of device pointers */
@@ -410,6 +435,11 @@ typedef struct {
   const double *rho_dzt_taum1, *rho_dztr, *rho_dzt_taup1;
   const double *dzwt, *diff_cbt;
   double rho0, dtime, aidif;
+  const int *scheme;                /* host array or NULL */
+  const double *min_limit, *max_limit;  /* host arrays, read for Quicker tracers */
+  int index_temp;                   /* 0-based, -1: none */
+  int tmask_limit_ts_same;
+  int limit_with_upwind;
 } mom_gpu_tracer_step_t;
 
 int mom_gpu_step_tracers(mom_gpu_ctx_t *ctx, const mom_gpu_tracer_step_t *s);

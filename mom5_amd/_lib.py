@@ -51,7 +51,10 @@ class TracerStep(C.Structure):
                 ("uhrho_et", _vp), ("vhrho_nt", _vp), ("wrho_bt", _vp), ("rho_dzt_tau", _vp),
                 ("rho_dzt_taum1", _vp), ("rho_dztr", _vp), ("rho_dzt_taup1", _vp),
                 ("dzwt", _vp), ("diff_cbt", _vp), ("rho0", C.c_double), ("dtime", C.c_double),
-                ("aidif", C.c_double)]
+                ("aidif", C.c_double), ("scheme", C.POINTER(C.c_int)),
+                ("min_limit", C.POINTER(C.c_double)), ("max_limit", C.POINTER(C.c_double)),
+                ("index_temp", C.c_int), ("tmask_limit_ts_same", C.c_int),
+                ("limit_with_upwind", C.c_int)]
 
 
 class BtFields(C.Structure):
@@ -92,6 +95,7 @@ _PROTOS = {
     "mom_gpu_graph_destroy": (C.c_int, [_vp]),
     "mom_gpu_memset": (C.c_int, [_vp, C.c_int, C.c_size_t]),
     "mom_gpu_fill": (C.c_int, [_vp, _vp, C.c_double, C.c_size_t]),
+    "mom_gpu_update_domains": (C.c_int, [_vp, _vp, C.c_int]),
     "mom_gpu_set_tuning": (C.c_int, [_vp, C.c_char_p, C.c_int]),
     "mom_gpu_get_tuning": (C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_int)]),
     "mom_gpu_mark": (C.c_int, [_vp, C.c_int]),

@@ -1173,6 +1173,38 @@ int launch_copy(mom_gpu_ctx *ctx, double *dst, const double *src, size_t n) {
   return MOM_GPU_OK;
 }
 
+// ---- single-PE halo update (mpp_update_domains, shared/mpp/include/
+// mpp_do_update.h:186-288): one thread per halo point of the data domain
+// and level; a point the update leaves alone (closed axis) is not written.
+__global__ void k_update_domains(KDom d, double *__restrict__ a, int nz) {
+  const int h = blockIdx.x * blockDim.x + threadIdx.x;     // halo point index
+  const int nhalo = 2 * d.NI + 2 * d.nj;                   // halo rows + halo columns
+  if (h >= nhalo) return;
+  int ii, jj;
+  if (h < 2 * d.NI) {              // rows jj = 0 and NJ-1, every column
+    ii = h % d.NI;
+    jj = (h < d.NI) ? 0 : d.NJ - 1;
+  } else {                         // columns ii = 0 and NI-1 of compute rows
+    const int c = h - 2 * d.NI;
+    jj = 1 + c % d.nj;
+    ii = (c < d.nj) ? 0 : d.NI - 1;
+  }
+  int si, sj;
+  if (!halo_source(d, ii, jj, si, sj) || (si == ii && sj == jj)) return;
+  const size_t o = (size_t)ii + (size_t)d.NI * jj, s = (size_t)si + (size_t)d.NI * sj;
+  const int k = blockIdx.y;
+  if (k < nz) a[o + (size_t)d.P2 * k] = a[s + (size_t)d.P2 * k];
+}
+
+int launch_update_domains(mom_gpu_ctx *ctx, double *a, int nz) {
+  const KDom &d = ctx->kd;
+  const int nhalo = 2 * d.NI + 2 * d.nj;
+  hipLaunchKernelGGL(k_update_domains, dim3((nhalo + 255) / 256, nz), dim3(256), 0, ctx->stream,
+                     d, a, nz);
+  MOM_LAUNCHED(ctx, "k_update_domains");
+  return MOM_GPU_OK;
+}
+
 int launch_fill(mom_gpu_ctx *ctx, double *p, double v, size_t n) {
   if (n == 0) return MOM_GPU_OK;
   const size_t nb = (n + 255) / 256;

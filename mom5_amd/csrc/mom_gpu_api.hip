@@ -183,6 +183,15 @@ int mom_gpu_fill(mom_gpu_ctx_t *ctx, double *ptr, double value, size_t n) {
   return launch_fill(ctx, ptr, value, n);
 }
 
+int mom_gpu_update_domains(mom_gpu_ctx_t *ctx, double *field, int nz) {
+  REQUIRE(nullptr, ctx, "mom_gpu_update_domains: NULL context");
+  side_join(ctx);
+  REQUIRE(ctx, field || nz == 0, "mom_gpu_update_domains: NULL field");
+  REQUIRE(ctx, nz >= 0 && nz <= 65535, "mom_gpu_update_domains: nz=%d outside [0, 65535]", nz);
+  if (nz == 0) return MOM_GPU_OK;
+  return launch_update_domains(ctx, field, nz);
+}
+
 int mom_gpu_advection_velocity(mom_gpu_ctx_t *ctx, const double *u, const double *rho_dzu,
                                const double *pme, const double *river,
                                const double *rho_dzt_tendency, const double *mass_source,
@@ -556,6 +565,11 @@ int mom_gpu_ensemble_step_tracers(mom_gpu_ensemble_t *e, const mom_gpu_tracer_st
   for (size_t d = 0; d < e->ctx.size(); ++d) {
     const int rc = check_step(e->ctx[d], &steps[d], "mom_gpu_ensemble_step_tracers");
     if (rc) return mom_fail(e->lead, rc, "domain %zu: %s", d, e->ctx[d]->err);
+    if (steps[d].scheme)
+      for (int n = 0; n < steps[d].ntr; ++n)
+        REQUIRE(e->lead, steps[d].scheme[n] == MOM_ADVECT_MDFL_SWEBY,
+                "mom_gpu_ensemble_step_tracers: domain %zu: ensembles take MDFL-Sweby tracers "
+                "only (tracer %d)", d, n);
   }
   bool capturing = false;
   int rc = ens_capturing(e, &capturing);
@@ -849,9 +863,26 @@ static int check_step(mom_gpu_ctx *ctx, const mom_gpu_tracer_step_t *s, const ch
   REQUIRE(ctx, s->dtime > 0.0, "%s: dtime must be positive", who);
   for (int n = 0; n < s->ntr; ++n) {
     REQUIRE(ctx, s->dtype[n] == 8 || s->dtype[n] == 4, "%s: dtype[%d]=%d", who, n, s->dtype[n]);
+    if (s->scheme) {
+      REQUIRE(ctx, s->scheme[n] == MOM_ADVECT_MDFL_SWEBY || s->scheme[n] == MOM_ADVECT_QUICKER,
+              "%s: tracer %d: scheme %d (MDFL_SWEBY or QUICKER)", who, n, s->scheme[n]);
+      REQUIRE(ctx, s->scheme[n] != MOM_ADVECT_QUICKER || s->dtype[n] == 8,
+              "%s: tracer %d: Quicker tracers are FP64", who, n);
+      REQUIRE(ctx, s->scheme[n] != MOM_ADVECT_QUICKER || (s->min_limit && s->max_limit),
+              "%s: tracer %d: Quicker needs min_limit and max_limit", who, n);
+    }
     REQUIRE(ctx, s->T_taum1[n] && s->T_taup1[n], "%s: tracer %d NULL", who, n);
     REQUIRE(ctx, s->T_taum1[n] != (const void *)s->T_taup1[n],
             "%s: tracer %d: taum1 and taup1 must differ", who, n);
+  }
+  if (s->scheme && s->tmask_limit_ts_same) {
+    REQUIRE(ctx, s->index_temp < s->ntr && s->index_salt < s->ntr,
+            "%s: index_temp/index_salt out of range", who);
+    const bool tq = s->index_temp >= 0 && s->scheme[s->index_temp] == MOM_ADVECT_QUICKER;
+    const bool sq = s->index_salt >= 0 && s->scheme[s->index_salt] == MOM_ADVECT_QUICKER;
+    REQUIRE(ctx, !(tq || sq) || (s->index_temp >= 0 && s->index_salt >= 0 &&
+                                 s->dtype[s->index_temp] == 8 && s->dtype[s->index_salt] == 8),
+            "%s: tmask_limit_ts_same needs FP64 temperature and salinity", who);
   }
   if (s->stf && s->btf && s->aidif != 0.0) {
     REQUIRE(ctx, s->dzwt && s->diff_cbt, "%s: dzwt/diff_cbt required", who);
@@ -866,10 +897,27 @@ static int check_step(mom_gpu_ctx *ctx, const mom_gpu_tracer_step_t *s, const ch
 // vertical diffusion.
 static int step_part(mom_gpu_ctx *ctx, const mom_gpu_tracer_step_t *s, const int *dtype) {
   const double *rm1 = s->rho_dzt_taum1 ? s->rho_dzt_taum1 : s->rho_dzt_tau;
-  int rc = launch_tracer_step(ctx, s->ntr, s->T_taum1, s->T_taup1, dtype, s->th_tendency,
+  // MDFL-Sweby tracers through the grouped passes, Quicker ones one pass each
+  std::vector<int> d_mdfl(dtype, dtype + s->ntr), q(s->ntr, 0);
+  bool any_q = false;
+  if (s->scheme)
+    for (int n = 0; n < s->ntr; ++n)
+      if (dtype[n] && s->scheme[n] == MOM_ADVECT_QUICKER) {
+        d_mdfl[n] = 0;
+        q[n] = MOM_ADVECT_QUICKER;
+        any_q = true;
+      }
+  int rc = launch_tracer_step(ctx, s->ntr, s->T_taum1, s->T_taup1, d_mdfl.data(), s->th_tendency,
                               s->uhrho_et, s->vhrho_nt, s->wrho_bt, s->rho_dzt_tau, rm1,
                               s->rho_dztr, s->dtime);
   if (rc) return rc;
+  if (any_q) {
+    rc = launch_quicker_step(ctx, s->ntr, s->T_taum1, s->T_taup1, q.data(), s->th_tendency,
+                             s->min_limit, s->max_limit, s->index_temp, s->index_salt,
+                             s->tmask_limit_ts_same, s->limit_with_upwind, s->uhrho_et,
+                             s->vhrho_nt, s->wrho_bt, s->rho_dzt_tau, rm1, s->rho_dztr, s->dtime);
+    if (rc) return rc;
+  }
   if (s->stf && s->btf && s->aidif != 0.0) {
     const double *dh = s->rho_dzt_taup1 ? s->rho_dzt_taup1 : s->rho_dzt_tau;
     rc = launch_vdiff_mixed(ctx, s->ntr, s->T_taup1, dtype, s->stf, s->btf, s->diff_cbt,

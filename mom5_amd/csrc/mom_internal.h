@@ -197,6 +197,11 @@ int launch_tracer_step(mom_gpu_ctx *ctx, int ntr, const void *const *T_in, void 
                        const int *dtype, const double *const *th_in, const double *u,
                        const double *v, const double *w, const double *rho,
                        const double *rho_taum1, const double *rho_dztr, double dtime);
+int launch_quicker_step(mom_gpu_ctx *ctx, int ntr, const void *const *T_in, void *const *T_out,
+                        const int *scheme, const double *const *th_in, const double *mn,
+                        const double *mx, int it, int is, int ts_same, int lwu,
+                        const double *u, const double *v, const double *w, const double *rho,
+                        const double *rho_taum1, const double *rho_dztr, double dtime);
 int launch_tmask_limit(mom_gpu_ctx *ctx, int ntr, const double *const *T,
                        const double *mn, const double *mx, int it, int is,
                        int ts_same, double *const *lim);
@@ -232,6 +237,7 @@ int launch_content(mom_gpu_ctx *ctx, const void *T, int dtype, const double *rho
 int launch_tracer_error(mom_gpu_ctx *ctx, const void *a, int dtype, const double *b,
                         double floor, double *out);
 int launch_fill(mom_gpu_ctx *ctx, double *p, double v, size_t n);
+int launch_update_domains(mom_gpu_ctx *ctx, double *a, int nz);
 int launch_bt_forcing(mom_gpu_ctx *ctx, const double *accel, const double *smf, const double *bmf,
                       double *forcing_bt);
 int launch_bt_variant(mom_gpu_ctx *ctx, const mom_gpu_bt_params_t *p,

@@ -65,7 +65,9 @@ module mom_gpu_mod
 
   ! the tracer loop of update_ocean_tracer + vert_diffuse_implicit
   ! (ocean_tracers/ocean_tracer.F90:2295-2350,2432): host tables of device
-  ! addresses (c_loc of type(c_ptr) arrays), dtype(n) = 8 (FP64) or 4 (FP32)
+  ! addresses (c_loc of type(c_ptr) arrays), dtype(n) = 8 (FP64) or 4 (FP32);
+  ! scheme (c_loc of an integer array, or c_null_ptr: all MDFL_SWEBY) and,
+  ! for QUICKER tracers, the tmask_limit ranges and flags
   type, bind(C), public :: mom_gpu_tracer_step_t
      integer(c_int) :: ntr
      type(c_ptr) :: T_taum1, T_taup1, dtype, th_tendency, stf, btf
@@ -74,11 +76,16 @@ module mom_gpu_mod
      type(c_ptr) :: rho_dzt_taum1, rho_dztr, rho_dzt_taup1
      type(c_ptr) :: dzwt, diff_cbt
      real(c_double) :: rho0, dtime, aidif
+     type(c_ptr) :: scheme = c_null_ptr
+     type(c_ptr) :: min_limit = c_null_ptr, max_limit = c_null_ptr
+     integer(c_int) :: index_temp = -1
+     integer(c_int) :: tmask_limit_ts_same = 1
+     integer(c_int) :: limit_with_upwind = 0
   end type mom_gpu_tracer_step_t
 
   public :: mom_gpu_create, mom_gpu_destroy, mom_gpu_sync, mom_gpu_last_error_f
   public :: mom_gpu_malloc, mom_gpu_free, mom_gpu_memcpy_htod, mom_gpu_memcpy_dtoh
-  public :: mom_gpu_fill, mom_gpu_set_tuning
+  public :: mom_gpu_fill, mom_gpu_set_tuning, mom_gpu_update_domains
   public :: mom_gpu_horz_advect_tracer, mom_gpu_vert_advect_tracer
   public :: mom_gpu_advect_sweby_all, mom_gpu_compute_tmask_limit
   public :: mom_gpu_update_tracer_explicit, mom_gpu_invtri
@@ -142,6 +149,14 @@ module mom_gpu_mod
        real(c_double), value :: val
        integer(c_size_t), value :: n
      end function mom_gpu_fill
+
+     ! mpp_update_domains(field, Dom%domain2d) on one PE, device field of nz levels
+     integer(c_int) function mom_gpu_update_domains(ctx, field, nz) &
+          bind(C, name="mom_gpu_update_domains")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, field
+       integer(c_int), value :: nz
+     end function mom_gpu_update_domains
 
      integer(c_int) function mom_gpu_set_tuning(ctx, knob, val) bind(C, name="mom_gpu_set_tuning")
        import :: c_int, c_ptr, c_char

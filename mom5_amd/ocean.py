@@ -109,12 +109,23 @@ def _ptab(arrs):
 def tracer_step_struct(T_taum1s, T_taup1s, *, uhrho_et, vhrho_nt, wrho_bt, rho_dzt_tau, dtime,
                        stfs=None, btfs=None, dzwt=None, diff_cbt=None, index_salt=1, rho0=1035.0,
                        aidif=1.0, th_tendencies=None, rho_dzt_taum1=None, rho_dztr=None,
-                       rho_dzt_taup1=None):
+                       rho_dzt_taup1=None, schemes=None, min_limits=None, max_limits=None,
+                       index_temp=0, ts_same=True, limit_with_upwind=False):
     """The mom_gpu_tracer_step_t of one domain and the ctypes tables it
-    points to (keep both alive until the call returns)."""
+    points to (keep both alive until the call returns).  schemes: per tracer
+    "mdfl_sweby" (default) or "quicker"; Quicker tracers need min_limits and
+    max_limits (their tmask_limit ranges)."""
     n = len(T_taum1s)
     dt = (C.c_int * max(n, 1))(*[_elem_size(a) for a in T_taum1s])
     keep = [dt]
+    sch = lo = hi = None
+    if schemes is not None:
+        sch = (C.c_int * max(n, 1))(*[SCHEMES[x] for x in schemes])
+        keep.append(sch)
+        if min_limits is not None:
+            lo = (C.c_double * max(n, 1))(*[float(x) for x in min_limits])
+            hi = (C.c_double * max(n, 1))(*[float(x) for x in max_limits])
+            keep += [lo, hi]
 
     def tab(xs):
         if xs is None:
@@ -129,7 +140,10 @@ def tracer_step_struct(T_taum1s, T_taup1s, *, uhrho_et, vhrho_nt, wrho_bt, rho_d
                    wrho_bt=_p(wrho_bt), rho_dzt_tau=_p(rho_dzt_tau),
                    rho_dzt_taum1=_p(rho_dzt_taum1), rho_dztr=_p(rho_dztr),
                    rho_dzt_taup1=_p(rho_dzt_taup1), dzwt=_p(dzwt), diff_cbt=_p(diff_cbt),
-                   rho0=float(rho0), dtime=float(dtime), aidif=float(aidif))
+                   rho0=float(rho0), dtime=float(dtime), aidif=float(aidif), scheme=sch,
+                   min_limit=lo, max_limit=hi, index_temp=int(index_temp),
+                   tmask_limit_ts_same=int(bool(ts_same)),
+                   limit_with_upwind=int(bool(limit_with_upwind)))
     return s, keep
 
 
@@ -237,6 +251,12 @@ class OceanGPU:
         """Fill a float64 device array on the context's stream."""
         n = arr.nbytes // 8
         self._chk(self.L.mom_gpu_fill(self.ctx, _p(arr), float(value), n), "fill")
+
+    def update_domains(self, arr):
+        """Single-PE halo update (mpp_update_domains) of a float64 device
+        array of one or more data-domain levels."""
+        nz = arr.nbytes // 8 // (self.shape2[0] * self.shape2[1])
+        self._chk(self.L.mom_gpu_update_domains(self.ctx, _p(arr), int(nz)), "update_domains")
 
     def copy(self, dst, src):
         """Copy a float64 device array into another on the context's stream."""
@@ -394,15 +414,20 @@ class OceanGPU:
     def step_tracers(self, T_taum1s, T_taup1s, *, uhrho_et, vhrho_nt, wrho_bt, rho_dzt_tau,
                      dtime, stfs=None, btfs=None, dzwt=None, diff_cbt=None, index_salt=1,
                      rho0=1035.0, aidif=1.0, th_tendencies=None, rho_dzt_taum1=None,
-                     rho_dztr=None, rho_dzt_taup1=None):
-        """Advection (MDFL-Sweby), explicit update and implicit vertical
-        diffusion of every tracer; each tracer's dtype is its array's."""
+                     rho_dztr=None, rho_dzt_taup1=None, schemes=None, min_limits=None,
+                     max_limits=None, index_temp=0, ts_same=True, limit_with_upwind=False):
+        """Advection (MDFL-Sweby, or Quicker where schemes says so), explicit
+        update and implicit vertical diffusion of every tracer; each
+        tracer's dtype is its array's."""
         s, keep = tracer_step_struct(T_taum1s, T_taup1s, uhrho_et=uhrho_et, vhrho_nt=vhrho_nt,
                                      wrho_bt=wrho_bt, rho_dzt_tau=rho_dzt_tau, dtime=dtime,
                                      stfs=stfs, btfs=btfs, dzwt=dzwt, diff_cbt=diff_cbt,
                                      index_salt=index_salt, rho0=rho0, aidif=aidif,
                                      th_tendencies=th_tendencies, rho_dzt_taum1=rho_dzt_taum1,
-                                     rho_dztr=rho_dztr, rho_dzt_taup1=rho_dzt_taup1)
+                                     rho_dztr=rho_dztr, rho_dzt_taup1=rho_dzt_taup1,
+                                     schemes=schemes, min_limits=min_limits,
+                                     max_limits=max_limits, index_temp=index_temp,
+                                     ts_same=ts_same, limit_with_upwind=limit_with_upwind)
         self._chk(self.L.mom_gpu_step_tracers(self.ctx, C.byref(s)), "step_tracers")
 
     # ---- hipGraph capture --------------------------------------------------------

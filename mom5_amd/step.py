@@ -14,12 +14,15 @@ Two-level time stepping (``taum1 = tau``) is assumed, as in the default
 configuration (``ocean_model.F90:1489-1491``): each step reads one tracer
 time level and writes the other, then the levels swap.
 
-:class:`PlainStep` makes one C-ABI call per reference routine and tracer;
-:class:`BatchedStep` advances every MDFL-Sweby tracer with one
-``step_tracers`` call (grouped, fused advection + explicit update + implicit
-vertical diffusion, FP64 or FP32 per tracer).  The batched step skips
-``compute_tmask_limit``: its only readers are the Quicker and QuickMOM3
-schemes (``ocean_tracer_advect.F90:2609-2625, 2738-2753, 3006-3111``).
+:class:`PlainStep` makes one C-ABI call per reference routine and tracer,
+and updates the tracer halos after the step as ``ocean_model.F90:2049``
+does; :class:`BatchedStep` advances every tracer with one ``step_tracers``
+call (grouped, fused advection + explicit update + implicit vertical
+diffusion, FP64 or FP32 per tracer).  The batched step forms the
+``tmask_limit`` of its Quicker tracers inside their pass (its only readers
+are the Quicker and QuickMOM3 schemes, ``ocean_tracer_advect.F90:2609-2625,
+2738-2753, 3006-3111``) and reads every halo value as the halo update would
+leave it, so it needs no halo updates of its own.
 """
 from __future__ import annotations
 
@@ -139,6 +142,8 @@ class PlainStep(_Captured):
             g.vert_diffuse_implicit(Tp, self.stf, self.btf, d["diff_cbt"], d["rho_dzt"],
                                     d["dzwt"], c.dt,
                                     index_salt=self.names.index("salt") if "salt" in self.names else -1)
+            for t in Tp:                     # ocean_model.F90:2049
+                g.update_domains(t)
             if c.dtbt > 0.0:
                 g.update_ocean_barotropic(
                     nts=c.nts, dtbt=c.dtbt, eta_t_bar_tau=self.eta_bar[now],
@@ -162,7 +167,7 @@ class BatchedStep(_Captured):
               "eta_smooth", "rho_g")
 
     def __init__(self, case, gpu: OceanGPU | None = None, tracers=None, diffuse=True,
-                 barotropic=True):
+                 barotropic=True, schemes=None):
         self.case = case
         self.gpu = gpu or OceanGPU(case.dom, case.grid)
         f = case.fields
@@ -170,6 +175,11 @@ class BatchedStep(_Captured):
         trs = case.tracers if tracers is None else tracers
         self.names = [n for n, _ in trs]
         self.ntr = len(self.names)
+        # per-tracer scheme ("mdfl_sweby" or "quicker") and tmask_limit ranges
+        self.schemes = None if schemes is None or all(x == "mdfl_sweby" for x in schemes) \
+            else list(schemes)
+        self.mins = [LIMITS.get(n, PASSIVE_LIMITS)[0] for n in self.names]
+        self.maxs = [LIMITS.get(n, PASSIVE_LIMITS)[1] for n in self.names]
         self.T = [[_resident(t), None] for _, t in trs]
         for pair in self.T:
             pair[1] = _zeros_like(pair[0])
@@ -199,7 +209,9 @@ class BatchedStep(_Captured):
                     rho_dzt_tau=d["rho_dzt"], rho_dztr=d["rho_dztr"], dtime=c.dt,
                     stfs=self.stf, btfs=self.btf, dzwt=d["dzwt"] if self.diffuse else None,
                     diff_cbt=d["diff_cbt"] if self.diffuse else None,
-                    index_salt=self.index_salt, th_tendencies=self.th)
+                    index_salt=self.index_salt, th_tendencies=self.th, schemes=self.schemes,
+                    min_limits=self.mins, max_limits=self.maxs,
+                    index_temp=self.names.index("temp") if "temp" in self.names else -1)
 
     def bt_args(self):
         """Barotropic parameters and fields of this step at the current level."""

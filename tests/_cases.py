@@ -70,15 +70,29 @@ def bt_variant_fields(c, rng, cgrid, scale=1.0):
                 smooth=np.zeros(sh))
 
 
-def oracle_tracer_step(orc, Ts, stfs, btfs, dt, *, index_salt=1, ths=None, fields=None):
-    """The oracle's sequence for one batched tracer step of FP64 tracers Ts:
-    per tracer th = (other tendencies or 0) + MDFL-Sweby advection, the
-    explicit update, then the implicit vertical diffusion of all of them
-    (ocean_tracer.F90:2295-2350, 2432).  Returns the T(taup1) arrays."""
+def oracle_tracer_step(orc, Ts, stfs, btfs, dt, *, index_salt=1, ths=None, fields=None,
+                       schemes=None, mins=None, maxs=None, index_temp=0, ts_same=True,
+                       limit_with_upwind=False):
+    """The oracle's sequence for one batched tracer step of FP64 tracers Ts
+    (halo-updated): compute_tmask_limit when a tracer is Quicker, per tracer
+    th = (other tendencies or 0) + its advection (MDFL-Sweby, or Quicker
+    horizontal then vertical), the explicit update, then the implicit
+    vertical diffusion of all of them (ocean_tracer.F90:2295-2350, 2432,
+    2788-2892).  Returns the T(taup1) arrays."""
+    lims = None
+    if schemes is not None and "quicker" in schemes:
+        lims = orc.tmask_limit(Ts, mins, maxs, index_temp=index_temp, index_salt=index_salt,
+                               ts_same=ts_same)
     outs = []
     for n, T in enumerate(Ts):
         th = np.zeros_like(T) if ths is None or ths[n] is None else np.array(ths[n], copy=True)
-        orc.horz_advect_mdfl(T, th, dt, lim=1.0, fields=fields)
+        if schemes is None or schemes[n] == "mdfl_sweby":
+            orc.horz_advect_mdfl(T, th, dt, lim=1.0, fields=fields)
+        else:
+            rh = orc.horz_quicker(T, T, tmask_limit=lims[n], limit_with_upwind=limit_with_upwind,
+                                  fields=fields)
+            rv = orc.vert_quicker(T, T, tmask_limit=lims[n], fields=fields)
+            th[:, 1:-1, 1:-1] = (th[:, 1:-1, 1:-1] + (-rh)) + (-rv)
         outs.append(orc.update_explicit(T, th, dt, fields=fields))
     orc.vert_diffuse_implicit(outs, stfs, btfs, dt, index_salt=index_salt, fields=fields)
     return outs
