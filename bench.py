@@ -24,13 +24,15 @@ tracer, 80 subcycles).  --workload small: 90 x 60 x 25, MDFL only.
 
 Prints one JSON line: tracer-cell-updates/s of the step (value), the roofline
 of the dominant kernel (HIP-event timed on the context's stream) and a CPU
-baseline (the CPU restatement on a bounded sample, single thread).
+baseline (the CPU restatement on a bounded sample of the workload, on one core
+and on every usable core, with the host's CPU model and core count).
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -77,25 +79,53 @@ def mdfl_bytes_per_launch(ni, nj, nk):
     return ni * nj * nk * (8 + 16 + 32) + ni * nj * 52
 
 
-def cpu_baseline(w, steps=8):
-    """The CPU restatement on a bounded sample of the same workload: at least
-    `steps` steps after one warm-up and at least 10 s of timed work."""
+def host_cpu():
+    """Model name (lscpu / /proc/cpuinfo), nproc and the cores this process
+    may use (its affinity mask, capped by OMP_NUM_THREADS when set)."""
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    if model is None and os.path.exists("/proc/cpuinfo"):
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    nproc = os.cpu_count()
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        usable = min(usable, int(os.environ["OMP_NUM_THREADS"]))
+    return dict(model=model, nproc=nproc, usable=max(1, usable))
+
+
+def cpu_step_time(w, threads, steps=4, budget=10.0):
+    """Median time of one whole step of the CPU restatement (serial build, or
+    its all-core OpenMP build) on the bounded sample of the workload: at
+    least `steps` steps after one warm-up and about `budget` s of timed work."""
     from oracle.oracle import Oracle
     s = w["sample"]
     case = make(w, ni=s["ni"], nj=s["nj"], seed=0x4D4F4D35 + 7)
-    o = Oracle(case)
+    o = Oracle(case, threads=threads)
     f = case.fields
     Ts = [np.array(t, dtype=np.float64) for _, t in case.tracers]
     names = [n for n, _ in case.tracers]
+    schemes = w.get("schemes") or ["mdfl_sweby"] * len(Ts)
     mins = [(-2.0 if n == "temp" else 1.0 if n == "salt" else -0.1) for n in names]
     maxs = [(32.0 if n == "temp" else 42.0 if n == "salt" else 1.0) for n in names]
 
     def step(Ts):
-        lim = o.tmask_limit(Ts, mins, maxs, 0, 1, True)
+        # compute_tmask_limit only feeds Quicker (ocean_tracer_advect.F90:
+        # 2609-2625, 3006-3111): formed when a Quicker tracer is present
+        lim = o.tmask_limit(Ts, mins, maxs, 0, 1, True) if "quicker" in schemes else None
         out = []
         for n, T in enumerate(Ts):
             th = np.zeros_like(T)
-            if (w.get("schemes") or ["mdfl_sweby"] * len(Ts))[n] == "quicker":
+            if schemes[n] == "quicker":
                 th[:, 1:-1, 1:-1] -= o.horz_quicker(T, T, lim[n])
                 th[:, 1:-1, 1:-1] -= o.vert_quicker(T, T, lim[n])
             else:
@@ -108,17 +138,30 @@ def cpu_baseline(w, steps=8):
 
     Ts = step(Ts)                       # warm-up
     times = []
-    while len(times) < steps or (sum(times) < 10.0 and len(times) < 64):
+    while len(times) < steps or (sum(times) < budget and len(times) < 64):
         t0 = time.perf_counter()
         Ts = step(Ts)
         times.append(time.perf_counter() - t0)
-    steps = len(times)
-    dt = float(np.median(times))
     cells = case.ni * case.nj * case.nk * len(Ts)
-    return dict(value=cells / dt, unit="tracer-cell-updates/s", cores=1, kind="port",
-                sample=f"{case.ni}x{case.nj}x{case.nk}, {len(Ts)} tracers, full step incl. "
-                       f"barotropic nts={case.nts}; median of {steps} steps after 1 warm-up "
-                       f"({sum(times):.1f} s timed)")
+    sample = (f"{case.ni}x{case.nj}x{case.nk}, {len(Ts)} tracers, full step incl. barotropic "
+              f"nts={case.nts}; median of {len(times)} steps after 1 warm-up "
+              f"({sum(times):.1f} s timed)")
+    return cells / float(np.median(times)), sample
+
+
+def cpu_baseline(w):
+    """The CPU restatement timed on this host: one core (the serial build)
+    and every usable core (the same source built with OpenMP, parallel over
+    levels, rows and points; OMP_NUM_THREADS threads).  The all-core figure
+    is the reported value."""
+    host = host_cpu()
+    n = host["usable"]
+    os.environ["OMP_NUM_THREADS"] = str(n)   # read when the OpenMP build loads
+    v1, s1 = cpu_step_time(w, threads=False)
+    vn, sn = cpu_step_time(w, threads=True)
+    return dict(value=vn, unit="tracer-cell-updates/s", cores=n, kind="port", sample=sn,
+                single_core=dict(value=v1, cores=1, sample=s1),
+                host=dict(model=host["model"], nproc=host["nproc"], threads_used=n))
 
 
 def log(msg):
@@ -411,6 +454,12 @@ def run_mixed(args, w):
     step_bytes = mixed_step_bytes(ni, nj, nk, n64, n32)
     step_frac = step_bytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS
 
+    bt_check = barotropic_blocked_vs_subcycle(g, bs) if bs.barotropic else None
+    if bt_check:
+        log(f"barotropic at full size: blocked (H={bt_check['blocked_subcycles_per_launch']}) "
+            f"vs per-subcycle: {bt_check['points_differ']} differing values; "
+            f"max |eta_t_bar| {bt_check['eta_t_bar_max_abs_m']:.3f} m")
+
     # keep the batched results (FP64 on the device, FP32 on the host), free the rest
     fin64 = {n: bs.tracer(n) for n in range(ntr) if is64[n]}
     fin32 = {n: bs.tracer(n).to_host() for n in range(ntr) if not is64[n]}
@@ -471,7 +520,8 @@ def run_mixed(args, w):
                  speedup_vs_plain_fp64=value / value_plain, step_algorithmic_bytes=step_bytes,
                  step_hbm_frac=round(step_frac, 4), ts_bitwise_equal_plain_fp64=ts_equal,
                  passive_max_rel_err_normwise=worst, errors_after_steps=warm + steps,
-                 tracer_errors=errors, content_drift_rel=drift)
+                 tracer_errors=errors, content_drift_rel=drift,
+                 barotropic_full_size_check=bt_check)
     name = w["name"] if not args.shape else f"{w['name']}@{args.shape}"
     cfg = {"workload": name, "grid": [ni, nj, nk], "tracers": ntr,
            "tracer_dtypes": {"f64": n64, "f32": n32}, "schemes": "mdfl_sweby (all)",
@@ -480,6 +530,39 @@ def run_mixed(args, w):
            "tuning": tuning_used}
     return dict(value=value, ms=ms, steps=steps, warmup=warm,
                 dtype="f64+f32" if n32 else "f64", cfg=cfg, roof=roof, extra=extra, ntr=ntr)
+
+
+def barotropic_blocked_vs_subcycle(g, bs):
+    """update_ocean_barotropic at the workload's full size and subcycle count
+    from the step's current state, once with the tuned subcycles per launch
+    (temporally blocked kernel) and once per subcycle (0): every value of
+    eta_t_bar(taup1), udrho(taup1) and conv_rho_ud_t compared bit for bit,
+    and the largest |eta_t_bar| reported (initial eta ~ N(0, 0.1 m))."""
+    from mom5_amd import DeviceArray
+    knob = "barotropic_subcycles_per_launch"
+    H = g.get_tuning(knob)
+    hs = (H, 0) if H != 0 else (0, 4)
+    res = {}
+    for h in hs:
+        g.set_tuning(knob, h)
+        eb, cv = DeviceArray(g.shape2), DeviceArray(g.shape2)
+        ud = DeviceArray((2,) + g.shape2)
+        g.update_ocean_barotropic(**dict(bs.bt_args(), eta_t_bar_taup1=eb, udrho_taup1=ud,
+                                         conv_rho_ud_t=cv))
+        g.sync()
+        res[h] = [x.to_host() for x in (eb, ud, cv)]
+        del eb, ud, cv
+    g.set_tuning(knob, H)
+    a, b = res[hs[0]], res[hs[1]]
+    differ = sum(int(np.count_nonzero(x.view(np.int64) != y.view(np.int64)))
+                 for x, y in zip(a, b))
+    eta = a[0]
+    finite = bool(np.isfinite(eta).all() and np.isfinite(a[1]).all())
+    emax = float(np.abs(eta).max()) if finite else float("inf")
+    return dict(blocked_subcycles_per_launch=hs[0], compared_with_subcycles_per_launch=hs[1],
+                nts=bs.case.nts, points_compared=int(sum(x.size for x in a)),
+                points_differ=differ, eta_t_bar_max_abs_m=emax,
+                eta_bounded=bool(finite and emax < 10.0))
 
 
 def _tune_summary(rep):

@@ -20,6 +20,12 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* Loops marked PAR_FOR have independent iterations (no carried values, each
+ * writes its own points), so the all-core build (-fopenmp, see Makefile)
+ * gives results identical to the serial one; without -fopenmp the pragma is
+ * ignored. */
+#define PAR_FOR _Pragma("omp parallel for schedule(static)")
+
 #define ONESIXTH (1.0 / 6.0)     /* ocean_core/ocean_parameters.F90:92 */
 #define EPS30 1.0e-30            /* literal used by the MDFL limiter */
 
@@ -56,6 +62,7 @@ void orc_halo_update(const orc_dom *d, int h, double *a, int nz, int flags) {
   const int nx = ni + 2 * h, ny = nj + 2 * h;
 #define HA(ii, jj, kk) a[(size_t)((ii) - (d->isc - h)) + (size_t)nx * \
     ((size_t)((jj) - (d->jsc - h)) + (size_t)ny * (size_t)(kk))]
+  PAR_FOR
   for (int k = 0; k < nz; ++k) {
     if ((flags & 1) && d->cyclic_x) {
       for (int j = d->jsc; j <= d->jec; ++j)
@@ -89,6 +96,7 @@ void orc_halo_update(const orc_dom *d, int h, double *a, int nz, int flags) {
 static double *make_tmask_h2(const orc_dom *d, const orc_grid *g) {
   const int nk = d->nk;
   double *tm = (double *)calloc((size_t)NI2 * NJ2 * nk, sizeof(double));
+  PAR_FOR
   for (int k = 1; k <= nk; ++k)
     for (int j = d->jsc; j <= d->jec; ++j)
       for (int i = d->isc; i <= d->iec; ++i) tm[H2(i, j, k)] = g->tmask[I3(i, j, k)];
@@ -128,11 +136,13 @@ void orc_advect_tracer_mdfl_sweby(const orc_dom *d, const orc_grid *g,
 
   /* z-sweep: flux through the bottom face of each cell, :3843-3909 */
   for (int k = 1; k <= nk; ++k) {
+    PAR_FOR
     for (int j = d->jsc; j <= d->jec; ++j)
       for (int i = d->isc; i <= d->iec; ++i) tr[H2(i, j, k)] = T[I3(i, j, k)];
     const int kp1 = (k + 1 < nk) ? k + 1 : nk;
     const int kp2 = (k + 2 < nk) ? k + 2 : nk;
     const int km1 = (k - 1 > 1) ? k - 1 : 1;
+    PAR_FOR
     for (int j = d->jsc; j <= d->jec; ++j)
       for (int i = d->isc; i <= d->iec; ++i) {
         const double Rjp = (T[I3(i, j, km1)] - T[I3(i, j, k)]) * tm[H2(i, j, km1)] * tm[H2(i, j, k)];
@@ -156,12 +166,14 @@ void orc_advect_tracer_mdfl_sweby(const orc_dom *d, const orc_grid *g,
         if (flux_z) flux_z[I3(i, j, k)] = fbt;
         ftp[C2(i, j)] = fbt;
       }
+    PAR_FOR
     for (int j = d->jsc; j <= d->jec; ++j)
       for (int i = d->isc; i <= d->iec; ++i) wkm1[C2(i, j)] = wrho_bt[I3W(i, j, k)];
   }
   orc_halo_update(d, 2, tr, nk, 1);
 
   /* x-sweep: flux through the eastern face, :3916-3969 */
+  PAR_FOR
   for (int k = 1; k <= nk; ++k) {
     for (int j = d->jsc; j <= d->jec; ++j)
       for (int i = d->isc - 1; i <= d->iec; ++i) {
@@ -196,6 +208,7 @@ void orc_advect_tracer_mdfl_sweby(const orc_dom *d, const orc_grid *g,
   /* y-sweep, then remove the divergence correction, :3974-4056 */
   memset(wkm1, 0, sizeof(double) * (size_t)NIC * NJC);
   for (int k = 1; k <= nk; ++k) {
+    PAR_FOR
     for (int j = d->jsc - 1; j <= d->jec; ++j)
       for (int i = d->isc; i <= d->iec; ++i) {
         const double Rjp = (tr[H2(i, j + 2, k)] - tr[H2(i, j + 1, k)]) * tm[H2(i, j + 2, k)] * tm[H2(i, j + 1, k)];
@@ -215,11 +228,13 @@ void orc_advect_tracer_mdfl_sweby(const orc_dom *d, const orc_grid *g,
                                  (massflux - fabs(massflux)) * (tr[H2(i, j + 1, k)] - psiM * Rj)) *
                           tm[H2(i, j, k)] * tm[H2(i, j + 1, k)];
       }
+    PAR_FOR
     for (int j = d->jsc; j <= d->jec; ++j)
       for (int i = d->isc; i <= d->iec; ++i)
         tr[H2(i, j, k)] = tr[H2(i, j, k)] +
             dtime * tm[H2(i, j, k)] * g->datr[I2(i, j)] / rho_dzt[I3(i, j, k)] *
                 (fy[I3(i, j - 1, k)] - fy[I3(i, j, k)]);
+    PAR_FOR
     for (int j = d->jsc; j <= d->jec; ++j)
       for (int i = d->isc; i <= d->iec; ++i) {
         const double Tc = T[I3(i, j, k)];
@@ -231,6 +246,7 @@ void orc_advect_tracer_mdfl_sweby(const orc_dom *d, const orc_grid *g,
                                       g->dyte[I2(i, j)] * uhrho_et[I3(i, j, k)]));
         advect[C3(i, j, k)] = -(rho * (tr[H2(i, j, k)] - Tc) / dtime * tm[H2(i, j, k)]);
       }
+    PAR_FOR
     for (int j = d->jsc; j <= d->jec; ++j)
       for (int i = d->isc; i <= d->iec; ++i) wkm1[C2(i, j)] = wrho_bt[I3W(i, j, k)];
   }
@@ -250,6 +266,7 @@ void orc_horz_advect_tracer_mdfl(const orc_dom *d, const orc_grid *g,
   orc_advect_tracer_mdfl_sweby(d, g, T, uhrho_et, vhrho_nt, wrho_bt, rho_dzt, dtime, lim,
                                adv, NULL, NULL, NULL);
   if (wrk1) memset(wrk1, 0, P2 * (size_t)nk * sizeof(double));
+  PAR_FOR
   for (int k = 1; k <= nk; ++k)
     for (int j = d->jsc; j <= d->jec; ++j)
       for (int i = d->isc; i <= d->iec; ++i) {
@@ -503,11 +520,13 @@ void orc_horz_advect_tracer_quicker(const orc_dom *d, const orc_grid *g,
   double *tmq = make_tmask_h2(d, g);   /* tmask_quick is built like tmask_mdfl */
   double *fx = (double *)calloc(n3, sizeof(double));
   double *fy = (double *)calloc(n3, sizeof(double));
+  PAR_FOR
   for (int k = 1; k <= nk; ++k)
     for (int j = d->jsc; j <= d->jec; ++j)
       for (int i = d->isc; i <= d->iec; ++i) tq[H2(i, j, k)] = Tm1[I3(i, j, k)];
   orc_halo_update(d, 2, tq, nk, 3);
 
+  PAR_FOR
   for (int k = 1; k <= nk; ++k) {
     for (int j = d->jsc - 1; j <= d->jec; ++j)
       for (int i = d->isc; i <= d->iec; ++i) {
@@ -560,6 +579,7 @@ void orc_horz_advect_tracer_quicker(const orc_dom *d, const orc_grid *g,
           }
     }
   }
+  PAR_FOR
   for (int k = 1; k <= nk; ++k)
     for (int j = d->jsc; j <= d->jec; ++j)
       for (int i = d->isc; i <= d->iec; ++i)
@@ -594,6 +614,7 @@ void orc_vert_advect_tracer_quicker(const orc_dom *d, const orc_grid *g,
     const int km1 = (k - 1 > 1) ? k - 1 : 1;
     const int kp1 = (k + 1 < nk) ? k + 1 : nk;
     const int p2 = (k + 2 < nk) ? k + 2 : nk;
+    PAR_FOR
     for (int j = d->jsc; j <= d->jec; ++j)
       for (int i = d->isc; i <= d->iec; ++i) {
         const double vel = wrho_bt[I3W(i, j, k)];
@@ -633,6 +654,7 @@ void orc_compute_tmask_limit(const orc_dom *d, const orc_grid *g, int ntr,
   for (int n = 0; n < ntr; ++n) {
     const double *T = Tt[n];
     double *L = lim[n];
+    PAR_FOR
     for (int k = 1; k <= nk; ++k)
       for (int j = d->jsc - 1; j <= d->jec; ++j)
         for (int i = d->isc - 1; i <= d->iec; ++i) {
@@ -644,6 +666,7 @@ void orc_compute_tmask_limit(const orc_dom *d, const orc_grid *g, int ntr,
           if (tm[I3(i, j + 1, k)] == 1.0 && (T[I3(i, j + 1, k)] < mn[n] || T[I3(i, j + 1, k)] > mx[n]))
             L[I3(i, j, k)] = tm[I3(i, j, k)];
         }
+    PAR_FOR
     for (int k = 1; k <= nk - 1; ++k)
       for (int j = d->jsc - 1; j <= d->jec; ++j)
         for (int i = d->isc - 1; i <= d->iec; ++i)
@@ -652,6 +675,7 @@ void orc_compute_tmask_limit(const orc_dom *d, const orc_grid *g, int ntr,
   }
   if (ts_same && index_temp >= 0 && index_salt >= 0) {
     double *Lt = lim[index_temp], *Ls = lim[index_salt];
+    PAR_FOR
     for (int k = 1; k <= nk; ++k)
       for (int j = d->jsc - 1; j <= d->jec; ++j)
         for (int i = d->isc - 1; i <= d->iec; ++i)
@@ -666,6 +690,7 @@ void orc_compute_tmask_limit(const orc_dom *d, const orc_grid *g, int ntr,
 void orc_update_tracer_explicit(const orc_dom *d, const double *Tm1,
     const double *tend, const double *rdz_m1, const double *rdzr,
     double dtime, double *Tp1) {
+  PAR_FOR
   for (int k = 1; k <= d->nk; ++k)
     for (int j = d->jsc; j <= d->jec; ++j)
       for (int i = d->isc; i <= d->iec; ++i) {
@@ -684,7 +709,10 @@ void orc_invtri(const orc_dom *d, double *z, const double *topbc,
   const int ni = NI;
   const double eps = 1.0e-30;
   const double tdt_aidif = tdt * aidif;
-  /* work rows a,b,c,e,f(isd:ied,0:nk), bet(isd:ied) */
+  /* work rows a,b,c,e,f(isd:ied,0:nk), bet(isd:ied): one set per thread
+   * of the all-core build, rows j independent */
+#pragma omp parallel
+  {
   double *a = (double *)calloc((size_t)ni * (nk + 1), sizeof(double));
   double *b = (double *)calloc((size_t)ni * (nk + 1), sizeof(double));
   double *c = (double *)calloc((size_t)ni * (nk + 1), sizeof(double));
@@ -692,6 +720,7 @@ void orc_invtri(const orc_dom *d, double *z, const double *topbc,
   double *f = (double *)calloc((size_t)ni * (nk + 1), sizeof(double));
   double *bet = (double *)calloc((size_t)ni, sizeof(double));
 #define W(arr, i, k) arr[(size_t)((i) - d->isd) + (size_t)ni * (size_t)(k)]
+#pragma omp for schedule(static)
   for (int j = d->jsc; j <= d->jec; ++j) {
     for (int k = 1; k <= nk; ++k) {
       const int km1 = (k - 1 > 1) ? k - 1 : 1;
@@ -730,6 +759,7 @@ void orc_invtri(const orc_dom *d, double *z, const double *topbc,
   }
 #undef W
   free(a); free(b); free(c); free(e); free(f); free(bet);
+  }
 }
 
 /* ocean_vert_mix.F90:3426-3513 (the part feeding invtri) */
@@ -745,6 +775,7 @@ void orc_vert_diffuse_implicit(const orc_dom *d, const orc_grid *g, int ntr,
   for (int n = 0; n < ntr; ++n) {
     const int nmix = (n == index_salt) ? 2 : 1;
     const double *dc = diff_cbt + n3 * (size_t)(nmix - 1);
+    PAR_FOR
     for (int k = 1; k <= nk; ++k)
       for (int j = d->jsc; j <= d->jec; ++j)
         for (int i = d->isc; i <= d->iec; ++i) {
@@ -764,6 +795,7 @@ static void orc_div_ud(const orc_dom *d, const orc_grid *g, const double *ud,
                        double *out) {
   const double *u1 = ud, *u2 = ud + P2;
   for (size_t o = 0; o < P2; ++o) out[o] = 0.0;
+  PAR_FOR
   for (int j = d->jsd + 1; j <= d->jed; ++j) {
     int i = d->isd + 1;
     double uhim = u1[I2(i - 1, j)] * g->dyu[I2(i - 1, j)];
@@ -839,13 +871,17 @@ void orc_update_ocean_barotropic(const orc_dom *d, const orc_grid *g,
     /* predictor (:3365-3376) */
     double *ubuf = &UD(0, 1, fstau);
     orc_div_ud(d, g, ubuf, tmp);
+    PAR_FOR
     for (size_t o = 0; o < P2; ++o) tmp[o] = -tmp[o] * tmask1[o];
+    PAR_FOR
     for (size_t o = 0; o < P2; ++o)
       ETA(o, fstaup1) = ETA(o, fstau) + dtbt_gamma_rho0r * (tmp[o] + sf[o]);
     /* legacy halo update of eta_t_bt(fstaup1) (:3383) */
     orc_halo_update(d, 1, &ETA(0, fstaup1), 1, 3);
+    PAR_FOR
     for (size_t o = 0; o < P2; ++o) ps[o] = rho_g[o] * ETA(o, fstaup1) + patm[o];
     orc_grad_barotropic_p(d, g, ps, gp);
+    PAR_FOR
     for (size_t o = 0; o < P2; ++o) {       /* :3464-3486 */
       const double pf1 = -(thicku[o] * gp[o]);
       const double pf2 = -(thicku[o] * gp[o + P2]);
@@ -857,11 +893,14 @@ void orc_update_ocean_barotropic(const orc_dom *d, const orc_grid *g,
     }
     /* corrector (:3497-3505) */
     orc_div_ud(d, g, &UD(0, 1, fstaup1), tmp);
+    PAR_FOR
     for (size_t o = 0; o < P2; ++o) tmp[o] = -tmp[o] * tmask1[o];
+    PAR_FOR
     for (size_t o = 0; o < P2; ++o)
       ETA(o, fstaup1) = ETA(o, fstau) + dtbt_rho0r * (tmp[o] + sf[o]);
     /* accumulate (:3532-3549) */
     const double wgt = p->stepping_a ? 1.0 : (double)(nts - itime + 1);
+    PAR_FOR
     for (size_t o = 0; o < P2; ++o) {
       eta_bar[o] = eta_bar[o] + ETA(o, fstaup1);
       if (p->stepping_a) {
@@ -883,6 +922,7 @@ void orc_update_ocean_barotropic(const orc_dom *d, const orc_grid *g,
   orc_halo_update(d, 1, udrho, 2, 3);
   if (conv_rho_ud_t) {
     orc_div_ud(d, g, udrho, conv_rho_ud_t);
+    PAR_FOR
     for (size_t o = 0; o < P2; ++o) conv_rho_ud_t[o] = -conv_rho_ud_t[o];
     orc_halo_update(d, 1, conv_rho_ud_t, 1, 3);
   }
@@ -1071,6 +1111,7 @@ static void orc_remap_bt_to_bu(const orc_dom *d, const double *a, const double *
 static void orc_grad_barotropic_p(const orc_dom *d, const orc_grid *g, const double *p,
                                   double *out) {
   for (size_t o = 0; o < 2 * P2; ++o) out[o] = 0.0;
+  PAR_FOR
   for (int j = d->jsd; j <= d->jed - 1; ++j)
     for (int i = d->isd; i <= d->ied - 1; ++i) {
       const double p_fay = 0.5 * (p[I2(i, j)] + p[I2(i, j + 1)]);

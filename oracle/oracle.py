@@ -14,6 +14,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libmom5_oracle.so")
+# the same source built with -fopenmp: loops over independent levels, rows or
+# points run on every host core (OMP_NUM_THREADS), results identical
+LIB_OMP_PATH = os.path.join(HERE, "_build", "libmom5_oracle_omp.so")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -39,16 +42,17 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
+def lib(threads=False):
+    """The serial restatement, or with threads=True its all-core build."""
+    path = LIB_OMP_PATH if threads else LIB_PATH
+    if path not in _libs:
+        if not os.path.exists(path):
             build()
-        _lib = C.CDLL(LIB_PATH)
-    return _lib
+        _libs[path] = C.CDLL(path)
+    return _libs[path]
 
 
 def _p(a):
@@ -72,7 +76,7 @@ def _parr(arrs):
 class Oracle:
     """Binds one synthetic case (mom5_amd.synth.Case) to the oracle."""
 
-    def __init__(self, case):
+    def __init__(self, case, threads=False):
         self.case = case
         self.dom = Dom(**case.dom)
         g = case.grid
@@ -81,7 +85,7 @@ class Oracle:
                             ("dxt", "dyt", "dat", "datr", "dxte", "dyte", "dxtn", "dytn",
                              "dxu", "dyu", "dxur", "dyur", "f", "tmask", "umask", "dzt")},
                          kmt=_pi(self._keep["kmt"]))
-        self.L = lib()
+        self.L = lib(threads)
         nk, NJ, NI = case.nk, case.nj + 2, case.ni + 2
         self.shape3 = (nk, NJ, NI)
         self.shape2 = (NJ, NI)
