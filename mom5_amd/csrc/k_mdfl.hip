@@ -16,9 +16,15 @@
 // arithmetic as the tile that owns them, which is what the two
 // mpp_update_domains calls of the reference (:3912, :3970) deliver.
 //
-// FLAVOR 0 rounds like advect_tracer_mdfl_sweby (per-tracer routine, with a
-// sweby_limiter argument); FLAVOR 1 like advect_tracer_sweby_all (:4104-4511).
+// FLAVOR 0 orders its updates like advect_tracer_mdfl_sweby (per-tracer
+// routine, with a sweby_limiter argument); FLAVOR 1 like
+// advect_tracer_sweby_all (:4104-4511).  Limited fluxes (sweby_limiter = 1)
+// and the update factors come from mdfl64.h, the arithmetic the batched
+// step shares; other sweby_limiter values (DST-linear: 0) keep the
+// reference's quotient form of psi, whose 1/Rj singularity the unlimited
+// scheme needs.
 #include "mom_internal.h"
+#include "mdfl64.h"
 
 namespace {
 
@@ -91,6 +97,8 @@ __global__ __launch_bounds__(RX * NW) void k_mdfl(
   const int wave = threadIdx.x / RX;
   const int nk = d.nk, P2 = d.P2;
   const int gi = gi0 - 2 + lane;
+  const bool limited = FLAVOR == 1 || lim == 1.0;   // the Sweby limiter in full
+  const double rdt = 1.0 / dtime;
 
   int si = gi;
   bool colok = true;
@@ -137,12 +145,13 @@ __global__ __launch_bounds__(RX * NW) void k_mdfl(
   for (int k = 1; k <= nk; ++k) {
     const int k0 = k - 1;
     const int kp1 = min(k + 1, nk), kp2 = min(k + 2, nk), km1 = max(k - 1, 1);
-    double tz[RPW], rhov[RPW], wv[RPW], fbtv[RPW], txv[RPW], mfe[RPW], mfw[RPW], fxv[RPW];
+    double tz[RPW], rhov[RPW], dtrv[RPW], wv[RPW], fbtv[RPW], txv[RPW], mfe[RPW], mfw[RPW];
+    double fxv[RPW], xfv[RPW];
 
     // ---- Z: flux through the bottom face and the z-sweep update ----------
 #pragma unroll
     for (int q = 0; q < RPW; ++q) {
-      tz[q] = 0.0; rhov[q] = 1.0; wv[q] = 0.0; fbtv[q] = 0.0;
+      tz[q] = 0.0; rhov[q] = 1.0; dtrv[q] = 0.0; wv[q] = 0.0; fbtv[q] = 0.0;
       if (ok[q]) {
         const int o = offz[q];
         const double w = W[o + P2 * k];
@@ -153,16 +162,20 @@ __global__ __launch_bounds__(RX * NW) void k_mdfl(
         const double Rj = (t1[q] - t2[q]) * mk0 * mkp1;
         const double Rjm = (t2[q] - t3[q]) * mkp1 * mkp2;
         const double massflux = datv[q] * w;
-        const double cfl = fabs(w * dtime / r);
-        const double fbt = face_flux<FLAVOR>(Rjm, Rj, Rjp, massflux, cfl, t2[q], t1[q],
-                                             mkp1, mk0, lim);
+        const double dtr = dtime * mdfl64::rcp64(r);
+        const double cfl = mdfl64::cfl_z(w, dtr);
+        const double fbt =
+            limited ? mdfl64::flux(mdfl64::face(massflux, cfl, mkp1 * mk0), Rjm, Rj, Rjp, t2[q],
+                                   t1[q])
+                    : face_flux<FLAVOR>(Rjm, Rj, Rjp, massflux, cfl, t2[q], t1[q], mkp1, mk0,
+                                        lim);
         if (FLAVOR == 0) {
-          tz[q] = t1[q] + dtime / r * (datrv[q] * (fbt - ftp[q]) + t1[q] * (wkm1[q] - w));
+          tz[q] = t1[q] + dtr * (datrv[q] * (fbt - ftp[q]) + t1[q] * (wkm1[q] - w));
         } else {
           const double wk = datrv[q] * (fbt - ftp[q]) + t1[q] * (wkm1[q] - w);
-          tz[q] = t1[q] + wk * dtime / r;
+          tz[q] = t1[q] + wk * dtr;
         }
-        rhov[q] = r; wv[q] = w; fbtv[q] = fbt;
+        rhov[q] = r; dtrv[q] = dtr; wv[q] = w; fbtv[q] = fbt;
       }
     }
 
@@ -182,28 +195,30 @@ __global__ __launch_bounds__(RX * NW) void k_mdfl(
         const double u = U[of + P2 * k0];
         mf = dyte[of] * u;
         const double rw = R[of + P2 * k0], re = R[of + 1 + P2 * k0];
-        const double cfl = fabs(u * dtime * 2.0 / ((rw + re) * dxte[of]));
+        const double cfl = mdfl64::cfl_h(u, dtime, rw + re, dxte[of]);
         const double mi = mk(k, km[q]), mim1 = mk(k, kmm1);
         const double mip1 = mk(k, kmp1), mip2 = mk(k, kmp2);
         const double Rjp = (tzp2 - tzp1) * mip2 * mip1;
         const double Rj = (tzp1 - tz[q]) * mip1 * mi;
         const double Rjm = (tz[q] - tzm1) * mi * mim1;
-        fx = face_flux<FLAVOR>(Rjm, Rj, Rjp, mf, cfl, tz[q], tzp1, mi, mip1, lim);
+        fx = limited ? mdfl64::flux(mdfl64::face(mf, cfl, mi * mip1), Rjm, Rj, Rjp, tz[q], tzp1)
+                     : face_flux<FLAVOR>(Rjm, Rj, Rjp, mf, cfl, tz[q], tzp1, mi, mip1, lim);
       }
       const double fxw = __shfl(fx, lane - 1);
       const double mw = __shfl(mf, lane - 1);
-      double tx = 0.0;
+      double tx = 0.0, xf = 0.0;
       if (rowok[q] && lane >= 2 && lane < RX - 2 && gi < d.ni) {
         const double mi = mk(k, km[q]);
         if (FLAVOR == 0) {
-          tx = tz[q] + dtime * mi * datrv[q] / rhov[q] * (fxw - fx + t1[q] * (mf - mw));
+          xf = mdfl64::sweep_factor(dtrv[q], mi, datrv[q]);
+          tx = tz[q] + xf * (fxw - fx + t1[q] * (mf - mw));
         } else {
           const double wk = mi * datrv[q] * (fxw - fx + t1[q] * (mf - mw));
-          tx = tz[q] + wk * dtime / rhov[q];
+          tx = tz[q] + wk * dtrv[q];
         }
       }
       s_tx[ry][lane] = tx;
-      txv[q] = tx; mfe[q] = mf; mfw[q] = mw; fxv[q] = fx;
+      txv[q] = tx; xfv[q] = xf; mfe[q] = mf; mfw[q] = mw; fxv[q] = fx;
     }
     __syncthreads();
 
@@ -220,7 +235,7 @@ __global__ __launch_bounds__(RX * NW) void k_mdfl(
         const double v = V[of + P2 * k0];
         const double mf = dxtn[of] * v;
         const double rs = R[of + P2 * k0], rn = R[of + d.NI + P2 * k0];
-        const double cfl = fabs(v * dtime * 2.0 / ((rs + rn) * dytn[of]));
+        const double cfl = mdfl64::cfl_h(v, dtime, rs + rn, dytn[of]);
         const double tjm1 = s_tx[ry - 1][lane], tj = s_tx[ry][lane];
         const double tjp1 = s_tx[ry + 1][lane], tjp2 = s_tx[ry + 2][lane];
         const double mjm1 = mk(k, s_km[ry - 1][lane]), mj = mk(k, s_km[ry][lane]);
@@ -228,7 +243,8 @@ __global__ __launch_bounds__(RX * NW) void k_mdfl(
         const double Rjp = (tjp2 - tjp1) * mjp2 * mjp1;
         const double Rj = (tjp1 - tj) * mjp1 * mj;
         const double Rjm = (tj - tjm1) * mj * mjm1;
-        fy = face_flux<FLAVOR>(Rjm, Rj, Rjp, mf, cfl, tj, tjp1, mj, mjp1, lim);
+        fy = limited ? mdfl64::flux(mdfl64::face(mf, cfl, mj * mjp1), Rjm, Rj, Rjp, tj, tjp1)
+                     : face_flux<FLAVOR>(Rjm, Rj, Rjp, mf, cfl, tj, tjp1, mj, mjp1, lim);
       }
       s_fy[ry][lane] = fy;
       fyv[q] = fy;
@@ -248,15 +264,14 @@ __global__ __launch_bounds__(RX * NW) void k_mdfl(
         const double r = rhov[q], Tc = t1[q];
         double w1;
         if (FLAVOR == 0) {
-          double tr = txv[q] + dtime * mi * datrv[q] / r * (fys - fyn);
-          tr = tr + dtime * Tc / r * (wv[q] - wkm1[q] + datrv[q] * (mfw[q] - mfe[q]));
-          const double adv = -(r * (tr - Tc) / dtime * mi);
-          w1 = -adv;
+          double tr = txv[q] + xfv[q] * (fys - fyn);
+          tr = tr + dtrv[q] * Tc * (wv[q] - wkm1[q] + datrv[q] * (mfw[q] - mfe[q]));
+          w1 = mdfl64::tendency(r, tr, Tc, rdt, mi);
         } else {
           const double wk = mi * datrv[q] * (fys - fyn) +
                             Tc * (wv[q] - wkm1[q] + datrv[q] * (mfw[q] - mfe[q]));
-          const double tr = txv[q] + wk * dtime / r;
-          w1 = r * (tr - Tc) / dtime * mi;
+          const double tr = txv[q] + wk * dtrv[q];
+          w1 = mdfl64::tendency(r, tr, Tc, rdt, mi);
         }
         const int o = offz[q] + P2 * k0;
         th[o] = th[o] + w1;

@@ -8,16 +8,18 @@
 // dt*tmask*datr/rho, the divergence brackets) are formed once per point and
 // level and reused for the G tracers of the group.
 //
-// FP64 groups keep the reference's operation order for every tracer-
-// dependent operation, so their T(taup1) equals, bit for bit, the per-tracer
-// path (horz_advect_tracer into a zeroed th_tendency, then the explicit
-// update).  FP32 groups store and compute the tracer in single precision and
-// use the division-free form of the limited flux:
+// FP64 groups take every face flux and update factor from mdfl64.h, the
+// arithmetic of the per-tracer kernel, in the same order, so their T(taup1)
+// equals, bit for bit, the per-tracer path (horz_advect_tracer into a zeroed
+// th_tendency, then the explicit update).  FP32 groups store and compute the
+// tracer in single precision and use the same division-free form of the
+// limited flux:
 //   psi*Rj = max(0, min(Rj, d0*Rj + d1*Rjm, k*Rjm))   for Rj >= 0,
 //   psi*Rj = min(0, max(Rj, d0*Rj + d1*Rjm, k*Rjm))   for Rj < 0,
 // which is the limiter of ocean_tracer_advect.F90:3877-3885 multiplied
 // through by Rj (the 1e-30 guard is below FP32 resolution).
 #include "mom_internal.h"
+#include "mdfl64.h"
 
 #include <algorithm>
 #include <type_traits>
@@ -34,64 +36,12 @@ constexpr double EPSLN = 1.0e-40;   // shared/constants/constants.F90:123
 template <typename TT>
 __device__ inline TT mk(int k, int kmt) { return (k <= kmt) ? TT(1) : TT(0); }
 
-// Neighbour lanes of a 64-wide region row through DPP wavefront shifts
-// (v_mov_b32_dpp wave_shr:1 / wave_shl:1, full rate, no LDS traffic).  The
-// first (last) lane receives 0: callers use lane-1 only from lane 1 up and
-// lane+1, lane+2 only up to lane 62, 61.
-__device__ inline int dpp_prev(int x) { return __builtin_amdgcn_mov_dpp(x, 0x138, 0xf, 0xf, true); }
-__device__ inline int dpp_next(int x) { return __builtin_amdgcn_mov_dpp(x, 0x130, 0xf, 0xf, true); }
-__device__ inline float lane_prev(float x) {
-  return __builtin_bit_cast(float, dpp_prev(__builtin_bit_cast(int, x)));
-}
-__device__ inline float lane_next(float x) {
-  return __builtin_bit_cast(float, dpp_next(__builtin_bit_cast(int, x)));
-}
-__device__ inline double lane_prev(double x) {
-  const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
-  const unsigned long long lo = (unsigned)dpp_prev((int)(unsigned)b);
-  const unsigned long long hi = (unsigned)dpp_prev((int)(unsigned)(b >> 32));
-  return __builtin_bit_cast(double, lo | (hi << 32));
-}
-__device__ inline double lane_next(double x) {
-  const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
-  const unsigned long long lo = (unsigned)dpp_next((int)(unsigned)b);
-  const unsigned long long hi = (unsigned)dpp_next((int)(unsigned)(b >> 32));
-  return __builtin_bit_cast(double, lo | (hi << 32));
-}
-__device__ inline int lane_prev(int x) { return dpp_prev(x); }
-__device__ inline int lane_next(int x) { return dpp_next(x); }
-
 // tracer-independent limiter coefficients of one face
 template <typename TT>
 struct FaceC {
   TT M;      // mass flux through the face
   TT d0, d1, kk;
 };
-
-__device__ inline FaceC<double> face_coef(double massflux, double cfl) {
-  FaceC<double> c;
-  c.M = massflux;
-  c.d0 = (2.0 - cfl) * (1.0 - cfl) * ONESIXTH;
-  c.d1 = (1.0 - cfl * cfl) * ONESIXTH;
-  c.kk = (1.0 - cfl) / (EPS30 + cfl);
-  return c;
-}
-
-// reference-order limited flux (FP64), lim = 1
-// mab = ma*mb of the two cells of the face: for masks in {0, 1},
-// x*ma*mb == x*(ma*mb) bit for bit (signed zeros and NaN included)
-__device__ inline double flux_exact(const FaceC<double> &c, double Rjm, double Rj, double Rjp,
-                                    double Tpos, double Tneg, double mab) {
-  const double thetaP = Rjm / (EPS30 + Rj);
-  const double thetaM = Rjp / (EPS30 + Rj);
-  const double pP = c.d0 + c.d1 * thetaP;
-  const double pM = c.d0 + c.d1 * thetaM;
-  // psi*(1-lim) + max(...)*lim with lim = 1, as in the per-tracer routine
-  const double psiP = pP * 0.0 + fmx(0.0, fmn(fmn(1.0, c.d0 + c.d1 * thetaP), c.kk * thetaP)) * 1.0;
-  const double psiM = pM * 0.0 + fmx(0.0, fmn(fmn(1.0, c.d0 + c.d1 * thetaM), c.kk * thetaM)) * 1.0;
-  return 0.5 * ((c.M + fabs(c.M)) * (Tpos + psiP * Rj) + (c.M - fabs(c.M)) * (Tneg - psiM * Rj)) *
-         mab;
-}
 
 // FP32 groups: the same coefficients in single precision, with the
 // hardware reciprocal (about 1 ulp) in place of the divisions
@@ -108,18 +58,23 @@ __device__ inline FaceC<float> face_coef_f(float massflux, float cfl) {
   return c;
 }
 
-// division-free limited flux (FP32).  The hardware min/max/median
-// (v_min_f32, v_max_f32, v_med3_f32 on finite data) replace compare-and-
-// select: FP32 tracers need no particular signed zero, only the same code
-// everywhere.
+// division-free limited flux (FP32).  The hardware median (v_med3_f32 on
+// finite data) replaces compare-and-select: FP32 tracers need no particular
+// signed zero, only the same code everywhere.
+//   a >= 0: max(0, min(a, lin, kr)) = med3(0, a, min(lin, kr)),
+//   a <  0: min(0, max(a, lin, kr)) = med3(0, a, max(lin, kr)),
+// and min(lin, kr) = -med3(-lin, -kr, +inf), max(lin, kr) = -med3(-lin, -kr,
+// -inf): sel = copysign(inf, a) picks the one the sign of a asks for (a = -0
+// takes the max, whose median with 0 and a is a zero all the same), and the
+// negations are source modifiers of the medians, so one face's limited terms
+// cost a bit insert and four medians.
+__device__ inline float minmax_sel(float a) { return __builtin_copysignf(__builtin_inff(), a); }
 // d0a = d0*a is shared by the two limited fluxes of a face
-__device__ inline float psi_r(const FaceC<float> &c, float d0a, float a, float r) {
+__device__ inline float psi_r(const FaceC<float> &c, float d0a, float a, float sel, float r) {
   const float lin = fmaf(c.d1, r, d0a);
   const float kr = c.kk * r;
-  // a >= 0: max(0, min(a, lin, kr)) = med3(0, a, min(lin, kr));
-  // a <  0: min(0, max(a, lin, kr)) = med3(0, a, max(lin, kr))
-  const float m = (a >= 0.0f) ? __builtin_fminf(lin, kr) : __builtin_fmaxf(lin, kr);
-  return __builtin_amdgcn_fmed3f(0.0f, a, m);
+  const float nm = __builtin_amdgcn_fmed3f(-lin, -kr, sel);
+  return __builtin_amdgcn_fmed3f(0.0f, a, -nm);
 }
 // The upwind weights of a face with the 1/2 and the face mask folded in:
 // mp = (M + |M|)/2 * mab, mn = (M - |M|)/2 * mab (power-of-two and 0/1
@@ -133,8 +88,9 @@ __device__ inline FaceW face_w(const FaceC<float> &c, float mab) {
 __device__ inline float flux_fast(const FaceC<float> &c, const FaceW &w, float Rjm, float Rj,
                                   float Rjp, float Tpos, float Tneg) {
   const float d0a = c.d0 * Rj;
-  const float pP = psi_r(c, d0a, Rj, Rjm);
-  const float pM = psi_r(c, d0a, Rj, Rjp);
+  const float sel = minmax_sel(Rj);
+  const float pP = psi_r(c, d0a, Rj, sel, Rjm);
+  const float pM = psi_r(c, d0a, Rj, sel, Rjp);
   return fmaf(w.mp, Tpos + pP, w.mn * (Tneg - pM));
 }
 
@@ -178,6 +134,7 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
   const double *__restrict__ dxtn = J.dxtn;
   const double *__restrict__ dytn = J.dytn;
   const double dtime = J.dtime;
+  const double rdt = 1.0 / dtime;
   const int ntx = J.ntx;
   constexpr bool EXACT = sizeof(TT) == 8;
   constexpr int RY = NW * RPW;
@@ -282,7 +239,7 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
   for (int k = 1; k <= nk; ++k) {
     const int k0 = k - 1;
     const int kp1 = min(k + 1, nk), kp2 = min(k + 2, nk), km1 = max(k - 1, 1);
-    double rhov[RPW], xf64v[RPW];
+    double rhov[RPW], xf64v[RPW], dtrv[RPW];   // dtrv: FP64 groups, dtime/rho_dzt
     RT wv[RPW];
     float rinv[RPW];   // FP32 groups: about 1/rho_dzt
     TT tz[G][RPW], fbtv[G][RPW], txv[G][RPW], fyv[G][RPW];
@@ -292,7 +249,7 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
     // ---- Z ----------------------------------------------------------------
 #pragma unroll
     for (int q = 0; q < RPW; ++q) {
-      rhov[q] = 1.0; wv[q] = RT(0); rinv[q] = 1.0f;
+      rhov[q] = 1.0; wv[q] = RT(0); rinv[q] = 1.0f; dtrv[q] = 0.0;
       if (ok[q]) {
         const int o = offz[q];
         const double w = W[o + P2 * k];
@@ -302,15 +259,16 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
         // mk(kp2): km1 <= k <= kp1 <= kp2 and masks in {0, 1}
         const TT mA = mk<TT>(k, km[q]), mB = mk<TT>(kp1, km[q]), mC = mk<TT>(kp2, km[q]);
         if constexpr (EXACT) {
-          const FaceC<double> cz = face_coef(datv[q] * w, fabs(w * dtime / r));
-          const double dtr = dtime / r;
+          const double dtr = dtime * mdfl64::rcp64(r);
+          dtrv[q] = dtr;
+          const mdfl64::Face cz = mdfl64::face(datv[q] * w, mdfl64::cfl_z(w, dtr), mB);
           const double wdiff = wkm1[q] - w;
 #pragma unroll
           for (int g = 0; g < G; ++g) {
             const double Rjp = (t0[g][q] - t1[g][q]) * mA;
             const double Rj = (t1[g][q] - t2[g][q]) * mB;
             const double Rjm = (t2[g][q] - t3[g][q]) * mC;
-            const double fbt = flux_exact(cz, Rjm, Rj, Rjp, t2[g][q], t1[g][q], mB);
+            const double fbt = mdfl64::flux(cz, Rjm, Rj, Rjp, t2[g][q], t1[g][q]);
             tz[g][q] = t1[g][q] + dtr * (datrv[q] * (fbt - ftp[g][q]) + t1[g][q] * wdiff);
             fbtv[g][q] = fbt;
           }
@@ -345,31 +303,33 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
     for (int q = 0; q < RPW; ++q) {
       const int ry = wave + NW * q;
       const bool face = rowok[q] && lane >= 1 && lane <= RX - 3 && gi >= -1 && gi <= d.ni - 1;
-      FaceC<double> cx{0.0, 0.0, 0.0, 0.0};
+      if constexpr (EXACT) face_depths(q, kx0[q], kx1[q], kx2[q], ky0[q], ky1[q], ky2[q]);
+      const TT mi = mk<TT>(k, km[q]);
+      const TT mX2 = mk<TT>(k, kx2[q]), mX1 = mk<TT>(k, kx1[q]), mX0 = mk<TT>(k, kx0[q]);
+      mdfl64::Face cx{0.0, 0.0, 0.0, 0.0, 0.0};
+      double Mx = 0.0;
       FaceC<float> cxf{0.0f, 0.0f, 0.0f, 0.0f};
       if (face) {
         const int of = d2(d, gi, sjr[q]);
         const double u = U[of + P2 * k0];
         const double rw = R[of + P2 * k0], re = R[of + 1 + P2 * k0];
         if constexpr (EXACT) {
-          cx = face_coef(dyte[of] * u, fabs(u * dtime * 2.0 / ((rw + re) * dxte[of])));
+          Mx = dyte[of] * u;
+          cx = mdfl64::face(Mx, mdfl64::cfl_h(u, dtime, rw + re, dxte[of]), (double)mX1);
         } else {
           const float uf = (float)u;
           cxf = face_coef_f(mdyte[q] * uf, fabsf(uf * (float)(dtime * 2.0) *
                                                  rcpf(((float)rw + (float)re) * mdxte[q])));
         }
       }
-      if constexpr (EXACT) face_depths(q, kx0[q], kx1[q], kx2[q], ky0[q], ky1[q], ky2[q]);
-      const TT mi = mk<TT>(k, km[q]);
-      const TT mX2 = mk<TT>(k, kx2[q]), mX1 = mk<TT>(k, kx1[q]), mX0 = mk<TT>(k, kx0[q]);
       const bool cell = rowok[q] && lane >= 2 && lane < RX - 2 && gi < d.ni;
       double mw64 = 0.0;
       if constexpr (EXACT) {
-        mw64 = lane_prev(cx.M);
+        mw64 = lane_prev(Mx);
         // dtime*tmask*datr/rho (x and y sweeps)
-        const double xf64 = dtime * (double)mi * datrv[q] / rhov[q];
+        const double xf64 = mdfl64::sweep_factor(dtrv[q], (double)mi, datrv[q]);
         xf64v[q] = xf64;
-        mfe[q] = (TT)cx.M;
+        mfe[q] = (TT)Mx;
         mfw[q] = (TT)mw64;
         xfac[q] = (TT)xf64;
       } else {
@@ -391,7 +351,7 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
           const TT Rj = EXACT ? (tzp1 - tz[g][q]) * mX1 : tzp1 - tz[g][q];  // FP32: face weights carry mX1
           const TT Rjm = (tz[g][q] - tzm1) * mX0;
           if constexpr (EXACT)
-            fx = flux_exact(cx, Rjm, Rj, Rjp, tz[g][q], tzp1, mX1);
+            fx = mdfl64::flux(cx, Rjm, Rj, Rjp, tz[g][q], tzp1);
           else
             fx = flux_fast(cxf, wx, Rjm, Rj, Rjp, tz[g][q], tzp1);
         }
@@ -407,7 +367,7 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
         TT tx = TT(0);
         if (cell) {
           if constexpr (EXACT)
-            tx = tz[g][q] + xf64v[q] * (fxw - fx + t1[g][q] * (cx.M - mw64));
+            tx = tz[g][q] + xf64v[q] * (fxw - fx + t1[g][q] * (Mx - mw64));
           else
             tx = fmaf(xfac[q], fmaf(t1[g][q], mfe[q] - mfw[q], fxw - fx), tz[g][q]);
         }
@@ -424,21 +384,22 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
       const int gj = gjr[q];
       const bool face = ry >= 1 && ry <= RY - 3 && lane >= 2 && lane < RX - 2 && gi < d.ni &&
                         gj >= -1 && gj <= d.nj - 1;
-      FaceC<double> cy{0.0, 0.0, 0.0, 0.0};
+      const TT mY2 = mk<TT>(k, ky2[q]), mY1 = mk<TT>(k, ky1[q]), mY0 = mk<TT>(k, ky0[q]);
+      mdfl64::Face cy{0.0, 0.0, 0.0, 0.0, 0.0};
       FaceC<float> cyf{0.0f, 0.0f, 0.0f, 0.0f};
       if (face) {
         const int of = d2(d, gi, gj);
         const double v = V[of + P2 * k0];
         const double rs = R[of + P2 * k0], rn = R[of + d.NI + P2 * k0];
         if constexpr (EXACT) {
-          cy = face_coef(dxtn[of] * v, fabs(v * dtime * 2.0 / ((rs + rn) * dytn[of])));
+          cy = mdfl64::face(dxtn[of] * v, mdfl64::cfl_h(v, dtime, rs + rn, dytn[of]),
+                            (double)mY1);
         } else {
           const float vf = (float)v;
           cyf = face_coef_f(mdxtn[q] * vf, fabsf(vf * (float)(dtime * 2.0) *
                                                  rcpf(((float)rs + (float)rn) * mdytn[q])));
         }
       }
-      const TT mY2 = mk<TT>(k, ky2[q]), mY1 = mk<TT>(k, ky1[q]), mY0 = mk<TT>(k, ky0[q]);
       FaceW wy{0.0f, 0.0f};
       if constexpr (!EXACT) wy = face_w(cyf, mY1);
 #pragma unroll
@@ -451,7 +412,7 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
           const TT Rj = EXACT ? (tjp1 - tj) * mY1 : tjp1 - tj;   // FP32: face weights carry mY1
           const TT Rjm = (tj - tjm1) * mY0;
           if constexpr (EXACT)
-            fy = flux_exact(cy, Rjm, Rj, Rjp, tj, tjp1, mY1);
+            fy = mdfl64::flux(cy, Rjm, Rj, Rjp, tj, tjp1);
           else
             fy = flux_fast(cyf, wy, Rjm, Rj, Rjp, tj, tjp1);
         }
@@ -492,9 +453,8 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
             const double thin = tab.th[g] ? tab.th[g][o] : 0.0;
             const double Tc = t1[g][q];
             double tr = txv[g][q] + xf64v[q] * (s_fy[g][ry - 1][lane] - fyv[g][q]);
-            tr = tr + dtime * Tc / r * brk;
-            const double adv = -(r * (tr - Tc) / dtime * mi);
-            const double th = thin + (-adv);
+            tr = tr + dtrv[q] * Tc * brk;
+            const double th = thin + mdfl64::tendency(r, tr, Tc, rdt, mi);
             tab.out[g][o] = (rm1 * Tc + dtime * th) * rr;
           } else {
             const float Tc = t1[g][q];

@@ -82,6 +82,34 @@ __device__ inline bool halo_source(const KDom &d, int ii, int jj, int &si, int &
 
 // Fortran MIN/MAX on non-NaN data.  The ternary form pins which operand
 // is returned for +0/-0 so that the result matches the CPU restatement.
+// Neighbour lanes of a 64-wide region row through DPP wavefront shifts
+// (v_mov_b32_dpp wave_shr:1 / wave_shl:1, full rate, no LDS traffic).  The
+// first (last) lane receives 0: callers use lane-1 only from lane 1 up and
+// lane+1, lane+2 only up to lane 62, 61.  A lane whose source lane is
+// inactive also receives 0, so callers shift outside divergent branches.
+__device__ inline int dpp_prev(int x) { return __builtin_amdgcn_mov_dpp(x, 0x138, 0xf, 0xf, true); }
+__device__ inline int dpp_next(int x) { return __builtin_amdgcn_mov_dpp(x, 0x130, 0xf, 0xf, true); }
+__device__ inline float lane_prev(float x) {
+  return __builtin_bit_cast(float, dpp_prev(__builtin_bit_cast(int, x)));
+}
+__device__ inline float lane_next(float x) {
+  return __builtin_bit_cast(float, dpp_next(__builtin_bit_cast(int, x)));
+}
+__device__ inline double lane_prev(double x) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+  const unsigned long long lo = (unsigned)dpp_prev((int)(unsigned)b);
+  const unsigned long long hi = (unsigned)dpp_prev((int)(unsigned)(b >> 32));
+  return __builtin_bit_cast(double, lo | (hi << 32));
+}
+__device__ inline double lane_next(double x) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+  const unsigned long long lo = (unsigned)dpp_next((int)(unsigned)b);
+  const unsigned long long hi = (unsigned)dpp_next((int)(unsigned)(b >> 32));
+  return __builtin_bit_cast(double, lo | (hi << 32));
+}
+__device__ inline int lane_prev(int x) { return dpp_prev(x); }
+__device__ inline int lane_next(int x) { return dpp_next(x); }
+
 __host__ __device__ inline double fmn(double a, double b) { return (b < a) ? b : a; }
 __host__ __device__ inline double fmx(double a, double b) { return (b > a) ? b : a; }
 __host__ __device__ inline float fmnf(float a, float b) { return (b < a) ? b : a; }

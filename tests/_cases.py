@@ -112,3 +112,46 @@ def host_case(case):
               for k, v in case.fields.items()}
     tracers = [(nm, conv(t).astype(np.float64)) for nm, t in case.tracers]
     return dataclasses.replace(case, grid=grid, fields=fields, tracers=tracers)
+
+
+# ---- FP64 tolerance of the division-free MDFL-Sweby arithmetic ---------------
+# The HIP kernels form the limited flux without its divisions and the update
+# factors with reciprocals (rounding differs from the reference's quotients by
+# a few ulp); the bar is 1e-12 relative per cell on the updated tracer, and
+# for intermediate quantities (tendencies, fluxes) 1e-12 of the quantity's
+# natural scale.
+RTOL64 = 1e-12
+
+
+def assert_close_cells(got, ref, rtol=RTOL64, what=""):
+    """|got - ref| <= rtol*|ref| per cell (a floor of 1e-6*max|ref| keeps cells
+    that are exactly zero in the reference from demanding exact zeros)."""
+    got = np.asarray(got, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    assert got.shape == ref.shape, (what, got.shape, ref.shape)
+    assert np.isfinite(got).all(), f"{what}: non-finite values"
+    scale = np.maximum(np.abs(ref), 1e-6 * np.abs(ref).max())
+    err = np.abs(got - ref)
+    bad = err > rtol * scale
+    if bad.any():
+        i = tuple(np.argwhere(bad)[0])
+        raise AssertionError(f"{what}: {int(bad.sum())} cells beyond {rtol:g} relative, first at "
+                             f"{i}: {got[i]!r} vs {ref[i]!r} (max rel "
+                             f"{(err / np.where(scale > 0, scale, 1.0)).max():.3e})")
+    return float((err / np.where(scale > 0, scale, 1.0)).max())
+
+
+def assert_close_scaled(got, ref, scale, rtol=RTOL64, what=""):
+    """|got - ref| <= rtol*scale elementwise (scale: an array or a number)."""
+    got = np.asarray(got, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    assert got.shape == ref.shape, (what, got.shape, ref.shape)
+    assert np.isfinite(got).all(), f"{what}: non-finite values"
+    err = np.abs(got - ref)
+    tol = rtol * np.broadcast_to(np.asarray(scale, dtype=np.float64), err.shape)
+    bad = err > tol
+    if bad.any():
+        i = tuple(np.argwhere(bad)[0])
+        raise AssertionError(f"{what}: {int(bad.sum())} values beyond {rtol:g} of scale, first "
+                             f"at {i}: {got[i]!r} vs {ref[i]!r} (err {err[i]:.3e}, tol "
+                             f"{tol[i]:.3e})")
