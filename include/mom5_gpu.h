@@ -510,6 +510,19 @@ int mom_gpu_tracer_content(mom_gpu_ctx_t *ctx, const double *T,
 int mom_gpu_tracer_content_f32(mom_gpu_ctx_t *ctx, const float *T,
     const double *rho_dzt, double *content);
 
+/* The masked content of mom_gpu_tracer_content (dtype 8) or
+ * mom_gpu_tracer_content_f32 (dtype 4), bit for bit the same value, written
+ * to DEVICE memory (content_dev: one double) and computed without allocating
+ * anything: no hipMalloc, hipFree, host copy or stream sync.  No counterpart
+ * in the reference: for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops).  The partial sums live in one fixed array of
+ * the library per GPU (1024 blocks' partials, 16 KiB, held until the process
+ * ends); a context whose reduction uses more blocks is refused.  Runs
+ * asynchronously on ctx's stream.  Calls that overlap on one GPU (other
+ * streams, other contexts on that GPU) share that array and race. */
+int mom_gpu_tracer_content_noalloc(mom_gpu_ctx_t *ctx, const void *T, int dtype,
+    const double *rho_dzt, double *content_dev);
+
 /* Error of a tracer a (dtype 8: FP64, 4: FP32) against an FP64 result b
  * over the wet compute cells (k <= kmt).  No reference counterpart: the
  * report of each result's error against an FP64 run.  Writes 7 doubles to

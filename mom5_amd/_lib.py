@@ -133,6 +133,7 @@ _PROTOS = {
                                        C.POINTER(C.c_double)]),
     "mom_gpu_tracer_content_f32": (C.c_int, [_vp, _vp, _vp, C.POINTER(C.c_double)]),
     "mom_gpu_tracer_content": (C.c_int, [_vp, _dp, _dp, C.POINTER(C.c_double)]),
+    "mom_gpu_tracer_content_noalloc": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp]),
 }
 
 EXPORTED = tuple(_PROTOS)

@@ -19,7 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmom5_gpu.so")
 SOURCES = ["mom_gpu_api.hip", "k_mdfl.hip", "k_quicker.hip", "k_column.hip", "k_barotropic.hip",
            "k_step.hip", "k_synth.hip", "k_advvel.hip", "k_btvariant.hip",
-           "k_quick_step.hip"]
+           "k_quick_step.hip", "k_noalloc.hip"]
 ARCH = os.environ.get("MOM5_OFFLOAD_ARCH", "gfx950")
 
 

@@ -1512,19 +1512,26 @@ int launch_tracer_error(mom_gpu_ctx *ctx, const void *a, int dtype, const double
   return MOM_GPU_OK;
 }
 
-int launch_content(mom_gpu_ctx *ctx, const void *T, int dtype, const double *rho, double *out) {
+int launch_content_part(mom_gpu_ctx *ctx, const void *T, int dtype, const double *rho,
+                        double *part, double *out_dev) {
   const KDom &d = ctx->kd;
   const int nb = ctx->red_n;
   if (dtype == 4)
     hipLaunchKernelGGL(k_content<float>, dim3(nb), dim3(RB), 0, ctx->stream, d,
-                       (const float *)T, rho, ctx->dat, ctx->kmt, ctx->red);
+                       (const float *)T, rho, ctx->dat, ctx->kmt, part);
   else
     hipLaunchKernelGGL(k_content<double>, dim3(nb), dim3(RB), 0, ctx->stream, d,
-                       (const double *)T, rho, ctx->dat, ctx->kmt, ctx->red);
+                       (const double *)T, rho, ctx->dat, ctx->kmt, part);
   MOM_LAUNCHED(ctx, "k_content");
-  hipLaunchKernelGGL(k_content_final, dim3(1), dim3(RB), 0, ctx->stream, ctx->red, nb,
-                     ctx->red + 2 * nb);
+  hipLaunchKernelGGL(k_content_final, dim3(1), dim3(RB), 0, ctx->stream, part, nb, out_dev);
   MOM_LAUNCHED(ctx, "k_content_final");
+  return MOM_GPU_OK;
+}
+
+int launch_content(mom_gpu_ctx *ctx, const void *T, int dtype, const double *rho, double *out) {
+  const int nb = ctx->red_n;
+  const int rc = launch_content_part(ctx, T, dtype, rho, ctx->red, ctx->red + 2 * nb);
+  if (rc != MOM_GPU_OK) return rc;
   MOM_HIP(ctx, hipMemcpyAsync(out, ctx->red + 2 * nb, sizeof(double), hipMemcpyDeviceToHost,
                               ctx->stream));
   MOM_HIP(ctx, hipStreamSynchronize(ctx->stream));

@@ -262,6 +262,10 @@ int launch_barotropic_blocked(mom_gpu_ctx *ctx, const mom_gpu_bt_params_t *p,
                               const double *rho_g, double *eta_bar, double *ud, double *conv,
                               int hsub);
 int launch_content(mom_gpu_ctx *ctx, const void *T, int dtype, const double *rho, double *out);
+// the content kernels with their partial sums (2 doubles per block, ctx->red_n
+// blocks) in `part` and the result in device memory `out_dev`; no copy, no sync
+int launch_content_part(mom_gpu_ctx *ctx, const void *T, int dtype, const double *rho,
+                        double *part, double *out_dev);
 int launch_tracer_error(mom_gpu_ctx *ctx, const void *a, int dtype, const double *b,
                         double floor, double *out);
 int launch_fill(mom_gpu_ctx *ctx, double *p, double v, size_t n);
