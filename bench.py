@@ -625,10 +625,11 @@ def run_plain(args, w):
 
 
 def profile_kernels(args, w):
-    """Launch only the kernels whose HBM traffic is measured with
-    rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE), three dispatches each:
-    k_copy of one 3-D FP64 field (the calibration: reads and writes
-    exactly its size) and one pass of each fused tracer-step group."""
+    """Launch only the kernels of one step, three dispatches each, for
+    rocprofv3 (--kernel-trace --stats, --pmc): k_copy of one 3-D FP64 field
+    (the calibration of FETCH_SIZE and WRITE_SIZE: it reads and writes exactly
+    its size), one pass of each fused tracer-step group, the implicit
+    diffusion of every tracer and the barotropic subcycles of one step."""
     from mom5_amd import DeviceArray
     from mom5_amd.synth import make_case_device
     ni, nj, nk = w["ni"], w["nj"], w["nk"]
@@ -666,6 +667,21 @@ def profile_kernels(args, w):
         g.step_tracers(trs, out, **kwd)
     g.sync()
     log(f"implicit diffusion of {ns} tracers x3 (k_vdiff_mix when merged)")
+    # the barotropic subcycles of one step (k_bt_block launches and their tails)
+    up2 = {k: up(f[k]) for k in ("eta_t_bar", "udrho", "forcing_bt", "thicku", "patm", "pme",
+                                  "river", "source", "eta_smooth", "rho_g")}
+    eb, cv = DeviceArray(g.shape2), DeviceArray(g.shape2)
+    ud = DeviceArray((2,) + g.shape2)
+    for _ in range(3):
+        g.update_ocean_barotropic(nts=case.nts, dtbt=case.dtbt, eta_t_bar_tau=up2["eta_t_bar"],
+                                  udrho_tau=up2["udrho"], forcing_bt=up2["forcing_bt"],
+                                  thicku=up2["thicku"], patm=up2["patm"], pme=up2["pme"],
+                                  river=up2["river"], source=up2["source"],
+                                  eta_smooth=up2["eta_smooth"], rho_g=up2["rho_g"],
+                                  eta_t_bar_taup1=eb, udrho_taup1=ud, conv_rho_ud_t=cv)
+    g.sync()
+    log(f"update_ocean_barotropic x3 (nts={case.nts}, "
+        f"{g.get_tuning('barotropic_subcycles_per_launch')} subcycles per launch)")
 
 
 def main():

@@ -56,19 +56,25 @@ __device__ __forceinline__ Face face(double massflux, double cfl, double mab) {
   return f;
 }
 
+// x with its sign bit flipped where sgn (0 or the sign bit) is set: an
+// exact multiplication by -1
+__device__ __forceinline__ double flip(double x, unsigned long long sgn) {
+  return __builtin_bit_cast(double, __builtin_bit_cast(unsigned long long, x) ^ sgn);
+}
+
 // psi*Rj for the neighbouring difference Rn; d0a = d0*Rj (shared by the two
-// limited terms of a face)
-__device__ __forceinline__ double psi_r(const Face &f, double d0a, double Rj, double Rn) {
+// limited terms of a face), sgn the sign bit of Rj.  With s = -1 where Rj's
+// sign bit is set and +1 elsewhere (exact sign flips),
+//   psi*Rj = s*max(0, min(|Rj|, s*lin, s*kr)),
+// which is max(0, min(Rj, lin, kr)) for Rj >= 0 and min(0, max(Rj, lin, kr))
+// for Rj < 0 without a branch.
+__device__ __forceinline__ double psi_r(const Face &f, double d0a, double Rj,
+                                        unsigned long long sgn, double Rn) {
   const double lin = fma(f.d1, Rn, d0a);
   const double kr = f.kk * Rn;
-  if (Rj >= 0.0) {
-    const double m = (kr < lin) ? kr : lin;
-    const double t = (m < Rj) ? m : Rj;
-    return (t > 0.0) ? t : 0.0;
-  }
-  const double m = (kr > lin) ? kr : lin;
-  const double t = (m > Rj) ? m : Rj;
-  return (t < 0.0) ? t : 0.0;
+  const double m = __builtin_fmin(flip(lin, sgn), flip(kr, sgn));
+  const double t = __builtin_fmax(0.0, __builtin_fmin(fabs(Rj), m));
+  return flip(t, sgn);
 }
 
 // The limited flux: Tpos is upstream for M > 0, Tneg for M < 0; Rjm, Rj, Rjp
@@ -77,8 +83,9 @@ __device__ __forceinline__ double psi_r(const Face &f, double d0a, double Rj, do
 __device__ __forceinline__ double flux(const Face &f, double Rjm, double Rj, double Rjp,
                                        double Tpos, double Tneg) {
   const double d0a = f.d0 * Rj;
-  const double pP = psi_r(f, d0a, Rj, Rjm);
-  const double pM = psi_r(f, d0a, Rj, Rjp);
+  const unsigned long long sgn = __builtin_bit_cast(unsigned long long, Rj) & (1ull << 63);
+  const double pP = psi_r(f, d0a, Rj, sgn, Rjm);
+  const double pM = psi_r(f, d0a, Rj, sgn, Rjp);
   return fma(f.mp, Tpos + pP, f.mn * (Tneg - pM));
 }
 
