@@ -181,13 +181,9 @@ __global__ void k_bt_conv(KDom d, const double *__restrict__ ud, const double *_
 
 // ---- temporally blocked subcycles -----------------------------------------
 // One work-group owns a BRY x BRX region of the data domain and advances it
-// nsub subcycles with the state (eta, U) and the accumulators in registers.
-// Each wavefront holds BPT consecutive region rows, one lane per column: the
-// western and eastern neighbours come from DPP lane shifts and the rows of
-// the same wavefront from registers, so only the rows at a wavefront's edges
-// go through LDS (U*dyu, U*dxu of its last row for the divergence below it,
-// the surface pressure of its first row for the gradient above it).  Each
-// subcycle invalidates two rows/columns on the low side and one on
+// nsub subcycles with the state (eta, U) and the accumulators in registers
+// and only the exchanged quantities (U*dyu, U*dxu, surface pressure) in
+// LDS.  Each subcycle invalidates two rows/columns on the low side and one on
 // the high side of the region, so after nsub subcycles the interior
 // (BRY-3*nsub) x (BRX-3*nsub) is exact; the rest is recomputed by the
 // neighbouring groups.  On a cyclic-x domain the region addresses the
@@ -195,11 +191,10 @@ __global__ void k_bt_conv(KDom d, const double *__restrict__ ud, const double *_
 // holding the wrap (what mpp_update_domains leaves), this reproduces the
 // reference's evolution of every point, the halo columns of the compute
 // rows excepted -- those are rewritten by the tail's halo update anyway.
-constexpr int BRX = 64;          // region columns (one lane per column)
+constexpr int BRX = 64;          // region columns (one wavefront per row)
 constexpr int BRY = 32;          // region rows
-constexpr int BPT = 2;           // consecutive region rows per wavefront
+constexpr int BPT = 2;           // region rows per wavefront
 constexpr int BNT = BRX * BRY / BPT;
-constexpr int BNW = BNT / BRX;   // wavefronts per work-group
 
 // sf: tmask * (pme + river + source - eta_smooth) and mask: bit 0 tmask(k=1),
 // bit 1 umask(k=1), formed once per call by k_bt_sfmask (constant over the
@@ -247,17 +242,9 @@ __device__ __forceinline__ void bt_tile(const BtJob &J, int tile) {
   const int s0 = J.s0, nsub = J.nsub, nts = J.nts, step_a = J.step_a;
   const double dtbt = J.dtbt, cp = J.cp, cc = J.cc;
   const int xbase = J.xbase, xcount = J.xcount, ntx = J.ntx;
-  // edge rows of each wavefront: U*dyu, U*dxu of its last row, surface
-  // pressure of its first row
-  __shared__ double s_uh[BNW][BRX];
-  __shared__ double s_vh[BNW][BRX];
-  __shared__ double s_ps[BNW][BRX];
-  // the time-average accumulators (eta_bar, udrho) of every region point,
-  // touched once per subcycle: in LDS, leaving the registers to the state
-  // and the per-point constants
-  __shared__ double s_eb[BRY][BRX];
-  __shared__ double s_ub1[BRY][BRX];
-  __shared__ double s_ub2[BRY][BRX];
+  __shared__ double s_ps[BRY][BRX];
+  __shared__ double s_uh[BRY][BRX];
+  __shared__ double s_vh[BRY][BRX];
   const int TX = BRX - 3 * nsub, TY = BRY - 3 * nsub;
   const int tix = tile % ntx, tiy = tile / ntx;
   const int x_int0 = xbase + tix * TX, y_int0 = tiy * TY;
@@ -265,12 +252,11 @@ __device__ __forceinline__ void bt_tile(const BtJob &J, int tile) {
   const int P2 = d.P2;
   const bool cyc = d.cyclic_x != 0;
 
-  int off[BPT], mbits[BPT];   // mbits: bit 0 tmask(k=1), bit 1 umask(k=1)
-  bool interior[BPT], div0[BPT], grad0[BPT];
-  double eta[BPT], u1[BPT], u2[BPT];
-  double sf[BPT], datr[BPT], dyu[BPT], dxu[BPT], rho_g[BPT], patm[BPT], thk[BPT];
-  double F1[BPT], F2[BPT], fc[BPT], cori2[BPT], dxur[BPT], dyur[BPT];
-  const double hdtbt = 0.5 * dtbt;   // cori1 = 0.5*dtbt*f, formed where used
+  int rr[BPT], off[BPT], vj[BPT];
+  bool present[BPT], interior[BPT], div0[BPT], grad0[BPT];
+  double eta[BPT], u1[BPT], u2[BPT], ebv[BPT], ub1[BPT], ub2[BPT];
+  double tm[BPT], sf[BPT], datr[BPT], dyu[BPT], dxu[BPT], rho_g[BPT], patm[BPT], thk[BPT];
+  double F1[BPT], F2[BPT], fc[BPT], cori1[BPT], cori2[BPT], dxur[BPT], dyur[BPT], um[BPT];
   const int vi = x_int0 - 2 * nsub + lane;
   int col = vi;
   bool colok = true;
@@ -278,138 +264,122 @@ __device__ __forceinline__ void bt_tile(const BtJob &J, int tile) {
   else colok = vi >= 0 && vi < d.NI;
 #pragma unroll
   for (int q = 0; q < BPT; ++q) {
-    const int r = BPT * wave + q;
+    const int r = wave + (BNT / BRX) * q;
+    rr[q] = r;
     const int j = y_int0 - 2 * nsub + r;
-    const bool present = colok && j >= 0 && j < d.NJ;
-    interior[q] = present && lane >= 2 * nsub && lane < 2 * nsub + TX &&
+    vj[q] = j;
+    present[q] = colok && j >= 0 && j < d.NJ;
+    interior[q] = present[q] && lane >= 2 * nsub && lane < 2 * nsub + TX &&
                   vi < xbase + xcount && r >= 2 * nsub && r < 2 * nsub + TY;
     div0[q] = (j == 0) || (!cyc && vi == 0);
     grad0[q] = (j == d.NJ - 1) || (!cyc && vi == d.NI - 1);
-    const int o = present ? col + d.NI * j : 0;
+    const int o = present[q] ? col + d.NI * j : 0;
     off[q] = o;
-    if (present) {
-      mbits[q] = in.mask[o];
+    if (present[q]) {
+      const unsigned char mb = in.mask[o];
+      tm[q] = (mb & 1) ? 1.0 : 0.0;
       sf[q] = in.sf[o];
       datr[q] = in.datr[o]; dyu[q] = in.dyu[o]; dxu[q] = in.dxu[o];
       rho_g[q] = in.rho_g[o]; patm[q] = in.patm[o]; thk[q] = in.thicku[o];
       F1[q] = in.forcing[o]; F2[q] = in.forcing[o + P2]; fc[q] = in.f[o];
-      const double cori1 = hdtbt * fc[q];
-      cori2[q] = 1.0 / (1.0 + cori1 * cori1);
+      cori1[q] = 0.5 * dtbt * fc[q];
+      cori2[q] = 1.0 / (1.0 + cori1[q] * cori1[q]);
       dxur[q] = in.dxur[o]; dyur[q] = in.dyur[o];
+      um[q] = (mb & 2) ? 1.0 : 0.0;
       eta[q] = eta_src[o]; u1[q] = ud_src[o]; u2[q] = ud_src[o + P2];
+      if (interior[q]) { ebv[q] = eb[o]; ub1[q] = ub[o]; ub2[q] = ub[o + P2]; }
+      else { ebv[q] = ub1[q] = ub2[q] = 0.0; }
     } else {
-      mbits[q] = 0;
-      sf[q] = datr[q] = dyu[q] = dxu[q] = rho_g[q] = patm[q] = thk[q] = 0.0;
-      F1[q] = F2[q] = fc[q] = cori2[q] = dxur[q] = dyur[q] = 0.0;
-      eta[q] = u1[q] = u2[q] = 0.0;
+      tm[q] = sf[q] = datr[q] = dyu[q] = dxu[q] = rho_g[q] = patm[q] = thk[q] = 0.0;
+      F1[q] = F2[q] = fc[q] = cori1[q] = cori2[q] = dxur[q] = dyur[q] = um[q] = 0.0;
+      eta[q] = u1[q] = u2[q] = ebv[q] = ub1[q] = ub2[q] = 0.0;
     }
-    const bool in_ = interior[q];
-    s_eb[r][lane] = in_ ? eb[o] : 0.0;
-    s_ub1[r][lane] = in_ ? ub[o] : 0.0;
-    s_ub2[r][lane] = in_ ? ub[o + P2] : 0.0;
+    s_uh[r][lane] = u1[q] * dyu[q];
+    s_vh[r][lane] = u2[q] * dxu[q];
   }
-  s_uh[wave][lane] = u1[BPT - 1] * dyu[BPT - 1];
-  s_vh[wave][lane] = u2[BPT - 1] * dxu[BPT - 1];
   __syncthreads();
 
-  // DIV_UD of one row from its U*dyu, U*dxu (uh, vh) and those of the row
-  // below (uhm, vhm); every lane shifts (no divergent DPP)
-  auto div_row = [&](int q, double uh, double vh, double uhm, double vhm) {
-    const double uhw = lane_prev(uh), vhw = lane_prev(vh);
-    const double uhmw = lane_prev(uhm), vhmw = lane_prev(vhm);
-    const double uh_bay = 0.5 * (uh + uhm);
-    const double vh_bax = 0.5 * (vh + vhw);
-    const double uhim_bay = 0.5 * (uhw + uhmw);
-    const double vhjm_bax = 0.5 * (vhm + vhmw);
-    const double div = ((uh_bay - uhim_bay) + (vh_bax - vhjm_bax)) * datr[q];
-    return div0[q] ? 0.0 : div;
-  };
-  // U*dyu, U*dxu of row q-1 (the row below in j): formed from this
-  // wavefront's registers, read as the last row of the wavefront below from
-  // LDS, or for the region's first row the row itself (an edge value, never
-  // interior)
-  auto below = [&](int q, double &uhm, double &vhm) {
-    const int qm = q > 0 ? q - 1 : 0;
-    if (q == 0 && wave > 0) { uhm = s_uh[wave - 1][lane]; vhm = s_vh[wave - 1][lane]; }
-    else { uhm = u1[qm] * dyu[qm]; vhm = u2[qm] * dxu[qm]; }
-  };
-  auto tmask1 = [&](int q) { return (mbits[q] & 1) ? 1.0 : 0.0; };
-  auto umask1 = [&](int q) { return (mbits[q] & 2) ? 1.0 : 0.0; };
-
+  const int lm = max(lane - 1, 0), lp = min(lane + 1, BRX - 1);
   for (int t = 0; t < nsub; ++t) {
     const int itime = s0 + t;
-    double psv[BPT];
     // A: predictor and surface pressure
 #pragma unroll
     for (int q = 0; q < BPT; ++q) {
-      double uhm, vhm;
-      below(q, uhm, vhm);
-      const double div = div_row(q, u1[q] * dyu[q], u2[q] * dxu[q], uhm, vhm);
-      const double tmp = -div * tmask1(q);
+      const int r = rr[q], rm = max(r - 1, 0);
+      double div = 0.0;
+      if (!div0[q]) {
+        const double uh_bay = 0.5 * (s_uh[r][lane] + s_uh[rm][lane]);
+        const double vh_bax = 0.5 * (s_vh[r][lane] + s_vh[r][lm]);
+        const double uhim_bay = 0.5 * (s_uh[r][lm] + s_uh[rm][lm]);
+        const double vhjm_bax = 0.5 * (s_vh[rm][lane] + s_vh[rm][lm]);
+        div = ((uh_bay - uhim_bay) + (vh_bax - vhjm_bax)) * datr[q];
+      }
+      const double tmp = -div * tm[q];
       const double es = eta[q] + cp * (tmp + sf[q]);
-      psv[q] = rho_g[q] * es + patm[q];
+      s_ps[r][lane] = rho_g[q] * es + patm[q];
     }
-    s_ps[wave][lane] = psv[0];
     __syncthreads();
     // B: pressure gradient and momentum with implicit Coriolis
 #pragma unroll
     for (int q = 0; q < BPT; ++q) {
-      // surface pressure of row q+1 (the row above in j), as in below()
-      double pn;
-      if (q + 1 < BPT) pn = psv[q + 1];
-      else if (wave + 1 < BNW) pn = s_ps[wave + 1][lane];
-      else pn = psv[q];
-      const double p = psv[q];
-      const double pe = lane_next(p), pne = lane_next(pn);
-      const double p_fay = 0.5 * (p + pn);
-      const double p_fax = 0.5 * (p + pe);
-      const double pip_fay = 0.5 * (pe + pne);
-      const double pjp_fax = 0.5 * (pn + pne);
-      double g1 = (pip_fay - p_fay) * dxur[q] * umask1(q);
-      double g2 = (pjp_fax - p_fax) * dyur[q] * umask1(q);
-      if (grad0[q]) { g1 = 0.0; g2 = 0.0; }
+      const int r = rr[q], rp = min(r + 1, BRY - 1);
+      double g1 = 0.0, g2 = 0.0;
+      if (!grad0[q]) {
+        const double p = s_ps[r][lane], pe = s_ps[r][lp];
+        const double pn = s_ps[rp][lane], pne = s_ps[rp][lp];
+        const double p_fay = 0.5 * (p + pn);
+        const double p_fax = 0.5 * (p + pe);
+        const double pip_fay = 0.5 * (pe + pne);
+        const double pjp_fax = 0.5 * (pn + pne);
+        g1 = (pip_fay - p_fay) * dxur[q] * um[q];
+        g2 = (pjp_fax - p_fax) * dyur[q] * um[q];
+      }
       const double pf1 = -(thk[q] * g1);
       const double pf2 = -(thk[q] * g2);
       const double t1 = u1[q] + dtbt * (0.5 * fc[q] * u2[q] + pf1 + F1[q] + 0.0);
       const double t2 = u2[q] + dtbt * (-(0.5 * fc[q] * u1[q]) + pf2 + F2[q] + 0.0);
-      const double cori1 = hdtbt * fc[q];
-      u1[q] = cori2[q] * (t1 + cori1 * t2);
-      u2[q] = cori2[q] * (t2 - cori1 * t1);
+      u1[q] = cori2[q] * (t1 + cori1[q] * t2);
+      u2[q] = cori2[q] * (t2 - cori1[q] * t1);
+      // phase A's reads of s_uh/s_vh finished before the barrier above
+      s_uh[r][lane] = u1[q] * dyu[q];
+      s_vh[r][lane] = u2[q] * dxu[q];
     }
-    // phase A's reads of s_uh/s_vh finished before the barrier above
-    s_uh[wave][lane] = u1[BPT - 1] * dyu[BPT - 1];
-    s_vh[wave][lane] = u2[BPT - 1] * dxu[BPT - 1];
     __syncthreads();
     // C: corrector and time-average accumulation
     const double wgt = (double)(nts - itime + 1);
 #pragma unroll
     for (int q = 0; q < BPT; ++q) {
-      double uhm, vhm;
-      below(q, uhm, vhm);
-      const double div = div_row(q, u1[q] * dyu[q], u2[q] * dxu[q], uhm, vhm);
-      const double tmp = -div * tmask1(q);
+      const int r = rr[q], rm = max(r - 1, 0);
+      double div = 0.0;
+      if (!div0[q]) {
+        const double uh_bay = 0.5 * (s_uh[r][lane] + s_uh[rm][lane]);
+        const double vh_bax = 0.5 * (s_vh[r][lane] + s_vh[r][lm]);
+        const double uhim_bay = 0.5 * (s_uh[r][lm] + s_uh[rm][lm]);
+        const double vhjm_bax = 0.5 * (s_vh[rm][lane] + s_vh[rm][lm]);
+        div = ((uh_bay - uhim_bay) + (vh_bax - vhjm_bax)) * datr[q];
+      }
+      const double tmp = -div * tm[q];
       eta[q] = eta[q] + cc * (tmp + sf[q]);
-      const int r = BPT * wave + q;   // each thread owns its rows' accumulators
-      s_eb[r][lane] = s_eb[r][lane] + eta[q];
+      ebv[q] = ebv[q] + eta[q];
       if (step_a) {
-        s_ub1[r][lane] = s_ub1[r][lane] + u1[q];
-        s_ub2[r][lane] = s_ub2[r][lane] + u2[q];
+        ub1[q] = ub1[q] + u1[q];
+        ub2[q] = ub2[q] + u2[q];
       } else {
-        s_ub1[r][lane] = s_ub1[r][lane] + wgt * u1[q];
-        s_ub2[r][lane] = s_ub2[r][lane] + wgt * u2[q];
+        ub1[q] = ub1[q] + wgt * u1[q];
+        ub2[q] = ub2[q] + wgt * u2[q];
       }
     }
   }
 #pragma unroll
   for (int q = 0; q < BPT; ++q) {
     if (interior[q]) {
-      const int o = off[q], r = BPT * wave + q;
+      const int o = off[q];
       eta_dst[o] = eta[q];
       ud_dst[o] = u1[q];
       ud_dst[o + P2] = u2[q];
-      eb[o] = s_eb[r][lane];
-      ub[o] = s_ub1[r][lane];
-      ub[o + P2] = s_ub2[r][lane];
+      eb[o] = ebv[q];
+      ub[o] = ub1[q];
+      ub[o + P2] = ub2[q];
     }
   }
 }

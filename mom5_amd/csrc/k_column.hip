@@ -269,15 +269,21 @@ __device__ __forceinline__ void vdiff_col1(const VdJob<double> &J, int gi, int g
   }
 }
 
+// Register tails: 40 levels (about 210 VGPRs, two waves a SIMD) for nk > 40,
+// 20 for 20 < nk <= 40; shallower columns take k_vdiff.  At 3600x2700x75 the
+// 40-level tail takes one FP64 tracer from 9.9 to 8.4 ms (the plane traffic of
+// 35 levels instead of 55).
 constexpr int VD_TAIL = 20;
+constexpr int VD_TAIL_DEEP = 40;
 
+template <int TAIL>
 __global__ __launch_bounds__(64) void k_vdiff1(VdJob<double> J, int per_xcd) {
   const int b = blockIdx.x;
   const int tile = (b & 7) * per_xcd + (b >> 3);
   if (tile >= J.nblocks) return;
   const int gi = (tile % J.nbx) * 64 + threadIdx.x;
   if (gi >= J.d.ni) return;
-  vdiff_col1<VD_TAIL>(J, gi, tile / J.nbx);
+  vdiff_col1<TAIL>(J, gi, tile / J.nbx);
 }
 
 // 64-column tiles in XCD-aware order (see k_vdiff_twin): J.nbx tiles a row,
@@ -1243,8 +1249,11 @@ static int vdiff_groups(mom_gpu_ctx *ctx, int ntr, void *const *T, const int *dt
       VdJob<TT> J{d, g, diff_cbt + p3 * (nmix - 1), rho0, ctx->kmt, dh, dhw, dt, aidif,
                   ctx->tri_e, nbx, nbx * d.nj};
       if constexpr (sizeof(TT) == 8) {
-        if (g.n == 1 && d.nk > VD_TAIL)
-          hipLaunchKernelGGL(k_vdiff1, dim3(8 * per_xcd), dim3(64), 0, ctx->stream,
+        if (g.n == 1 && d.nk > VD_TAIL_DEEP)
+          hipLaunchKernelGGL(k_vdiff1<VD_TAIL_DEEP>, dim3(8 * per_xcd), dim3(64), 0,
+                             ctx->stream, (const VdJob<double> &)J, per_xcd);
+        else if (g.n == 1 && d.nk > VD_TAIL)
+          hipLaunchKernelGGL(k_vdiff1<VD_TAIL>, dim3(8 * per_xcd), dim3(64), 0, ctx->stream,
                              (const VdJob<double> &)J, per_xcd);
         else
           hipLaunchKernelGGL(k_vdiff<TT>, dim3(8 * per_xcd), dim3(64), 0, ctx->stream, J,
