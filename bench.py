@@ -42,6 +42,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# vector ALU issue peak: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 FP32 (not
+# packed) or FP64 instruction per SIMD every 4 cycles
+VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
 
 WORKLOADS = {
     "mixed": dict(name="mixed_3600x2700x75", ni=3600, nj=2700, nk=75, land_frac=0.30,
@@ -315,6 +318,25 @@ def measured_traffic(workload, key):
     return k["traffic_bytes"], os.path.relpath(path, ROOT)
 
 
+def measured_valu(workload, key, kernel_ms):
+    """The vector-ALU issue fraction of kernel `key`: its SQ_INSTS_VALU per
+    dispatch (wave instructions, whole device, from the rocprofv3 --pmc run
+    committed under profiles/) over what the device issues at peak in
+    kernel_ms; None if not measured."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_mixed_3600x2700x75.json")
+    if workload is None or not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        rec = json.load(fh)
+    k = rec.get("kernels", {}).get(key)
+    if rec.get("workload") != workload or k is None or "sq" not in k:
+        return None
+    insts = k["sq"]["insts_valu"]
+    return dict(insts_per_launch=insts, peak_wave_insts_per_s=VALU_PEAK_WAVE_INSTS,
+                frac=round(insts / (kernel_ms / 1e3) / VALU_PEAK_WAVE_INSTS, 4),
+                source=os.path.relpath(path, ROOT))
+
+
 def run_mixed(args, w):
     import dataclasses
     import gc
@@ -445,12 +467,22 @@ def run_mixed(args, w):
         del src, dst
     dom = max(cand, key=lambda c: c["kernel_ms"] * c["launches_per_step"])
     achieved = dom["bytes_per_launch"] / (dom["kernel_ms"] / 1e3) / 1e9
-    traffic, tsrc = measured_traffic(w["name"] if not args.shape else None, dom.pop("key"))
+    dkey = dom.pop("key")
+    wname = w["name"] if not args.shape else None
+    traffic, tsrc = measured_traffic(wname, dkey)
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, **{
                     k: (round(v, 4) if isinstance(v, float) else v) for k, v in dom.items()})
     if tsrc:
         roof["traffic_source"] = tsrc
+    # what limits the kernel: the larger of its HBM fraction (real traffic over
+    # the kernel time) and its vector-ALU issue fraction
+    valu = measured_valu(wname, dkey, dom["kernel_ms"])
+    if valu:
+        roof["valu_issue"] = valu
+        hbm_real = (traffic / (dom["kernel_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS) if traffic else 0.0
+        roof["hbm_traffic_frac"] = round(hbm_real, 4)
+        roof["limiter"] = "valu issue" if valu["frac"] > hbm_real else "hbm"
     step_bytes = mixed_step_bytes(ni, nj, nk, n64, n32)
     step_frac = step_bytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS
 
