@@ -957,6 +957,229 @@ void k_vdiff_twin_ens(const VdJobF *__restrict__ jobs, VdLaunch L, int per_xcd) 
   twin_tile<H>(jobs[lo], t - L.first[lo]);
 }
 
+// ---- FP64 tracers, every level of a column in registers ---------------------
+// The system of invtri (ocean_util.F90:175-236) for FP64 tracers, solved by two
+// wavefronts per 64 columns.  The first eliminates levels 1..nt (nt =
+// ceil(nk/2)) downward with the reference's recurrence
+//   bet_k = mask_k/(b_k - a_k e_k + eps),  e_k = c_{k-1} bet_{k-1},
+//   z'_k = (f_k - a_k z'_{k-1}) bet_k,
+// the second levels nk..nt+1 upward, the same recurrence with the sub- and
+// super-diagonals swapped (p = a, q = c on top; p = c, q = a below):
+//   bet_k = mask_k/(b_k - p_k e_k + eps),  e_k = q_{prev} bet_{prev},
+//   v_k = (f_k - p_k v_{prev}) bet_k.
+// Each wavefront's last level then satisfies v = x + (q bet) x_other; the two
+// rows at the meeting are solved with the other wavefront's (v, q bet) through
+// LDS, and each wavefront back-substitutes x = v - (q bet) x_next over its own
+// levels.  a, b, c, f and every division are invtri's expressions; only the
+// order of elimination differs from its Thomas sweep (the matrix is diagonally
+// dominant: the two agree to rounding).  v and q bet of every level stay in
+// registers, so each tracer is read and written once and there is no
+// factorisation plane.  The levels of a wavefront fill slots ph..H-1 (ph = H -
+// its level count), so both meet in slot H-1; slots below ph are inert.  A
+// group's tracers share the coefficients' addresses, not their factorisation:
+// each tracer repeats it (the coefficients come from L2 the second time).
+template <int H>
+__device__ __forceinline__ void twin64_tile(const VdJob<double> &J, int tile) {
+  __shared__ double xv[2][2][64], xq[2][2][64];   // [tracer parity][half][column]
+  const KDom d = J.d;
+  const int half = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const bool top = half == 0;
+  const int lane = threadIdx.x & 63;
+  const int col = (tile % J.nbx) * 64 + lane;
+  const bool valid = col < d.ni;
+  const int nk = d.nk;
+  const int nt = (nk + 1) >> 1;
+  const int nact = top ? nt : nk - nt;
+  const int ph = H - nact;
+  const long P2 = d.P2;
+  const long step = top ? P2 : -P2;            // plane stride from slot to slot
+  const long p0 = top ? 0 : (long)(nk - 1) * P2;   // plane of slot ph
+  const unsigned o = (unsigned)d2(d, valid ? col : d.ni - 1, tile / J.nbx);
+  const unsigned ob8 = o * 8u;                 // lane byte offset in a plane
+  const int kmo = ld_at(J.kmt, o * 4u);
+  const int kb = max(2, kmo);
+  const double tda = J.tdt * J.aidif;
+  const double rho0 = J.rho0;
+  const double *k33 = J.g.k33;
+  // dcb = rho0*diff_cbt(nmix) + K33 of the plane at offset pl (vdiff_col's
+  // expression)
+  auto dcb_at = [&](long pl) {
+    return rho0 * ld_at(J.dcbase + pl, ob8) + (k33 ? ld_at(k33 + pl, ob8) : 0.0);
+  };
+  for (int n = 0; n < J.g.n; ++n) {
+    double *z = J.g.z[n];
+    const double stf = ld_at(J.g.stf[n], ob8), btf = ld_at(J.g.btf[n], ob8);
+    double vs[H], qs[H];
+    // the active levels, slot ph's plane first; one running plane offset.  The
+    // offset and ph are opaque to the compiler at every slot, so that the
+    // slots' offsets and their active/inert tests are formed one at a time
+    // instead of all being kept (as scalar pairs: they would spill)
+    long pl = p0;
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+      vs[j] = 0.0;
+      int phj = ph;
+      asm volatile("" : "+s"(phj));
+      if (j < phj) continue;           // inert slot (wavefront-uniform)
+      asm volatile("" : "+s"(pl));
+      vs[j] = ld_at(z + pl, ob8);
+      pl += step;
+    }
+    // The coefficients of a chunk of CH slots are loaded while the previous
+    // chunk is computed (its divisions wait for them otherwise): per slot the
+    // level's dh, the dzwt interface and the diff_cbt plane that the slot
+    // adds (top: the interface below and the level's plane; bottom: the
+    // interface above and the plane above), with the first slot's other
+    // interface and plane loaded before the loop.  Inert slots load slot ph's.
+    constexpr int CH = 4, NCH = (H + CH - 1) / CH;
+    const long dwo = top ? P2 : 0, dco = top ? 0 : -P2;
+    double Ldh[NCH * CH], Ldw[NCH * CH], Ldc[NCH * CH];
+    auto cload = [&](int c) {
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        const int j = c * CH + u;
+        if (j >= H) break;
+        int phj = ph;
+        asm volatile("" : "+s"(phj));
+        long plj = p0 + (long)max(j - phj, 0) * step;
+        asm volatile("" : "+s"(plj));
+        Ldh[j] = ld_at(J.dh + plj, ob8);
+        Ldw[j] = ld_at(J.dhw + plj + dwo, ob8);
+        Ldc[j] = ld_at(J.dcbase + plj + dco, ob8);
+      }
+    };
+    // the first slot's other interface and plane (top: dzwt(0) and none; the
+    // bottom: dzwt(nk) and diff_cbt(nk))
+    const double dhw_first = ld_at(J.dhw + p0 + (top ? 0 : P2), ob8);
+    const double dcb_first = top ? 0.0 : dcb_at(p0);
+    double bet = 0.0, qprev = 0.0, vprev = 0.0;
+    double dcb_c = 0.0, dhw_c = 0.0;   // the interface shared with the next slot
+    pl = p0;
+    cload(0);
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+      if (c + 1 < NCH) cload(c + 1);
+      asm volatile("" ::: "memory");
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        const int j = c * CH + u;
+        if (j >= H) break;
+        qs[j] = 0.0;
+        int phj = ph;
+        asm volatile("" : "+s"(phj));
+        if (j < phj) continue;
+        asm volatile("" : "+s"(pl));
+        const int r = j - phj;
+        const int k = top ? r + 1 : nk - r;     // pl = P2*(k - 1)
+        const double dhk = Ldh[j];
+        // the slot's diff_cbt plane: rho0*diff_cbt (+ K33, read here when a
+        // tracer has its own)
+        const double dcb_new = J.rho0 * Ldc[j] + (k33 ? ld_at(k33 + pl + dco, ob8) : 0.0);
+        // dzwt plane k-1 is the interface above level k, plane k the one below
+        double dcb_km1, dcb_k, dhw_km1, dhw_k;
+        if (top) {
+          dcb_k = dcb_new;
+          dhw_k = Ldw[j];
+          dcb_km1 = (r == 0) ? dcb_k : dcb_c;
+          dhw_km1 = (r == 0) ? dhw_first : dhw_c;
+          dcb_c = dcb_k;
+          dhw_c = dhw_k;
+        } else {
+          dcb_k = (r == 0) ? dcb_first : dcb_c;
+          dhw_k = (r == 0) ? dhw_first : dhw_c;
+          dcb_km1 = dcb_new;
+          dhw_km1 = Ldw[j];
+          dcb_c = dcb_km1;
+          dhw_c = dhw_km1;
+        }
+        pl += step;
+        const int kp1 = min(k + 1, nk);
+        const double factu = tda / (dhw_km1 * dhk);
+        const double factl = tda / (dhw_k * dhk);
+        const double mkk = (k <= kmo) ? 1.0 : 0.0;
+        const double mkp1 = (kp1 <= kmo) ? 1.0 : 0.0;
+        double a = -dcb_km1 * factu * mkk;
+        double c2 = -dcb_k * factl * mkp1;
+        if (k == 1) a = 0.0;
+        if (k == nk) c2 = 0.0;
+        const double b = 1.0 - a - c2;
+        const double zo = vs[j];
+        double f;
+        if (k == 1)
+          f = zo + stf * tda * mkk / dhk;
+        else if (k == kb)
+          f = zo - btf * tda * mkk / dhk;
+        else
+          f = zo * mkk;
+        const double p = top ? a : c2, q = top ? c2 : a;
+        double v;
+        if (r == 0) {
+          bet = mkk / (b + EPS_TRI);
+          v = f * bet;
+        } else {
+          const double e = qprev * bet;
+          bet = mkk / (b - p * e + EPS_TRI);
+          v = (f - p * vprev) * bet;
+        }
+        vs[j] = v;
+        qs[j] = q * bet;
+        qprev = q;
+        vprev = v;
+        // one slot at a time: later slots' divisions are not hoisted
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    // the meeting: x_t + Qt x_b = Vt (top's last level), x_b + Qb x_t = Vb
+    xv[n & 1][half][lane] = vs[H - 1];
+    xq[n & 1][half][lane] = qs[H - 1];
+    lds_barrier();
+    const double Vo = xv[n & 1][1 - half][lane], Qo = xq[n & 1][1 - half][lane];
+    const double Vt = top ? vs[H - 1] : Vo, Qt = top ? qs[H - 1] : Qo;
+    const double Vb = top ? Vo : vs[H - 1], Qb = top ? Qo : qs[H - 1];
+    const double xt = (Vt - Qt * Vb) / (1.0 - Qt * Qb);
+    double x = top ? xt : Vb - Qb * xt;
+    // back substitution from slot H-1 (plane p0 + (nact-1)*step) outward
+    pl = p0 + (long)(nact - 1) * step;
+#pragma unroll
+    for (int j = H - 1; j >= 0; --j) {
+      int phj = ph;
+      asm volatile("" : "+s"(phj));
+      if (j < phj) continue;
+      if (j < H - 1) x = vs[j] - qs[j] * x;
+      asm volatile("" : "+s"(pl));
+      if (valid) st_at(z + pl, ob8, x);
+      pl -= step;
+    }
+  }
+}
+
+template <int H>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
+void k_vdiff_twin64(VdJob<double> J, int per_xcd) {
+  const int b = blockIdx.x;
+  const int tile = (b & 7) * per_xcd + (b >> 3);
+  if (tile >= J.nblocks) return;             // the whole block
+  twin64_tile<H>(J, tile);
+}
+
+// Ensemble: every FP64 group of every domain whose nk the register solver
+// takes, in one launch; tiles in XCD-aware order over the whole table.
+template <int H>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
+void k_vdiff_twin64_ens(const VdJob<double> *__restrict__ jobs, VdLaunch L, int per_xcd) {
+  const int b = blockIdx.x;
+  const int t = (b & 7) * per_xcd + (b >> 3);
+  if (t >= L.first[L.njobs]) return;         // the whole block
+  int lo = 0, hi = L.njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (L.first[mid] <= t) lo = mid; else hi = mid - 1;
+  }
+  lo = __builtin_amdgcn_readfirstlane(lo);
+  twin64_tile<H>(jobs[lo], t - L.first[lo]);
+}
+
 __global__ void k_set_ints(int *__restrict__ p, int v, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -1230,6 +1453,12 @@ int launch_invtri(mom_gpu_ctx *ctx, double *z, const double *topbc, const double
   return MOM_GPU_OK;
 }
 
+static int twin_slots(int nk);
+static void launch_twin64(mom_gpu_ctx *ctx, VdJob<double> J);
+
+// FP64 tracers take the register-resident solver (k_vdiff_twin64) whenever nk
+// is in its range (2..80), whatever the knobs, so that FP64 results never
+// depend on them; the Thomas kernels (k_vdiff1, k_vdiff) serve other nk.
 template <typename TT>
 static int vdiff_groups(mom_gpu_ctx *ctx, int ntr, void *const *T, const int *dtype, int want,
                         const double *const *stf, const double *const *btf,
@@ -1249,7 +1478,9 @@ static int vdiff_groups(mom_gpu_ctx *ctx, int ntr, void *const *T, const int *dt
       VdJob<TT> J{d, g, diff_cbt + p3 * (nmix - 1), rho0, ctx->kmt, dh, dhw, dt, aidif,
                   ctx->tri_e, nbx, nbx * d.nj};
       if constexpr (sizeof(TT) == 8) {
-        if (g.n == 1 && d.nk > VD_TAIL_DEEP)
+        if (twin_slots(d.nk) > 0)
+          launch_twin64(ctx, (const VdJob<double> &)J);
+        else if (g.n == 1 && d.nk > VD_TAIL_DEEP)
           hipLaunchKernelGGL(k_vdiff1<VD_TAIL_DEEP>, dim3(8 * per_xcd), dim3(64), 0,
                              ctx->stream, (const VdJob<double> &)J, per_xcd);
         else if (g.n == 1 && d.nk > VD_TAIL)
@@ -1401,6 +1632,19 @@ static int twin_slots(int nk) {
   return h <= 8 ? 8 : h <= 16 ? 16 : h <= 24 ? 24 : h <= 32 ? 32 : 40;
 }
 
+static void launch_twin64(mom_gpu_ctx *ctx, VdJob<double> J) {
+  const KDom &d = J.d;
+  const int per_xcd = (J.nblocks + 7) / 8;
+  const dim3 grid(8 * per_xcd), block(128);
+  switch (twin_slots(d.nk)) {
+    case 8: hipLaunchKernelGGL(k_vdiff_twin64<8>, grid, block, 0, ctx->stream, J, per_xcd); break;
+    case 16: hipLaunchKernelGGL(k_vdiff_twin64<16>, grid, block, 0, ctx->stream, J, per_xcd); break;
+    case 24: hipLaunchKernelGGL(k_vdiff_twin64<24>, grid, block, 0, ctx->stream, J, per_xcd); break;
+    case 32: hipLaunchKernelGGL(k_vdiff_twin64<32>, grid, block, 0, ctx->stream, J, per_xcd); break;
+    default: hipLaunchKernelGGL(k_vdiff_twin64<40>, grid, block, 0, ctx->stream, J, per_xcd); break;
+  }
+}
+
 static void launch_twin(mom_gpu_ctx *ctx, VdJobF J) {
   const KDom &d = J.d;
   J.nbx = (d.ni + 63) / 64;
@@ -1425,7 +1669,7 @@ int launch_vdiff_mixed(mom_gpu_ctx *ctx, int ntr, void *const *T, const int *dty
                        const double *diff_cbt, int index_salt, double rho0, const double *dh,
                        const double *dhw, double dt, double aidif) {
   const bool twin = ctx->tune.vdiff_solver == 1 && twin_slots(ctx->kd.nk) > 0;
-  if (ctx->tune.vdiff_merge && !twin) {
+  if (ctx->tune.vdiff_merge && !twin && twin_slots(ctx->kd.nk) == 0) {
     bool done = false;
     const int rc0 = vdiff_mix_one(ctx, ntr, T, dtype, stf, btf, diff_cbt, index_salt, rho0, dh,
                                   dhw, dt, aidif, done);
@@ -1676,6 +1920,45 @@ int launch_vdiff_ensemble(mom_gpu_ensemble *e, const mom_gpu_tracer_step_t *step
       }
       flush64();
       flush32();
+    }
+  }
+  // FP64: the register solver for every domain whose nk it takes (all their
+  // groups in one launch, nothing shared), rounds of the Thomas kernel for
+  // the others -- each domain as it would be solved alone
+  {
+    std::vector<VdJob<double>> all64;
+    int h64 = 0;
+    for (size_t dd = 0; dd < nd; ++dd) {
+      const int h = twin_slots(e->ctx[dd]->kd.nk);
+      if (h == 0) continue;
+      h64 = std::max(h64, h);
+      all64.insert(all64.end(), g64[dd].begin(), g64[dd].end());
+      g64[dd].clear();
+    }
+    for (size_t at = 0; at < all64.size(); at += VD_MAXJ) {
+      const size_t nj = std::min((size_t)VD_MAXJ, all64.size() - at);
+      VdLaunch L{};
+      int total = 0;
+      for (size_t j = 0; j < nj; ++j) {
+        L.first[j] = total;
+        total += all64[at + j].nblocks;
+      }
+      L.first[nj] = total;
+      L.njobs = (int)nj;
+      const void *dev = ens_table(e, all64.data() + at, sizeof(VdJob<double>) * nj);
+      if (!dev) return MOM_GPU_ERR_HIP;
+      const int per_xcd = (total + 7) / 8;
+      const dim3 grid(8 * per_xcd), block(128);
+      hipStream_t st = e->lead->stream;
+      const VdJob<double> *jt = (const VdJob<double> *)dev;
+      switch (h64) {
+        case 8: hipLaunchKernelGGL(k_vdiff_twin64_ens<8>, grid, block, 0, st, jt, L, per_xcd); break;
+        case 16: hipLaunchKernelGGL(k_vdiff_twin64_ens<16>, grid, block, 0, st, jt, L, per_xcd); break;
+        case 24: hipLaunchKernelGGL(k_vdiff_twin64_ens<24>, grid, block, 0, st, jt, L, per_xcd); break;
+        case 32: hipLaunchKernelGGL(k_vdiff_twin64_ens<32>, grid, block, 0, st, jt, L, per_xcd); break;
+        default: hipLaunchKernelGGL(k_vdiff_twin64_ens<40>, grid, block, 0, st, jt, L, per_xcd); break;
+      }
+      MOM_LAUNCHED(e->lead, "k_vdiff_twin64_ens");
     }
   }
   int rc = vd_rounds(e, g64, k_vdiff_ens, "k_vdiff_ens");

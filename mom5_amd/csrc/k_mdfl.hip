@@ -65,7 +65,9 @@ __device__ inline double face_flux(double Rjm, double Rj, double Rjp,
                 (massflux - fabs(massflux)) * (Tneg - psiM * Rj)) * ma * mb;
 }
 
-template <int FLAVOR, int RPW>
+// LIM1: sweby_limiter = 1 (the limited flux of mdfl64.h); otherwise the
+// quotient form with the limiter weight lim
+template <int FLAVOR, int RPW, bool LIM1>
 __global__ __launch_bounds__(RX * NW) void k_mdfl(
     KDom d, const double *__restrict__ T, const double *__restrict__ U,
     const double *__restrict__ V, const double *__restrict__ W,
@@ -97,7 +99,7 @@ __global__ __launch_bounds__(RX * NW) void k_mdfl(
   const int wave = threadIdx.x / RX;
   const int nk = d.nk, P2 = d.P2;
   const int gi = gi0 - 2 + lane;
-  const bool limited = FLAVOR == 1 || lim == 1.0;   // the Sweby limiter in full
+  constexpr bool limited = LIM1;
   const double rdt = 1.0 / dtime;
 
   int si = gi;
@@ -300,7 +302,7 @@ __global__ __launch_bounds__(RX * NW) void k_mdfl(
 
 }  // namespace
 
-template <int FLAVOR, int RPW>
+template <int FLAVOR, int RPW, bool LIM1>
 static void launch_mdfl_t(mom_gpu_ctx *ctx, const double *T, const double *u, const double *v,
                           const double *w, const double *rho, double dtime, double lim,
                           double *th, double *wrk1, double *fx, double *fy, double *fz) {
@@ -310,7 +312,8 @@ static void launch_mdfl_t(mom_gpu_ctx *ctx, const double *T, const double *u, co
   const int nty = (d.nj + TY - 1) / TY;
   const int ntiles = ntx * nty;
   const int per = (ntiles + 7) / 8;
-  hipLaunchKernelGGL((k_mdfl<FLAVOR, RPW>), dim3(8 * per), dim3(RX * NW), 0, ctx->stream, d, T,
+  hipLaunchKernelGGL((k_mdfl<FLAVOR, RPW, LIM1>), dim3(8 * per), dim3(RX * NW), 0, ctx->stream,
+                     d, T,
                      u, v, w, rho, ctx->kmt, ctx->dat, ctx->datr, ctx->dxte, ctx->dyte,
                      ctx->dxtn, ctx->dytn, dtime, lim, th, wrk1, fx, fy, fz, ntx, ntiles, per);
 }
@@ -320,14 +323,19 @@ int launch_mdfl(mom_gpu_ctx *ctx, int flavor, const double *T,
                 const double *rho, double dtime, double lim, double *th,
                 double *wrk1, double *fx, double *fy, double *fz) {
   const int rpw = ctx->tune.mdfl_rows_per_wave;
-  if (flavor == 0 && rpw == 2)
-    launch_mdfl_t<0, 2>(ctx, T, u, v, w, rho, dtime, lim, th, wrk1, fx, fy, fz);
+  const bool lim1 = lim == 1.0;
+  if (flavor == 0 && lim1 && rpw == 2)
+    launch_mdfl_t<0, 2, true>(ctx, T, u, v, w, rho, dtime, lim, th, wrk1, fx, fy, fz);
+  else if (flavor == 0 && lim1)
+    launch_mdfl_t<0, 4, true>(ctx, T, u, v, w, rho, dtime, lim, th, wrk1, fx, fy, fz);
+  else if (flavor == 0 && rpw == 2)
+    launch_mdfl_t<0, 2, false>(ctx, T, u, v, w, rho, dtime, lim, th, wrk1, fx, fy, fz);
   else if (flavor == 0)
-    launch_mdfl_t<0, 4>(ctx, T, u, v, w, rho, dtime, lim, th, wrk1, fx, fy, fz);
+    launch_mdfl_t<0, 4, false>(ctx, T, u, v, w, rho, dtime, lim, th, wrk1, fx, fy, fz);
   else if (rpw == 2)
-    launch_mdfl_t<1, 2>(ctx, T, u, v, w, rho, dtime, lim, th, wrk1, fx, fy, fz);
+    launch_mdfl_t<1, 2, true>(ctx, T, u, v, w, rho, dtime, lim, th, wrk1, fx, fy, fz);
   else
-    launch_mdfl_t<1, 4>(ctx, T, u, v, w, rho, dtime, lim, th, wrk1, fx, fy, fz);
+    launch_mdfl_t<1, 4, true>(ctx, T, u, v, w, rho, dtime, lim, th, wrk1, fx, fy, fz);
   MOM_LAUNCHED(ctx, "k_mdfl");
   return MOM_GPU_OK;
 }

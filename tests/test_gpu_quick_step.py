@@ -6,9 +6,9 @@ implicit vertical diffusion, against
 * the oracle's per-routine sequence (compute_tmask_limit,
   horz_advect_tracer_quicker, vert_advect_tracer_quicker, the explicit
   update, vert_diffuse_implicit; ocean_tracer.F90:2295-2350, 2432,
-  2788-2892; ocean_tracer_advect.F90:2538-2653, 2981-3031), Quicker tracers
-  bit for bit, MDFL-Sweby ones (division-free limited fluxes) within 1e-12
-  per cell;
+  2788-2892; ocean_tracer_advect.F90:2538-2653, 2981-3031), within 1e-12
+  per cell (the FP64 column solver eliminates from both ends; MDFL-Sweby's
+  limited fluxes are division-free);
 * the plain per-tracer GPU path over two steps (which updates the tracer
   halos between steps with mom_gpu_update_domains), bit for bit.
 
@@ -89,11 +89,10 @@ def test_quicker_batched_equals_oracle(oracle_lib, kind, schemes, lwu):
                              mins=mins, maxs=maxs, index_temp=0, ts_same=True,
                              limit_with_upwind=lwu)
     for n in range(4):
-        a, b = got[n][:, 1:-1, 1:-1], ref[n][:, 1:-1, 1:-1]
-        if schemes[n] == "quicker":
-            assert np.array_equal(a, b), (n, schemes[n])
-        else:       # division-free limited fluxes: 1e-12 per cell
-            assert_close_cells(a, b, what=f"tracer {n} ({schemes[n]})")
+        # the FP64 column solver's two-ended elimination (and MDFL-Sweby's
+        # division-free limited fluxes): 1e-12 per cell
+        assert_close_cells(got[n][:, 1:-1, 1:-1], ref[n][:, 1:-1, 1:-1],
+                           what=f"tracer {n} ({schemes[n]})")
 
 
 @pytest.mark.parametrize("kind", ["cyclic", "closed"])
