@@ -99,6 +99,11 @@ struct VdJob {
   double tdt, aidif;
   double *ework;
   int nbx, nblocks;
+  // k_vdiff_twin64 only: tracers n >= n2 of the group take their diffusivity
+  // from dcbase2 (a job holding both mixing classes; other kernels: n2 = 0
+  // and dcbase2 unused)
+  const double *dcbase2;
+  int n2;
 };
 
 template <typename TT>
@@ -1001,12 +1006,14 @@ __device__ __forceinline__ void twin64_tile(const VdJob<double> &J, int tile) {
   const double tda = J.tdt * J.aidif;
   const double rho0 = J.rho0;
   const double *k33 = J.g.k33;
-  // dcb = rho0*diff_cbt(nmix) + K33 of the plane at offset pl (vdiff_col's
-  // expression)
-  auto dcb_at = [&](long pl) {
-    return rho0 * ld_at(J.dcbase + pl, ob8) + (k33 ? ld_at(k33 + pl, ob8) : 0.0);
-  };
   for (int n = 0; n < J.g.n; ++n) {
+    // the tracer's diffusivity plane: its mixing class's diff_cbt
+    const double *dcbase = (J.dcbase2 && n >= J.n2) ? J.dcbase2 : J.dcbase;
+    // dcb = rho0*diff_cbt(nmix) + K33 of the plane at offset pl (vdiff_col's
+    // expression)
+    auto dcb_at = [&](long pl) {
+      return rho0 * ld_at(dcbase + pl, ob8) + (k33 ? ld_at(k33 + pl, ob8) : 0.0);
+    };
     double *z = J.g.z[n];
     const double stf = ld_at(J.g.stf[n], ob8), btf = ld_at(J.g.btf[n], ob8);
     double vs[H], qs[H];
@@ -1045,7 +1052,7 @@ __device__ __forceinline__ void twin64_tile(const VdJob<double> &J, int tile) {
         asm volatile("" : "+s"(plj));
         Ldh[j] = ld_at(J.dh + plj, ob8);
         Ldw[j] = ld_at(J.dhw + plj + dwo, ob8);
-        Ldc[j] = ld_at(J.dcbase + plj + dco, ob8);
+        Ldc[j] = ld_at(dcbase + plj + dco, ob8);
       }
     };
     // the first slot's other interface and plane (top: dzwt(0) and none; the
@@ -1459,6 +1466,58 @@ static void launch_twin64(mom_gpu_ctx *ctx, VdJob<double> J);
 // FP64 tracers take the register-resident solver (k_vdiff_twin64) whenever nk
 // is in its range (2..80), whatever the knobs, so that FP64 results never
 // depend on them; the Thomas kernels (k_vdiff1, k_vdiff) serve other nk.
+// Jobs of up to NT tracers hold both mixing classes (class 1 first, n2 the
+// first of class 2), so that T and S share one launch and the tile's rho_dzt
+// and dzwt come from L2 for the second; a tracer with its own K33 goes alone.
+static int twin64_groups(mom_gpu_ctx *ctx, int ntr, void *const *T, const int *dtype, int want,
+                         const double *const *stf, const double *const *btf,
+                         const double *diff_cbt, const double *const *K33, int index_salt,
+                         double rho0, const double *dh, const double *dhw, double dt,
+                         double aidif) {
+  const KDom &d = ctx->kd;
+  const size_t p3 = (size_t)d.P2 * d.nk;
+  const int nbx = (d.ni + 63) / 64;
+  auto job = [&]() {
+    VdJob<double> J{};
+    J.d = d; J.dcbase = diff_cbt; J.dcbase2 = diff_cbt + p3; J.rho0 = rho0; J.kmt = ctx->kmt;
+    J.dh = dh; J.dhw = dhw; J.tdt = dt; J.aidif = aidif; J.ework = ctx->tri_e;
+    J.nbx = nbx; J.nblocks = nbx * d.nj;
+    return J;
+  };
+  std::vector<int> c1, c2;
+  for (int n = 0; n < ntr; ++n) {
+    if ((dtype ? dtype[n] : 8) != want) continue;
+    const int nm = (n == index_salt) ? 2 : 1;
+    const double *k33 = (K33 && K33[n]) ? K33[n] : nullptr;
+    if (k33) {
+      VdJob<double> J = job();
+      J.g.z[0] = (double *)T[n]; J.g.stf[0] = stf[n]; J.g.btf[0] = btf[n]; J.g.n = 1;
+      J.g.k33 = k33;
+      J.n2 = (nm == 2) ? 0 : 1;
+      launch_twin64(ctx, J);
+      MOM_LAUNCHED(ctx, "k_vdiff_twin64");
+      continue;
+    }
+    (nm == 1 ? c1 : c2).push_back(n);
+  }
+  const int n1 = (int)c1.size();
+  std::vector<int> all(c1);
+  all.insert(all.end(), c2.begin(), c2.end());
+  for (size_t at = 0; at < all.size(); at += NT) {
+    const int cnt = (int)std::min((size_t)NT, all.size() - at);
+    VdJob<double> J = job();
+    for (int q = 0; q < cnt; ++q) {
+      const int n = all[at + q];
+      J.g.z[q] = (double *)T[n]; J.g.stf[q] = stf[n]; J.g.btf[q] = btf[n];
+    }
+    J.g.n = cnt;
+    J.n2 = std::max(0, std::min(cnt, n1 - (int)at));
+    launch_twin64(ctx, J);
+    MOM_LAUNCHED(ctx, "k_vdiff_twin64");
+  }
+  return MOM_GPU_OK;
+}
+
 template <typename TT>
 static int vdiff_groups(mom_gpu_ctx *ctx, int ntr, void *const *T, const int *dtype, int want,
                         const double *const *stf, const double *const *btf,
@@ -1466,6 +1525,11 @@ static int vdiff_groups(mom_gpu_ctx *ctx, int ntr, void *const *T, const int *dt
                         double rho0, const double *dh, const double *dhw, double dt,
                         double aidif) {
   const KDom &d = ctx->kd;
+  if constexpr (sizeof(TT) == 8) {
+    if (twin_slots(d.nk) > 0)
+      return twin64_groups(ctx, ntr, T, dtype, want, stf, btf, diff_cbt, K33, index_salt, rho0,
+                           dh, dhw, dt, aidif);
+  }
   const size_t p3 = (size_t)d.P2 * d.nk;
   // tracers with their own K33 go alone; the others are grouped by nmix
   for (int nmix = 1; nmix <= 2; ++nmix) {
@@ -1478,9 +1542,7 @@ static int vdiff_groups(mom_gpu_ctx *ctx, int ntr, void *const *T, const int *dt
       VdJob<TT> J{d, g, diff_cbt + p3 * (nmix - 1), rho0, ctx->kmt, dh, dhw, dt, aidif,
                   ctx->tri_e, nbx, nbx * d.nj};
       if constexpr (sizeof(TT) == 8) {
-        if (twin_slots(d.nk) > 0)
-          launch_twin64(ctx, (const VdJob<double> &)J);
-        else if (g.n == 1 && d.nk > VD_TAIL_DEEP)
+        if (g.n == 1 && d.nk > VD_TAIL_DEEP)
           hipLaunchKernelGGL(k_vdiff1<VD_TAIL_DEEP>, dim3(8 * per_xcd), dim3(64), 0,
                              ctx->stream, (const VdJob<double> &)J, per_xcd);
         else if (g.n == 1 && d.nk > VD_TAIL)
