@@ -114,7 +114,7 @@ int mom_gpu_copy(mom_gpu_ctx_t *ctx, double *dst, const double *src, size_t n);
  * drives these; results do not depend on them, bit for bit, except FP32
  * results on vdiff_solver).  Knobs:
  *   "barotropic_subcycles_per_launch"  0: three launches per subcycle;
- *                                      n>0: n subcycles per launch (1..9)
+ *                                      n>0: n subcycles per launch (1..12)
  *   "mdfl_rows_per_wave"               2 or 4: tile height of the MDFL kernel
  *   "step_group_f64"                   1, 2 or 4 FP64 tracers per batched pass
  *                                      (4 with 1 row x 16 or 2 x 8 wavefronts)

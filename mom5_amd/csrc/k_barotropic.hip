@@ -183,10 +183,14 @@ __global__ void k_bt_conv(KDom d, const double *__restrict__ ud, const double *_
 // One work-group owns a BRY x BRX region of the data domain and advances it
 // nsub subcycles with the state (eta, U) and the accumulators in registers
 // and only the exchanged quantities (U*dyu, U*dxu, surface pressure) in
-// LDS.  Each subcycle invalidates two rows/columns on the low side and one on
-// the high side of the region, so after nsub subcycles the interior
-// (BRY-3*nsub) x (BRX-3*nsub) is exact; the rest is recomputed by the
-// neighbouring groups.  On a cyclic-x domain the region addresses the
+// LDS.  Per subcycle the velocity's exact region shrinks by one point on
+// each side (its pressure gradient reaches one point up, the divergence in
+// that pressure one point down) and the height, formed from the divergence
+// of the new velocity, lags one more point on the low side: after nsub
+// subcycles the points from nsub + 1 above the region's low edges to nsub
+// below its high edges are exact, an interior of (BRY-2*nsub-1) x
+// (BRX-2*nsub-1); the rest is recomputed by the neighbouring groups.  On a
+// cyclic-x domain the region addresses the
 // periodic images of the compute columns: with the inputs' halo columns
 // holding the wrap (what mpp_update_domains leaves), this reproduces the
 // reference's evolution of every point, the halo columns of the compute
@@ -195,6 +199,11 @@ constexpr int BRX = 64;          // region columns (one wavefront per row)
 constexpr int BRY = 32;          // region rows
 constexpr int BPT = 2;           // region rows per wavefront
 constexpr int BNT = BRX * BRY / BPT;
+
+// the halo a region keeps below its interior, and the interior's extent, for
+// nsub subcycles
+__host__ __device__ constexpr int bt_lo(int nsub) { return nsub + 1; }
+__host__ __device__ constexpr int bt_interior(int n, int nsub) { return n - 2 * nsub - 1; }
 
 // sf: tmask * (pme + river + source - eta_smooth) and mask: bit 0 tmask(k=1),
 // bit 1 umask(k=1), formed once per call by k_bt_sfmask (constant over the
@@ -245,7 +254,7 @@ __device__ __forceinline__ void bt_tile(const BtJob &J, int tile) {
   __shared__ double s_ps[BRY][BRX];
   __shared__ double s_uh[BRY][BRX];
   __shared__ double s_vh[BRY][BRX];
-  const int TX = BRX - 3 * nsub, TY = BRY - 3 * nsub;
+  const int TX = bt_interior(BRX, nsub), TY = bt_interior(BRY, nsub), LO = bt_lo(nsub);
   const int tix = tile % ntx, tiy = tile / ntx;
   const int x_int0 = xbase + tix * TX, y_int0 = tiy * TY;
   const int lane = threadIdx.x % BRX, wave = threadIdx.x / BRX;
@@ -255,9 +264,12 @@ __device__ __forceinline__ void bt_tile(const BtJob &J, int tile) {
   int rr[BPT], off[BPT], vj[BPT];
   bool present[BPT], interior[BPT], div0[BPT], grad0[BPT];
   double eta[BPT], u1[BPT], u2[BPT], ebv[BPT], ub1[BPT], ub2[BPT];
-  double tm[BPT], sf[BPT], datr[BPT], dyu[BPT], dxu[BPT], rho_g[BPT], patm[BPT], thk[BPT];
-  double F1[BPT], F2[BPT], fc[BPT], cori1[BPT], cori2[BPT], dxur[BPT], dyur[BPT], um[BPT];
-  const int vi = x_int0 - 2 * nsub + lane;
+  // tmask and umask at k = 1 as lane masks: x*mask is x where the mask is 1
+  // and x*0.0 where it is 0 (the same bits as multiplying by the 0/1 double)
+  bool tmb[BPT], umb[BPT];
+  double sf[BPT], datr[BPT], dyu[BPT], dxu[BPT], rho_g[BPT], patm[BPT], thk[BPT];
+  double F1[BPT], F2[BPT], fc[BPT], cori1[BPT], cori2[BPT], dxur[BPT], dyur[BPT];
+  const int vi = x_int0 - LO + lane;
   int col = vi;
   bool colok = true;
   if (cyc) col = 1 + wrapi(vi - 1, d.ni);
@@ -266,18 +278,18 @@ __device__ __forceinline__ void bt_tile(const BtJob &J, int tile) {
   for (int q = 0; q < BPT; ++q) {
     const int r = wave + (BNT / BRX) * q;
     rr[q] = r;
-    const int j = y_int0 - 2 * nsub + r;
+    const int j = y_int0 - LO + r;
     vj[q] = j;
     present[q] = colok && j >= 0 && j < d.NJ;
-    interior[q] = present[q] && lane >= 2 * nsub && lane < 2 * nsub + TX &&
-                  vi < xbase + xcount && r >= 2 * nsub && r < 2 * nsub + TY;
+    interior[q] = present[q] && lane >= LO && lane < LO + TX &&
+                  vi < xbase + xcount && r >= LO && r < LO + TY;
     div0[q] = (j == 0) || (!cyc && vi == 0);
     grad0[q] = (j == d.NJ - 1) || (!cyc && vi == d.NI - 1);
     const int o = present[q] ? col + d.NI * j : 0;
     off[q] = o;
     if (present[q]) {
       const unsigned char mb = in.mask[o];
-      tm[q] = (mb & 1) ? 1.0 : 0.0;
+      tmb[q] = (mb & 1) != 0;
       sf[q] = in.sf[o];
       datr[q] = in.datr[o]; dyu[q] = in.dyu[o]; dxu[q] = in.dxu[o];
       rho_g[q] = in.rho_g[o]; patm[q] = in.patm[o]; thk[q] = in.thicku[o];
@@ -285,13 +297,14 @@ __device__ __forceinline__ void bt_tile(const BtJob &J, int tile) {
       cori1[q] = 0.5 * dtbt * fc[q];
       cori2[q] = 1.0 / (1.0 + cori1[q] * cori1[q]);
       dxur[q] = in.dxur[o]; dyur[q] = in.dyur[o];
-      um[q] = (mb & 2) ? 1.0 : 0.0;
+      umb[q] = (mb & 2) != 0;
       eta[q] = eta_src[o]; u1[q] = ud_src[o]; u2[q] = ud_src[o + P2];
       if (interior[q]) { ebv[q] = eb[o]; ub1[q] = ub[o]; ub2[q] = ub[o + P2]; }
       else { ebv[q] = ub1[q] = ub2[q] = 0.0; }
     } else {
-      tm[q] = sf[q] = datr[q] = dyu[q] = dxu[q] = rho_g[q] = patm[q] = thk[q] = 0.0;
-      F1[q] = F2[q] = fc[q] = cori1[q] = cori2[q] = dxur[q] = dyur[q] = um[q] = 0.0;
+      tmb[q] = umb[q] = false;
+      sf[q] = datr[q] = dyu[q] = dxu[q] = rho_g[q] = patm[q] = thk[q] = 0.0;
+      F1[q] = F2[q] = fc[q] = cori1[q] = cori2[q] = dxur[q] = dyur[q] = 0.0;
       eta[q] = u1[q] = u2[q] = ebv[q] = ub1[q] = ub2[q] = 0.0;
     }
     s_uh[r][lane] = u1[q] * dyu[q];
@@ -300,21 +313,30 @@ __device__ __forceinline__ void bt_tile(const BtJob &J, int tile) {
   __syncthreads();
 
   const int lm = max(lane - 1, 0), lp = min(lane + 1, BRX - 1);
+  // The divergence of U*dy, U*dx at each point.  Phase A of a subcycle needs
+  // the divergence of the velocity phase C of the previous subcycle formed,
+  // from the same LDS values by the same operations, so it is carried over;
+  // only the first subcycle of a launch forms it from the loaded velocity.
+  double dv[BPT];
+#pragma unroll
+  for (int q = 0; q < BPT; ++q) {
+    const int r = rr[q], rm = max(r - 1, 0);
+    dv[q] = 0.0;
+    if (!div0[q]) {
+      const double uh_bay = 0.5 * (s_uh[r][lane] + s_uh[rm][lane]);
+      const double vh_bax = 0.5 * (s_vh[r][lane] + s_vh[r][lm]);
+      const double uhim_bay = 0.5 * (s_uh[r][lm] + s_uh[rm][lm]);
+      const double vhjm_bax = 0.5 * (s_vh[rm][lane] + s_vh[rm][lm]);
+      dv[q] = ((uh_bay - uhim_bay) + (vh_bax - vhjm_bax)) * datr[q];
+    }
+  }
   for (int t = 0; t < nsub; ++t) {
     const int itime = s0 + t;
     // A: predictor and surface pressure
 #pragma unroll
     for (int q = 0; q < BPT; ++q) {
-      const int r = rr[q], rm = max(r - 1, 0);
-      double div = 0.0;
-      if (!div0[q]) {
-        const double uh_bay = 0.5 * (s_uh[r][lane] + s_uh[rm][lane]);
-        const double vh_bax = 0.5 * (s_vh[r][lane] + s_vh[r][lm]);
-        const double uhim_bay = 0.5 * (s_uh[r][lm] + s_uh[rm][lm]);
-        const double vhjm_bax = 0.5 * (s_vh[rm][lane] + s_vh[rm][lm]);
-        div = ((uh_bay - uhim_bay) + (vh_bax - vhjm_bax)) * datr[q];
-      }
-      const double tmp = -div * tm[q];
+      const int r = rr[q];
+      const double tmp = tmb[q] ? -dv[q] : -dv[q] * 0.0;
       const double es = eta[q] + cp * (tmp + sf[q]);
       s_ps[r][lane] = rho_g[q] * es + patm[q];
     }
@@ -331,8 +353,10 @@ __device__ __forceinline__ void bt_tile(const BtJob &J, int tile) {
         const double p_fax = 0.5 * (p + pe);
         const double pip_fay = 0.5 * (pe + pne);
         const double pjp_fax = 0.5 * (pn + pne);
-        g1 = (pip_fay - p_fay) * dxur[q] * um[q];
-        g2 = (pjp_fax - p_fax) * dyur[q] * um[q];
+        const double x1 = (pip_fay - p_fay) * dxur[q];
+        const double x2 = (pjp_fax - p_fax) * dyur[q];
+        g1 = umb[q] ? x1 : x1 * 0.0;
+        g2 = umb[q] ? x2 : x2 * 0.0;
       }
       const double pf1 = -(thk[q] * g1);
       const double pf2 = -(thk[q] * g2);
@@ -340,7 +364,7 @@ __device__ __forceinline__ void bt_tile(const BtJob &J, int tile) {
       const double t2 = u2[q] + dtbt * (-(0.5 * fc[q] * u1[q]) + pf2 + F2[q] + 0.0);
       u1[q] = cori2[q] * (t1 + cori1[q] * t2);
       u2[q] = cori2[q] * (t2 - cori1[q] * t1);
-      // phase A's reads of s_uh/s_vh finished before the barrier above
+      // the previous phase C's reads of s_uh/s_vh finished before the barrier above
       s_uh[r][lane] = u1[q] * dyu[q];
       s_vh[r][lane] = u2[q] * dxu[q];
     }
@@ -358,7 +382,8 @@ __device__ __forceinline__ void bt_tile(const BtJob &J, int tile) {
         const double vhjm_bax = 0.5 * (s_vh[rm][lane] + s_vh[rm][lm]);
         div = ((uh_bay - uhim_bay) + (vh_bax - vhjm_bax)) * datr[q];
       }
-      const double tmp = -div * tm[q];
+      dv[q] = div;
+      const double tmp = tmb[q] ? -div : -div * 0.0;
       eta[q] = eta[q] + cc * (tmp + sf[q]);
       ebv[q] = ebv[q] + eta[q];
       if (step_a) {
@@ -418,7 +443,7 @@ __device__ __host__ inline void bt_shape(const BtDom &D, int ell, int hsub, int 
   const int left = D.nts - ell * hsub;
   nsub = left < hsub ? left : hsub;
   if (nsub <= 0) { nsub = 0; ntx = 0; ntiles = 0; return; }
-  const int TX = BRX - 3 * nsub, TY = BRY - 3 * nsub;
+  const int TX = bt_interior(BRX, nsub), TY = bt_interior(BRY, nsub);
   ntx = (D.xcount + TX - 1) / TX;
   ntiles = ntx * ((D.d.NJ + TY - 1) / TY);
 }
@@ -706,7 +731,7 @@ int launch_barotropic_blocked(mom_gpu_ctx *ctx, const mom_gpu_bt_params_t *p,
   int which = 0;
   for (int s0 = 1; s0 <= p->nts; s0 += hsub) {
     const int nsub = std::min(hsub, p->nts - s0 + 1);
-    const int TX = BRX - 3 * nsub, TY = BRY - 3 * nsub;
+    const int TX = bt_interior(BRX, nsub), TY = bt_interior(BRY, nsub);
     const int ntx = (xcount + TX - 1) / TX, nty = (d.NJ + TY - 1) / TY;
     const BtJob J{d, in, eta_src, ud_src, eta_buf[which], ud_buf[which], eta_bar, ud, s0, nsub,
                   p->nts, p->stepping_a, p->dtbt, c_pred, c_corr, xbase, xcount, ntx, ntx * nty};

@@ -85,7 +85,7 @@ int mom_gpu_memset(void *ptr, int value, size_t bytes) {
 static int *tuning_slot(mom_gpu_ctx *ctx, const char *knob, int *lo, int *hi) {
   if (!knob) return nullptr;
   if (!std::strcmp(knob, "barotropic_subcycles_per_launch")) {
-    *lo = 0; *hi = 9;
+    *lo = 0; *hi = 12;
     return &ctx->tune.bt_subcycles_per_launch;
   }
   if (!std::strcmp(knob, "mdfl_rows_per_wave")) {
@@ -344,7 +344,7 @@ int mom_gpu_create(const mom_gpu_domain_t *dom, const mom_gpu_grid_t *grid, mom_
   k.P2 = k.NI * k.NJ;
   k.cyclic_x = dom->cyclic_x ? 1 : 0;
   k.cyclic_y = dom->cyclic_y ? 1 : 0;
-  c->tune.bt_subcycles_per_launch = 4;
+  c->tune.bt_subcycles_per_launch = 5;
   c->tune.mdfl_rows_per_wave = 4;
   c->tune.step_group_f64 = 2;
   c->tune.step_group_f32 = 10;

@@ -104,7 +104,7 @@ def autotune(step, reps=3, barotropic=True):
     if barotropic and getattr(step, "barotropic", c.dtbt > 0.0) and c.dtbt > 0.0:
         cur, nxt = step.cur, 1 - step.cur
         report["barotropic"] = _search(
-            g, ("barotropic_subcycles_per_launch",), [(n,) for n in range(0, 10)],
+            g, ("barotropic_subcycles_per_launch",), [(n,) for n in range(0, 13)],
             lambda: g.update_ocean_barotropic(
                 nts=c.nts, dtbt=c.dtbt, eta_t_bar_tau=step.eta_bar[cur],
                 udrho_tau=step.udrho[cur], forcing_bt=d["forcing_bt"], thicku=d["thicku"],

@@ -223,7 +223,7 @@ def test_vert_diffuse_implicit(setup):
                            what=f"tracer {n}")
 
 
-@pytest.mark.parametrize("per_launch", [0, 1, 4, 7])
+@pytest.mark.parametrize("per_launch", [0, 1, 4, 7, 12])
 @pytest.mark.parametrize("stepping_a,initsum", [(False, True), (True, False)])
 def test_update_ocean_barotropic(setup, stepping_a, initsum, per_launch):
     from mom5_amd import DeviceArray
