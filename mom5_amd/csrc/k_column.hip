@@ -540,18 +540,23 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// One 64-column tile (J.nbx tiles a row) of job J; every thread of the
-// block calls it (it holds two block barriers per tracer).
-template <int H>
-__device__ __forceinline__ void twin_tile(const VdJobF &J, int tile) {
-  __shared__ float xch[2][2][64];            // [tracer parity][half][column]
-  __shared__ double ech[2][64];
+// One 64-column tile (J.nbx tiles a row) of job J, by wavefront pair
+// threadIdx.x / 128 of a block of NP pairs; every thread of the block calls it
+// (it holds block barriers: one, then one per tracer).  A pair whose tile is
+// past the end (live false) runs a valid tile without storing.
+template <int H, int NP = 1>
+__device__ __forceinline__ void twin_tile(const VdJobF &J, int tile, bool live = true) {
+  __shared__ float xch_all[NP][2][2][64];    // [pair][tracer parity][half][column]
+  __shared__ double ech_all[NP][2][64];
+  const int pr = NP > 1 ? __builtin_amdgcn_readfirstlane(threadIdx.x >> 7) : 0;
+  float (*xch)[2][64] = xch_all[pr];
+  double (*ech)[64] = ech_all[pr];
   const KDom d = J.d;
-  const int half = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int half = __builtin_amdgcn_readfirstlane((threadIdx.x >> 6) & 1);
   const bool top = half == 0;
   const int lane = threadIdx.x & 63;
   const int col = (tile % J.nbx) * 64 + lane;
-  const bool valid = col < d.ni;
+  const bool valid = live && col < d.ni;
   const int nk = d.nk;
   const int nt = (nk + 1) >> 1;
   const int nact = top ? nt : nk - nt;
@@ -702,14 +707,18 @@ __device__ __forceinline__ void twin_tile(const VdJobF &J, int tile) {
 
 // Blocks are dealt to the 8 XCDs round-robin; each XCD takes a contiguous
 // run of 64-column tiles, so the cache lines that two neighbouring tiles
-// share (rows are not line-aligned) are fetched into one L2, once.
+// share (rows are not line-aligned) are fetched into one L2.  A block holds
+// two neighbouring tiles (per_xcd counts tile pairs), so that the two halves
+// of such a line are also written together, before it leaves L2, rather
+// than microseconds apart from two blocks as two partial writes.
 template <int H>
-__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 void k_vdiff_twin(VdJobF J, int per_xcd) {
   const int b = blockIdx.x;
-  const int tile = (b & 7) * per_xcd + (b >> 3);
-  if (tile >= J.nblocks) return;             // the whole block
-  twin_tile<H>(J, tile);
+  const int t0 = 2 * ((b & 7) * per_xcd + (b >> 3));
+  if (t0 >= J.nblocks) return;               // the whole block
+  const int tile = t0 + (threadIdx.x >> 7);
+  twin_tile<H, 2>(J, min(tile, J.nblocks - 1), tile < J.nblocks);
 }
 
 // Every tracer of a mixed step in one launch: FP32 tracers (mixing class 1,
@@ -1711,8 +1720,8 @@ static void launch_twin(mom_gpu_ctx *ctx, VdJobF J) {
   const KDom &d = J.d;
   J.nbx = (d.ni + 63) / 64;
   J.nblocks = J.nbx * d.nj;
-  const int per_xcd = (J.nblocks + 7) / 8;
-  const dim3 grid(8 * per_xcd), block(128);
+  const int per_xcd = ((J.nblocks + 1) / 2 + 7) / 8;   // tile pairs per XCD
+  const dim3 grid(8 * per_xcd), block(256);
   switch (twin_slots(d.nk)) {
     case 8: hipLaunchKernelGGL(k_vdiff_twin<8>, grid, block, 0, ctx->stream, J, per_xcd); break;
     case 16: hipLaunchKernelGGL(k_vdiff_twin<16>, grid, block, 0, ctx->stream, J, per_xcd); break;
