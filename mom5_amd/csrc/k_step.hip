@@ -342,14 +342,19 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
       if constexpr (!EXACT) wx = face_w(cxf, mX1);
 #pragma unroll
       for (int g = 0; g < G; ++g) {
-        const TT tzm1 = lane_prev(tz[g][q]);
+        // Each lane forms its east face's masked difference once; the faces
+        // either side have theirs one lane over: (T(i+2) - T(i+1))*mX2 is the
+        // next lane's difference and (T(i) - T(i-1))*mX0 the previous lane's
+        // (mX2, mX0 are the neighbouring lanes' mX1), the same values bit for bit.
         const TT tzp1 = lane_next(tz[g][q]);
-        const TT tzp2 = lane_next(tzp1);
+        const TT Rjl = (tzp1 - tz[g][q]) * mX1;
+        const TT Rjp = lane_next(Rjl);
+        const TT Rjm = lane_prev(Rjl);
         TT fx = TT(0);
         if (face) {
-          const TT Rjp = (tzp2 - tzp1) * mX2;
-          const TT Rj = EXACT ? (tzp1 - tz[g][q]) * mX1 : tzp1 - tz[g][q];  // FP32: face weights carry mX1
-          const TT Rjm = (tz[g][q] - tzm1) * mX0;
+          // FP32: the unmasked difference would do (the face weights carry
+          // mX1), the masked one gives the same flux
+          const TT Rj = Rjl;
           if constexpr (EXACT)
             fx = mdfl64::flux(cx, Rjm, Rj, Rjp, tz[g][q], tzp1);
           else
