@@ -992,15 +992,20 @@ void k_vdiff_twin_ens(const VdJobF *__restrict__ jobs, VdLaunch L, int per_xcd) 
 // its level count), so both meet in slot H-1; slots below ph are inert.  A
 // group's tracers share the coefficients' addresses, not their factorisation:
 // each tracer repeats it (the coefficients come from L2 the second time).
-template <int H>
-__device__ __forceinline__ void twin64_tile(const VdJob<double> &J, int tile) {
-  __shared__ double xv[2][2][64], xq[2][2][64];   // [tracer parity][half][column]
+// Wavefront pair threadIdx.x / 128 of a block of NP pairs; a pair whose
+// tile is past the end (live false) runs a valid tile without storing.
+template <int H, int NP = 1>
+__device__ __forceinline__ void twin64_tile(const VdJob<double> &J, int tile, bool live = true) {
+  __shared__ double xv_all[NP][2][2][64], xq_all[NP][2][2][64];   // [pair][parity][half][column]
+  const int pr = NP > 1 ? __builtin_amdgcn_readfirstlane(threadIdx.x >> 7) : 0;
+  double (*xv)[2][64] = xv_all[pr];
+  double (*xq)[2][64] = xq_all[pr];
   const KDom d = J.d;
-  const int half = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int half = __builtin_amdgcn_readfirstlane((threadIdx.x >> 6) & 1);
   const bool top = half == 0;
   const int lane = threadIdx.x & 63;
   const int col = (tile % J.nbx) * 64 + lane;
-  const bool valid = col < d.ni;
+  const bool valid = live && col < d.ni;
   const int nk = d.nk;
   const int nt = (nk + 1) >> 1;
   const int nact = top ? nt : nk - nt;
@@ -1170,13 +1175,16 @@ __device__ __forceinline__ void twin64_tile(const VdJob<double> &J, int tile) {
   }
 }
 
+// Two neighbouring tiles a block (per_xcd counts tile pairs), as for the
+// FP32 solver: the halves of the cache lines they share are written together.
 template <int H>
-__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 void k_vdiff_twin64(VdJob<double> J, int per_xcd) {
   const int b = blockIdx.x;
-  const int tile = (b & 7) * per_xcd + (b >> 3);
-  if (tile >= J.nblocks) return;             // the whole block
-  twin64_tile<H>(J, tile);
+  const int t0 = 2 * ((b & 7) * per_xcd + (b >> 3));
+  if (t0 >= J.nblocks) return;               // the whole block
+  const int tile = t0 + (threadIdx.x >> 7);
+  twin64_tile<H, 2>(J, min(tile, J.nblocks - 1), tile < J.nblocks);
 }
 
 // Ensemble: every FP64 group of every domain whose nk the register solver
@@ -1705,8 +1713,8 @@ static int twin_slots(int nk) {
 
 static void launch_twin64(mom_gpu_ctx *ctx, VdJob<double> J) {
   const KDom &d = J.d;
-  const int per_xcd = (J.nblocks + 7) / 8;
-  const dim3 grid(8 * per_xcd), block(128);
+  const int per_xcd = ((J.nblocks + 1) / 2 + 7) / 8;   // tile pairs per XCD
+  const dim3 grid(8 * per_xcd), block(256);
   switch (twin_slots(d.nk)) {
     case 8: hipLaunchKernelGGL(k_vdiff_twin64<8>, grid, block, 0, ctx->stream, J, per_xcd); break;
     case 16: hipLaunchKernelGGL(k_vdiff_twin64<16>, grid, block, 0, ctx->stream, J, per_xcd); break;
