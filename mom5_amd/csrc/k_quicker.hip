@@ -78,119 +78,171 @@ __global__ void k_quicker_init_z(int nk, const double *__restrict__ dzt, double 
   qz[k - 1 + 7 * nk] = z0 * z1 / ((z0 + 2.0 * z1 + z2) * (z0 + z1));
 }
 
-// tracer_quick / tmask_quick semantics: compute-domain value, cyclic wrap,
-// zero outside a closed axis; corners only when both axes are cyclic.
-__device__ inline bool h2src(const KDom &d, int gi, int gj, int &o) {
-  const bool in_i = gi >= 0 && gi < d.ni, in_j = gj >= 0 && gj < d.nj;
-  if (!in_i && !d.cyclic_x) return false;
-  if (!in_j && !d.cyclic_y) return false;
-  if (!in_i && !in_j && !(d.cyclic_x && d.cyclic_y)) return false;
-  o = d2(d, in_i ? gi : wrapi(gi, d.ni), in_j ? gj : wrapi(gj, d.nj));
-  return true;
-}
+// horz_advect_tracer_quicker (:2538-2653), marching down the levels.  A
+// work-group owns a 64-wide region of NW rows (one row per wavefront; a
+// 2-point halo on each side, so 60 x (NW-4) interior points) and walks k:
+// the 16 Quicker weights, dyte, dxtn and datr of a column are two-dimensional
+// and stay in registers for every level, the x neighbours come by DPP lane
+// shifts and the y neighbours through LDS.  Each point forms the flux through
+// its east face and its north face; the cell takes the west and south ones
+// from its neighbours.
+//
+// The inputs keep the reference's two readings: the upwind-biased stencil
+// reads T(taum1) and the masks with tracer_quick / tmask_quick semantics
+// (compute-domain value, cyclic wrap, zero beyond a closed axis, corners only
+// when both axes are cyclic), while T(tau), the velocities, tmask_limit and
+// the upwind fallback's T(taum1) and kmt are read at the face's own data-
+// domain points, halo included.  Every flux and divergence is the
+// reference's expression, operation for operation.
+constexpr int QRX = 64, QTX = QRX - 4;
 
-struct H2Val {
-  double t, m;
-};
-
-__device__ inline H2Val h2val(const KDom &d, const double *Tm1, const int *kmt,
-                              int gi, int gj, int k) {
-  int o;
-  if (!h2src(d, gi, gj, o)) return {0.0, 0.0};
-  return {Tm1[o + d.P2 * (k - 1)], (k <= kmt[o]) ? 1.0 : 0.0};
-}
-
-constexpr int QBX = 64, QBY = 4;
-
-// One k level of one 64x4 tile per block; fluxes of the tile's faces in LDS.
-__global__ __launch_bounds__(QBX *QBY) void k_quicker_horz(
+template <int NW>
+__global__ __launch_bounds__(QRX *NW) void k_quicker_horz(
     KDom d, const double *__restrict__ Tm1, const double *__restrict__ Tt,
     const double *__restrict__ U, const double *__restrict__ V,
     const double *__restrict__ tlim, int limit_with_upwind,
-    const int *__restrict__ kmt, const double *__restrict__ dyte,
-    const double *__restrict__ dxtn, const double *__restrict__ datr,
+    const int *__restrict__ kmt, const double *__restrict__ dyte_a,
+    const double *__restrict__ dxtn_a, const double *__restrict__ datr_a,
     const double *__restrict__ qw, double *__restrict__ th,
-    double *__restrict__ wrk1, double *__restrict__ fxo, double *__restrict__ fyo) {
-  __shared__ double s_fx[QBY][QBX + 1];
-  __shared__ double s_fy[QBY + 1][QBX];
-  const int tx = threadIdx.x, ty = threadIdx.y;
-  const int gi = blockIdx.x * QBX + tx, gj = blockIdx.y * QBY + ty;
-  const int k = blockIdx.z + 1;
-  const int P2 = d.P2, k0 = k - 1;
-  const double *quick_x = qw, *quick_y = qw + 2 * P2, *cxp = qw + 4 * P2;
-  const double *cxn = qw + 7 * P2, *cyp = qw + 10 * P2, *cyn = qw + 13 * P2;
+    double *__restrict__ wrk1, double *__restrict__ fxo, double *__restrict__ fyo,
+    int ntx, int ntiles, int per_xcd) {
+  constexpr int RY = NW, TY = RY - 4;
+  const int b = blockIdx.x;
+  const int t = (b & 7) * per_xcd + (b >> 3);
+  if (t >= ntiles) return;
+  const int tile = strip_tile<8>(t, ntx, ntiles / ntx);
+  const int tix = tile % ntx, tiy = tile / ntx;
+  const int lane = threadIdx.x & (QRX - 1), row = threadIdx.x / QRX;
+  const int gi = tix * QTX - 2 + lane, gj = tiy * TY - 2 + row;
+  const int nk = d.nk, P2 = d.P2;
 
-  // x flux through the face east of column fi, row gj (reference :2590-2606)
-  auto xflux = [&](int fi) -> double {
-    const int o = d2(d, fi, gj);
-    const double vel = dyte[o] * U[o + P2 * k0];
-    const double upos = 0.5 * (vel + fabs(vel));
-    const double uneg = 0.5 * (vel - fabs(vel));
-    const H2Val a_m1 = h2val(d, Tm1, kmt, fi - 1, gj, k), a_0 = h2val(d, Tm1, kmt, fi, gj, k);
-    const H2Val a_p1 = h2val(d, Tm1, kmt, fi + 1, gj, k), a_p2 = h2val(d, Tm1, kmt, fi + 2, gj, k);
-    const double eastmsk = a_0.m * (1.0 - a_m1.m);
-    const double westmsk = a_p1.m * (1.0 - a_p2.m);
-    double fx = vel * (quick_x[o] * Tt[o + P2 * k0] + quick_x[o + P2] * Tt[o + 1 + P2 * k0]) -
-                upos * (cxp[o] * a_p1.t + cxp[o + P2] * a_0.t +
-                        cxp[o + 2 * P2] * (a_m1.t * (1.0 - eastmsk) + a_0.t * eastmsk)) -
-                uneg * (cxn[o] * (a_p2.t * (1.0 - westmsk) + a_p1.t * westmsk) +
-                        cxn[o + P2] * a_p1.t + cxn[o + 2 * P2] * a_0.t);
-    if (limit_with_upwind && tlim[o + P2 * k0] == 1.0) {
-      const double u = U[o + P2 * k0];
-      const double up = 0.5 * (u + fabs(u)), un = 0.5 * (u - fabs(u));
-      const double m0 = (k <= kmt[o]) ? 1.0 : 0.0, m1 = (k <= kmt[o + 1]) ? 1.0 : 0.0;
-      fx = dyte[o] * (up * Tm1[o + P2 * k0] + un * Tm1[o + 1 + P2 * k0]) * m0 * m1;
-    }
-    return fx;
-  };
-  // y flux through the face north of row fj, column gi (:2573-2587)
-  auto yflux = [&](int fj) -> double {
-    const int o = d2(d, gi, fj);
-    const double vel = dxtn[o] * V[o + P2 * k0];
-    const double upos = 0.5 * (vel + fabs(vel));
-    const double uneg = 0.5 * (vel - fabs(vel));
-    const H2Val a_m1 = h2val(d, Tm1, kmt, gi, fj - 1, k), a_0 = h2val(d, Tm1, kmt, gi, fj, k);
-    const H2Val a_p1 = h2val(d, Tm1, kmt, gi, fj + 1, k), a_p2 = h2val(d, Tm1, kmt, gi, fj + 2, k);
-    const double rnormsk = a_0.m * (1.0 - a_m1.m);
-    const double soutmsk = a_p1.m * (1.0 - a_p2.m);
-    double fy = vel * (quick_y[o] * Tt[o + P2 * k0] + quick_y[o + P2] * Tt[o + d.NI + P2 * k0]) -
-                upos * (cyp[o] * a_p1.t + cyp[o + P2] * a_0.t +
-                        cyp[o + 2 * P2] * (a_m1.t * (1.0 - rnormsk) + a_0.t * rnormsk)) -
-                uneg * (cyn[o] * (a_p2.t * (1.0 - soutmsk) + a_p1.t * soutmsk) +
-                        cyn[o + P2] * a_p1.t + cyn[o + 2 * P2] * a_0.t);
-    if (limit_with_upwind && tlim[o + P2 * k0] == 1.0) {
-      const double v = V[o + P2 * k0];
-      const double up = 0.5 * (v + fabs(v)), un = 0.5 * (v - fabs(v));
-      const double m0 = (k <= kmt[o]) ? 1.0 : 0.0, m1 = (k <= kmt[o + d.NI]) ? 1.0 : 0.0;
-      fy = dxtn[o] * (up * Tm1[o + P2 * k0] + un * Tm1[o + d.NI + P2 * k0]) * m0 * m1;
-    }
-    return fy;
-  };
+  __shared__ double s_t[RY][QRX];    // T(taum1), quick semantics
+  __shared__ double s_tt[RY][QRX];   // T(tau) at the data-domain point
+  __shared__ double s_fy[RY][QRX];   // flux through the north face of each point
+  __shared__ int s_km[RY][QRX];
 
-  const bool in = gi < d.ni && gj < d.nj;
-  if (in) {
-    s_fx[ty][tx + 1] = xflux(gi);
-    s_fy[ty + 1][tx] = yflux(gj);
-    if (tx == 0) s_fx[ty][0] = xflux(gi - 1);
-    if (ty == 0) s_fy[0][tx] = yflux(gj - 1);
+  // the point with quick semantics
+  int si = gi, sj = gj;
+  bool colok = true, rowok = true;
+  if (si < 0 || si >= d.ni) {
+    if (d.cyclic_x) si = wrapi(si, d.ni); else colok = false;
   }
+  if (sj < 0 || sj >= d.nj) {
+    if (d.cyclic_y) sj = wrapi(sj, d.nj); else rowok = false;
+  }
+  const bool ok = colok && rowok;
+  const int off = ok ? d2(d, si, sj) : 0;
+  const int km = ok ? kmt[off] : 0;
+  s_km[row][lane] = km;
+  // the point as a data-domain index (halo included), where it exists
+  const bool indata = gi >= -1 && gi <= d.ni && gj >= -1 && gj <= d.nj;
+  const int raw = indata ? d2(d, gi, gj) : 0;
+
+  // faces this point forms: the east face of a compute row, the north face
+  // of a compute column; the cells of the region's interior
+  const bool in_j = gj >= 0 && gj < d.nj, in_i = gi >= 0 && gi < d.ni;
+  const bool xface = in_j && lane >= 1 && lane <= QRX - 3 && gi >= -1 && gi <= d.ni - 1;
+  const bool yface = in_i && row >= 1 && row <= RY - 3 && gj >= -1 && gj <= d.nj - 1;
+  const bool cell = in_i && in_j && row >= 2 && row < RY - 2 && lane >= 2 && lane < QRX - 2;
+  double qx0 = 0.0, qx1 = 0.0, cxp0 = 0.0, cxp1 = 0.0, cxp2 = 0.0, cxn0 = 0.0, cxn1 = 0.0,
+         cxn2 = 0.0, dyte = 0.0;
+  double qy0 = 0.0, qy1 = 0.0, cyp0 = 0.0, cyp1 = 0.0, cyp2 = 0.0, cyn0 = 0.0, cyn1 = 0.0,
+         cyn2 = 0.0, dxtn = 0.0;
+  if (xface) {
+    const double *w = qw + raw;
+    qx0 = w[0]; qx1 = w[P2];
+    cxp0 = w[4 * P2]; cxp1 = w[5 * P2]; cxp2 = w[6 * P2];
+    cxn0 = w[7 * P2]; cxn1 = w[8 * P2]; cxn2 = w[9 * P2];
+    dyte = dyte_a[raw];
+  }
+  if (yface) {
+    const double *w = qw + raw;
+    qy0 = w[2 * P2]; qy1 = w[3 * P2];
+    cyp0 = w[10 * P2]; cyp1 = w[11 * P2]; cyp2 = w[12 * P2];
+    cyn0 = w[13 * P2]; cyn1 = w[14 * P2]; cyn2 = w[15 * P2];
+    dxtn = dxtn_a[raw];
+  }
+  const double datr = cell ? datr_a[raw] : 0.0;
+  const int kraw = cell ? kmt[raw] : 0;
   __syncthreads();
-  if (!in) return;
-  const int o = d2(d, gi, gj);
-  const double fx = s_fx[ty][tx + 1], fxw = s_fx[ty][tx];
-  const double fy = s_fy[ty + 1][tx], fys = s_fy[ty][tx];
-  const double m = (k <= kmt[o]) ? 1.0 : 0.0;
-  const double res = m * (fx - fxw + fy - fys) * datr[o];
-  const double w1 = -res;
-  th[o + P2 * k0] = th[o + P2 * k0] + w1;
-  if (wrk1) wrk1[o + P2 * k0] = w1;
-  if (fxo) {
-    fxo[o + P2 * k0] = fx;
-    if (gi == 0) fxo[o - 1 + P2 * k0] = fxw;
-  }
-  if (fyo) {
-    fyo[o + P2 * k0] = fy;
-    if (gj == 0) fyo[o - d.NI + P2 * k0] = fys;
+  // depths of the neighbours the coastal substitutions read
+  const int km_w = dpp_prev(km);
+  const int km_e = dpp_next(km), km_e2 = dpp_next(km_e);
+  const int km_s = s_km[max(row - 1, 0)][lane];
+  const int km_n = s_km[min(row + 1, RY - 1)][lane];
+  const int km_n2 = s_km[min(row + 2, RY - 1)][lane];
+
+  for (int k = 1; k <= nk; ++k) {
+    const int k0 = k - 1;
+    const size_t pk = (size_t)P2 * k0;
+    const double t1 = ok ? Tm1[off + pk] : 0.0;
+    const double tt = indata ? Tt[raw + pk] : 0.0;
+    s_t[row][lane] = t1;
+    s_tt[row][lane] = tt;
+    __syncthreads();
+    const double te = lane_next(t1), te2 = lane_next(te), tw = lane_prev(t1);
+    const double tte = lane_next(tt);
+    const double tn = s_t[min(row + 1, RY - 1)][lane];
+    const double tn2 = s_t[min(row + 2, RY - 1)][lane];
+    const double ts = s_t[max(row - 1, 0)][lane];
+    const double ttn = s_tt[min(row + 1, RY - 1)][lane];
+    const double m0 = (k <= km) ? 1.0 : 0.0;
+
+    // x flux through the east face of this point (:2590-2606, 2610-2622)
+    double fx = 0.0;
+    if (xface) {
+      const double u = U[raw + pk];
+      const double vel = dyte * u;
+      const double upos = 0.5 * (vel + fabs(vel));
+      const double uneg = 0.5 * (vel - fabs(vel));
+      const double eastmsk = m0 * (1.0 - ((k <= km_w) ? 1.0 : 0.0));
+      const double westmsk = ((k <= km_e) ? 1.0 : 0.0) * (1.0 - ((k <= km_e2) ? 1.0 : 0.0));
+      fx = vel * (qx0 * tt + qx1 * tte) -
+           upos * (cxp0 * te + cxp1 * t1 + cxp2 * (tw * (1.0 - eastmsk) + t1 * eastmsk)) -
+           uneg * (cxn0 * (te2 * (1.0 - westmsk) + te * westmsk) + cxn1 * te + cxn2 * t1);
+      if (limit_with_upwind && tlim[raw + pk] == 1.0) {
+        const double up = 0.5 * (u + fabs(u)), un = 0.5 * (u - fabs(u));
+        const double ma = (k <= kmt[raw]) ? 1.0 : 0.0, mb = (k <= kmt[raw + 1]) ? 1.0 : 0.0;
+        fx = dyte * (up * Tm1[raw + pk] + un * Tm1[raw + 1 + pk]) * ma * mb;
+      }
+    }
+    const double fxw = lane_prev(fx);
+    // y flux through the north face (:2573-2587, 2623-2635)
+    if (yface) {
+      const double v = V[raw + pk];
+      const double vel = dxtn * v;
+      const double upos = 0.5 * (vel + fabs(vel));
+      const double uneg = 0.5 * (vel - fabs(vel));
+      const double rnormsk = m0 * (1.0 - ((k <= km_s) ? 1.0 : 0.0));
+      const double soutmsk = ((k <= km_n) ? 1.0 : 0.0) * (1.0 - ((k <= km_n2) ? 1.0 : 0.0));
+      double fy = vel * (qy0 * tt + qy1 * ttn) -
+                  upos * (cyp0 * tn + cyp1 * t1 + cyp2 * (ts * (1.0 - rnormsk) + t1 * rnormsk)) -
+                  uneg * (cyn0 * (tn2 * (1.0 - soutmsk) + tn * soutmsk) + cyn1 * tn + cyn2 * t1);
+      if (limit_with_upwind && tlim[raw + pk] == 1.0) {
+        const double up = 0.5 * (v + fabs(v)), un = 0.5 * (v - fabs(v));
+        const double ma = (k <= kmt[raw]) ? 1.0 : 0.0, mb = (k <= kmt[raw + d.NI]) ? 1.0 : 0.0;
+        fy = dxtn * (up * Tm1[raw + pk] + un * Tm1[raw + d.NI + pk]) * ma * mb;
+      }
+      s_fy[row][lane] = fy;
+    }
+    __syncthreads();
+
+    if (cell) {
+      const double fy = s_fy[row][lane], fys = s_fy[row - 1][lane];
+      const double m = (k <= kraw) ? 1.0 : 0.0;
+      const double res = m * (fx - fxw + fy - fys) * datr;
+      const double w1 = -res;
+      th[raw + pk] = th[raw + pk] + w1;
+      if (wrk1) wrk1[raw + pk] = w1;
+      if (fxo) {
+        fxo[raw + pk] = fx;
+        if (gi == 0) fxo[raw - 1 + pk] = fxw;
+      }
+      if (fyo) {
+        fyo[raw + pk] = fy;
+        if (gj == 0) fyo[raw - d.NI + pk] = fys;
+      }
+    }
   }
 }
 
@@ -258,10 +310,12 @@ int launch_quicker_horz(mom_gpu_ctx *ctx, const double *Tm1, const double *Tt,
                         int limit_with_upwind, double *th, double *wrk1, double *fx,
                         double *fy) {
   const KDom &d = ctx->kd;
-  const dim3 grid((d.ni + QBX - 1) / QBX, (d.nj + QBY - 1) / QBY, d.nk);
-  hipLaunchKernelGGL(k_quicker_horz, grid, dim3(QBX, QBY), 0, ctx->stream, d, Tm1, Tt, u, v,
-                     tlim, limit_with_upwind, ctx->kmt, ctx->dyte, ctx->dxtn, ctx->datr, ctx->qw,
-                     th, wrk1, fx, fy);
+  constexpr int NW = 16;
+  const int ntx = (d.ni + QTX - 1) / QTX, nty = (d.nj + NW - 4 - 1) / (NW - 4);
+  const int ntiles = ntx * nty, per_xcd = (ntiles + 7) / 8;
+  hipLaunchKernelGGL(k_quicker_horz<NW>, dim3(8 * per_xcd), dim3(QRX * NW), 0, ctx->stream, d,
+                     Tm1, Tt, u, v, tlim, limit_with_upwind, ctx->kmt, ctx->dyte, ctx->dxtn,
+                     ctx->datr, ctx->qw, th, wrk1, fx, fy, ntx, ntiles, per_xcd);
   MOM_LAUNCHED(ctx, "k_quicker_horz");
   return MOM_GPU_OK;
 }
