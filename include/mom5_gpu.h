@@ -165,7 +165,7 @@ int mom_gpu_copy(mom_gpu_ctx_t *ctx, double *dst, const double *src, size_t n);
  *                                      mom_gpu_mark first wait for that
  *                                      stream, so results are the same.
  * Defaults (the fastest measured at 3600x2700x75 with T, S and ten FP32
- * tracers): 4, 4, 2, 10, 2, 1, 16, 12, 1, 256, 0, 1, 1 in the order above.  Unknown knobs and
+ * tracers): 5, 4, 2, 10, 1, 1, 16, 16, 1, 256, 0, 1, 1 in the order above.  Unknown knobs and
  * out-of-range values return MOM_GPU_ERR_ARG. */
 int mom_gpu_set_tuning(mom_gpu_ctx_t *ctx, const char *knob, int value);
 int mom_gpu_get_tuning(const mom_gpu_ctx_t *ctx, const char *knob, int *value);

@@ -236,9 +236,57 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
     }
   }
 
-  for (int k = 1; k <= nk; ++k) {
+  // FP64 groups issue all of a level's 3-D loads when the level starts: w
+  // and rho_dzt of the point, u and the two rho_dzt of its east face, v and
+  // the two of its north face, rho_dzt(taum1) and 1/rho_dzt(taup1) (or
+  // rho_dzt(taup1)), so that the x, y and update phases (each after a
+  // barrier) do not each wait for their own loads with every wavefront of the
+  // work-group at once.  The loads are unconditional, from offset 0 where the
+  // face or point does not exist (the value is then unused).  The face
+  // metrics stay in registers as for FP32 groups.
+  struct LvIn {
+    double w, r, u, rw, re, v, rs, rn, rm1, rr;
+  };
+  int oxq[RPW], oyq[RPW];
+  double hdyte[RPW], hdxte[RPW], hdxtn[RPW], hdytn[RPW];
+  if constexpr (EXACT) {
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+      const int ry = wave + NW * q;
+      const bool fx = rowok[q] && lane >= 1 && lane <= RX - 3 && gi >= -1 && gi <= d.ni - 1;
+      const bool fy = ry >= 1 && ry <= RY - 3 && lane >= 2 && lane < RX - 2 && gi < d.ni &&
+                      gjr[q] >= -1 && gjr[q] <= d.nj - 1;
+      oxq[q] = fx ? d2(d, gi, sjr[q]) : 0;
+      oyq[q] = fy ? d2(d, gi, gjr[q]) : 0;
+      hdyte[q] = dyte[oxq[q]];
+      hdxte[q] = dxte[oxq[q]];
+      hdxtn[q] = dxtn[oyq[q]];
+      hdytn[q] = dytn[oyq[q]];
+    }
+  }
+  auto lv_load = [&](int k, LvIn *L) {
+    const size_t pk = (size_t)P2 * (k - 1);
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+      const int o = offz[q], ox = oxq[q], oy = oyq[q];
+      L[q].w = W[o + pk + P2];
+      L[q].r = R[o + pk];
+      L[q].u = U[ox + pk];
+      L[q].rw = R[ox + pk];
+      L[q].re = R[ox + 1 + pk];
+      L[q].v = V[oy + pk];
+      L[q].rs = R[oy + pk];
+      L[q].rn = R[oy + d.NI + pk];
+      L[q].rm1 = Rm1[o + pk];
+      L[q].rr = Rr ? Rr[o + pk] : R[o + pk];
+    }
+  };
+
+  auto level = [&](const int k) {
+    LvIn cur[RPW];
+    if constexpr (EXACT) lv_load(k, cur);
     const int k0 = k - 1;
-    const int kp1 = min(k + 1, nk), kp2 = min(k + 2, nk), km1 = max(k - 1, 1);
+    const int kp1 = min(k + 1, nk), kp2 = min(k + 2, nk);
     double rhov[RPW], xf64v[RPW], dtrv[RPW];   // dtrv: FP64 groups, dtime/rho_dzt
     RT wv[RPW];
     float rinv[RPW];   // FP32 groups: about 1/rho_dzt
@@ -252,8 +300,8 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
       rhov[q] = 1.0; wv[q] = RT(0); rinv[q] = 1.0f; dtrv[q] = 0.0;
       if (ok[q]) {
         const int o = offz[q];
-        const double w = W[o + P2 * k];
-        const double r = R[o + P2 * k0];
+        const double w = EXACT ? cur[q].w : W[o + P2 * k];
+        const double r = EXACT ? cur[q].r : R[o + P2 * k0];
         rhov[q] = r; wv[q] = (RT)w;
         // mk(km1)*mk(k) = mk(k), mk(k)*mk(kp1) = mk(kp1), mk(kp1)*mk(kp2) =
         // mk(kp2): km1 <= k <= kp1 <= kp2 and masks in {0, 1}
@@ -311,12 +359,13 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
       FaceC<float> cxf{0.0f, 0.0f, 0.0f, 0.0f};
       if (face) {
         const int of = d2(d, gi, sjr[q]);
-        const double u = U[of + P2 * k0];
-        const double rw = R[of + P2 * k0], re = R[of + 1 + P2 * k0];
         if constexpr (EXACT) {
-          Mx = dyte[of] * u;
-          cx = mdfl64::face(Mx, mdfl64::cfl_h(u, dtime, rw + re, dxte[of]), (double)mX1);
+          const double u = cur[q].u, rw = cur[q].rw, re = cur[q].re;
+          Mx = hdyte[q] * u;
+          cx = mdfl64::face(Mx, mdfl64::cfl_h(u, dtime, rw + re, hdxte[q]), (double)mX1);
         } else {
+          const double u = U[of + P2 * k0];
+          const double rw = R[of + P2 * k0], re = R[of + 1 + P2 * k0];
           const float uf = (float)u;
           cxf = face_coef_f(mdyte[q] * uf, fabsf(uf * (float)(dtime * 2.0) *
                                                  rcpf(((float)rw + (float)re) * mdxte[q])));
@@ -394,12 +443,13 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
       FaceC<float> cyf{0.0f, 0.0f, 0.0f, 0.0f};
       if (face) {
         const int of = d2(d, gi, gj);
-        const double v = V[of + P2 * k0];
-        const double rs = R[of + P2 * k0], rn = R[of + d.NI + P2 * k0];
         if constexpr (EXACT) {
-          cy = mdfl64::face(dxtn[of] * v, mdfl64::cfl_h(v, dtime, rs + rn, dytn[of]),
+          const double v = cur[q].v, rs = cur[q].rs, rn = cur[q].rn;
+          cy = mdfl64::face(hdxtn[q] * v, mdfl64::cfl_h(v, dtime, rs + rn, hdytn[q]),
                             (double)mY1);
         } else {
+          const double v = V[of + P2 * k0];
+          const double rs = R[of + P2 * k0], rn = R[of + d.NI + P2 * k0];
           const float vf = (float)v;
           cyf = face_coef_f(mdxtn[q] * vf, fabsf(vf * (float)(dtime * 2.0) *
                                                  rcpf(((float)rs + (float)rn) * mdytn[q])));
@@ -439,8 +489,9 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
         const double r = rhov[q];
         const double mi = mk<double>(k, km[q]);
         const double brk = wv[q] - wkm1[q] + datrv[q] * ((double)mfw[q] - (double)mfe[q]);
-        const double rm1 = Rm1[o];
-        const double rr = Rr ? Rr[o] : 1.0 / (R[o] + EPSLN);
+        const double rm1 = EXACT ? cur[q].rm1 : Rm1[o];
+        const double rr = Rr ? (EXACT ? cur[q].rr : Rr[o])
+                             : 1.0 / ((EXACT ? cur[q].rr : R[o]) + EPSLN);
         // FP32: T(taup1) = T*(rho_taum1/rho_taup1) + dtime*datr/rho_taup1
         //                  * (sum of the face flux differences) + th*dtime/rho_taup1.
         // The update is rebuilt from the limited fluxes alone (flux form); the
@@ -484,7 +535,8 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
         if (ok[q] && k < nk) t3[g][q] = tab.T[g][offz[q] + P2 * (min(k + 3, nk) - 1)];
       }
     }
-  }
+  };
+  for (int k = 1; k <= nk; ++k) level(k);
 }
 
 

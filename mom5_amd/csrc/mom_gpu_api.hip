@@ -348,10 +348,10 @@ int mom_gpu_create(const mom_gpu_domain_t *dom, const mom_gpu_grid_t *grid, mom_
   c->tune.mdfl_rows_per_wave = 4;
   c->tune.step_group_f64 = 2;
   c->tune.step_group_f32 = 10;
-  c->tune.step_rows_per_wave = 2;
+  c->tune.step_rows_per_wave = 1;
   c->tune.step_rows_per_wave_f32 = 1;
   c->tune.step_waves_f32 = 16;
-  c->tune.step_waves_f64 = 12;
+  c->tune.step_waves_f64 = 16;
   c->tune.vdiff_prefetch = 1;
   c->tune.vdiff_block = 256;
   c->tune.vdiff_merge = 0;
