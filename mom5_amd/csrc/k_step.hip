@@ -249,15 +249,18 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
   };
   int oxq[RPW], oyq[RPW];
   double hdyte[RPW], hdxte[RPW], hdxtn[RPW], hdytn[RPW];
+#pragma unroll
+  for (int q = 0; q < RPW; ++q) {
+    const int ry = wave + NW * q;
+    const bool fx = rowok[q] && lane >= 1 && lane <= RX - 3 && gi >= -1 && gi <= d.ni - 1;
+    const bool fy = ry >= 1 && ry <= RY - 3 && lane >= 2 && lane < RX - 2 && gi < d.ni &&
+                    gjr[q] >= -1 && gjr[q] <= d.nj - 1;
+    oxq[q] = fx ? d2(d, gi, sjr[q]) : 0;
+    oyq[q] = fy ? d2(d, gi, gjr[q]) : 0;
+  }
   if constexpr (EXACT) {
 #pragma unroll
     for (int q = 0; q < RPW; ++q) {
-      const int ry = wave + NW * q;
-      const bool fx = rowok[q] && lane >= 1 && lane <= RX - 3 && gi >= -1 && gi <= d.ni - 1;
-      const bool fy = ry >= 1 && ry <= RY - 3 && lane >= 2 && lane < RX - 2 && gi < d.ni &&
-                      gjr[q] >= -1 && gjr[q] <= d.nj - 1;
-      oxq[q] = fx ? d2(d, gi, sjr[q]) : 0;
-      oyq[q] = fy ? d2(d, gi, gjr[q]) : 0;
       hdyte[q] = dyte[oxq[q]];
       hdxte[q] = dxte[oxq[q]];
       hdxtn[q] = dxtn[oyq[q]];
@@ -284,7 +287,23 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
 
   auto level = [&](const int k) {
     LvIn cur[RPW];
-    if constexpr (EXACT) lv_load(k, cur);
+    if constexpr (EXACT) {
+      lv_load(k, cur);
+    } else {
+      // FP32 groups (registers for six more values, not ten): the x and y
+      // phases' loads
+      const size_t pk = (size_t)P2 * (k - 1);
+#pragma unroll
+      for (int q = 0; q < RPW; ++q) {
+        const int ox = oxq[q], oy = oyq[q];
+        cur[q].u = U[ox + pk];
+        cur[q].rw = R[ox + pk];
+        cur[q].re = R[ox + 1 + pk];
+        cur[q].v = V[oy + pk];
+        cur[q].rs = R[oy + pk];
+        cur[q].rn = R[oy + d.NI + pk];
+      }
+    }
     const int k0 = k - 1;
     const int kp1 = min(k + 1, nk), kp2 = min(k + 2, nk);
     double rhov[RPW], xf64v[RPW], dtrv[RPW];   // dtrv: FP64 groups, dtime/rho_dzt
@@ -353,19 +372,17 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
       const bool face = rowok[q] && lane >= 1 && lane <= RX - 3 && gi >= -1 && gi <= d.ni - 1;
       if constexpr (EXACT) face_depths(q, kx0[q], kx1[q], kx2[q], ky0[q], ky1[q], ky2[q]);
       const TT mi = mk<TT>(k, km[q]);
-      const TT mX2 = mk<TT>(k, kx2[q]), mX1 = mk<TT>(k, kx1[q]), mX0 = mk<TT>(k, kx0[q]);
+      const TT mX1 = mk<TT>(k, kx1[q]);
       mdfl64::Face cx{0.0, 0.0, 0.0, 0.0, 0.0};
       double Mx = 0.0;
       FaceC<float> cxf{0.0f, 0.0f, 0.0f, 0.0f};
       if (face) {
-        const int of = d2(d, gi, sjr[q]);
         if constexpr (EXACT) {
           const double u = cur[q].u, rw = cur[q].rw, re = cur[q].re;
           Mx = hdyte[q] * u;
           cx = mdfl64::face(Mx, mdfl64::cfl_h(u, dtime, rw + re, hdxte[q]), (double)mX1);
         } else {
-          const double u = U[of + P2 * k0];
-          const double rw = R[of + P2 * k0], re = R[of + 1 + P2 * k0];
+          const double u = cur[q].u, rw = cur[q].rw, re = cur[q].re;
           const float uf = (float)u;
           cxf = face_coef_f(mdyte[q] * uf, fabsf(uf * (float)(dtime * 2.0) *
                                                  rcpf(((float)rw + (float)re) * mdxte[q])));
@@ -442,14 +459,12 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
       mdfl64::Face cy{0.0, 0.0, 0.0, 0.0, 0.0};
       FaceC<float> cyf{0.0f, 0.0f, 0.0f, 0.0f};
       if (face) {
-        const int of = d2(d, gi, gj);
         if constexpr (EXACT) {
           const double v = cur[q].v, rs = cur[q].rs, rn = cur[q].rn;
           cy = mdfl64::face(hdxtn[q] * v, mdfl64::cfl_h(v, dtime, rs + rn, hdytn[q]),
                             (double)mY1);
         } else {
-          const double v = V[of + P2 * k0];
-          const double rs = R[of + P2 * k0], rn = R[of + d.NI + P2 * k0];
+          const double v = cur[q].v, rs = cur[q].rs, rn = cur[q].rn;
           const float vf = (float)v;
           cyf = face_coef_f(mdxtn[q] * vf, fabsf(vf * (float)(dtime * 2.0) *
                                                  rcpf(((float)rs + (float)rn) * mdytn[q])));
@@ -500,7 +515,7 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
         // factors are formed in FP64 without division, so every cell's content
         // weight rho*dat*factor is dtime to 1e-16 and the fluxes telescope:
         // what is left is the unbiased rounding of the FP32 result.
-        const float a32 = (float)(rm1 * rr), c32 = (float)(dtime * rr);
+        const float a32 = (float)(rm1 * rr);
         double f64 = 0.0;
         if constexpr (!EXACT) f64 = dtime * rr * mdatr[q] * mi;
 #pragma unroll
