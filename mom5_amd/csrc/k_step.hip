@@ -449,6 +449,16 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
     __syncthreads();
 
     // ---- Y ----------------------------------------------------------------
+    // FP32 groups: the update's rho_dzt(taum1) and 1/rho_dzt(taup1) (or
+    // rho_dzt(taup1)), issued now that the x phase's values are dead
+    if constexpr (!EXACT) {
+      const size_t pk = (size_t)P2 * k0;
+#pragma unroll
+      for (int q = 0; q < RPW; ++q) {
+        cur[q].rm1 = Rm1[offz[q] + pk];
+        cur[q].rr = Rr ? Rr[offz[q] + pk] : R[offz[q] + pk];
+      }
+    }
 #pragma unroll
     for (int q = 0; q < RPW; ++q) {
       const int ry = wave + NW * q;
@@ -504,9 +514,8 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
         const double r = rhov[q];
         const double mi = mk<double>(k, km[q]);
         const double brk = wv[q] - wkm1[q] + datrv[q] * ((double)mfw[q] - (double)mfe[q]);
-        const double rm1 = EXACT ? cur[q].rm1 : Rm1[o];
-        const double rr = Rr ? (EXACT ? cur[q].rr : Rr[o])
-                             : 1.0 / ((EXACT ? cur[q].rr : R[o]) + EPSLN);
+        const double rm1 = cur[q].rm1;
+        const double rr = Rr ? cur[q].rr : 1.0 / (cur[q].rr + EPSLN);
         // FP32: T(taup1) = T*(rho_taum1/rho_taup1) + dtime*datr/rho_taup1
         //                  * (sum of the face flux differences) + th*dtime/rho_taup1.
         // The update is rebuilt from the limited fluxes alone (flux form); the
