@@ -115,7 +115,8 @@ int mom_gpu_copy(mom_gpu_ctx_t *ctx, double *dst, const double *src, size_t n);
  * results on vdiff_solver).  Knobs:
  *   "barotropic_subcycles_per_launch"  0: three launches per subcycle;
  *                                      n>0: n subcycles per launch (1..12)
- *   "mdfl_rows_per_wave"               2 or 4: tile height of the MDFL kernel
+ *   "mdfl_rows_per_wave"               1, 2 or 4: rows a wavefront of the MDFL kernel
+ *                                      (16 wavefronts with 1 row, 8 otherwise)
  *   "step_group_f64"                   1, 2 or 4 FP64 tracers per batched pass
  *                                      (4 with 1 row x 16 or 2 x 8 wavefronts)
  *   "step_group_f32"                   1, 2, 4, 5, 6, 8 or 10 FP32 tracers per
@@ -165,7 +166,7 @@ int mom_gpu_copy(mom_gpu_ctx_t *ctx, double *dst, const double *src, size_t n);
  *                                      mom_gpu_mark first wait for that
  *                                      stream, so results are the same.
  * Defaults (the fastest measured at 3600x2700x75 with T, S and ten FP32
- * tracers): 5, 4, 2, 10, 1, 1, 16, 16, 1, 256, 0, 1, 1 in the order above.  Unknown knobs and
+ * tracers): 5, 1, 2, 10, 1, 1, 16, 16, 1, 256, 0, 1, 1 in the order above.  Unknown knobs and
  * out-of-range values return MOM_GPU_ERR_ARG. */
 int mom_gpu_set_tuning(mom_gpu_ctx_t *ctx, const char *knob, int value);
 int mom_gpu_get_tuning(const mom_gpu_ctx_t *ctx, const char *knob, int *value);

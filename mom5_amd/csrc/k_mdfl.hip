@@ -30,7 +30,8 @@ namespace {
 
 constexpr int RX = 64;            // region width (one wavefront)
 constexpr int TX = RX - 4;        // interior tile width
-constexpr int NW = 8;             // wavefronts per work-group
+// wavefronts per work-group: 8 with 2 or 4 region rows a wavefront, 16 with 1
+constexpr int nw_of(int rpw) { return rpw == 1 ? 16 : 8; }
 constexpr double ONESIXTH = 1.0 / 6.0;   // ocean_parameters.F90:92
 constexpr double EPS30 = 1.0e-30;
 
@@ -68,7 +69,7 @@ __device__ inline double face_flux(double Rjm, double Rj, double Rjp,
 // LIM1: sweby_limiter = 1 (the limited flux of mdfl64.h); otherwise the
 // quotient form with the limiter weight lim
 template <int FLAVOR, int RPW, bool LIM1>
-__global__ __launch_bounds__(RX * NW) void k_mdfl(
+__global__ __launch_bounds__(RX * nw_of(RPW)) void k_mdfl(
     KDom d, const double *__restrict__ T, const double *__restrict__ U,
     const double *__restrict__ V, const double *__restrict__ W,
     const double *__restrict__ R, const int *__restrict__ kmt,
@@ -79,6 +80,7 @@ __global__ __launch_bounds__(RX * NW) void k_mdfl(
     double *__restrict__ wrk1, double *__restrict__ fxo,
     double *__restrict__ fyo, double *__restrict__ fzo, int ntx, int ntiles,
     int per_xcd) {
+  constexpr int NW = nw_of(RPW);
   constexpr int RY = NW * RPW;
   constexpr int TY = RY - 4;
   __shared__ double s_tx[RY][RX];
@@ -144,11 +146,52 @@ __global__ __launch_bounds__(RX * NW) void k_mdfl(
   }
   __syncthreads();
 
+  // The faces' offsets (0 where a face does not exist) and metrics, and a
+  // level's 3-D loads issued when the level starts (u and the east face's two
+  // rho_dzt, v and the north face's two, th_tendency of the point), so that
+  // the x, y and final phases, each after a barrier, do not each wait for
+  // their own loads with every wavefront of the work-group at once.  The
+  // loads are unconditional (a value of a face that does not exist is
+  // unused), so that no branch merge waits for them.
+  int oxq[RPW], oyq[RPW];
+  double mdyte[RPW], mdxte[RPW], mdxtn[RPW], mdytn[RPW];
+#pragma unroll
+  for (int q = 0; q < RPW; ++q) {
+    const int ry = wave + NW * q;
+    const int gj = gjr[q];
+    const bool fx = rowok[q] && lane >= 1 && lane <= RX - 3 && gi >= -1 && gi <= d.ni - 1;
+    const bool fy = ry >= 1 && ry <= RY - 3 && lane >= 2 && lane < RX - 2 && gi < d.ni &&
+                    gj >= -1 && gj <= d.nj - 1;
+    oxq[q] = fx ? d2(d, gi, sjr[q]) : 0;
+    oyq[q] = fy ? d2(d, gi, gj) : 0;
+    mdyte[q] = dyte[oxq[q]];
+    mdxte[q] = dxte[oxq[q]];
+    mdxtn[q] = dxtn[oyq[q]];
+    mdytn[q] = dytn[oyq[q]];
+  }
+
   for (int k = 1; k <= nk; ++k) {
     const int k0 = k - 1;
     const int kp1 = min(k + 1, nk), kp2 = min(k + 2, nk), km1 = max(k - 1, 1);
     double tz[RPW], rhov[RPW], dtrv[RPW], wv[RPW], fbtv[RPW], txv[RPW], mfe[RPW], mfw[RPW];
     double fxv[RPW], xfv[RPW];
+    // (4-row shapes have no registers for them and load in the phases)
+    constexpr bool EARLY = RPW <= 2;
+    const size_t pk = (size_t)P2 * k0;
+    double lu[RPW], lrw[RPW], lre[RPW], lv[RPW], lrs[RPW], lrn[RPW], lth[RPW];
+    if constexpr (EARLY) {
+#pragma unroll
+      for (int q = 0; q < RPW; ++q) {
+        const int ox = oxq[q], oy = oyq[q];
+        lu[q] = U[ox + pk];
+        lrw[q] = R[ox + pk];
+        lre[q] = R[ox + 1 + pk];
+        lv[q] = V[oy + pk];
+        lrs[q] = R[oy + pk];
+        lrn[q] = R[oy + d.NI + pk];
+        lth[q] = th[offz[q] + pk];
+      }
+    }
 
     // ---- Z: flux through the bottom face and the z-sweep update ----------
 #pragma unroll
@@ -185,19 +228,19 @@ __global__ __launch_bounds__(RX * NW) void k_mdfl(
 #pragma unroll
     for (int q = 0; q < RPW; ++q) {
       const int ry = wave + NW * q;
-      const double tzm1 = __shfl(tz[q], lane - 1);
-      const double tzp1 = __shfl(tz[q], lane + 1);
-      const double tzp2 = __shfl(tz[q], lane + 2);
-      const int kmm1 = __shfl(km[q], lane - 1);
-      const int kmp1 = __shfl(km[q], lane + 1);
-      const int kmp2 = __shfl(km[q], lane + 2);
+      const double tzm1 = lane_prev(tz[q]);
+      const double tzp1 = lane_next(tz[q]);
+      const double tzp2 = lane_next(tzp1);
+      const int kmm1 = dpp_prev(km[q]);
+      const int kmp1 = dpp_next(km[q]);
+      const int kmp2 = dpp_next(kmp1);
       double fx = 0.0, mf = 0.0;
       if (rowok[q] && lane >= 1 && lane <= RX - 3 && gi >= -1 && gi <= d.ni - 1) {
-        const int of = d2(d, gi, sjr[q]);
-        const double u = U[of + P2 * k0];
-        mf = dyte[of] * u;
-        const double rw = R[of + P2 * k0], re = R[of + 1 + P2 * k0];
-        const double cfl = mdfl64::cfl_h(u, dtime, rw + re, dxte[of]);
+        const int ox = oxq[q];
+        const double u = EARLY ? lu[q] : U[ox + pk];
+        mf = mdyte[q] * u;
+        const double rw = EARLY ? lrw[q] : R[ox + pk], re = EARLY ? lre[q] : R[ox + 1 + pk];
+        const double cfl = mdfl64::cfl_h(u, dtime, rw + re, mdxte[q]);
         const double mi = mk(k, km[q]), mim1 = mk(k, kmm1);
         const double mip1 = mk(k, kmp1), mip2 = mk(k, kmp2);
         const double Rjp = (tzp2 - tzp1) * mip2 * mip1;
@@ -206,8 +249,8 @@ __global__ __launch_bounds__(RX * NW) void k_mdfl(
         fx = limited ? mdfl64::flux(mdfl64::face(mf, cfl, mi * mip1), Rjm, Rj, Rjp, tz[q], tzp1)
                      : face_flux<FLAVOR>(Rjm, Rj, Rjp, mf, cfl, tz[q], tzp1, mi, mip1, lim);
       }
-      const double fxw = __shfl(fx, lane - 1);
-      const double mw = __shfl(mf, lane - 1);
+      const double fxw = lane_prev(fx);
+      const double mw = lane_prev(mf);
       double tx = 0.0, xf = 0.0;
       if (rowok[q] && lane >= 2 && lane < RX - 2 && gi < d.ni) {
         const double mi = mk(k, km[q]);
@@ -233,11 +276,12 @@ __global__ __launch_bounds__(RX * NW) void k_mdfl(
       double fy = 0.0;
       if (ry >= 1 && ry <= RY - 3 && lane >= 2 && lane < RX - 2 && gi < d.ni &&
           gj >= -1 && gj <= d.nj - 1) {
-        const int of = d2(d, gi, gj);
-        const double v = V[of + P2 * k0];
-        const double mf = dxtn[of] * v;
-        const double rs = R[of + P2 * k0], rn = R[of + d.NI + P2 * k0];
-        const double cfl = mdfl64::cfl_h(v, dtime, rs + rn, dytn[of]);
+        const int oy = oyq[q];
+        const double v = EARLY ? lv[q] : V[oy + pk];
+        const double mf = mdxtn[q] * v;
+        const double rs = EARLY ? lrs[q] : R[oy + pk];
+        const double rn = EARLY ? lrn[q] : R[oy + d.NI + pk];
+        const double cfl = mdfl64::cfl_h(v, dtime, rs + rn, mdytn[q]);
         const double tjm1 = s_tx[ry - 1][lane], tj = s_tx[ry][lane];
         const double tjp1 = s_tx[ry + 1][lane], tjp2 = s_tx[ry + 2][lane];
         const double mjm1 = mk(k, s_km[ry - 1][lane]), mj = mk(k, s_km[ry][lane]);
@@ -276,7 +320,7 @@ __global__ __launch_bounds__(RX * NW) void k_mdfl(
           w1 = mdfl64::tendency(r, tr, Tc, rdt, mi);
         }
         const int o = offz[q] + P2 * k0;
-        th[o] = th[o] + w1;
+        th[o] = (EARLY ? lth[q] : th[o]) + w1;
         if (wrk1) wrk1[o] = w1;
         if (fzo) fzo[o] = fbtv[q];
         if (fxo) fxo[o] = fxv[q];
@@ -306,6 +350,7 @@ template <int FLAVOR, int RPW, bool LIM1>
 static void launch_mdfl_t(mom_gpu_ctx *ctx, const double *T, const double *u, const double *v,
                           const double *w, const double *rho, double dtime, double lim,
                           double *th, double *wrk1, double *fx, double *fy, double *fz) {
+  constexpr int NW = nw_of(RPW);
   constexpr int TY = NW * RPW - 4;
   const KDom &d = ctx->kd;
   const int ntx = (d.ni + TX - 1) / TX;
@@ -324,7 +369,13 @@ int launch_mdfl(mom_gpu_ctx *ctx, int flavor, const double *T,
                 double *wrk1, double *fx, double *fy, double *fz) {
   const int rpw = ctx->tune.mdfl_rows_per_wave;
   const bool lim1 = lim == 1.0;
-  if (flavor == 0 && lim1 && rpw == 2)
+  if (flavor == 0 && lim1 && rpw == 1)
+    launch_mdfl_t<0, 1, true>(ctx, T, u, v, w, rho, dtime, lim, th, wrk1, fx, fy, fz);
+  else if (flavor == 0 && rpw == 1)
+    launch_mdfl_t<0, 1, false>(ctx, T, u, v, w, rho, dtime, lim, th, wrk1, fx, fy, fz);
+  else if (flavor == 1 && rpw == 1)
+    launch_mdfl_t<1, 1, true>(ctx, T, u, v, w, rho, dtime, lim, th, wrk1, fx, fy, fz);
+  else if (flavor == 0 && lim1 && rpw == 2)
     launch_mdfl_t<0, 2, true>(ctx, T, u, v, w, rho, dtime, lim, th, wrk1, fx, fy, fz);
   else if (flavor == 0 && lim1)
     launch_mdfl_t<0, 4, true>(ctx, T, u, v, w, rho, dtime, lim, th, wrk1, fx, fy, fz);

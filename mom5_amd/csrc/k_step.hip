@@ -94,6 +94,28 @@ __device__ inline float flux_fast(const FaceC<float> &c, const FaceW &w, float R
   return fmaf(w.mp, Tpos + pP, w.mn * (Tneg - pM));
 }
 
+// Level-sliced addressing: a buffer descriptor of the level (the array plus
+// the level's offset, in scalar registers) and a 32-bit byte offset of the
+// point within the level, so that no lane carries a 64-bit address.  A level
+// of the largest domain is far below 4 GiB.
+__device__ inline __amdgpu_buffer_rsrc_t lvl_rsrc(const void *p) {
+  // raw buffer: stride 0, 4 GiB of records, 32-bit data format (CDNA)
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, -1, 0x00020000);
+}
+__device__ inline float ld_lv(const float *base, unsigned boff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(lvl_rsrc(base), (int)boff, 0, 0));
+}
+__device__ inline double ld_lv(const double *base, unsigned boff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(lvl_rsrc(base), (int)boff, 0, 0));
+}
+__device__ inline void st_lv(float *base, unsigned boff, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), lvl_rsrc(base), (int)boff, 0, 0);
+}
+__device__ inline void st_lv(double *base, unsigned boff, double v) {
+  using V2 = unsigned __attribute__((ext_vector_type(2)));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), lvl_rsrc(base), (int)boff, 0, 0);
+}
+
 template <typename TT>
 struct StepTab {
   const TT *T[12];      // tracer at taum1 (= tau)
@@ -267,21 +289,31 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
       hdytn[q] = dytn[oyq[q]];
     }
   }
+  // byte offsets within a level: of the point (FP64 fields and tracers), of
+  // its east face and of its north face (FP64 fields)
+  unsigned bz[RPW], bt[RPW], bx[RPW], by[RPW];
+#pragma unroll
+  for (int q = 0; q < RPW; ++q) {
+    bz[q] = (unsigned)offz[q] * 8u;
+    bt[q] = (unsigned)offz[q] * (unsigned)sizeof(TT);
+    bx[q] = (unsigned)oxq[q] * 8u;
+    by[q] = (unsigned)oyq[q] * 8u;
+  }
+  const double *Rx = Rr ? Rr : R;
   auto lv_load = [&](int k, LvIn *L) {
     const size_t pk = (size_t)P2 * (k - 1);
 #pragma unroll
     for (int q = 0; q < RPW; ++q) {
-      const int o = offz[q], ox = oxq[q], oy = oyq[q];
-      L[q].w = W[o + pk + P2];
-      L[q].r = R[o + pk];
-      L[q].u = U[ox + pk];
-      L[q].rw = R[ox + pk];
-      L[q].re = R[ox + 1 + pk];
-      L[q].v = V[oy + pk];
-      L[q].rs = R[oy + pk];
-      L[q].rn = R[oy + d.NI + pk];
-      L[q].rm1 = Rm1[o + pk];
-      L[q].rr = Rr ? Rr[o + pk] : R[o + pk];
+      L[q].w = ld_lv(W + pk + P2, bz[q]);
+      L[q].r = ld_lv(R + pk, bz[q]);
+      L[q].u = ld_lv(U + pk, bx[q]);
+      L[q].rw = ld_lv(R + pk, bx[q]);
+      L[q].re = ld_lv(R + pk + 1, bx[q]);
+      L[q].v = ld_lv(V + pk, by[q]);
+      L[q].rs = ld_lv(R + pk, by[q]);
+      L[q].rn = ld_lv(R + pk + d.NI, by[q]);
+      L[q].rm1 = ld_lv(Rm1 + pk, bz[q]);
+      L[q].rr = ld_lv(Rx + pk, bz[q]);
     }
   };
 
@@ -290,18 +322,19 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
     if constexpr (EXACT) {
       lv_load(k, cur);
     } else {
-      // FP32 groups (registers for six more values, not ten): the x and y
-      // phases' loads
+      // FP32 groups (registers for eight more values, not ten): the z, x and
+      // y phases' loads
       const size_t pk = (size_t)P2 * (k - 1);
 #pragma unroll
       for (int q = 0; q < RPW; ++q) {
-        const int ox = oxq[q], oy = oyq[q];
-        cur[q].u = U[ox + pk];
-        cur[q].rw = R[ox + pk];
-        cur[q].re = R[ox + 1 + pk];
-        cur[q].v = V[oy + pk];
-        cur[q].rs = R[oy + pk];
-        cur[q].rn = R[oy + d.NI + pk];
+        cur[q].w = ld_lv(W + pk + P2, bz[q]);
+        cur[q].r = ld_lv(R + pk, bz[q]);
+        cur[q].u = ld_lv(U + pk, bx[q]);
+        cur[q].rw = ld_lv(R + pk, bx[q]);
+        cur[q].re = ld_lv(R + pk + 1, bx[q]);
+        cur[q].v = ld_lv(V + pk, by[q]);
+        cur[q].rs = ld_lv(R + pk, by[q]);
+        cur[q].rn = ld_lv(R + pk + d.NI, by[q]);
       }
     }
     const int k0 = k - 1;
@@ -317,47 +350,49 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
 #pragma unroll
     for (int q = 0; q < RPW; ++q) {
       rhov[q] = 1.0; wv[q] = RT(0); rinv[q] = 1.0f; dtrv[q] = 0.0;
-      if (ok[q]) {
-        const int o = offz[q];
-        const double w = EXACT ? cur[q].w : W[o + P2 * k];
-        const double r = EXACT ? cur[q].r : R[o + P2 * k0];
+      // mk(km1)*mk(k) = mk(k), mk(k)*mk(kp1) = mk(kp1), mk(kp1)*mk(kp2) =
+      // mk(kp2): km1 <= k <= kp1 <= kp2 and masks in {0, 1}
+      const TT mA = mk<TT>(k, km[q]), mB = mk<TT>(kp1, km[q]), mC = mk<TT>(kp2, km[q]);
+      if constexpr (!EXACT) {
+        // FP32 groups take no branch: at a point outside the domain (t, dat,
+        // datr and the masks 0) w = 0 and rho_dzt = 1 keep every value
+        // finite, its tz 0 and its flux a zero
+        const double w = ok[q] ? cur[q].w : 0.0;
+        const double r = ok[q] ? cur[q].r : 1.0;
         rhov[q] = r; wv[q] = (RT)w;
-        // mk(km1)*mk(k) = mk(k), mk(k)*mk(kp1) = mk(kp1), mk(kp1)*mk(kp2) =
-        // mk(kp2): km1 <= k <= kp1 <= kp2 and masks in {0, 1}
-        const TT mA = mk<TT>(k, km[q]), mB = mk<TT>(kp1, km[q]), mC = mk<TT>(kp2, km[q]);
-        if constexpr (EXACT) {
-          const double dtr = dtime * mdfl64::rcp64(r);
-          dtrv[q] = dtr;
-          const mdfl64::Face cz = mdfl64::face(datv[q] * w, mdfl64::cfl_z(w, dtr), mB);
-          const double wdiff = wkm1[q] - w;
+        const float ri = rcpf((float)r);
+        rinv[q] = ri;
+        const float wf = (float)w;
+        const FaceC<float> c = face_coef_f((float)datv[q] * wf, fabsf(wf * (float)dtime * ri));
+        const FaceW wz = face_w(c, mB);
+        const float dtrf = (float)dtime * ri, datrf = (float)datrv[q];
+        const float wdiff = (float)wkm1[q] - wf;
 #pragma unroll
-          for (int g = 0; g < G; ++g) {
-            const double Rjp = (t0[g][q] - t1[g][q]) * mA;
-            const double Rj = (t1[g][q] - t2[g][q]) * mB;
-            const double Rjm = (t2[g][q] - t3[g][q]) * mC;
-            const double fbt = mdfl64::flux(cz, Rjm, Rj, Rjp, t2[g][q], t1[g][q]);
-            tz[g][q] = t1[g][q] + dtr * (datrv[q] * (fbt - ftp[g][q]) + t1[g][q] * wdiff);
-            fbtv[g][q] = fbt;
-          }
-        } else {
-          const float ri = rcpf((float)r);
-          rinv[q] = ri;
-          const float wf = (float)w;
-          const FaceC<float> c = face_coef_f((float)datv[q] * wf, fabsf(wf * (float)dtime * ri));
-          const FaceW wz = face_w(c, mB);
-          const float dtrf = (float)dtime * ri, datrf = (float)datrv[q];
-          const float wdiff = (float)wkm1[q] - wf;
+        // where the face is wet (mB = 1) so is the cell above (mA = 1);
+        // where it is dry its weights are 0: only mC needs applying
+        for (int g = 0; g < G; ++g) {
+          const float Rjp = t0[g][q] - t1[g][q];
+          const float Rj = t1[g][q] - t2[g][q];
+          const float Rjm = (t2[g][q] - t3[g][q]) * mC;
+          const float fbt = flux_fast(c, wz, Rjm, Rj, Rjp, t2[g][q], t1[g][q]);
+          tz[g][q] = fmaf(dtrf, fmaf(datrf, fbt - ftp[g][q], t1[g][q] * wdiff), t1[g][q]);
+          fbtv[g][q] = fbt;
+        }
+      } else if (ok[q]) {
+        const double w = cur[q].w, r = cur[q].r;
+        rhov[q] = r; wv[q] = (RT)w;
+        const double dtr = dtime * mdfl64::rcp64(r);
+        dtrv[q] = dtr;
+        const mdfl64::Face cz = mdfl64::face(datv[q] * w, mdfl64::cfl_z(w, dtr), mB);
+        const double wdiff = wkm1[q] - w;
 #pragma unroll
-          // where the face is wet (mB = 1) so is the cell above (mA = 1);
-          // where it is dry its weights are 0: only mC needs applying
-          for (int g = 0; g < G; ++g) {
-            const float Rjp = t0[g][q] - t1[g][q];
-            const float Rj = t1[g][q] - t2[g][q];
-            const float Rjm = (t2[g][q] - t3[g][q]) * mC;
-            const float fbt = flux_fast(c, wz, Rjm, Rj, Rjp, t2[g][q], t1[g][q]);
-            tz[g][q] = fmaf(dtrf, fmaf(datrf, fbt - ftp[g][q], t1[g][q] * wdiff), t1[g][q]);
-            fbtv[g][q] = fbt;
-          }
+        for (int g = 0; g < G; ++g) {
+          const double Rjp = (t0[g][q] - t1[g][q]) * mA;
+          const double Rj = (t1[g][q] - t2[g][q]) * mB;
+          const double Rjm = (t2[g][q] - t3[g][q]) * mC;
+          const double fbt = mdfl64::flux(cz, Rjm, Rj, Rjp, t2[g][q], t1[g][q]);
+          tz[g][q] = t1[g][q] + dtr * (datrv[q] * (fbt - ftp[g][q]) + t1[g][q] * wdiff);
+          fbtv[g][q] = fbt;
         }
       } else {
 #pragma unroll
@@ -376,17 +411,17 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
       mdfl64::Face cx{0.0, 0.0, 0.0, 0.0, 0.0};
       double Mx = 0.0;
       FaceC<float> cxf{0.0f, 0.0f, 0.0f, 0.0f};
-      if (face) {
-        if constexpr (EXACT) {
-          const double u = cur[q].u, rw = cur[q].rw, re = cur[q].re;
-          Mx = hdyte[q] * u;
-          cx = mdfl64::face(Mx, mdfl64::cfl_h(u, dtime, rw + re, hdxte[q]), (double)mX1);
-        } else {
-          const double u = cur[q].u, rw = cur[q].rw, re = cur[q].re;
-          const float uf = (float)u;
-          cxf = face_coef_f(mdyte[q] * uf, fabsf(uf * (float)(dtime * 2.0) *
-                                                 rcpf(((float)rw + (float)re) * mdxte[q])));
-        }
+      if constexpr (!EXACT) {
+        // FP32 groups take no branch: where there is no face its metrics
+        // are 0, so its mass flux is 0 and its CFL number is set to 0
+        const float uf = (float)cur[q].u;
+        const float cfl = fabsf(uf * (float)(dtime * 2.0) *
+                                rcpf(((float)cur[q].rw + (float)cur[q].re) * mdxte[q]));
+        cxf = face_coef_f(mdyte[q] * uf, face ? cfl : 0.0f);
+      } else if (face) {
+        const double u = cur[q].u, rw = cur[q].rw, re = cur[q].re;
+        Mx = hdyte[q] * u;
+        cx = mdfl64::face(Mx, mdfl64::cfl_h(u, dtime, rw + re, hdxte[q]), (double)mX1);
       }
       const bool cell = rowok[q] && lane >= 2 && lane < RX - 2 && gi < d.ni;
       double mw64 = 0.0;
@@ -416,16 +451,14 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
         const TT Rjl = (tzp1 - tz[g][q]) * mX1;
         const TT Rjp = lane_next(Rjl);
         const TT Rjm = lane_prev(Rjl);
+        // FP32: the unmasked difference would do (the face weights carry
+        // mX1), the masked one gives the same flux; where there is no face
+        // the weights are 0 and the flux a zero
         TT fx = TT(0);
-        if (face) {
-          // FP32: the unmasked difference would do (the face weights carry
-          // mX1), the masked one gives the same flux
-          const TT Rj = Rjl;
-          if constexpr (EXACT)
-            fx = mdfl64::flux(cx, Rjm, Rj, Rjp, tz[g][q], tzp1);
-          else
-            fx = flux_fast(cxf, wx, Rjm, Rj, Rjp, tz[g][q], tzp1);
-        }
+        if constexpr (!EXACT)
+          fx = flux_fast(cxf, wx, Rjm, Rjl, Rjp, tz[g][q], tzp1);
+        else if (face)
+          fx = mdfl64::flux(cx, Rjm, Rjl, Rjp, tz[g][q], tzp1);
         const TT fxw = lane_prev(fx);
         if constexpr (EXACT) {
           dxf[g][q] = fxw - fx;
@@ -435,13 +468,13 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
           dxf[g][q] = (fbtv[g][q] - ftp[g][q]) + (fxw - fx);
           ftp[g][q] = fbtv[g][q];
         }
+        // FP32: formed everywhere (outside the domain xfac = 0 and tz = 0;
+        // the halo columns' values are not read)
         TT tx = TT(0);
-        if (cell) {
-          if constexpr (EXACT)
-            tx = tz[g][q] + xf64v[q] * (fxw - fx + t1[g][q] * (Mx - mw64));
-          else
-            tx = fmaf(xfac[q], fmaf(t1[g][q], mfe[q] - mfw[q], fxw - fx), tz[g][q]);
-        }
+        if constexpr (!EXACT)
+          tx = fmaf(xfac[q], fmaf(t1[g][q], mfe[q] - mfw[q], fxw - fx), tz[g][q]);
+        else if (cell)
+          tx = tz[g][q] + xf64v[q] * (fxw - fx + t1[g][q] * (Mx - mw64));
         s_tx[g][ry][lane] = tx;
         txv[g][q] = tx;
       }
@@ -455,8 +488,8 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
       const size_t pk = (size_t)P2 * k0;
 #pragma unroll
       for (int q = 0; q < RPW; ++q) {
-        cur[q].rm1 = Rm1[offz[q] + pk];
-        cur[q].rr = Rr ? Rr[offz[q] + pk] : R[offz[q] + pk];
+        cur[q].rm1 = ld_lv(Rm1 + pk, bz[q]);
+        cur[q].rr = ld_lv(Rx + pk, bz[q]);
       }
     }
 #pragma unroll
@@ -468,33 +501,39 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
       const TT mY2 = mk<TT>(k, ky2[q]), mY1 = mk<TT>(k, ky1[q]), mY0 = mk<TT>(k, ky0[q]);
       mdfl64::Face cy{0.0, 0.0, 0.0, 0.0, 0.0};
       FaceC<float> cyf{0.0f, 0.0f, 0.0f, 0.0f};
-      if (face) {
-        if constexpr (EXACT) {
-          const double v = cur[q].v, rs = cur[q].rs, rn = cur[q].rn;
-          cy = mdfl64::face(hdxtn[q] * v, mdfl64::cfl_h(v, dtime, rs + rn, hdytn[q]),
-                            (double)mY1);
-        } else {
-          const double v = cur[q].v, rs = cur[q].rs, rn = cur[q].rn;
-          const float vf = (float)v;
-          cyf = face_coef_f(mdxtn[q] * vf, fabsf(vf * (float)(dtime * 2.0) *
-                                                 rcpf(((float)rs + (float)rn) * mdytn[q])));
-        }
+      if constexpr (!EXACT) {
+        // FP32 groups take no branch (as in the x sweep)
+        const float vf = (float)cur[q].v;
+        const float cfl = fabsf(vf * (float)(dtime * 2.0) *
+                                rcpf(((float)cur[q].rs + (float)cur[q].rn) * mdytn[q]));
+        cyf = face_coef_f(mdxtn[q] * vf, face ? cfl : 0.0f);
+      } else if (face) {
+        const double v = cur[q].v, rs = cur[q].rs, rn = cur[q].rn;
+        cy = mdfl64::face(hdxtn[q] * v, mdfl64::cfl_h(v, dtime, rs + rn, hdytn[q]),
+                          (double)mY1);
       }
       FaceW wy{0.0f, 0.0f};
       if constexpr (!EXACT) wy = face_w(cyf, mY1);
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         TT fy = TT(0);
-        if (face) {
+        if constexpr (!EXACT) {
+          // rows clamped to the region: a row without a face reads values
+          // that its zero weights discard
+          const TT tjm1 = s_tx[g][max(ry - 1, 0)][lane], tj = s_tx[g][ry][lane];
+          const TT tjp1 = s_tx[g][min(ry + 1, RY - 1)][lane];
+          const TT tjp2 = s_tx[g][min(ry + 2, RY - 1)][lane];
+          const TT Rjp = (tjp2 - tjp1) * mY2;
+          const TT Rj = tjp1 - tj;   // the face weights carry mY1
+          const TT Rjm = (tj - tjm1) * mY0;
+          fy = flux_fast(cyf, wy, Rjm, Rj, Rjp, tj, tjp1);
+        } else if (face) {
           const TT tjm1 = s_tx[g][ry - 1][lane], tj = s_tx[g][ry][lane];
           const TT tjp1 = s_tx[g][ry + 1][lane], tjp2 = s_tx[g][ry + 2][lane];
           const TT Rjp = (tjp2 - tjp1) * mY2;
-          const TT Rj = EXACT ? (tjp1 - tj) * mY1 : tjp1 - tj;   // FP32: face weights carry mY1
+          const TT Rj = (tjp1 - tj) * mY1;
           const TT Rjm = (tj - tjm1) * mY0;
-          if constexpr (EXACT)
-            fy = mdfl64::flux(cy, Rjm, Rj, Rjp, tj, tjp1);
-          else
-            fy = flux_fast(cyf, wy, Rjm, Rj, Rjp, tj, tjp1);
+          fy = mdfl64::flux(cy, Rjm, Rj, Rjp, tj, tjp1);
         }
         s_fy[g][ry][lane] = fy;
         if constexpr (EXACT) fyv[g][q] = fy;   // FP32 reads it back from LDS
@@ -510,7 +549,7 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
       const bool cell = ry >= 2 && ry < RY - 2 && lane >= 2 && lane < RX - 2 && gi < d.ni &&
                         gj < d.nj;
       if (cell) {
-        const int o = offz[q] + P2 * k0;
+        const size_t pk = (size_t)P2 * k0;
         const double r = rhov[q];
         const double mi = mk<double>(k, km[q]);
         const double brk = wv[q] - wkm1[q] + datrv[q] * ((double)mfw[q] - (double)mfe[q]);
@@ -530,25 +569,28 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
 #pragma unroll
         for (int g = 0; g < G; ++g) {
           if constexpr (EXACT) {
-            const double thin = tab.th[g] ? tab.th[g][o] : 0.0;
+            const double thin = tab.th[g] ? ld_lv(tab.th[g] + pk, bz[q]) : 0.0;
             const double Tc = t1[g][q];
             double tr = txv[g][q] + xf64v[q] * (s_fy[g][ry - 1][lane] - fyv[g][q]);
             tr = tr + dtrv[q] * Tc * brk;
             const double th = thin + mdfl64::tendency(r, tr, Tc, rdt, mi);
-            tab.out[g][o] = (rm1 * Tc + dtime * th) * rr;
+            st_lv(tab.out[g] + pk, bt[q], (rm1 * Tc + dtime * th) * rr);
           } else {
             const float Tc = t1[g][q];
             const float dflux = dxf[g][q] + (s_fy[g][ry - 1][lane] - s_fy[g][ry][lane]);
             // other th_tendency contributions: k_add_th_f32 adds
             // (float)th * c32 to this value afterwards, bit for bit the
             // same as here, so that FP32 passes hold no th pointers
-            tab.out[g][o] = Tc * a32 + (float)(f64 * (double)dflux);
+            st_lv(tab.out[g] + pk, bt[q], Tc * a32 + (float)(f64 * (double)dflux));
           }
         }
       }
     }
 
     // ---- advance the k window -----------------------------------------------
+    // (t3 of a point outside the domain stays 0; the last level reloads its
+    // own, unused)
+    const size_t p3 = (size_t)P2 * (min(k + 3, nk) - 1);
 #pragma unroll
     for (int q = 0; q < RPW; ++q) {
       wkm1[q] = wv[q];
@@ -556,7 +598,10 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
       for (int g = 0; g < G; ++g) {
         if constexpr (EXACT) ftp[g][q] = fbtv[g][q];
         t0[g][q] = t1[g][q]; t1[g][q] = t2[g][q]; t2[g][q] = t3[g][q];
-        if (ok[q] && k < nk) t3[g][q] = tab.T[g][offz[q] + P2 * (min(k + 3, nk) - 1)];
+      }
+      if (ok[q]) {
+#pragma unroll
+        for (int g = 0; g < G; ++g) t3[g][q] = ld_lv(tab.T[g] + p3, bt[q]);
       }
     }
   };

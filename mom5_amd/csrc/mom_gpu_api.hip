@@ -89,7 +89,7 @@ static int *tuning_slot(mom_gpu_ctx *ctx, const char *knob, int *lo, int *hi) {
     return &ctx->tune.bt_subcycles_per_launch;
   }
   if (!std::strcmp(knob, "mdfl_rows_per_wave")) {
-    *lo = 2; *hi = 4;
+    *lo = 1; *hi = 4;
     return &ctx->tune.mdfl_rows_per_wave;
   }
   if (!std::strcmp(knob, "step_group_f64")) {
@@ -146,8 +146,8 @@ int mom_gpu_set_tuning(mom_gpu_ctx_t *ctx, const char *knob, int value) {
   REQUIRE(ctx, slot, "mom_gpu_set_tuning: unknown knob '%s'", knob ? knob : "(null)");
   REQUIRE(ctx, value >= lo && value <= hi, "mom_gpu_set_tuning: %s=%d outside [%d,%d]", knob,
           value, lo, hi);
-  REQUIRE(ctx, slot != &ctx->tune.mdfl_rows_per_wave || value == 2 || value == 4,
-          "mom_gpu_set_tuning: %s must be 2 or 4", knob);
+  REQUIRE(ctx, slot != &ctx->tune.mdfl_rows_per_wave || value == 1 || value == 2 || value == 4,
+          "mom_gpu_set_tuning: %s must be 1, 2 or 4", knob);
   REQUIRE(ctx, (slot != &ctx->tune.step_rows_per_wave_f32 && slot != &ctx->tune.step_rows_per_wave) ||
                    value == 1 || value == 2 || value == 4,
           "mom_gpu_set_tuning: %s must be 1, 2 or 4", knob);
@@ -345,7 +345,7 @@ int mom_gpu_create(const mom_gpu_domain_t *dom, const mom_gpu_grid_t *grid, mom_
   k.cyclic_x = dom->cyclic_x ? 1 : 0;
   k.cyclic_y = dom->cyclic_y ? 1 : 0;
   c->tune.bt_subcycles_per_launch = 5;
-  c->tune.mdfl_rows_per_wave = 4;
+  c->tune.mdfl_rows_per_wave = 1;
   c->tune.step_group_f64 = 2;
   c->tune.step_group_f32 = 10;
   c->tune.step_rows_per_wave = 1;

@@ -86,7 +86,7 @@ def autotune(step, reps=3, barotropic=True):
         th = step.th
         T0 = step.tracer(0)
         report["mdfl"] = _search(
-            g, ("mdfl_rows_per_wave",), [(2,), (4,)],
+            g, ("mdfl_rows_per_wave",), [(1,), (2,), (4,)],
             lambda: g.horz_advect_tracer("mdfl_sweby", T0, th, uhrho_et=d["uhrho_et"],
                                          vhrho_nt=d["vhrho_nt"], wrho_bt=d["wrho_bt"],
                                          rho_dzt_tau=d["rho_dzt"], dtime=c.dt), reps)

@@ -54,7 +54,7 @@ def same(a, b):
                              f"(max abs diff {np.nanmax(np.abs(a - b)):.3e})")
 
 
-@pytest.mark.parametrize("rows_per_wave", [4, 2])
+@pytest.mark.parametrize("rows_per_wave", [4, 2, 1])
 @pytest.mark.parametrize("scheme,lim", [("mdfl_sweby", 1.0), ("dst_linear", 0.0)])
 def test_horz_advect_mdfl(setup, scheme, lim, rows_per_wave):
     from mom5_amd import DeviceArray
