@@ -168,7 +168,8 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
   const int tix = tile % ntx, tiy = tile / ntx;
   const int gi0 = tix * TX, gj0 = tiy * TY;
   const int lane = threadIdx.x & (RX - 1);
-  const int wave = threadIdx.x / RX;
+  // the row of a wavefront, in a scalar register
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / RX);
   const int nk = d.nk, P2 = d.P2;
   const int gi = gi0 - 2 + lane;
 
@@ -496,47 +497,51 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
     for (int q = 0; q < RPW; ++q) {
       const int ry = wave + NW * q;
       const int gj = gjr[q];
-      const bool face = ry >= 1 && ry <= RY - 3 && lane >= 2 && lane < RX - 2 && gi < d.ni &&
-                        gj >= -1 && gj <= d.nj - 1;
       const TT mY2 = mk<TT>(k, ky2[q]), mY1 = mk<TT>(k, ky1[q]), mY0 = mk<TT>(k, ky0[q]);
-      mdfl64::Face cy{0.0, 0.0, 0.0, 0.0, 0.0};
-      FaceC<float> cyf{0.0f, 0.0f, 0.0f, 0.0f};
       if constexpr (!EXACT) {
-        // FP32 groups take no branch (as in the x sweep)
-        const float vf = (float)cur[q].v;
-        const float cfl = fabsf(vf * (float)(dtime * 2.0) *
-                                rcpf(((float)cur[q].rs + (float)cur[q].rn) * mdytn[q]));
-        cyf = face_coef_f(mdxtn[q] * vf, face ? cfl : 0.0f);
-      } else if (face) {
-        const double v = cur[q].v, rs = cur[q].rs, rn = cur[q].rn;
-        cy = mdfl64::face(hdxtn[q] * v, mdfl64::cfl_h(v, dtime, rs + rn, hdytn[q]),
-                          (double)mY1);
-      }
-      FaceW wy{0.0f, 0.0f};
-      if constexpr (!EXACT) wy = face_w(cyf, mY1);
+        // FP32 groups: a row (one wavefront) with no face skips the sweep;
+        // within a row no branch, as in the x sweep.  The update reads the
+        // faces of rows 1 .. RY-3 only.
+        if (ry >= 1 && ry <= RY - 3 && gj >= -1 && gj <= d.nj - 1) {
+          const bool face = lane >= 2 && lane < RX - 2 && gi < d.ni;
+          const float vf = (float)cur[q].v;
+          const float cfl = fabsf(vf * (float)(dtime * 2.0) *
+                                  rcpf(((float)cur[q].rs + (float)cur[q].rn) * mdytn[q]));
+          const FaceC<float> cyf = face_coef_f(mdxtn[q] * vf, face ? cfl : 0.0f);
+          const FaceW wy = face_w(cyf, mY1);
 #pragma unroll
-      for (int g = 0; g < G; ++g) {
-        TT fy = TT(0);
-        if constexpr (!EXACT) {
-          // rows clamped to the region: a row without a face reads values
-          // that its zero weights discard
-          const TT tjm1 = s_tx[g][max(ry - 1, 0)][lane], tj = s_tx[g][ry][lane];
-          const TT tjp1 = s_tx[g][min(ry + 1, RY - 1)][lane];
-          const TT tjp2 = s_tx[g][min(ry + 2, RY - 1)][lane];
-          const TT Rjp = (tjp2 - tjp1) * mY2;
-          const TT Rj = tjp1 - tj;   // the face weights carry mY1
-          const TT Rjm = (tj - tjm1) * mY0;
-          fy = flux_fast(cyf, wy, Rjm, Rj, Rjp, tj, tjp1);
-        } else if (face) {
-          const TT tjm1 = s_tx[g][ry - 1][lane], tj = s_tx[g][ry][lane];
-          const TT tjp1 = s_tx[g][ry + 1][lane], tjp2 = s_tx[g][ry + 2][lane];
-          const TT Rjp = (tjp2 - tjp1) * mY2;
-          const TT Rj = (tjp1 - tj) * mY1;
-          const TT Rjm = (tj - tjm1) * mY0;
-          fy = mdfl64::flux(cy, Rjm, Rj, Rjp, tj, tjp1);
+          for (int g = 0; g < G; ++g) {
+            const TT tjm1 = s_tx[g][ry - 1][lane], tj = s_tx[g][ry][lane];
+            const TT tjp1 = s_tx[g][ry + 1][lane], tjp2 = s_tx[g][ry + 2][lane];
+            const TT Rjp = (tjp2 - tjp1) * mY2;
+            const TT Rj = tjp1 - tj;   // the face weights carry mY1
+            const TT Rjm = (tj - tjm1) * mY0;
+            s_fy[g][ry][lane] = flux_fast(cyf, wy, Rjm, Rj, Rjp, tj, tjp1);
+          }
         }
-        s_fy[g][ry][lane] = fy;
-        if constexpr (EXACT) fyv[g][q] = fy;   // FP32 reads it back from LDS
+      } else {
+        const bool face = ry >= 1 && ry <= RY - 3 && lane >= 2 && lane < RX - 2 && gi < d.ni &&
+                          gj >= -1 && gj <= d.nj - 1;
+        mdfl64::Face cy{0.0, 0.0, 0.0, 0.0, 0.0};
+        if (face) {
+          const double v = cur[q].v, rs = cur[q].rs, rn = cur[q].rn;
+          cy = mdfl64::face(hdxtn[q] * v, mdfl64::cfl_h(v, dtime, rs + rn, hdytn[q]),
+                            (double)mY1);
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          TT fy = TT(0);
+          if (face) {
+            const TT tjm1 = s_tx[g][ry - 1][lane], tj = s_tx[g][ry][lane];
+            const TT tjp1 = s_tx[g][ry + 1][lane], tjp2 = s_tx[g][ry + 2][lane];
+            const TT Rjp = (tjp2 - tjp1) * mY2;
+            const TT Rj = (tjp1 - tj) * mY1;
+            const TT Rjm = (tj - tjm1) * mY0;
+            fy = mdfl64::flux(cy, Rjm, Rj, Rjp, tj, tjp1);
+          }
+          s_fy[g][ry][lane] = fy;
+          fyv[g][q] = fy;
+        }
       }
     }
     __syncthreads();
