@@ -160,8 +160,14 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
   const int ntx = J.ntx;
   constexpr bool EXACT = sizeof(TT) == 8;
   constexpr int RY = NW * RPW;
-  __shared__ TT s_tx[G][RY][RX];
-  __shared__ TT s_fy[G][RY][RX];
+  // FP64 groups keep two copies of the x-swept values and of the y fluxes,
+  // by level parity, where the LDS holds them: one barrier a level, and the
+  // update of a level done after the next level's barrier
+  constexpr bool ONEB =
+      EXACT && 2 * 2 * G * RY * RX * sizeof(TT) + RY * RX * sizeof(int) <= 160 * 1024;
+  constexpr int NB = ONEB ? 2 : 1;
+  __shared__ TT s_tx[NB][G][RY][RX];
+  __shared__ TT s_fy[NB][G][RY][RX];
   __shared__ int s_km[RY][RX];
 
   constexpr int TY = RY - 4;
@@ -303,6 +309,9 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
   const double *Rx = Rr ? Rr : R;
   auto lv_load = [&](int k, LvIn *L) {
     const size_t pk = (size_t)P2 * (k - 1);
+    // the update's rho_dzt(taum1) and 1/rho_dzt(taup1): of the level before
+    // when the update is deferred (unused at the first level)
+    const size_t pu = ONEB ? (size_t)P2 * max(k - 2, 0) : pk;
 #pragma unroll
     for (int q = 0; q < RPW; ++q) {
       L[q].w = ld_lv(W + pk + P2, bz[q]);
@@ -313,12 +322,44 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
       L[q].v = ld_lv(V + pk, by[q]);
       L[q].rs = ld_lv(R + pk, by[q]);
       L[q].rn = ld_lv(R + pk + d.NI, by[q]);
-      L[q].rm1 = ld_lv(Rm1 + pk, bz[q]);
-      L[q].rr = ld_lv(Rx + pk, bz[q]);
+      L[q].rm1 = ld_lv(Rm1 + pu, bz[q]);
+      L[q].rr = ld_lv(Rx + pu, bz[q]);
+    }
+  };
+
+  // Deferred FP64 update of level kf (ONEB): the tracer at kf is t0 (the
+  // window has moved on), the y fluxes are in copy kf & 1, the rest was kept
+  // from level kf in pf_*.  The arithmetic is that of the in-level update.
+  double pf_xf[RPW], pf_dtr[RPW], pf_r[RPW], pf_brk[RPW];
+  TT pf_tx[G][RPW];
+  auto update_deferred = [&](int kf, const LvIn *L) {
+    const int bp = kf & 1;
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+      const int ry = wave + NW * q;
+      const int gj = gjr[q];
+      const bool cell = ry >= 2 && ry < RY - 2 && lane >= 2 && lane < RX - 2 && gi < d.ni &&
+                        gj < d.nj;
+      if (cell) {
+        const size_t pk = (size_t)P2 * (kf - 1);
+        const double mi = mk<double>(kf, km[q]);
+        const double rm1 = L[q].rm1;
+        const double rr = Rr ? L[q].rr : 1.0 / (L[q].rr + EPSLN);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          const double thin = tab.th[g] ? ld_lv(tab.th[g] + pk, bz[q]) : 0.0;
+          const double Tc = t0[g][q];
+          double tr = pf_tx[g][q] + pf_xf[q] * (s_fy[bp][g][ry - 1][lane] - s_fy[bp][g][ry][lane]);
+          tr = tr + pf_dtr[q] * Tc * pf_brk[q];
+          const double th = thin + mdfl64::tendency(pf_r[q], tr, Tc, rdt, mi);
+          st_lv(tab.out[g] + pk, bt[q], (rm1 * Tc + dtime * th) * rr);
+        }
+      }
     }
   };
 
   auto level = [&](const int k) {
+    const int bb = ONEB ? (k & 1) : 0;
     LvIn cur[RPW];
     if constexpr (EXACT) {
       lv_load(k, cur);
@@ -476,11 +517,14 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
           tx = fmaf(xfac[q], fmaf(t1[g][q], mfe[q] - mfw[q], fxw - fx), tz[g][q]);
         else if (cell)
           tx = tz[g][q] + xf64v[q] * (fxw - fx + t1[g][q] * (Mx - mw64));
-        s_tx[g][ry][lane] = tx;
+        s_tx[bb][g][ry][lane] = tx;
         txv[g][q] = tx;
       }
     }
     __syncthreads();
+    if constexpr (ONEB) {
+      if (k > 1) update_deferred(k - 1, cur);
+    }
 
     // ---- Y ----------------------------------------------------------------
     // FP32 groups: the update's rho_dzt(taum1) and 1/rho_dzt(taup1) (or
@@ -511,12 +555,12 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
           const FaceW wy = face_w(cyf, mY1);
 #pragma unroll
           for (int g = 0; g < G; ++g) {
-            const TT tjm1 = s_tx[g][ry - 1][lane], tj = s_tx[g][ry][lane];
-            const TT tjp1 = s_tx[g][ry + 1][lane], tjp2 = s_tx[g][ry + 2][lane];
+            const TT tjm1 = s_tx[bb][g][ry - 1][lane], tj = s_tx[bb][g][ry][lane];
+            const TT tjp1 = s_tx[bb][g][ry + 1][lane], tjp2 = s_tx[bb][g][ry + 2][lane];
             const TT Rjp = (tjp2 - tjp1) * mY2;
             const TT Rj = tjp1 - tj;   // the face weights carry mY1
             const TT Rjm = (tj - tjm1) * mY0;
-            s_fy[g][ry][lane] = flux_fast(cyf, wy, Rjm, Rj, Rjp, tj, tjp1);
+            s_fy[bb][g][ry][lane] = flux_fast(cyf, wy, Rjm, Rj, Rjp, tj, tjp1);
           }
         }
       } else {
@@ -532,18 +576,30 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
         for (int g = 0; g < G; ++g) {
           TT fy = TT(0);
           if (face) {
-            const TT tjm1 = s_tx[g][ry - 1][lane], tj = s_tx[g][ry][lane];
-            const TT tjp1 = s_tx[g][ry + 1][lane], tjp2 = s_tx[g][ry + 2][lane];
+            const TT tjm1 = s_tx[bb][g][ry - 1][lane], tj = s_tx[bb][g][ry][lane];
+            const TT tjp1 = s_tx[bb][g][ry + 1][lane], tjp2 = s_tx[bb][g][ry + 2][lane];
             const TT Rjp = (tjp2 - tjp1) * mY2;
             const TT Rj = (tjp1 - tj) * mY1;
             const TT Rjm = (tj - tjm1) * mY0;
             fy = mdfl64::flux(cy, Rjm, Rj, Rjp, tj, tjp1);
           }
-          s_fy[g][ry][lane] = fy;
+          s_fy[bb][g][ry][lane] = fy;
           fyv[g][q] = fy;
         }
       }
     }
+    if constexpr (ONEB) {
+      // keep what the deferred update of this level needs
+#pragma unroll
+      for (int q = 0; q < RPW; ++q) {
+        pf_xf[q] = xf64v[q];
+        pf_dtr[q] = dtrv[q];
+        pf_r[q] = rhov[q];
+        pf_brk[q] = wv[q] - wkm1[q] + datrv[q] * ((double)mfw[q] - (double)mfe[q]);
+#pragma unroll
+        for (int g = 0; g < G; ++g) pf_tx[g][q] = txv[g][q];
+      }
+    } else {
     __syncthreads();
 
     // ---- final: y update, correction removal, explicit update --------------
@@ -576,13 +632,13 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
           if constexpr (EXACT) {
             const double thin = tab.th[g] ? ld_lv(tab.th[g] + pk, bz[q]) : 0.0;
             const double Tc = t1[g][q];
-            double tr = txv[g][q] + xf64v[q] * (s_fy[g][ry - 1][lane] - fyv[g][q]);
+            double tr = txv[g][q] + xf64v[q] * (s_fy[0][g][ry - 1][lane] - fyv[g][q]);
             tr = tr + dtrv[q] * Tc * brk;
             const double th = thin + mdfl64::tendency(r, tr, Tc, rdt, mi);
             st_lv(tab.out[g] + pk, bt[q], (rm1 * Tc + dtime * th) * rr);
           } else {
             const float Tc = t1[g][q];
-            const float dflux = dxf[g][q] + (s_fy[g][ry - 1][lane] - s_fy[g][ry][lane]);
+            const float dflux = dxf[g][q] + (s_fy[0][g][ry - 1][lane] - s_fy[0][g][ry][lane]);
             // other th_tendency contributions: k_add_th_f32 adds
             // (float)th * c32 to this value afterwards, bit for bit the
             // same as here, so that FP32 passes hold no th pointers
@@ -591,6 +647,7 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
         }
       }
     }
+    }   // !ONEB
 
     // ---- advance the k window -----------------------------------------------
     // (t3 of a point outside the domain stays 0; the last level reloads its
@@ -611,6 +668,17 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
     }
   };
   for (int k = 1; k <= nk; ++k) level(k);
+  if constexpr (ONEB) {
+    LvIn last[RPW];
+    const size_t pk = (size_t)P2 * (nk - 1);
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+      last[q].rm1 = ld_lv(Rm1 + pk, bz[q]);
+      last[q].rr = ld_lv(Rx + pk, bz[q]);
+    }
+    __syncthreads();
+    update_deferred(nk, last);
+  }
 }
 
 
