@@ -190,7 +190,7 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
   // groups (fewer registers; their conservation weight reloads datr in FP64)
   using RT = typename std::conditional<EXACT, double, float>::type;
   RT datv[RPW], datrv[RPW], wkm1[RPW];
-  TT t0[G][RPW], t1[G][RPW], t2[G][RPW], t3[G][RPW], ftp[G][RPW];
+  TT t0[G][RPW], t1[G][RPW], t2[G][RPW], t3[G][RPW], t4[G][RPW], ftp[G][RPW];
 #pragma unroll
   for (int q = 0; q < RPW; ++q) {
     const int ry = wave + NW * q;
@@ -221,6 +221,7 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
       } else {
         t0[g][q] = t1[g][q] = t2[g][q] = t3[g][q] = TT(0);
       }
+      t4[g][q] = TT(0);
     }
   }
   __syncthreads();
@@ -442,6 +443,20 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
       }
     }
 
+    // The tracers of level k+3, issued now that the z sweep has used t3:
+    // they have the x and y sweeps to arrive.  (A point outside the domain
+    // keeps 0; the last levels reload level nk, unused.)
+    {
+      const size_t p3 = (size_t)P2 * (min(k + 3, nk) - 1);
+#pragma unroll
+      for (int q = 0; q < RPW; ++q) {
+        if (ok[q]) {
+#pragma unroll
+          for (int g = 0; g < G; ++g) t4[g][q] = ld_lv(tab.T[g] + p3, bt[q]);
+        }
+      }
+    }
+
     // ---- X ----------------------------------------------------------------
 #pragma unroll
     for (int q = 0; q < RPW; ++q) {
@@ -650,20 +665,13 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
     }   // !ONEB
 
     // ---- advance the k window -----------------------------------------------
-    // (t3 of a point outside the domain stays 0; the last level reloads its
-    // own, unused)
-    const size_t p3 = (size_t)P2 * (min(k + 3, nk) - 1);
 #pragma unroll
     for (int q = 0; q < RPW; ++q) {
       wkm1[q] = wv[q];
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         if constexpr (EXACT) ftp[g][q] = fbtv[g][q];
-        t0[g][q] = t1[g][q]; t1[g][q] = t2[g][q]; t2[g][q] = t3[g][q];
-      }
-      if (ok[q]) {
-#pragma unroll
-        for (int g = 0; g < G; ++g) t3[g][q] = ld_lv(tab.T[g] + p3, bt[q]);
+        t0[g][q] = t1[g][q]; t1[g][q] = t2[g][q]; t2[g][q] = t3[g][q]; t3[g][q] = t4[g][q];
       }
     }
   };
