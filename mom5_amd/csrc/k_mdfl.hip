@@ -112,7 +112,7 @@ __global__ __launch_bounds__(RX * nw_of(RPW)) void k_mdfl(
 
   int km[RPW], offz[RPW], gjr[RPW], sjr[RPW];
   bool ok[RPW], rowok[RPW];
-  double t0[RPW], t1[RPW], t2[RPW], t3[RPW], ftp[RPW], wkm1[RPW];
+  double t0[RPW], t1[RPW], t2[RPW], t3[RPW], t4[RPW], ftp[RPW], wkm1[RPW];
   double datv[RPW], datrv[RPW];
 #pragma unroll
   for (int q = 0; q < RPW; ++q) {
@@ -143,6 +143,7 @@ __global__ __launch_bounds__(RX * nw_of(RPW)) void k_mdfl(
     } else {
       t0[q] = t1[q] = t2[q] = t3[q] = 0.0;
     }
+    t4[q] = 0.0;
   }
   __syncthreads();
 
@@ -178,11 +179,13 @@ __global__ __launch_bounds__(RX * nw_of(RPW)) void k_mdfl(
     // (4-row shapes have no registers for them and load in the phases)
     constexpr bool EARLY = RPW <= 2;
     const size_t pk = (size_t)P2 * k0;
-    double lu[RPW], lrw[RPW], lre[RPW], lv[RPW], lrs[RPW], lrn[RPW], lth[RPW];
+    double lu[RPW], lrw[RPW], lre[RPW], lv[RPW], lrs[RPW], lrn[RPW], lth[RPW], lw[RPW], lr[RPW];
     if constexpr (EARLY) {
 #pragma unroll
       for (int q = 0; q < RPW; ++q) {
         const int ox = oxq[q], oy = oyq[q];
+        lw[q] = W[offz[q] + P2 * k];
+        lr[q] = R[offz[q] + pk];
         lu[q] = U[ox + pk];
         lrw[q] = R[ox + pk];
         lre[q] = R[ox + 1 + pk];
@@ -199,8 +202,8 @@ __global__ __launch_bounds__(RX * nw_of(RPW)) void k_mdfl(
       tz[q] = 0.0; rhov[q] = 1.0; dtrv[q] = 0.0; wv[q] = 0.0; fbtv[q] = 0.0;
       if (ok[q]) {
         const int o = offz[q];
-        const double w = W[o + P2 * k];
-        const double r = R[o + P2 * k0];
+        const double w = EARLY ? lw[q] : W[o + P2 * k];
+        const double r = EARLY ? lr[q] : R[o + P2 * k0];
         const double mkm1 = mk(km1, km[q]), mk0 = mk(k, km[q]);
         const double mkp1 = mk(kp1, km[q]), mkp2 = mk(kp2, km[q]);
         const double Rjp = (t0[q] - t1[q]) * mkm1 * mk0;
@@ -222,6 +225,14 @@ __global__ __launch_bounds__(RX * nw_of(RPW)) void k_mdfl(
         }
         rhov[q] = r; dtrv[q] = dtr; wv[q] = w; fbtv[q] = fbt;
       }
+    }
+
+    // the tracer of level k+3, issued now that the z sweep has used t3 (a
+    // point outside the domain keeps 0; the last levels reload level nk)
+    if constexpr (EARLY) {
+#pragma unroll
+      for (int q = 0; q < RPW; ++q)
+        if (ok[q]) t4[q] = T[offz[q] + P2 * (min(k + 3, nk) - 1)];
     }
 
     // ---- X: eastern-face flux (shuffles) and the x-sweep update ----------
@@ -339,7 +350,8 @@ __global__ __launch_bounds__(RX * nw_of(RPW)) void k_mdfl(
       ftp[q] = fbtv[q];
       wkm1[q] = wv[q];
       t0[q] = t1[q]; t1[q] = t2[q]; t2[q] = t3[q];
-      if (ok[q] && k < nk) t3[q] = T[offz[q] + P2 * (min(k + 3, nk) - 1)];
+      if constexpr (EARLY) t3[q] = t4[q];
+      else if (ok[q] && k < nk) t3[q] = T[offz[q] + P2 * (min(k + 3, nk) - 1)];
     }
   }
 }
