@@ -66,32 +66,37 @@ __device__ inline FaceC<float> face_coef_f(float massflux, float cfl) {
 // and min(lin, kr) = -med3(-lin, -kr, +inf), max(lin, kr) = -med3(-lin, -kr,
 // -inf): sel = copysign(inf, a) picks the one the sign of a asks for (a = -0
 // takes the max, whose median with 0 and a is a zero all the same), and the
-// negations are source modifiers of the medians, so one face's limited terms
-// cost a bit insert and four medians.
+// negations are source modifiers of the medians, so a limited term costs a
+// bit insert and two medians.
 __device__ inline float minmax_sel(float a) { return __builtin_copysignf(__builtin_inff(), a); }
-// d0a = d0*a is shared by the two limited fluxes of a face
+// psi*a for the neighbouring difference r; d0a = d0*a
 __device__ inline float psi_r(const FaceC<float> &c, float d0a, float a, float sel, float r) {
   const float lin = fmaf(c.d1, r, d0a);
   const float kr = c.kk * r;
   const float nm = __builtin_amdgcn_fmed3f(-lin, -kr, sel);
   return __builtin_amdgcn_fmed3f(0.0f, a, -nm);
 }
-// The upwind weights of a face with the 1/2 and the face mask folded in:
-// mp = (M + |M|)/2 * mab, mn = (M - |M|)/2 * mab (power-of-two and 0/1
-// scalings are exact, so the flux is the same value as the unfolded form).
+// The upwind weight of a face with the face mask folded in.  Of the terms
+// (M + |M|)/2*(Tpos + psiP) and (M - |M|)/2*(Tneg - psiM) only the upwind one
+// has a nonzero weight, exactly M: the flux is M*mab*(Tu +- psi) with the
+// upwind side's value and limiter, the same value as the two-term sum (up
+// to the sign of a zero).
 struct FaceW {
-  float mp, mn;
+  float wm;    // M*mab
+  float sg;    // +1 where M >= 0, -1 where M < 0
+  bool pos;    // M >= 0
 };
 __device__ inline FaceW face_w(const FaceC<float> &c, float mab) {
-  return FaceW{0.5f * (c.M + fabsf(c.M)) * mab, 0.5f * (c.M - fabsf(c.M)) * mab};
+  const bool pos = c.M >= 0.0f;
+  return FaceW{c.M * mab, pos ? 1.0f : -1.0f, pos};
 }
 __device__ inline float flux_fast(const FaceC<float> &c, const FaceW &w, float Rjm, float Rj,
                                   float Rjp, float Tpos, float Tneg) {
   const float d0a = c.d0 * Rj;
   const float sel = minmax_sel(Rj);
-  const float pP = psi_r(c, d0a, Rj, sel, Rjm);
-  const float pM = psi_r(c, d0a, Rj, sel, Rjp);
-  return fmaf(w.mp, Tpos + pP, w.mn * (Tneg - pM));
+  const float Rn = w.pos ? Rjm : Rjp;
+  const float Tu = w.pos ? Tpos : Tneg;
+  return w.wm * fmaf(w.sg, psi_r(c, d0a, Rj, sel, Rn), Tu);
 }
 
 // Level-sliced addressing: a buffer descriptor of the level (the array plus
@@ -160,11 +165,15 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
   const int ntx = J.ntx;
   constexpr bool EXACT = sizeof(TT) == 8;
   constexpr int RY = NW * RPW;
-  // FP64 groups keep two copies of the x-swept values and of the y fluxes,
-  // by level parity, where the LDS holds them: one barrier a level, and the
-  // update of a level done after the next level's barrier
-  constexpr bool ONEB =
-      EXACT && 2 * 2 * G * RY * RX * sizeof(TT) + RY * RX * sizeof(int) <= 160 * 1024;
+  // Shapes with registers to spare (one row a wavefront, at most two FP64
+  // tracers) issue the tracers of level k+3 right after the z sweep
+  // (EARLYT); the others load them at the end of the level.  Their FP64
+  // groups also keep two copies of the x-swept values and of the y fluxes, by
+  // level parity, where the LDS holds them (ONEB): one barrier a level, and
+  // the update of a level done after the next level's barrier.
+  constexpr bool EARLYT = RPW == 1 && (!EXACT || G <= 2);
+  constexpr bool ONEB = EARLYT && EXACT &&
+                        2 * 2 * G * RY * RX * sizeof(TT) + RY * RX * sizeof(int) <= 160 * 1024;
   constexpr int NB = ONEB ? 2 : 1;
   __shared__ TT s_tx[NB][G][RY][RX];
   __shared__ TT s_fy[NB][G][RY][RX];
@@ -446,7 +455,7 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
     // The tracers of level k+3, issued now that the z sweep has used t3:
     // they have the x and y sweeps to arrive.  (A point outside the domain
     // keeps 0; the last levels reload level nk, unused.)
-    {
+    if constexpr (EARLYT) {
       const size_t p3 = (size_t)P2 * (min(k + 3, nk) - 1);
 #pragma unroll
       for (int q = 0; q < RPW; ++q) {
@@ -465,7 +474,7 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
       if constexpr (EXACT) face_depths(q, kx0[q], kx1[q], kx2[q], ky0[q], ky1[q], ky2[q]);
       const TT mi = mk<TT>(k, km[q]);
       const TT mX1 = mk<TT>(k, kx1[q]);
-      mdfl64::Face cx{0.0, 0.0, 0.0, 0.0, 0.0};
+      mdfl64::Face cx{};
       double Mx = 0.0;
       FaceC<float> cxf{0.0f, 0.0f, 0.0f, 0.0f};
       if constexpr (!EXACT) {
@@ -581,7 +590,7 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
       } else {
         const bool face = ry >= 1 && ry <= RY - 3 && lane >= 2 && lane < RX - 2 && gi < d.ni &&
                           gj >= -1 && gj <= d.nj - 1;
-        mdfl64::Face cy{0.0, 0.0, 0.0, 0.0, 0.0};
+        mdfl64::Face cy{};
         if (face) {
           const double v = cur[q].v, rs = cur[q].rs, rn = cur[q].rn;
           cy = mdfl64::face(hdxtn[q] * v, mdfl64::cfl_h(v, dtime, rs + rn, hdytn[q]),
@@ -671,7 +680,15 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         if constexpr (EXACT) ftp[g][q] = fbtv[g][q];
-        t0[g][q] = t1[g][q]; t1[g][q] = t2[g][q]; t2[g][q] = t3[g][q]; t3[g][q] = t4[g][q];
+        t0[g][q] = t1[g][q]; t1[g][q] = t2[g][q]; t2[g][q] = t3[g][q];
+        if constexpr (EARLYT) t3[g][q] = t4[g][q];
+      }
+      if constexpr (!EARLYT) {
+        const size_t p3 = (size_t)P2 * (min(k + 3, nk) - 1);
+        if (ok[q]) {
+#pragma unroll
+          for (int g = 0; g < G; ++g) t3[g][q] = ld_lv(tab.T[g] + p3, bt[q]);
+        }
       }
     }
   };

@@ -12,8 +12,12 @@
 //   Rj <  0:  min(0, max(Rj, d0*Rj + d1*Rn, kk*Rn)),
 // (Rn = R_{j-1} or R_{j+1}, kk = (1-c)/(eps+c)), equal in exact arithmetic
 // wherever |Rj| >> 1e-30 and within 1e-30 elsewhere.  The face terms
-// (c, d0, d1, kk and the upwind weights with the 1/2 and the face mask
-// folded in) are tracer independent.  The remaining reciprocals (of rho_dzt,
+// (c, d0, d1, kk and the upwind weight) are tracer independent.  Of the two
+// terms only the upwind one has a nonzero weight: 0.5*(M+|M|) and
+// 0.5*(M-|M|) are exactly M and 0 for M >= 0, 0 and M for M < 0, so the flux
+// is M*mask*(T_up + psiP*Rj) or M*mask*(T_dn - psiM*Rj), the same value as
+// the two-term sum (up to the sign of a zero), with one limited term formed
+// instead of two.  The remaining reciprocals (of rho_dzt,
 // of the face's thickness sum times its width, of eps + c) use rcp64: the
 // hardware estimate refined by two Newton-Raphson steps, within one ulp of
 // the correctly rounded quotient.  Every use is a plain FP64 multiply, add or
@@ -40,7 +44,9 @@ __device__ __forceinline__ double rcp64(double x) {
 
 // Tracer-independent terms of one face.
 struct Face {
-  double mp, mn;   // 0.5*(M + |M|)*mask, 0.5*(M - |M|)*mask
+  double wm;                // M*mask, the weight of the upwind term
+  unsigned long long nsg;   // the sign bit where M < 0 (the term enters negated)
+  bool pos;                 // M >= 0: upwind is the Tpos side
   double d0, d1, kk;
 };
 
@@ -50,9 +56,9 @@ __device__ __forceinline__ Face face(double massflux, double cfl, double mab) {
   f.d0 = (2.0 - cfl) * (1.0 - cfl) * ONESIXTH;
   f.d1 = (1.0 - cfl * cfl) * ONESIXTH;
   f.kk = (1.0 - cfl) * rcp64(EPS30 + cfl);
-  const double a = fabs(massflux);
-  f.mp = 0.5 * (massflux + a) * mab;
-  f.mn = 0.5 * (massflux - a) * mab;
+  f.pos = massflux >= 0.0;
+  f.wm = massflux * mab;
+  f.nsg = f.pos ? 0ull : (1ull << 63);
   return f;
 }
 
@@ -62,8 +68,8 @@ __device__ __forceinline__ double flip(double x, unsigned long long sgn) {
   return __builtin_bit_cast(double, __builtin_bit_cast(unsigned long long, x) ^ sgn);
 }
 
-// psi*Rj for the neighbouring difference Rn; d0a = d0*Rj (shared by the two
-// limited terms of a face), sgn the sign bit of Rj.  With s = -1 where Rj's
+// psi*Rj for the neighbouring difference Rn; d0a = d0*Rj, sgn the sign bit
+// of Rj.  With s = -1 where Rj's
 // sign bit is set and +1 elsewhere (exact sign flips),
 //   psi*Rj = s*max(0, min(|Rj|, s*lin, s*kr)),
 // which is max(0, min(Rj, lin, kr)) for Rj >= 0 and min(0, max(Rj, lin, kr))
@@ -79,14 +85,15 @@ __device__ __forceinline__ double psi_r(const Face &f, double d0a, double Rj,
 
 // The limited flux: Tpos is upstream for M > 0, Tneg for M < 0; Rjm, Rj, Rjp
 // the masked differences, as the reference forms them (a dry face has Rj = 0
-// and zero weights).
+// and a zero weight).  Rjm enters the limiter of the M > 0 term, Rjp that of
+// the M < 0 term.
 __device__ __forceinline__ double flux(const Face &f, double Rjm, double Rj, double Rjp,
                                        double Tpos, double Tneg) {
   const double d0a = f.d0 * Rj;
   const unsigned long long sgn = __builtin_bit_cast(unsigned long long, Rj) & (1ull << 63);
-  const double pP = psi_r(f, d0a, Rj, sgn, Rjm);
-  const double pM = psi_r(f, d0a, Rj, sgn, Rjp);
-  return fma(f.mp, Tpos + pP, f.mn * (Tneg - pM));
+  const double Rn = f.pos ? Rjm : Rjp;
+  const double Tu = f.pos ? Tpos : Tneg;
+  return f.wm * (Tu + flip(psi_r(f, d0a, Rj, sgn, Rn), f.nsg));
 }
 
 // Courant numbers: |w*dtime/rho_dzt| of a bottom face (dtr = dtime/rho_dzt)
