@@ -317,6 +317,17 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
     by[q] = (unsigned)oyq[q] * 8u;
   }
   const double *Rx = Rr ? Rr : R;
+  // ONEB shapes issue a level's w and rho_dzt during the level before (after
+  // its z sweep, with the tracers of level k+3): the z sweep that starts a
+  // level then waits for nothing
+  double nw[RPW], nr[RPW];
+  if constexpr (ONEB) {
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+      nw[q] = ld_lv(W + P2, bz[q]);
+      nr[q] = ld_lv(R, bz[q]);
+    }
+  }
   auto lv_load = [&](int k, LvIn *L) {
     const size_t pk = (size_t)P2 * (k - 1);
     // the update's rho_dzt(taum1) and 1/rho_dzt(taup1): of the level before
@@ -324,8 +335,13 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
     const size_t pu = ONEB ? (size_t)P2 * max(k - 2, 0) : pk;
 #pragma unroll
     for (int q = 0; q < RPW; ++q) {
-      L[q].w = ld_lv(W + pk + P2, bz[q]);
-      L[q].r = ld_lv(R + pk, bz[q]);
+      if constexpr (ONEB) {
+        L[q].w = nw[q];
+        L[q].r = nr[q];
+      } else {
+        L[q].w = ld_lv(W + pk + P2, bz[q]);
+        L[q].r = ld_lv(R + pk, bz[q]);
+      }
       L[q].u = ld_lv(U + pk, bx[q]);
       L[q].rw = ld_lv(R + pk, bx[q]);
       L[q].re = ld_lv(R + pk + 1, bx[q]);
@@ -462,6 +478,15 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
         if (ok[q]) {
 #pragma unroll
           for (int g = 0; g < G; ++g) t4[g][q] = ld_lv(tab.T[g] + p3, bt[q]);
+        }
+      }
+      if constexpr (ONEB) {
+        // w and rho_dzt of level k+1 (the last level reloads its own)
+        const size_t pw = (size_t)P2 * min(k + 1, nk), pr = (size_t)P2 * min(k, nk - 1);
+#pragma unroll
+        for (int q = 0; q < RPW; ++q) {
+          nw[q] = ld_lv(W + pw, bz[q]);
+          nr[q] = ld_lv(R + pr, bz[q]);
         }
       }
     }
