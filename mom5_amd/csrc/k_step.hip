@@ -165,15 +165,17 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
   const int ntx = J.ntx;
   constexpr bool EXACT = sizeof(TT) == 8;
   constexpr int RY = NW * RPW;
-  // Shapes with registers to spare (one row a wavefront, at most two FP64
-  // tracers) issue the tracers of level k+3 right after the z sweep
-  // (EARLYT); the others load them at the end of the level.  Their FP64
-  // groups also keep two copies of the x-swept values and of the y fluxes, by
-  // level parity, where the LDS holds them (ONEB): one barrier a level, and
-  // the update of a level done after the next level's barrier.
-  constexpr bool EARLYT = RPW == 1 && (!EXACT || G <= 2);
-  constexpr bool ONEB = EARLYT && EXACT &&
+  // FP64 groups of one row a wavefront and at most two tracers (registers to
+  // spare) issue the tracers of level k+3 right after the z sweep (EARLYT;
+  // the others load them at the end of the level), keep two copies of the
+  // x-swept values and of the y fluxes, by level parity, where the LDS holds
+  // them (ONEB: one barrier a level, the update of a level done after the
+  // next level's barrier), and issue a level's w and rho_dzt during the level
+  // before (PREWR).  FP32 groups of one row a wavefront take PREWR only.
+  constexpr bool EARLYT = RPW == 1 && EXACT && G <= 2;
+  constexpr bool ONEB = EARLYT &&
                         2 * 2 * G * RY * RX * sizeof(TT) + RY * RX * sizeof(int) <= 160 * 1024;
+  constexpr bool PREWR = EXACT ? ONEB : RPW == 1;
   constexpr int NB = ONEB ? 2 : 1;
   __shared__ TT s_tx[NB][G][RY][RX];
   __shared__ TT s_fy[NB][G][RY][RX];
@@ -317,11 +319,10 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
     by[q] = (unsigned)oyq[q] * 8u;
   }
   const double *Rx = Rr ? Rr : R;
-  // ONEB shapes issue a level's w and rho_dzt during the level before (after
-  // its z sweep, with the tracers of level k+3): the z sweep that starts a
-  // level then waits for nothing
+  // PREWR shapes issue a level's w and rho_dzt during the level before (after
+  // its z sweep): the z sweep that starts a level then waits for nothing
   double nw[RPW], nr[RPW];
-  if constexpr (ONEB) {
+  if constexpr (PREWR) {
 #pragma unroll
     for (int q = 0; q < RPW; ++q) {
       nw[q] = ld_lv(W + P2, bz[q]);
@@ -335,7 +336,7 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
     const size_t pu = ONEB ? (size_t)P2 * max(k - 2, 0) : pk;
 #pragma unroll
     for (int q = 0; q < RPW; ++q) {
-      if constexpr (ONEB) {
+      if constexpr (PREWR) {
         L[q].w = nw[q];
         L[q].r = nr[q];
       } else {
@@ -395,8 +396,13 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
       const size_t pk = (size_t)P2 * (k - 1);
 #pragma unroll
       for (int q = 0; q < RPW; ++q) {
-        cur[q].w = ld_lv(W + pk + P2, bz[q]);
-        cur[q].r = ld_lv(R + pk, bz[q]);
+        if constexpr (PREWR) {
+          cur[q].w = nw[q];
+          cur[q].r = nr[q];
+        } else {
+          cur[q].w = ld_lv(W + pk + P2, bz[q]);
+          cur[q].r = ld_lv(R + pk, bz[q]);
+        }
         cur[q].u = ld_lv(U + pk, bx[q]);
         cur[q].rw = ld_lv(R + pk, bx[q]);
         cur[q].re = ld_lv(R + pk + 1, bx[q]);
@@ -480,14 +486,14 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
           for (int g = 0; g < G; ++g) t4[g][q] = ld_lv(tab.T[g] + p3, bt[q]);
         }
       }
-      if constexpr (ONEB) {
-        // w and rho_dzt of level k+1 (the last level reloads its own)
-        const size_t pw = (size_t)P2 * min(k + 1, nk), pr = (size_t)P2 * min(k, nk - 1);
+    }
+    if constexpr (PREWR) {
+      // w and rho_dzt of level k+1 (the last level reloads its own)
+      const size_t pw = (size_t)P2 * min(k + 1, nk), pr = (size_t)P2 * min(k, nk - 1);
 #pragma unroll
-        for (int q = 0; q < RPW; ++q) {
-          nw[q] = ld_lv(W + pw, bz[q]);
-          nr[q] = ld_lv(R + pr, bz[q]);
-        }
+      for (int q = 0; q < RPW; ++q) {
+        nw[q] = ld_lv(W + pw, bz[q]);
+        nr[q] = ld_lv(R + pr, bz[q]);
       }
     }
 
