@@ -171,7 +171,8 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
   // x-swept values and of the y fluxes, by level parity, where the LDS holds
   // them (ONEB: one barrier a level, the update of a level done after the
   // next level's barrier), and issue a level's w and rho_dzt during the level
-  // before (PREWR).  FP32 groups of one row a wavefront take PREWR only.
+  // before (PREWR).  FP32 groups of one row a wavefront take PREWR only, and
+  // with it the loads of the east face (u and its two rho_dzt).
   constexpr bool EARLYT = RPW == 1 && EXACT && G <= 2;
   constexpr bool ONEB = EARLYT &&
                         2 * 2 * G * RY * RX * sizeof(TT) + RY * RX * sizeof(int) <= 160 * 1024;
@@ -319,14 +320,20 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
     by[q] = (unsigned)oyq[q] * 8u;
   }
   const double *Rx = Rr ? Rr : R;
-  // PREWR shapes issue a level's w and rho_dzt during the level before (after
-  // its z sweep): the z sweep that starts a level then waits for nothing
-  double nw[RPW], nr[RPW];
+  // PREWR shapes issue a level's w and rho_dzt (FP32: and the east face's
+  // u and rho_dzt) during the level before, after its z sweep: the sweeps
+  // that start a level then wait for nothing
+  double nw[RPW], nr[RPW], nu[RPW], nrw[RPW], nre[RPW];
   if constexpr (PREWR) {
 #pragma unroll
     for (int q = 0; q < RPW; ++q) {
       nw[q] = ld_lv(W + P2, bz[q]);
       nr[q] = ld_lv(R, bz[q]);
+      if constexpr (!EXACT) {
+        nu[q] = ld_lv(U, bx[q]);
+        nrw[q] = ld_lv(R, bx[q]);
+        nre[q] = ld_lv(R + 1, bx[q]);
+      }
     }
   }
   auto lv_load = [&](int k, LvIn *L) {
@@ -403,9 +410,15 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
           cur[q].w = ld_lv(W + pk + P2, bz[q]);
           cur[q].r = ld_lv(R + pk, bz[q]);
         }
-        cur[q].u = ld_lv(U + pk, bx[q]);
-        cur[q].rw = ld_lv(R + pk, bx[q]);
-        cur[q].re = ld_lv(R + pk + 1, bx[q]);
+        if constexpr (PREWR) {
+          cur[q].u = nu[q];
+          cur[q].rw = nrw[q];
+          cur[q].re = nre[q];
+        } else {
+          cur[q].u = ld_lv(U + pk, bx[q]);
+          cur[q].rw = ld_lv(R + pk, bx[q]);
+          cur[q].re = ld_lv(R + pk + 1, bx[q]);
+        }
         cur[q].v = ld_lv(V + pk, by[q]);
         cur[q].rs = ld_lv(R + pk, by[q]);
         cur[q].rn = ld_lv(R + pk + d.NI, by[q]);
@@ -494,6 +507,11 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
       for (int q = 0; q < RPW; ++q) {
         nw[q] = ld_lv(W + pw, bz[q]);
         nr[q] = ld_lv(R + pr, bz[q]);
+        if constexpr (!EXACT) {
+          nu[q] = ld_lv(U + pr, bx[q]);
+          nrw[q] = ld_lv(R + pr, bx[q]);
+          nre[q] = ld_lv(R + pr + 1, bx[q]);
+        }
       }
     }
 
