@@ -171,6 +171,21 @@ __global__ __launch_bounds__(RX * nw_of(RPW)) void k_mdfl(
     mdytn[q] = dytn[oyq[q]];
   }
 
+  // The 1-row shape (registers to spare) issues a level's w, rho_dzt and east
+  // face loads (u and its two rho_dzt) during the level before, after its z
+  // sweep: the sweeps that start a level then wait for nothing.
+  constexpr bool PRE = RPW == 1;
+  double nw[RPW], nr[RPW], nu[RPW], nrw[RPW], nre[RPW];
+  if constexpr (PRE) {
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+      nw[q] = W[offz[q] + P2];
+      nr[q] = R[offz[q]];
+      nu[q] = U[oxq[q]];
+      nrw[q] = R[oxq[q]];
+      nre[q] = R[oxq[q] + 1];
+    }
+  }
   for (int k = 1; k <= nk; ++k) {
     const int k0 = k - 1;
     const int kp1 = min(k + 1, nk), kp2 = min(k + 2, nk), km1 = max(k - 1, 1);
@@ -184,11 +199,16 @@ __global__ __launch_bounds__(RX * nw_of(RPW)) void k_mdfl(
 #pragma unroll
       for (int q = 0; q < RPW; ++q) {
         const int ox = oxq[q], oy = oyq[q];
-        lw[q] = W[offz[q] + P2 * k];
-        lr[q] = R[offz[q] + pk];
-        lu[q] = U[ox + pk];
-        lrw[q] = R[ox + pk];
-        lre[q] = R[ox + 1 + pk];
+        if constexpr (PRE) {
+          lw[q] = nw[q]; lr[q] = nr[q];
+          lu[q] = nu[q]; lrw[q] = nrw[q]; lre[q] = nre[q];
+        } else {
+          lw[q] = W[offz[q] + P2 * k];
+          lr[q] = R[offz[q] + pk];
+          lu[q] = U[ox + pk];
+          lrw[q] = R[ox + pk];
+          lre[q] = R[ox + 1 + pk];
+        }
         lv[q] = V[oy + pk];
         lrs[q] = R[oy + pk];
         lrn[q] = R[oy + d.NI + pk];
@@ -233,6 +253,18 @@ __global__ __launch_bounds__(RX * nw_of(RPW)) void k_mdfl(
 #pragma unroll
       for (int q = 0; q < RPW; ++q)
         if (ok[q]) t4[q] = T[offz[q] + P2 * (min(k + 3, nk) - 1)];
+    }
+    if constexpr (PRE) {
+      // level k+1's (the last level reloads its own)
+      const size_t pw = (size_t)P2 * min(k + 1, nk), pr = (size_t)P2 * min(k, nk - 1);
+#pragma unroll
+      for (int q = 0; q < RPW; ++q) {
+        nw[q] = W[offz[q] + pw];
+        nr[q] = R[offz[q] + pr];
+        nu[q] = U[oxq[q] + pr];
+        nrw[q] = R[oxq[q] + pr];
+        nre[q] = R[oxq[q] + 1 + pr];
+      }
     }
 
     // ---- X: eastern-face flux (shuffles) and the x-sweep update ----------
