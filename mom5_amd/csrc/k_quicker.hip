@@ -172,11 +172,23 @@ __global__ __launch_bounds__(QRX *NW) void k_quicker_horz(
   const int km_n = s_km[min(row + 1, RY - 1)][lane];
   const int km_n2 = s_km[min(row + 2, RY - 1)][lane];
 
+  // A level's T(taum1), T(tau), u and v are issued during the level before
+  // (unconditionally, from offset 0 where the point does not exist), so
+  // that the phases of a level wait for none of them.
+  double nt1 = Tm1[off], ntt = Tt[raw], nu = U[raw], nv = V[raw];
   for (int k = 1; k <= nk; ++k) {
     const int k0 = k - 1;
     const size_t pk = (size_t)P2 * k0;
-    const double t1 = ok ? Tm1[off + pk] : 0.0;
-    const double tt = indata ? Tt[raw + pk] : 0.0;
+    const double t1 = ok ? nt1 : 0.0;
+    const double tt = indata ? ntt : 0.0;
+    const double u = nu, v = nv;
+    {
+      const size_t pn = (size_t)P2 * min(k, nk - 1);   // the last level reloads its own
+      nt1 = Tm1[off + pn];
+      ntt = Tt[raw + pn];
+      nu = U[raw + pn];
+      nv = V[raw + pn];
+    }
     s_t[row][lane] = t1;
     s_tt[row][lane] = tt;
     __syncthreads();
@@ -191,7 +203,6 @@ __global__ __launch_bounds__(QRX *NW) void k_quicker_horz(
     // x flux through the east face of this point (:2590-2606, 2610-2622)
     double fx = 0.0;
     if (xface) {
-      const double u = U[raw + pk];
       const double vel = dyte * u;
       const double upos = 0.5 * (vel + fabs(vel));
       const double uneg = 0.5 * (vel - fabs(vel));
@@ -209,7 +220,6 @@ __global__ __launch_bounds__(QRX *NW) void k_quicker_horz(
     const double fxw = lane_prev(fx);
     // y flux through the north face (:2573-2587, 2623-2635)
     if (yface) {
-      const double v = V[raw + pk];
       const double vel = dxtn * v;
       const double upos = 0.5 * (vel + fabs(vel));
       const double uneg = 0.5 * (vel - fabs(vel));
