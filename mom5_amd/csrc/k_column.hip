@@ -1220,11 +1220,12 @@ struct LimTable {
 
 __global__ void k_tmask_limit(KDom d, LimTable t, int it, int is, int ts_same,
                               const int *__restrict__ kmt) {
-  const int ii = blockIdx.x * blockDim.x + threadIdx.x;   // gi = ii - 1
-  const int gj = blockIdx.y - 1;
+  // 64 x 4 points of one level a work-group
+  const int ii = blockIdx.x * 64 + threadIdx.x;   // gi = ii - 1
+  const int gj = blockIdx.y * 4 + threadIdx.y - 1;
   const int k = blockIdx.z + 1;
   const int gi = ii - 1;
-  if (gi > d.ni - 1) return;
+  if (gi > d.ni - 1 || gj > d.nj - 1) return;
   const int o = d2(d, gi, gj), P2 = d.P2, ok = o + P2 * (k - 1);
   const double m0 = (k <= kmt[o]) ? 1.0 : 0.0;
   const double me = (k <= kmt[o + 1]) ? 1.0 : 0.0;
@@ -1805,8 +1806,8 @@ int launch_tmask_limit(mom_gpu_ctx *ctx, int ntr, const double *const *T, const 
     t.T[n] = T[n]; t.lim[n] = lim[n]; t.mn[n] = mn[n]; t.mx[n] = mx[n];
   }
   // i in isc-1..iec, j in jsc-1..jec
-  hipLaunchKernelGGL(k_tmask_limit, dim3((d.ni + 1 + 63) / 64, d.nj + 1, d.nk), dim3(64), 0,
-                     ctx->stream, d, t, it, is, ts_same, ctx->kmt);
+  hipLaunchKernelGGL(k_tmask_limit, dim3((d.ni + 1 + 63) / 64, (d.nj + 1 + 3) / 4, d.nk),
+                     dim3(64, 4), 0, ctx->stream, d, t, it, is, ts_same, ctx->kmt);
   MOM_LAUNCHED(ctx, "k_tmask_limit");
   return MOM_GPU_OK;
 }
