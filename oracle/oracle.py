@@ -183,6 +183,10 @@ class Oracle:
 
     def vert_diffuse_implicit(self, Tp1s, stfs, btfs, dtime_t, index_salt=1, aidif=1.0,
                               K33=None, rho0=1035.0, fields=None):
+        if self.case.dom["nk"] < 2:
+            # invtri (ocean_util.F90:175-236) sets the bottom row at level
+            # max(2, kmt): a column of one level has no such row
+            raise ValueError("vert_diffuse_implicit: nk >= 2 required")
         f = fields or self.case.fields
         k33 = _parr(K33) if K33 is not None else None
         self.L.orc_vert_diffuse_implicit(

@@ -343,3 +343,44 @@ def test_fp64_full_groups_of_four(oracle_lib, rows, waves):
     ref = oracle_tracer_step(oracle_lib.Oracle(case), Ts, f["stf"][:8], f["btf"][:8], case.dt)
     for n in range(8):
         assert_close_cells(got[n][:, 1:-1, 1:-1], ref[n][:, 1:-1, 1:-1], what=f"tracer {n}")
+
+
+@pytest.mark.parametrize("nk", [2, 3])
+def test_shallow_domains_default_shapes(oracle_lib, nk):
+    """Two and three levels (the fewest the library and invtri take), where
+    every level index the passes clamp (the tracers three levels down, w and
+    rho_dzt one level ahead, the deferred update of the level before) meets the
+    bottom: FP64 tracers bit for bit the per-tracer path and within 1e-12 per
+    cell of the CPU restatement, FP32 tracers within the FP32 bound of it."""
+    from mom5_amd import DeviceArray, OceanGPU
+    from mom5_amd.synth import make_case
+    case = make_case(70, 18, nk, land_frac=0.3, nts=8, seed=300 + nk, n_passive=3,
+                     stf_scale=1e-3)
+    print(f"[shallow domain 70x18x{nk}]", flush=True)
+    gpu = OceanGPU(case.dom, case.grid)
+    dev = {k: DeviceArray.from_host(v) for k, v in case.fields.items()
+           if isinstance(v, np.ndarray)}
+    Ts = [t for _, t in case.tracers]
+    dtypes = [np.float64, np.float64] + [np.float32] * (len(Ts) - 2)
+    got = [g[:, 1:-1, 1:-1] for g in _run_step(case, gpu, dev, Ts, dtypes)]
+    f = case.fields
+    plain = []
+    for T in Ts[:2]:
+        dT, th, out = DeviceArray.from_host(T), gpu.field3(), gpu.field3()
+        gpu.horz_advect_tracer("mdfl_sweby", dT, th, uhrho_et=dev["uhrho_et"],
+                               vhrho_nt=dev["vhrho_nt"], wrho_bt=dev["wrho_bt"],
+                               rho_dzt_tau=dev["rho_dzt"], dtime=case.dt)
+        gpu.update_tracer_explicit(dT, th, dev["rho_dzt"], dev["rho_dztr"], case.dt, out)
+        plain.append(out)
+    gpu.vert_diffuse_implicit(plain, [DeviceArray.from_host(s) for s in f["stf"][:2]],
+                              [DeviceArray.from_host(s) for s in f["btf"][:2]], dev["diff_cbt"],
+                              dev["rho_dzt"], dev["dzwt"], case.dt, index_salt=1)
+    gpu.sync()
+    ref = [r[:, 1:-1, 1:-1] for r in _oracle_step(case, oracle_lib.Oracle(case), Ts)]
+    for n in range(2):
+        assert np.array_equal(got[n], plain[n].to_host()[:, 1:-1, 1:-1]), f"tracer {n} vs plain"
+        assert_close_cells(got[n], ref[n], what=f"tracer {n} vs oracle")
+    for n in range(2, len(Ts)):
+        err = np.abs(got[n].astype(np.float64) - ref[n]).max() / np.abs(ref[n]).max()
+        assert err <= 1e-5, (n, err)
+    gpu.close()

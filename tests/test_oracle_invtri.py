@@ -107,3 +107,15 @@ def test_vert_diffuse_implicit_uses_nmix(oracle_lib):
     wrong = o.invtri(Ts[1].copy(), f["stf"][1], f["btf"][1], 1035.0 * f["diff_cbt"][0],
                      case.dt, f["rho_dzt"], f["dzwt"])
     assert not np.array_equal(out[1][:, 1:-1, 1:-1], wrong[:, 1:-1, 1:-1])
+
+
+def test_one_level_refused(oracle_lib):
+    """invtri sets its bottom row at level max(2, kmt): the restatement
+    refuses a one-level domain (as the library does) instead of writing past
+    the column."""
+    from mom5_amd.synth import make_case
+    case = make_case(12, 6, 1, land_frac=0.2, nts=2, seed=5, n_passive=0)
+    o = oracle_lib.Oracle(case)
+    T = case.tracers[0][1].copy()
+    with pytest.raises(ValueError):
+        o.vert_diffuse_implicit([T], case.fields["stf"][:1], case.fields["btf"][:1], case.dt)
