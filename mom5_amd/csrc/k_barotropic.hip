@@ -213,7 +213,25 @@ struct BtIn {
   const double *dxu, *dyu, *datr, *dxur, *dyur;
   const double *sf;
   const unsigned char *mask;
+  int recip;   // dxur, dyur formed as 1/(dxu + epsln), 1/(dyu + epsln), not loaded
 };
+
+constexpr double EPSLN_GRID = 1.0e-40;   // shared/constants/constants.F90:123
+
+// 1 at a data-domain point whose dxur or dyur is not, bit for bit, the
+// reciprocal ocean_grids.F90:949-951 forms; the host adds them up.
+__global__ void k_check_recip(KDom d, const double *__restrict__ dxu,
+                              const double *__restrict__ dxur, const double *__restrict__ dyu,
+                              const double *__restrict__ dyur, int *__restrict__ bad) {
+  const int ii = blockIdx.x * blockDim.x + threadIdx.x;
+  const int o = ii + d.NI * blockIdx.y;
+  if (ii >= d.NI) return;
+  const double rx = 1.0 / (dxu[o] + EPSLN_GRID), ry = 1.0 / (dyu[o] + EPSLN_GRID);
+  bad[o] = (__builtin_bit_cast(unsigned long long, rx) !=
+            __builtin_bit_cast(unsigned long long, dxur[o])) ||
+           (__builtin_bit_cast(unsigned long long, ry) !=
+            __builtin_bit_cast(unsigned long long, dyur[o]));
+}
 
 __global__ void k_bt_sfmask(KDom d, const double *__restrict__ pme,
                             const double *__restrict__ river, const double *__restrict__ source,
@@ -296,7 +314,12 @@ __device__ __forceinline__ void bt_tile(const BtJob &J, int tile) {
       F1[q] = in.forcing[o]; F2[q] = in.forcing[o + P2]; fc[q] = in.f[o];
       cori1[q] = 0.5 * dtbt * fc[q];
       cori2[q] = 1.0 / (1.0 + cori1[q] * cori1[q]);
-      dxur[q] = in.dxur[o]; dyur[q] = in.dyur[o];
+      if (in.recip) {
+        dxur[q] = 1.0 / (dxu[q] + EPSLN_GRID);
+        dyur[q] = 1.0 / (dyu[q] + EPSLN_GRID);
+      } else {
+        dxur[q] = in.dxur[o]; dyur[q] = in.dyur[o];
+      }
       umb[q] = (mb & 2) != 0;
       eta[q] = eta_src[o]; u1[q] = ud_src[o]; u2[q] = ud_src[o + P2];
       if (interior[q]) { ebv[q] = eb[o]; ub1[q] = ub[o]; ub2[q] = ub[o + P2]; }
@@ -699,6 +722,35 @@ __global__ void k_bt_closure(KDom d, ClosureIn c, double *__restrict__ eta_u,
 
 }  // namespace
 
+// Sets ctx->bt_recip: whether the context's dxur and dyur are the reciprocals
+// the blocked barotropic may form itself (then it loads two planes fewer).
+int launch_check_recip(mom_gpu_ctx *ctx) {
+  const KDom &d = ctx->kd;
+  const size_t n = (size_t)d.P2;
+  ctx->bt_recip = 0;
+  int *dev = nullptr;
+  MOM_HIP(ctx, hipMalloc(&dev, n * sizeof(int)));
+  hipLaunchKernelGGL(k_check_recip, dim3((d.NI + 127) / 128, d.NJ), dim3(128), 0, ctx->stream, d,
+                     ctx->dxu, ctx->dxur, ctx->dyu, ctx->dyur, dev);
+  hipError_t le = hipGetLastError();
+  int *host = (int *)malloc(n * sizeof(int));
+  hipError_t e1 = host ? hipMemcpyAsync(host, dev, n * sizeof(int), hipMemcpyDeviceToHost,
+                                        ctx->stream)
+                       : hipErrorOutOfMemory;
+  hipError_t e2 = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(dev);
+  if (le != hipSuccess || e1 != hipSuccess || e2 != hipSuccess) {
+    free(host);
+    return mom_fail(ctx, MOM_GPU_ERR_HIP, "reciprocal check failed: %s",
+                    hipGetErrorString(le != hipSuccess ? le : (e1 != hipSuccess ? e1 : e2)));
+  }
+  long long bad = 0;
+  for (size_t q = 0; q < n; ++q) bad += host[q];
+  free(host);
+  ctx->bt_recip = bad == 0;
+  return MOM_GPU_OK;
+}
+
 int launch_barotropic_blocked(mom_gpu_ctx *ctx, const mom_gpu_bt_params_t *p,
                               const double *eta_t, const double *eta_bar_tau,
                               const double *ud_tau, const double *forcing, const double *thicku,
@@ -723,7 +775,7 @@ int launch_barotropic_blocked(mom_gpu_ctx *ctx, const mom_gpu_bt_params_t *p,
   hipLaunchKernelGGL(k_bt_sfmask, grid, block, 0, s, d, pme, river, source, eta_smooth,
                      ctx->kmt, ctx->kmu, sfp, maskp);
   BtIn in{rho_g, patm, thicku, forcing, ctx->f, ctx->dxu, ctx->dyu, ctx->datr, ctx->dxur,
-          ctx->dyur, sfp, maskp};
+          ctx->dyur, sfp, maskp, ctx->bt_recip};
   const int xbase = d.cyclic_x ? 1 : 0, xcount = d.cyclic_x ? d.ni : d.NI;
   double *eta_buf[2] = {ctx->bt, ctx->bt + P2};
   double *ud_buf[2] = {ctx->bt + 2 * P2, ctx->bt + 4 * P2};
@@ -841,7 +893,7 @@ int launch_barotropic_ensemble(mom_gpu_ensemble *e, const mom_gpu_bt_params_t *p
                        e->lead->stream, d, x.pme, x.river, x.source, x.eta_smooth, ctx->kmt,
                        ctx->kmu, sfp, maskp);
     D.in = BtIn{x.rho_g, x.patm, x.thicku, x.forcing_bt, ctx->f, ctx->dxu, ctx->dyu, ctx->datr,
-                ctx->dxur, ctx->dyur, sfp, maskp};
+                ctx->dxur, ctx->dyur, sfp, maskp, ctx->bt_recip};
     D.eta_bar_tau = x.eta_t_bar_tau;
     D.ud_tau = x.udrho_tau;
     D.eta_buf[0] = ctx->bt;

@@ -409,6 +409,7 @@ int mom_gpu_create(const mom_gpu_domain_t *dom, const mom_gpu_grid_t *grid, mom_
                   "mom_gpu_create: %lld mask entries disagree with kmt/kmu "
                   "(expected tmask = k<=kmt, umask = k<=kmu, ocean_grids.F90:1374-1386)", bad);
   if (rc == MOM_GPU_OK) rc = launch_quicker_init(c);
+  if (rc == MOM_GPU_OK) rc = launch_check_recip(c);
   if (rc == MOM_GPU_OK) {
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) rc = mom_fail(c, MOM_GPU_ERR_HIP, "mom_gpu_create: %s", hipGetErrorString(e));

@@ -145,6 +145,9 @@ struct mom_gpu_ctx {
   // barotropic work: eta_bt(2 levels), udrho_bt(2 comps x 2 levels),
   // steady forcing, cori1, cori2
   double *bt;
+  // the grid's dxur, dyur are 1/(dxu + epsln), 1/(dyu + epsln) bit for bit
+  // (ocean_grids.F90:949-951): the blocked barotropic forms them, not loads
+  int bt_recip;
   double bt_cori_dtbt;
   // REMAP_BT_TO_BU metrics dte, dtw, dus, dun, daur (5 planes, optional)
   double *remap;
@@ -286,6 +289,7 @@ int launch_adv_vel(mom_gpu_ctx *ctx, const double *u, const double *rho_dzu, con
                    const double *river, const double *rtend, const double *msrc, double *ue,
                    double *vn, double *wb, double *div);
 int launch_copy(mom_gpu_ctx *ctx, double *dst, const double *src, size_t n);
+int launch_check_recip(mom_gpu_ctx *ctx);
 int launch_check_masks(mom_gpu_ctx *ctx, const double *tmask,
                        const double *umask, long long *bad);
 

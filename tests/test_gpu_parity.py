@@ -252,6 +252,47 @@ def test_update_ocean_barotropic(setup, stepping_a, initsum, per_launch):
     same(cv.to_host(), conv_ref)
 
 
+@pytest.mark.parametrize("exact", [True, False])
+def test_blocked_barotropic_grid_reciprocals(oracle_lib, exact):
+    """The blocked barotropic forms dxur, dyur as 1/(dxu + epsln), 1/(dyu +
+    epsln) (ocean_grids.F90:949-951) only when the context's grid holds those
+    values bit for bit, and loads them otherwise: with the grid's own
+    reciprocals, and with a third of them moved by 2**-40 relative (which
+    forming them would not reproduce; the halo columns then take the cyclic
+    wrap, the barotropic precondition), the result equals the CPU
+    restatement, which reads the grid's arrays, bit for bit."""
+    from mom5_amd import DeviceArray, OceanGPU
+    from mom5_amd.synth import make_case
+    case = make_case(60, 40, 4, land_frac=0.25, nts=12, seed=77, pme_scale=1e-4)
+    if not exact:
+        rng = np.random.default_rng(3)
+        for key in ("dxur", "dyur"):
+            a = case.grid[key].copy()
+            pick = rng.random(a.shape) < 0.3
+            a[pick] = a[pick] * (1.0 + 2.0 ** -40)
+            a[:, 0], a[:, -1] = a[:, -2], a[:, 1]
+            case.grid[key] = a
+    print(f"[blocked barotropic, grid reciprocals exact={exact}]", flush=True)
+    orc = oracle_lib.Oracle(case)
+    gpu = OceanGPU(case.dom, case.grid)
+    f = {k: DeviceArray.from_host(v) for k, v in case.fields.items() if isinstance(v, np.ndarray)}
+    gpu.set_tuning("barotropic_subcycles_per_launch", 5)
+    eta_ref, ud_ref, conv_ref = orc.update_ocean_barotropic(case.nts, case.dtbt)
+    shp = orc.shape2
+    eb, ud, cv = DeviceArray(shp), DeviceArray((2,) + shp), DeviceArray(shp)
+    gpu.update_ocean_barotropic(nts=case.nts, dtbt=case.dtbt, eta_t_tau=f["eta_t"],
+                                eta_t_bar_tau=f["eta_t_bar"], udrho_tau=f["udrho"],
+                                forcing_bt=f["forcing_bt"], thicku=f["thicku"], patm=f["patm"],
+                                pme=f["pme"], river=f["river"], source=f["source"],
+                                eta_smooth=f["eta_smooth"], rho_g=f["rho_g"], eta_t_bar_taup1=eb,
+                                udrho_taup1=ud, conv_rho_ud_t=cv)
+    gpu.sync()
+    same(eb.to_host(), eta_ref)
+    same(ud.to_host(), ud_ref)
+    same(cv.to_host(), conv_ref)
+    gpu.close()
+
+
 def test_tracer_content(setup):
     from mom5_amd import DeviceArray
     case, orc, gpu, dev = setup
