@@ -1232,20 +1232,20 @@ __global__ void k_tmask_limit(KDom d, LimTable t, int it, int is, int ts_same,
   const double mn_ = (k <= kmt[o + d.NI]) ? 1.0 : 0.0;
   const double mb = (k < d.nk && k + 1 <= kmt[o]) ? 1.0 : 0.0;
   double flagT = 0.0, flagS = 0.0;
+  // the level below's value is read at the bottom level too (from the point
+  // itself; mb = 0 there), so that a tracer's four loads, and those of the
+  // next tracers, are issued together
+  const int below = k < d.nk ? P2 : 0;
+#pragma unroll 4
   for (int n = 0; n < t.n; ++n) {
     const double *T = t.T[n];
     const double lo = t.mn[n], hi = t.mx[n];
+    const double x0 = T[ok], x1 = T[ok + 1], x2 = T[ok + d.NI], x3 = T[ok + below];
     double L = 0.0;
-    double x = T[ok];
-    if (m0 == 1.0 && (x < lo || x > hi)) L = m0;
-    x = T[ok + 1];
-    if (me == 1.0 && (x < lo || x > hi)) L = m0;
-    x = T[ok + d.NI];
-    if (mn_ == 1.0 && (x < lo || x > hi)) L = m0;
-    if (k < d.nk) {
-      x = T[ok + P2];
-      if (mb == 1.0 && (x < lo || x > hi)) L = m0;
-    }
+    if (m0 == 1.0 && (x0 < lo || x0 > hi)) L = m0;
+    if (me == 1.0 && (x1 < lo || x1 > hi)) L = m0;
+    if (mn_ == 1.0 && (x2 < lo || x2 > hi)) L = m0;
+    if (mb == 1.0 && (x3 < lo || x3 > hi)) L = m0;
     if (n == it) flagT = L;
     if (n == is) flagS = L;
     t.lim[n][ok] = L;
