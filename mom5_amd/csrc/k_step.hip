@@ -278,14 +278,16 @@ __device__ __forceinline__ void step_tile(const StepJob<TT> &J, int tile) {
     }
   }
 
-  // FP64 groups issue all of a level's 3-D loads when the level starts: w
-  // and rho_dzt of the point, u and the two rho_dzt of its east face, v and
-  // the two of its north face, rho_dzt(taum1) and 1/rho_dzt(taup1) (or
-  // rho_dzt(taup1)), so that the x, y and update phases (each after a
-  // barrier) do not each wait for their own loads with every wavefront of the
-  // work-group at once.  The loads are unconditional, from offset 0 where the
-  // face or point does not exist (the value is then unused).  The face
-  // metrics stay in registers as for FP32 groups.
+  // FP64 groups issue all of a level's 3-D loads when the level starts (w
+  // and rho_dzt of the point during the level before, PREWR): u and the two
+  // rho_dzt of its east face, v and the two of its north face,
+  // rho_dzt(taum1) and 1/rho_dzt(taup1) (or rho_dzt(taup1); with ONEB those
+  // of the level before, for its deferred update), so that the x, y and
+  // update phases (each after a barrier) do not each wait for their own loads
+  // with every wavefront of the work-group at once.  The loads are
+  // unconditional, from offset 0 where the face or point does not exist (the
+  // value is then unused).  The face metrics stay in registers as for FP32
+  // groups.
   struct LvIn {
     double w, r, u, rw, re, v, rs, rn, rm1, rr;
   };
