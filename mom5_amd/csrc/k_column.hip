@@ -1257,15 +1257,22 @@ __global__ void k_tmask_limit(KDom d, LimTable t, int it, int is, int ts_same,
 }
 
 // ---- explicit update, ocean_tracer.F90:2341-2350 ---------------------------
-__global__ void k_explicit(KDom d, const double *__restrict__ Tm1, const double *__restrict__ th,
-                           const double *__restrict__ rdz, const double *__restrict__ rdzr,
-                           double dtime, double *__restrict__ Tp1) {
-  const int gi = blockIdx.x * blockDim.x + threadIdx.x;
-  const int gj = blockIdx.y;
-  const int k = blockIdx.z;
-  if (gi >= d.ni) return;
-  const int o = d2(d, gi, gj) + d.P2 * k;
-  Tp1[o] = (rdz[o] * Tm1[o] + dtime * th[o]) * rdzr[o];
+// One work-group per compute-domain row of a level, its threads striding
+// along the row (four points in flight each): each array is read as one
+// long run per row rather than in 256-point pieces.
+__global__ __launch_bounds__(256) void k_explicit(KDom d, const double *__restrict__ Tm1,
+                                                  const double *__restrict__ th,
+                                                  const double *__restrict__ rdz,
+                                                  const double *__restrict__ rdzr,
+                                                  double dtime, double *__restrict__ Tp1) {
+  const int gj = blockIdx.x;
+  const int k = blockIdx.y;
+  const size_t row = (size_t)d2(d, 0, gj) + (size_t)d.P2 * k;
+#pragma unroll 4
+  for (int gi = threadIdx.x; gi < d.ni; gi += 256) {
+    const size_t o = row + gi;
+    Tp1[o] = (rdz[o] * Tm1[o] + dtime * th[o]) * rdzr[o];
+  }
 }
 
 // ---- masked content with double-double accumulation -------------------------
@@ -1815,8 +1822,8 @@ int launch_tmask_limit(mom_gpu_ctx *ctx, int ntr, const double *const *T, const 
 int launch_explicit(mom_gpu_ctx *ctx, const double *Tm1, const double *th, const double *rdz,
                     const double *rdzr, double dtime, double *Tp1) {
   const KDom &d = ctx->kd;
-  hipLaunchKernelGGL(k_explicit, dim3((d.ni + 255) / 256, d.nj, d.nk), dim3(256), 0, ctx->stream,
-                     d, Tm1, th, rdz, rdzr, dtime, Tp1);
+  hipLaunchKernelGGL(k_explicit, dim3(d.nj, d.nk), dim3(256), 0, ctx->stream, d, Tm1, th, rdz,
+                     rdzr, dtime, Tp1);
   MOM_LAUNCHED(ctx, "k_explicit");
   return MOM_GPU_OK;
 }
