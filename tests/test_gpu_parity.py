@@ -131,6 +131,29 @@ def test_compute_tmask_limit(setup):
     assert ref[0].sum() > 0, "limits chosen so that some cells are flagged"
 
 
+@pytest.mark.parametrize("ntr", [4, 7])
+def test_compute_tmask_limit_many_tracers(setup, ntr):
+    """Four and seven tracers (the kernel takes several tracers' loads at
+    once: whole groups of four, and a remainder), each with its own range, T
+    and S masks joined: the flags equal the CPU restatement's bit for bit."""
+    from mom5_amd import DeviceArray
+    case, orc, gpu, dev = setup
+    base = [t for _, t in case.tracers]
+    rng = np.random.default_rng(ntr)
+    Ts = [base[n % len(base)] * (1.0 + 0.1 * n) + rng.standard_normal(base[0].shape) * 0.05
+          for n in range(ntr)]
+    mins = [float(np.quantile(t, 0.1)) for t in Ts]
+    maxs = [float(np.quantile(t, 0.9)) for t in Ts]
+    ref = orc.tmask_limit(Ts, mins, maxs, 0, 1, True)
+    dTs = [DeviceArray.from_host(t) for t in Ts]
+    outs = [gpu.field3() for _ in Ts]
+    gpu.compute_tmask_limit(dTs, mins, maxs, outs, 0, 1, True)
+    gpu.sync()
+    for n in range(ntr):
+        same(outs[n].to_host(), ref[n])
+    assert all(r.sum() > 0 for r in ref)
+
+
 @pytest.mark.parametrize("limit_with_upwind", [False, True])
 def test_quicker(setup, limit_with_upwind):
     from mom5_amd import DeviceArray
