@@ -6,8 +6,8 @@
 //      two-cell halo (ocean_tracer_advect.F90:3843-3909); the T window
 //      T(k-1..k+2), the flux through the cell top and w(k-1) stay in
 //      registers from one level to the next;
-//   X  forms the flux through each cell's eastern face from wavefront
-//      shuffles (one region row = one 64-lane wavefront) and the x-sweep
+//   X  forms the flux through each cell's eastern face from DPP lane
+//      shifts (one region row = one 64-lane wavefront) and the x-sweep
 //      update (:3916-3969) -- no LDS and no barrier;
 //   Y  stages the x-swept tracer in LDS, forms the northern-face fluxes and
 //      finishes with the y-sweep, the divergence-correction removal and the
