@@ -1232,20 +1232,20 @@ __global__ void k_tmask_limit(KDom d, LimTable t, int it, int is, int ts_same,
   const double mn_ = (k <= kmt[o + d.NI]) ? 1.0 : 0.0;
   const double mb = (k < d.nk && k + 1 <= kmt[o]) ? 1.0 : 0.0;
   double flagT = 0.0, flagS = 0.0;
-  // the level below's value is read at the bottom level too (from the point
-  // itself; mb = 0 there), so that a tracer's four loads, and those of the
-  // next tracers, are issued together
-  const int below = k < d.nk ? P2 : 0;
-#pragma unroll 4
   for (int n = 0; n < t.n; ++n) {
     const double *T = t.T[n];
     const double lo = t.mn[n], hi = t.mx[n];
-    const double x0 = T[ok], x1 = T[ok + 1], x2 = T[ok + d.NI], x3 = T[ok + below];
     double L = 0.0;
-    if (m0 == 1.0 && (x0 < lo || x0 > hi)) L = m0;
-    if (me == 1.0 && (x1 < lo || x1 > hi)) L = m0;
-    if (mn_ == 1.0 && (x2 < lo || x2 > hi)) L = m0;
-    if (mb == 1.0 && (x3 < lo || x3 > hi)) L = m0;
+    double x = T[ok];
+    if (m0 == 1.0 && (x < lo || x > hi)) L = m0;
+    x = T[ok + 1];
+    if (me == 1.0 && (x < lo || x > hi)) L = m0;
+    x = T[ok + d.NI];
+    if (mn_ == 1.0 && (x < lo || x > hi)) L = m0;
+    if (k < d.nk) {
+      x = T[ok + P2];
+      if (mb == 1.0 && (x < lo || x > hi)) L = m0;
+    }
     if (n == it) flagT = L;
     if (n == is) flagS = L;
     t.lim[n][ok] = L;
@@ -1257,22 +1257,15 @@ __global__ void k_tmask_limit(KDom d, LimTable t, int it, int is, int ts_same,
 }
 
 // ---- explicit update, ocean_tracer.F90:2341-2350 ---------------------------
-// One work-group per compute-domain row of a level, its threads striding
-// along the row (four points in flight each): each array is read as one
-// long run per row rather than in 256-point pieces.
-__global__ __launch_bounds__(256) void k_explicit(KDom d, const double *__restrict__ Tm1,
-                                                  const double *__restrict__ th,
-                                                  const double *__restrict__ rdz,
-                                                  const double *__restrict__ rdzr,
-                                                  double dtime, double *__restrict__ Tp1) {
-  const int gj = blockIdx.x;
-  const int k = blockIdx.y;
-  const size_t row = (size_t)d2(d, 0, gj) + (size_t)d.P2 * k;
-#pragma unroll 4
-  for (int gi = threadIdx.x; gi < d.ni; gi += 256) {
-    const size_t o = row + gi;
-    Tp1[o] = (rdz[o] * Tm1[o] + dtime * th[o]) * rdzr[o];
-  }
+__global__ void k_explicit(KDom d, const double *__restrict__ Tm1, const double *__restrict__ th,
+                           const double *__restrict__ rdz, const double *__restrict__ rdzr,
+                           double dtime, double *__restrict__ Tp1) {
+  const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gj = blockIdx.y;
+  const int k = blockIdx.z;
+  if (gi >= d.ni) return;
+  const int o = d2(d, gi, gj) + d.P2 * k;
+  Tp1[o] = (rdz[o] * Tm1[o] + dtime * th[o]) * rdzr[o];
 }
 
 // ---- masked content with double-double accumulation -------------------------
@@ -1822,8 +1815,8 @@ int launch_tmask_limit(mom_gpu_ctx *ctx, int ntr, const double *const *T, const 
 int launch_explicit(mom_gpu_ctx *ctx, const double *Tm1, const double *th, const double *rdz,
                     const double *rdzr, double dtime, double *Tp1) {
   const KDom &d = ctx->kd;
-  hipLaunchKernelGGL(k_explicit, dim3(d.nj, d.nk), dim3(256), 0, ctx->stream, d, Tm1, th, rdz,
-                     rdzr, dtime, Tp1);
+  hipLaunchKernelGGL(k_explicit, dim3((d.ni + 255) / 256, d.nj, d.nk), dim3(256), 0, ctx->stream,
+                     d, Tm1, th, rdz, rdzr, dtime, Tp1);
   MOM_LAUNCHED(ctx, "k_explicit");
   return MOM_GPU_OK;
 }
