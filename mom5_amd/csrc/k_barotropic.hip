@@ -218,32 +218,55 @@ struct BtIn {
 
 constexpr double EPSLN_GRID = 1.0e-40;   // shared/constants/constants.F90:123
 
-// 1 at a data-domain point whose dxur or dyur is not, bit for bit, the
-// reciprocal ocean_grids.F90:949-951 forms; the host adds them up.
+// Per data-domain point: bit 0 where dxur or dyur is not, bit for bit, the
+// reciprocal ocean_grids.F90:949-951 forms; bit 1 where a metric the blocked
+// barotropic multiplies is outside LAND_BOUND in magnitude (or not a number)
+// or dxu, dyu, datr carries a sign bit.  The host ORs them up.
+constexpr double LAND_BOUND = 1.0e100;
+__device__ inline bool bounded(double x) { return fabs(x) <= LAND_BOUND; }
 __global__ void k_check_recip(KDom d, const double *__restrict__ dxu,
                               const double *__restrict__ dxur, const double *__restrict__ dyu,
-                              const double *__restrict__ dyur, int *__restrict__ bad) {
+                              const double *__restrict__ dyur, const double *__restrict__ datr,
+                              const double *__restrict__ f, int *__restrict__ bad) {
   const int ii = blockIdx.x * blockDim.x + threadIdx.x;
   const int o = ii + d.NI * blockIdx.y;
   if (ii >= d.NI) return;
   const double rx = 1.0 / (dxu[o] + EPSLN_GRID), ry = 1.0 / (dyu[o] + EPSLN_GRID);
-  bad[o] = (__builtin_bit_cast(unsigned long long, rx) !=
-            __builtin_bit_cast(unsigned long long, dxur[o])) ||
-           (__builtin_bit_cast(unsigned long long, ry) !=
-            __builtin_bit_cast(unsigned long long, dyur[o]));
+  const bool recip_bad = (__builtin_bit_cast(unsigned long long, rx) !=
+                          __builtin_bit_cast(unsigned long long, dxur[o])) ||
+                         (__builtin_bit_cast(unsigned long long, ry) !=
+                          __builtin_bit_cast(unsigned long long, dyur[o]));
+  const bool grid_bad = !(bounded(dxu[o]) && bounded(dyu[o]) && bounded(datr[o]) &&
+                          bounded(dxur[o]) && bounded(dyur[o]) && bounded(f[o]) &&
+                          !signbit(dxu[o]) && !signbit(dyu[o]) && !signbit(datr[o]));
+  bad[o] = (recip_bad ? 1 : 0) | (grid_bad ? 2 : 0);
 }
 
+// The per-call inputs that decide whether a point is inert under the
+// subcycles (mask bit 2); null in Inert::eta: no point is.
+struct Inert {
+  const double *eta, *ud, *forcing, *rho_g, *patm, *thicku;
+};
 __global__ void k_bt_sfmask(KDom d, const double *__restrict__ pme,
                             const double *__restrict__ river, const double *__restrict__ source,
                             const double *__restrict__ eta_smooth, const int *__restrict__ kmt,
                             const int *__restrict__ kmu, double *__restrict__ sf,
-                            unsigned char *__restrict__ mask) {
+                            unsigned char *__restrict__ mask, Inert x) {
   const int ii = blockIdx.x * blockDim.x + threadIdx.x, jj = blockIdx.y;
   if (ii >= d.NI) return;
   const int o = ii + d.NI * jj;
   const double tm = (1 <= kmt[o]) ? 1.0 : 0.0;
-  sf[o] = tm * (pme[o] + river[o] + source[o] - eta_smooth[o]);
-  mask[o] = (unsigned char)((1 <= kmt[o] ? 1 : 0) | (1 <= kmu[o] ? 2 : 0));
+  const double s = tm * (pme[o] + river[o] + source[o] - eta_smooth[o]);
+  sf[o] = s;
+  const bool t = 1 <= kmt[o], u = 1 <= kmu[o];
+  // inert: both masks 0, forcing and velocity zeros (either sign), sf a
+  // zero, the fields multiplied into the pressure bounded
+  const size_t P2 = (size_t)d.P2;
+  const bool inert = x.eta && !t && !u && s == 0.0 && x.forcing[o] == 0.0 &&
+                     x.forcing[o + P2] == 0.0 && x.ud[o] == 0.0 && x.ud[o + P2] == 0.0 &&
+                     bounded(x.eta[o]) && bounded(x.rho_g[o]) && bounded(x.patm[o]) &&
+                     bounded(x.thicku[o]);
+  mask[o] = (unsigned char)((t ? 1 : 0) | (u ? 2 : 0) | (inert ? 4 : 0));
 }
 
 // One launch of the blocked subcycle for one domain: nsub subcycles from s0.
@@ -255,6 +278,7 @@ struct BtJob {
   int s0, nsub, nts, step_a;
   double dtbt, cp, cc;
   int xbase, xcount, ntx, ntiles;
+  const int *land;   // per tile: 1 where bt_tile may take its land path (null: none)
 };
 
 __device__ __forceinline__ void bt_tile(const BtJob &J, int tile) {
@@ -278,6 +302,41 @@ __device__ __forceinline__ void bt_tile(const BtJob &J, int tile) {
   const int lane = threadIdx.x % BRX, wave = threadIdx.x / BRX;
   const int P2 = d.P2;
   const bool cyc = d.cyclic_x != 0;
+
+  if (J.land && J.land[tile]) {
+    // Land path: every point of the region is inert (k_bt_sfmask bit 2).
+    // There the velocity after each momentum phase is +0 (the pressure
+    // gradient is x*0.0, the forcing and the old velocity zeros, dtbt > 0,
+    // and the Coriolis factors finite), so U*dyu and U*dxu are +0 with
+    // dxu, dyu >= +0 and the divergence is +0; the corrector adds
+    // cc*(-0*0.0 + sf) = cc*sf, a signed zero, to eta, and the time sums
+    // add eta and +0.  The velocity sums start the call at +0
+    // (k_bt_acc_init) and every subcycle adds +0 to them in either path, so
+    // they are left as they are.  The same values as the full path.
+    const int vi = x_int0 - LO + lane;
+    const int col = cyc ? 1 + wrapi(vi - 1, d.ni) : vi;
+#pragma unroll
+    for (int q = 0; q < BPT; ++q) {
+      const int r = wave + (BNT / BRX) * q;
+      const int j = y_int0 - LO + r;
+      const bool inside = (cyc || (vi >= 0 && vi < d.NI)) && j >= 0 && j < d.NJ &&
+                          lane >= LO && lane < LO + TX && vi < xbase + xcount && r >= LO &&
+                          r < LO + TY;
+      if (!inside) continue;
+      const int o = col + d.NI * j;
+      double e = eta_src[o], ebq = eb[o];
+      const double s = in.sf[o];
+      for (int t = 0; t < nsub; ++t) {
+        e = e + cc * s;
+        ebq = ebq + e;
+      }
+      eta_dst[o] = e;
+      ud_dst[o] = 0.0;
+      ud_dst[o + P2] = 0.0;
+      eb[o] = ebq;
+    }
+    return;
+  }
 
   int rr[BPT], off[BPT], vj[BPT];
   bool present[BPT], interior[BPT], div0[BPT], grad0[BPT];
@@ -432,6 +491,29 @@ __device__ __forceinline__ void bt_tile(const BtJob &J, int tile) {
   }
 }
 
+// 1 for a tile of nsub subcycles whose whole region (the points bt_tile
+// loads) is inert, 0 otherwise: one work-group a tile.
+__global__ __launch_bounds__(256) void k_bt_land(KDom d, const unsigned char *__restrict__ mask,
+                                                 int nsub, int xbase, int ntx, int ntiles,
+                                                 int *__restrict__ land) {
+  const int tile = blockIdx.x;
+  if (tile >= ntiles) return;                  // the whole block
+  const int TX = bt_interior(BRX, nsub), TY = bt_interior(BRY, nsub), LO = bt_lo(nsub);
+  const int x_int0 = xbase + (tile % ntx) * TX, y_int0 = (tile / ntx) * TY;
+  const int lane = threadIdx.x % BRX;
+  const int vi = x_int0 - LO + lane;
+  const bool cyc = d.cyclic_x != 0;
+  const int col = cyc ? 1 + wrapi(vi - 1, d.ni) : vi;
+  const bool colok = cyc || (vi >= 0 && vi < d.NI);
+  int ok = 1;
+  for (int r = threadIdx.x / BRX; r < BRY; r += 256 / BRX) {
+    const int j = y_int0 - LO + r;
+    if (colok && j >= 0 && j < d.NJ && !(mask[col + d.NI * j] & 4)) ok = 0;
+  }
+  ok = __syncthreads_and(ok);
+  if (threadIdx.x == 0) land[tile] = ok;
+}
+
 // Tiles in XCD-aware order: blocks are dealt to the 8 XCDs round-robin, so
 // each XCD takes per_xcd consecutive positions of the order of 8-wide strips
 // (strip_tile) and the halo rows and columns that neighbouring tiles both
@@ -502,6 +584,7 @@ __global__ __launch_bounds__(BNT) void k_bt_block_ens(const BtDom *__restrict__ 
   J.xcount = D.xcount;
   J.ntx = ntx;
   J.ntiles = ntiles;
+  J.land = nullptr;
   bt_tile(J, b - L.first[lo]);
 }
 
@@ -731,7 +814,7 @@ int launch_check_recip(mom_gpu_ctx *ctx) {
   int *dev = nullptr;
   MOM_HIP(ctx, hipMalloc(&dev, n * sizeof(int)));
   hipLaunchKernelGGL(k_check_recip, dim3((d.NI + 127) / 128, d.NJ), dim3(128), 0, ctx->stream, d,
-                     ctx->dxu, ctx->dxur, ctx->dyu, ctx->dyur, dev);
+                     ctx->dxu, ctx->dxur, ctx->dyu, ctx->dyur, ctx->datr, ctx->f, dev);
   hipError_t le = hipGetLastError();
   int *host = (int *)malloc(n * sizeof(int));
   hipError_t e1 = host ? hipMemcpyAsync(host, dev, n * sizeof(int), hipMemcpyDeviceToHost,
@@ -744,10 +827,11 @@ int launch_check_recip(mom_gpu_ctx *ctx) {
     return mom_fail(ctx, MOM_GPU_ERR_HIP, "reciprocal check failed: %s",
                     hipGetErrorString(le != hipSuccess ? le : (e1 != hipSuccess ? e1 : e2)));
   }
-  long long bad = 0;
-  for (size_t q = 0; q < n; ++q) bad += host[q];
+  int bad = 0;
+  for (size_t q = 0; q < n; ++q) bad |= host[q];
   free(host);
-  ctx->bt_recip = bad == 0;
+  ctx->bt_recip = (bad & 1) == 0;
+  ctx->bt_grid_ok = (bad & 2) == 0;
   return MOM_GPU_OK;
 }
 
@@ -772,10 +856,18 @@ int launch_barotropic_blocked(mom_gpu_ctx *ctx, const mom_gpu_bt_params_t *p,
   // ping-pong state)
   double *sfp = ctx->bt + 6 * P2;
   unsigned char *maskp = (unsigned char *)(ctx->bt + 7 * P2);
+  // the land path's conditions that do not depend on the data
+  const bool land_ok = ctx->tune.bt_land_tiles && ctx->bt_grid_ok && p->dtbt > 0.0 &&
+                       std::isfinite(c_pred) && std::isfinite(c_corr);
+  const Inert inert = land_ok ? Inert{eta_bar_tau, ud_tau, forcing, rho_g, patm, thicku} : Inert{};
   hipLaunchKernelGGL(k_bt_sfmask, grid, block, 0, s, d, pme, river, source, eta_smooth,
-                     ctx->kmt, ctx->kmu, sfp, maskp);
+                     ctx->kmt, ctx->kmu, sfp, maskp, inert);
   BtIn in{rho_g, patm, thicku, forcing, ctx->f, ctx->dxu, ctx->dyu, ctx->datr, ctx->dxur,
           ctx->dyur, sfp, maskp, ctx->bt_recip};
+  // tile flags in plane 8 (unused by the blocked path): at most one int a
+  // point, as a tile's interior holds at least one point
+  int *landp = land_ok ? (int *)(ctx->bt + 8 * P2) : nullptr;
+  int land_nsub = 0;
   const int xbase = d.cyclic_x ? 1 : 0, xcount = d.cyclic_x ? d.ni : d.NI;
   double *eta_buf[2] = {ctx->bt, ctx->bt + P2};
   double *ud_buf[2] = {ctx->bt + 2 * P2, ctx->bt + 4 * P2};
@@ -785,8 +877,16 @@ int launch_barotropic_blocked(mom_gpu_ctx *ctx, const mom_gpu_bt_params_t *p,
     const int nsub = std::min(hsub, p->nts - s0 + 1);
     const int TX = bt_interior(BRX, nsub), TY = bt_interior(BRY, nsub);
     const int ntx = (xcount + TX - 1) / TX, nty = (d.NJ + TY - 1) / TY;
+    if (landp && nsub != land_nsub) {
+      // stream order: the launches that read the previous flags are done
+      hipLaunchKernelGGL(k_bt_land, dim3(ntx * nty), dim3(256), 0, s, d, maskp, nsub, xbase,
+                         ntx, ntx * nty, landp);
+      MOM_LAUNCHED(ctx, "k_bt_land");
+      land_nsub = nsub;
+    }
     const BtJob J{d, in, eta_src, ud_src, eta_buf[which], ud_buf[which], eta_bar, ud, s0, nsub,
-                  p->nts, p->stepping_a, p->dtbt, c_pred, c_corr, xbase, xcount, ntx, ntx * nty};
+                  p->nts, p->stepping_a, p->dtbt, c_pred, c_corr, xbase, xcount, ntx, ntx * nty,
+                  landp};
     const int per_xcd = (ntx * nty + 7) / 8;
     hipLaunchKernelGGL(k_bt_block, dim3(8 * per_xcd), dim3(BNT), 0, s, J, per_xcd);
     MOM_LAUNCHED(ctx, "k_bt_block");
@@ -891,7 +991,7 @@ int launch_barotropic_ensemble(mom_gpu_ensemble *e, const mom_gpu_bt_params_t *p
     unsigned char *maskp = (unsigned char *)(ctx->bt + 7 * P2);
     hipLaunchKernelGGL(k_bt_sfmask, dim3((d.NI + 255) / 256, d.NJ), dim3(256), 0,
                        e->lead->stream, d, x.pme, x.river, x.source, x.eta_smooth, ctx->kmt,
-                       ctx->kmu, sfp, maskp);
+                       ctx->kmu, sfp, maskp, Inert{});
     D.in = BtIn{x.rho_g, x.patm, x.thicku, x.forcing_bt, ctx->f, ctx->dxu, ctx->dyu, ctx->datr,
                 ctx->dxur, ctx->dyur, sfp, maskp, ctx->bt_recip};
     D.eta_bar_tau = x.eta_t_bar_tau;

@@ -136,6 +136,10 @@ static int *tuning_slot(mom_gpu_ctx *ctx, const char *knob, int *lo, int *hi) {
     *lo = 1; *hi = 2;
     return &ctx->tune.step_streams;
   }
+  if (!std::strcmp(knob, "barotropic_land_tiles")) {
+    *lo = 0; *hi = 1;
+    return &ctx->tune.bt_land_tiles;
+  }
   return nullptr;
 }
 
@@ -357,6 +361,7 @@ int mom_gpu_create(const mom_gpu_domain_t *dom, const mom_gpu_grid_t *grid, mom_
   c->tune.vdiff_merge = 0;
   c->tune.vdiff_solver = 1;
   c->tune.step_streams = 1;
+  c->tune.bt_land_tiles = 1;
   const size_t P2 = (size_t)k.P2, nk = (size_t)k.nk;
 
 #define CK(call)                                                                      \

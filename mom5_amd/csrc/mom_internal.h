@@ -148,6 +148,9 @@ struct mom_gpu_ctx {
   // the grid's dxur, dyur are 1/(dxu + epsln), 1/(dyu + epsln) bit for bit
   // (ocean_grids.F90:949-951): the blocked barotropic forms them, not loads
   int bt_recip;
+  // the grid metrics the blocked barotropic multiplies are bounded by 1e100
+  // and dxu, dyu, datr carry no sign bit: the condition its land path needs
+  int bt_grid_ok;
   double bt_cori_dtbt;
   // REMAP_BT_TO_BU metrics dte, dtw, dus, dun, daur (5 planes, optional)
   double *remap;
@@ -178,6 +181,7 @@ struct mom_gpu_ctx {
     int vdiff_merge;               // mixed steps: all tracers in one diffusion launch (0, 1)
     int vdiff_solver;              // FP32 diffusion: 0 lane per column + scratch, 1 registers
     int step_streams;              // 2: a step's FP32 tracers on the second stream
+    int bt_land_tiles;             // blocked barotropic: all-land tiles take the short path (0, 1)
   } tune;
   char err[512];
 };

@@ -316,6 +316,66 @@ def test_blocked_barotropic_grid_reciprocals(oracle_lib, exact):
     gpu.close()
 
 
+def _land_tiles(case, nsub):
+    """Tiles of the blocked barotropic (64x32 regions, nsub subcycles) whose
+    every point is inert: both masks 0, forcing and velocity zeros."""
+    g, f = case.grid, case.fields
+    inert = ((g["kmt"] == 0) & (g["kmu"] == 0) & (f["forcing_bt"][0] == 0.0)
+             & (f["forcing_bt"][1] == 0.0) & (f["udrho"][0] == 0.0) & (f["udrho"][1] == 0.0))
+    NJ, NI = inert.shape
+    ni = NI - 2
+    TX, TY, LO = 64 - 2 * nsub - 1, 32 - 2 * nsub - 1, nsub + 1
+    n = 0
+    for y0 in range(0, NJ, TY):
+        for x0 in range(1, 1 + ni, TX):
+            cols = 1 + (np.arange(x0 - LO, x0 - LO + 64) - 1) % ni
+            rows = np.arange(y0 - LO, y0 - LO + 32)
+            rows = rows[(rows >= 0) & (rows < NJ)]
+            n += bool(inert[np.ix_(rows, cols)].all())
+    return n
+
+
+@pytest.mark.parametrize("stepping_a", [False, True])
+def test_blocked_barotropic_land_tiles(oracle_lib, stepping_a):
+    """On a grid with 60% land, regions of the blocked barotropic made of
+    inert points only (both masks 0, zero forcing and velocity) take the
+    short path that only accumulates the height: with it and without it
+    (launches of 5, 5 and 2 subcycles, so the tile flags are formed for two
+    tilings) the result equals the CPU restatement bit for bit, signed zeros
+    included (eta, sf and the forcing hold zeros of both signs on land)."""
+    from mom5_amd import DeviceArray, OceanGPU
+    from mom5_amd.synth import make_case
+    case = make_case(600, 300, 4, land_frac=0.6, nts=12, seed=5, pme_scale=1e-4)
+    assert _land_tiles(case, 5) > 0 and _land_tiles(case, 2) > 0
+    print(f"[blocked barotropic, land tiles, stepping_a={stepping_a}]", flush=True)
+    orc = oracle_lib.Oracle(case)
+    gpu = OceanGPU(case.dom, case.grid)
+    f = {k: DeviceArray.from_host(v) for k, v in case.fields.items() if isinstance(v, np.ndarray)}
+    gpu.set_tuning("barotropic_subcycles_per_launch", 5)
+    eta_ref, ud_ref, conv_ref = orc.update_ocean_barotropic(case.nts, case.dtbt,
+                                                            stepping_a=stepping_a)
+    shp = orc.shape2
+    got = {}
+    for land in (1, 0):
+        gpu.set_tuning("barotropic_land_tiles", land)
+        eb, ud, cv = DeviceArray(shp), DeviceArray((2,) + shp), DeviceArray(shp)
+        gpu.update_ocean_barotropic(nts=case.nts, dtbt=case.dtbt, eta_t_tau=f["eta_t"],
+                                    eta_t_bar_tau=f["eta_t_bar"], udrho_tau=f["udrho"],
+                                    forcing_bt=f["forcing_bt"], thicku=f["thicku"],
+                                    patm=f["patm"], pme=f["pme"], river=f["river"],
+                                    source=f["source"], eta_smooth=f["eta_smooth"],
+                                    rho_g=f["rho_g"], eta_t_bar_taup1=eb, udrho_taup1=ud,
+                                    conv_rho_ud_t=cv, stepping_a=stepping_a)
+        gpu.sync()
+        got[land] = [x.to_host() for x in (eb, ud, cv)]
+        same(got[land][0], eta_ref)
+        same(got[land][1], ud_ref)
+        same(got[land][2], conv_ref)
+    for a, b in zip(got[1], got[0]):     # the short path's bits, zero signs too
+        assert np.array_equal(a.view(np.int64), b.view(np.int64))
+    gpu.close()
+
+
 def test_tracer_content(setup):
     from mom5_amd import DeviceArray
     case, orc, gpu, dev = setup
