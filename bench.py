@@ -608,7 +608,7 @@ def _tune_summary(rep):
 TUNING_KEYS = ("barotropic_subcycles_per_launch", "mdfl_rows_per_wave", "step_group_f64",
                "step_group_f32", "step_rows_per_wave", "step_rows_per_wave_f32",
                "step_waves_f32", "step_waves_f64", "vdiff_prefetch", "vdiff_block",
-               "vdiff_merge", "vdiff_solver", "step_streams")
+               "vdiff_merge", "vdiff_solver", "step_streams", "barotropic_land_tiles")
 
 
 def run_plain(args, w):
