@@ -316,7 +316,7 @@ def test_blocked_barotropic_grid_reciprocals(oracle_lib, exact):
     gpu.close()
 
 
-def _land_tiles(case, nsub):
+def _land_tiles(case, nsub, cyclic=True):
     """Tiles of the blocked barotropic (64x32 regions, nsub subcycles) whose
     every point is inert: both masks 0, forcing and velocity zeros."""
     g, f = case.grid, case.fields
@@ -325,29 +325,34 @@ def _land_tiles(case, nsub):
     NJ, NI = inert.shape
     ni = NI - 2
     TX, TY, LO = 64 - 2 * nsub - 1, 32 - 2 * nsub - 1, nsub + 1
+    xs = range(1, 1 + ni, TX) if cyclic else range(0, NI, TX)
     n = 0
     for y0 in range(0, NJ, TY):
-        for x0 in range(1, 1 + ni, TX):
-            cols = 1 + (np.arange(x0 - LO, x0 - LO + 64) - 1) % ni
+        for x0 in xs:
+            cols = np.arange(x0 - LO, x0 - LO + 64)
+            cols = 1 + (cols - 1) % ni if cyclic else cols[(cols >= 0) & (cols < NI)]
             rows = np.arange(y0 - LO, y0 - LO + 32)
             rows = rows[(rows >= 0) & (rows < NJ)]
             n += bool(inert[np.ix_(rows, cols)].all())
     return n
 
 
-@pytest.mark.parametrize("stepping_a", [False, True])
-def test_blocked_barotropic_land_tiles(oracle_lib, stepping_a):
+@pytest.mark.parametrize("stepping_a,cyclic", [(False, True), (True, True), (False, False)])
+def test_blocked_barotropic_land_tiles(oracle_lib, stepping_a, cyclic):
     """On a grid with 60% land, regions of the blocked barotropic made of
     inert points only (both masks 0, zero forcing and velocity) take the
     short path that only accumulates the height: with it and without it
     (launches of 5, 5 and 2 subcycles, so the tile flags are formed for two
     tilings) the result equals the CPU restatement bit for bit, signed zeros
-    included (eta, sf and the forcing hold zeros of both signs on land)."""
+    included (eta, sf and the forcing hold zeros of both signs on land), on
+    a cyclic domain and on a closed basin."""
     from mom5_amd import DeviceArray, OceanGPU
     from mom5_amd.synth import make_case
-    case = make_case(600, 300, 4, land_frac=0.6, nts=12, seed=5, pme_scale=1e-4)
-    assert _land_tiles(case, 5) > 0 and _land_tiles(case, 2) > 0
-    print(f"[blocked barotropic, land tiles, stepping_a={stepping_a}]", flush=True)
+    case = make_case(600, 300, 4, land_frac=0.6, nts=12, seed=5, pme_scale=1e-4,
+                     cyclic_x=cyclic)
+    assert _land_tiles(case, 5, cyclic) > 0 and _land_tiles(case, 2, cyclic) > 0
+    print(f"[blocked barotropic, land tiles, stepping_a={stepping_a}, cyclic={cyclic}]",
+          flush=True)
     orc = oracle_lib.Oracle(case)
     gpu = OceanGPU(case.dom, case.grid)
     f = {k: DeviceArray.from_host(v) for k, v in case.fields.items() if isinstance(v, np.ndarray)}
